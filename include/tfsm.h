@@ -1,0 +1,182 @@
+/* tfsm.h -- C ABI of the MI355X ETD time-stepping engine (libtfsm.so).
+ *
+ * Plain C types only: device buffers are raw pointers (typically torch tensor data_ptr()), the
+ * stream is a hipStream_t passed as void*, sizes are integers.  Every function returns 0 on
+ * success or a negative TFSM_E* code; tfsm_last_error() returns a thread-local message for the
+ * last failure.  All work is asynchronous on the given stream; nothing synchronises the host.
+ *
+ * The reference (qiauil/torchfsm) is pure Python over PyTorch and binds no FFI; each entry point
+ * below names the reference Python interface whose work it replaces.
+ *
+ * Layout: a half spectrum of a batch is [nbatch][C][N0]...[N_{D-2}][P] complex64 (interleaved
+ * float2), the last axis holding k = 0..N_{D-1}/2 and P = tfsm_plan_pitch() >= N_{D-1}/2+1.
+ * Real fields are [nbatch][C][N0]...[N_{D-1}] float32, contiguous.
+ */
+#ifndef TFSM_H
+#define TFSM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TFSM_OK 0
+#define TFSM_E_INVALID -1     /* bad argument or unsupported configuration (ValueError) */
+#define TFSM_E_STATE -2       /* call out of order, e.g. stepping before a scheme is set */
+#define TFSM_E_DEVICE -3      /* HIP runtime error (launch, copy, allocation) */
+#define TFSM_E_OOM -4         /* device allocation failed (RuntimeError wrapping OOM) */
+
+/* nonlinear term of the operator (the dealiased NonlinearFunc of the closure) */
+#define TFSM_TERM_NONE 0
+#define TFSM_TERM_CONV 1      /* Convection: sum_i u_i d_i u_c, C == D   (_convection.py:18-46) */
+#define TFSM_TERM_VC 2        /* VorticityConvection, D == 2, C == 1     (_navier_stokes.py:27-46) */
+
+/* linear symbols (LinearCoef cores) */
+#define TFSM_LIN_LAP 1        /* Laplacian       (_laplacian.py:12-15)  */
+#define TFSM_LIN_BIHARM 2     /* Biharmonic      (_biharmonic.py:13-16) */
+#define TFSM_LIN_UNIT 3       /* ImplicitSource() unit symbol (_source.py:9-17) */
+
+/* time integration schemes (ETDRKIntegrator / SETDRKIntegrator members) */
+#define TFSM_ETDRK0 0
+#define TFSM_ETDRK1 1
+#define TFSM_ETDRK2 2
+#define TFSM_SETDRK1 3
+#define TFSM_SETDRK2 4
+#define TFSM_SETDRK3 5
+#define TFSM_SETDRK4 6
+
+typedef struct tfsm_plan tfsm_plan;
+typedef struct tfsm_graph tfsm_graph;
+
+/* Periodic box and batch: MeshGrid / FourierMesh (mesh.py:9-146, 294-344) plus the (B, C) of the
+ * state registered by OperatorLike.register_mesh (_base.py:581-624). */
+typedef struct tfsm_geometry {
+  int32_t ndim;                      /* 1, 2 or 3 */
+  int32_t n[3];                      /* points per axis, powers of two */
+  double length[3];                  /* end - start per axis */
+  int32_t batch;                     /* B */
+  int32_t channels;                  /* C */
+  const double* length_per_sample;   /* 1D only, optional host [batch]: per-sample domain length */
+} tfsm_geometry;
+
+/* Operator composition after register_mesh (_base.py:339-406): L = sum coef_t * symbol_t in
+ * generator order, N = nonlinear_coef * term(dealiased u^) (+ explicit sources, added with
+ * tfsm_plan_add_source). */
+typedef struct tfsm_terms {
+  int32_t nonlinear;                 /* TFSM_TERM_* */
+  float nonlinear_coef;
+  double dealias_rate;               /* 2/3 by default (_base.py:317) */
+  int32_t n_linear;                  /* 0..4 */
+  int32_t linear_kind[4];            /* TFSM_LIN_* */
+  const float* linear_coef;          /* host [n_linear][coef_batch] */
+  int32_t coef_batch;                /* 1, or batch for per-sample coefficients (batched nu) */
+} tfsm_terms;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* tfsm_last_error(void);
+
+/* Library version, (major << 16) | minor. */
+int tfsm_version(void);
+
+/* Compile an operator on a mesh into a plan: symbols, dealias limits, twiddle tables, L.
+ * Replaces OperatorLike.register_mesh + _build_linear_coefs + _build_nonlinear_funcs
+ * (_base.py:339-406, 581-624) and the FourierMesh symbol caches (mesh.py:399-461). */
+int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* stream, tfsm_plan** out);
+
+/* Free a plan and its tables.  (Reference: garbage collection of _state_dict, _base.py:307-315.) */
+int tfsm_plan_destroy(tfsm_plan* plan);
+
+/* Pitch P of the last spectral axis (complex elements). */
+int64_t tfsm_plan_pitch(const tfsm_plan* plan);
+
+/* Complex elements of one half-spectrum state [batch][C][...][P]. */
+int64_t tfsm_plan_state_elems(const tfsm_plan* plan);
+
+/* Bytes of caller-provided workspace that tfsm_step / tfsm_apply need. */
+int64_t tfsm_plan_workspace_bytes(const tfsm_plan* plan);
+
+/* Bytes of scratch tfsm_inverse needs for nbatch blocks of [C][N...]. */
+int64_t tfsm_plan_inverse_scratch_bytes(const tfsm_plan* plan, int32_t nbatch);
+
+/* Column tile width of the fused 2D column kernel (8 or 16); set by the autotuner. */
+int tfsm_plan_set_tile(tfsm_plan* plan, int32_t col_tile);
+
+/* ETD coefficients on the device: SETDRK1-4 as float64 contour means of n_points points on a
+ * circle of radius `radius`; ETDRK0-2 with the reference's complex64 closed forms.
+ * Replaces OperatorLike._build_integrator (_base.py:441-526) with ETDRK0/1/2.__init__
+ * (_etdrk.py:15-70) and UnCachedSETDRK1-4.__init__ (_uncached.py:20-211). */
+int tfsm_plan_set_scheme(tfsm_plan* plan, int32_t scheme, double dt, int32_t n_points, double radius, void* stream);
+
+/* Copy coefficient table `slot` (0 E, 1 E2, 2..7 coef_1..coef_6) into out: float32
+ * [coef_batch][modes] with modes = prod(N[:-1]) * P.  Inspection of the integrator's
+ * _exp_term / _half_exp_term / _coef_i attributes (_etdrk.py:21-70, _uncached.py:30-211). */
+int tfsm_plan_coefficients(tfsm_plan* plan, int32_t slot, float* out, void* stream);
+
+/* Add an explicit source term coef * fftn(src) to every nonlinear evaluation.  src: device
+ * float32 [src_batch][src_channels][N...] (src_batch in {1, B}, src_channels in {1, C}); the
+ * spectrum is computed here on the device.  Replaces _ExplicitSourceCore (_base.py:994-1015). */
+int tfsm_plan_add_source(tfsm_plan* plan, const float* src, int32_t src_batch, int32_t src_channels, float coef,
+                         void* stream);
+
+/* Real fields -> half spectra, unnormalised: nbatch blocks of [C][N...].
+ * Replaces FourierMesh.fft (mesh.py:481-485) for real input. */
+int tfsm_forward(tfsm_plan* plan, const float* u, void* uh, int32_t nbatch, void* stream);
+
+/* Half spectra -> real fields, scaled by 1 / prod(N): ifftn(.).real (mesh.py:487-491, and the
+ * output path of OperatorLike.integrate, _base.py:751).  scratch >= tfsm_plan_inverse_scratch_bytes. */
+int tfsm_inverse(tfsm_plan* plan, const void* uh, float* u, int32_t nbatch, void* scratch, void* stream);
+
+/* Complex fields -> complex spectra (dir = -1, unnormalised) or back (dir = +1, scaled by
+ * 1 / prod(N)) over the last D axes of contiguous [nbatch][C][N...] complex64 arrays; out may
+ * equal in.  Replaces FourierMesh.fft / .ifft for complex input (mesh.py:481-491). */
+int tfsm_c2c(tfsm_plan* plan, const void* in, void* out, int32_t nbatch, int32_t dir, void* stream);
+
+/* Half -> full complex spectrum with Hermitian fill (return_in_fourier, _base.py:748-749). */
+int tfsm_half_to_full(tfsm_plan* plan, const void* uh, void* full, int32_t nbatch, void* stream);
+
+/* Full complex spectrum -> half spectrum (Hermitian part; exact for spectra of real fields).
+ * Entry of u_0_fft into OperatorLike.integrate (_base.py:727-728). */
+int tfsm_full_to_half(tfsm_plan* plan, const void* full, void* uh, int32_t nbatch, void* stream);
+
+/* Advance the state uh (in place) by nsteps steps of the plan's scheme.  If rec_slot (device
+ * int32 [nsteps + 1]) is given, the state before step i is copied to traj[b][rec_slot[i]] when
+ * rec_slot[i] >= 0 (i = nsteps: the final state); traj is [B][T][C][...][P].
+ * Replaces the hot loop of OperatorLike.integrate (_base.py:729-746) with integrator.step
+ * (_setdrk_step.py:5-82, _etdrk.py:23-82), the nonlinear closure (_base.py:375-403) and
+ * _TrajRecorder.record (traj_recorder.py:46-55). */
+int tfsm_step(tfsm_plan* plan, void* uh, int32_t nsteps, const int32_t* rec_slot, void* traj, int32_t T, void* ws,
+              void* stream);
+
+/* Like tfsm_step for 1D plans, with a per-sample step count (device int32 [B]); samples stop
+ * independently (continuous batching of 1D requests). */
+int tfsm_step_varying(tfsm_plan* plan, void* uh, const int32_t* nsteps_per_sample, int32_t max_steps, void* ws,
+                      void* stream);
+
+/* out = L uh + N(uh) (half spectra), or only the linear / nonlinear part when the operator has
+ * only that part.  Replaces OperatorLike.__call__ (_base.py:753-790) via _build_operator
+ * (_base.py:408-439). */
+int tfsm_apply(tfsm_plan* plan, const void* uh, void* out, void* ws, void* stream);
+
+/* out = uh * where(L == 0, 1, 1 / L) (half spectra).  Replaces LinearOperator.solve
+ * (_base.py:217-262). */
+int tfsm_solve(tfsm_plan* plan, const void* uh, void* out, void* stream);
+
+/* Launch one pass of the fused step on the current state/workspace, for profiling: pass 0 = the
+ * row kernel (2D), pass 1 = the column kernel with stage code `stage` (0 prologue, 7..10 the four
+ * SETDRK4 evaluations), pass 2 = the 1D persistent stepper for one step.  (No reference
+ * counterpart: instrumentation of the engine's own kernels.) */
+int tfsm_launch_pass(tfsm_plan* plan, void* uh, void* ws, int32_t pass_id, int32_t stage, void* stream);
+
+/* Capture nsteps of tfsm_step (no recording) on uh/ws into a HIP graph for repeated launch.
+ * (No reference counterpart: the reference launches ~116 kernels per step from Python.) */
+int tfsm_graph_create(tfsm_plan* plan, void* uh, int32_t nsteps, void* ws, void* stream, tfsm_graph** out);
+int tfsm_graph_launch(tfsm_graph* graph, void* stream);
+int tfsm_graph_destroy(tfsm_graph* graph);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TFSM_H */

@@ -1,0 +1,351 @@
+"""refpath: PyTorch restatement of torchfsm's ETD stepping schedule (oracle; test infrastructure).
+
+This module reproduces what the reference computes, in the reference's order and dtypes, so that
+on the same CPU it matches the reference's float32 output bit for bit (checked against the golden
+fixtures in tests/golden).  It is the checker for the HIP engine, the torch-GPU baseline and the CPU
+baseline; it is never imported by the product package.
+
+An operator is described by a list of terms in generator order, each ``(coef, kind, payload)``:
+  kind "lap"     Laplacian symbol, concatenated over channels   (_laplacian.py:12-15)
+  kind "biharm"  lap*lap, concatenated over channels            (_biharmonic.py:13-16)
+  kind "unit"    ones_like(bf_x) (ImplicitSource() drag)        (_source.py:9-17)
+  kind "conv"    Convection, dealiased input                    (_convection.py:18-46)
+  kind "vc"      VorticityConvection, dealiased input           (_navier_stokes.py:27-46)
+  kind "src"     ExplicitSource; payload = spatial source field (_base.py:994-1015)
+"""
+
+import math
+from typing import Callable, List, Optional, Sequence, Tuple
+
+import torch
+
+Term = Tuple[object, str, Optional[torch.Tensor]]
+LINEAR_KINDS = ("lap", "biharm", "unit")
+NONLINEAR_KINDS = ("conv", "vc", "src")
+
+
+class Mesh:
+    """Wavenumbers, symbols, dealias mask and transforms of one periodic box (mesh.py:148-491)."""
+
+    def __init__(self, mesh_info: Sequence[Tuple[float, float, int]], device="cpu", dtype=torch.float32):
+        self.mesh_info = [tuple(m) for m in mesh_info]
+        self.device = torch.device(device)
+        self.dtype = dtype
+        self.n_dim = len(self.mesh_info)
+        self.fft_dim = tuple(-(i + 1) for i in range(self.n_dim))  # mesh.py:343
+        self._bf = {}
+        self._cache = {}
+
+    # FFTFrequency.__getitem__ (mesh.py:184-191): fftfreq(n, d=(end-start)/n)
+    def freq(self, i):
+        start, end, n = self.mesh_info[i]
+        return torch.fft.fftfreq(n, (end - start) / n, device=self.device, dtype=self.dtype)
+
+    # BroadcastedFFTFrequency.__getitem__ (mesh.py:250-254)
+    def bf(self, i):
+        if i not in self._bf:
+            shape = [1] * (self.n_dim + 2)
+            shape[i + 2] = self.mesh_info[i][2]
+            self._bf[i] = self.freq(i).reshape(*shape)
+        return self._bf[i]
+
+    # BroadcastedFFTFrequency.bf_vector (mesh.py:256-266)
+    def bf_vector(self):
+        if "bfv" not in self._cache:
+            full = (1, 1) + tuple(m[2] for m in self.mesh_info)
+            parts = []
+            for i in range(self.n_dim):
+                rep = list(full)
+                rep[i + 2] = 1
+                parts.append(self.bf(i).repeat(*rep))
+            self._cache["bfv"] = torch.cat(parts, dim=1)
+        return self._cache["bfv"]
+
+    # FourierMesh.grad (mesh.py:399-404)
+    def grad(self, i, order):
+        key = ("grad", i, order)
+        if key not in self._cache:
+            self._cache[key] = (2j * torch.pi * self.bf(i)) ** order
+        return self._cache[key]
+
+    # FourierMesh.nabla / laplacian (mesh.py:406-426): python sum over dims starting from 0
+    def lap(self):
+        if "lap" not in self._cache:
+            self._cache["lap"] = sum([self.grad(i, 2) for i in range(self.n_dim)])
+        return self._cache["lap"]
+
+    # FourierMesh.invert_laplacian (mesh.py:413-419)
+    def inv_lap(self):
+        if "ilap" not in self._cache:
+            lp = self.lap()
+            self._cache["ilap"] = torch.where(lp == 0, 1.0, 1 / lp)
+        return self._cache["ilap"]
+
+    # FourierMesh.nabla_vector (mesh.py:436-441)
+    def nabla_vector(self, order):
+        key = ("nv", order)
+        if key not in self._cache:
+            self._cache[key] = (2j * torch.pi * self.bf_vector()) ** order
+        return self._cache[key]
+
+    # FourierMesh.low_pass_filter (mesh.py:443-461): per-dimension box mask
+    def mask(self, rate):
+        key = ("mask", rate)
+        if key not in self._cache:
+            m = torch.ones_like(self.lap().real)
+            for i in range(self.n_dim):
+                a = self.bf(i).abs()
+                m *= torch.where(a > a.max() * rate, 0, 1)
+            self._cache[key] = m.to(device=self.device, dtype=self.dtype)
+        return self._cache[key]
+
+    def fft(self, u):  # mesh.py:481-485
+        return torch.fft.fftn(u, dim=self.fft_dim)
+
+    def ifft(self, u_fft):  # mesh.py:487-491
+        return torch.fft.ifftn(u_fft, dim=self.fft_dim)
+
+
+# ----------------------------------------------------------------------------- terms
+def linear_symbol(mesh: Mesh, kind: str, n_channel: int):
+    if kind == "lap":  # _laplacian.py:12-15
+        return torch.cat([mesh.lap()] * n_channel, dim=1)
+    if kind == "biharm":  # _biharmonic.py:13-16
+        return torch.cat([mesh.lap() * mesh.lap()] * n_channel, dim=1)
+    if kind == "unit":  # _source.py:9-17
+        return torch.ones_like(mesh.bf(0))
+    raise ValueError(kind)
+
+
+def convection(u_fft, mesh: Mesh, u=None):
+    """(u . grad) u_c (_convection.py:18-46)."""
+    if u is None:
+        u = mesh.ifft(u_fft).real
+    nabla_u = mesh.nabla_vector(1).unsqueeze(2) * u_fft.unsqueeze(1)
+    return mesh.fft((u.unsqueeze(2) * mesh.ifft(nabla_u).real).sum(1))
+
+
+def vorticity_convection(u_fft, mesh: Mesh, u=None):
+    """(u . grad) omega through the streamfunction (_navier_stokes.py:27-46)."""
+    psi = -u_fft * mesh.inv_lap()
+    ux = mesh.ifft(mesh.grad(1, 1) * psi).real
+    uy = mesh.ifft(-mesh.grad(0, 1) * psi).real
+    wx = mesh.ifft(mesh.grad(0, 1) * u_fft).real
+    wy = mesh.ifft(mesh.grad(1, 1) * u_fft).real
+    return mesh.fft(ux * wx + uy * wy)
+
+
+def source_spectrum(src: torch.Tensor):
+    """_ExplicitSourceCore.__init__ (_base.py:1002-1005): fftn over all dims after (B, C)."""
+    return torch.fft.fftn(src, dim=[i + 2 for i in range(src.dim() - 2)])
+
+
+def check_terms(mesh: Mesh, terms: List[Term], n_channel: int):
+    for _, kind, _ in terms:
+        if kind == "conv" and mesh.n_dim != n_channel:  # _convection.py:57-64
+            raise ValueError("convection operator only works for vector field with the same dimension as mesh")
+        if kind == "vc" and (mesh.n_dim != 2 or n_channel != 1):  # _navier_stokes.py:56-59
+            raise ValueError("Only vorticity in 2Dmesh is supported")
+
+
+def build_linear(mesh: Mesh, terms: List[Term], n_channel: int):
+    """_build_linear_coefs (_base.py:339-357): sum(coef * symbol) in generator order."""
+    lin = [(c, k) for c, k, _ in terms if k in LINEAR_KINDS]
+    if not lin:
+        return None
+    return sum([c * linear_symbol(mesh, k, n_channel) for c, k in lin])
+
+
+def build_closure(mesh: Mesh, terms: List[Term], dealias_rate=2 / 3) -> Optional[Callable]:
+    """nonlinear_funcs_all (_base.py:359-406): dealias only the inputs of dealiased terms."""
+    nl = []
+    for c, k, p in terms:
+        if k == "src":
+            nl.append((c, k, source_spectrum(p).to(mesh.device)))
+        elif k in ("conv", "vc"):
+            nl.append((c, k, None))
+    if not nl:
+        return None
+
+    def closure(u_fft):
+        result = 0.0
+        d_fft = None
+        d_u = None
+        u = None
+        for c, k, p in nl:
+            if k in ("conv", "vc"):
+                if d_fft is None:
+                    d_fft = u_fft * mesh.mask(dealias_rate)
+                    d_u = mesh.ifft(d_fft).real
+                f = convection if k == "conv" else vorticity_convection
+                result += c * f(d_fft, mesh, d_u)
+            else:  # explicit source: no dealiasing, u computed but unused (_base.py:394-401)
+                if u is None:
+                    u = mesh.ifft(u_fft).real
+                result += c * p
+        return result
+
+    return closure
+
+
+# ----------------------------------------------------------------------------- integrators
+def roots_of_unity(m, device, dtype):
+    """_uncached.py:8-15: exp(2 pi i (j - 1/2)/M), j = 1..M."""
+    return torch.exp(2j * torch.pi * (torch.arange(1, m + 1, device=device, dtype=dtype) - 0.5) / m)
+
+
+class Scheme:
+    """Coefficients and step of one ETD scheme, following the reference integrator classes."""
+
+    def __init__(self, name: str, dt: float, L, closure, n_points=16, radius=1.0):
+        self.name = name
+        self.dt = dt
+        self.N = closure
+        self.E = torch.exp(dt * L)  # _etdrk.py:21, _uncached.py:30
+        if name == "ETDRK0":
+            return
+        if name in ("ETDRK1", "ETDRK2"):  # _etdrk.py:43-45, 66-70
+            self.c1 = torch.where(L == 0, dt, (self.E - 1) / L)
+            if name == "ETDRK2":
+                self.c2 = torch.where(L == 0, dt / 2, (self.E - 1 - L * dt) / (L**2 * dt))
+            return
+        lr = radius * roots_of_unity(n_points, L.device, L.real.dtype) + L.unsqueeze(-1) * dt  # _uncached.py:34-43
+
+        def mean(z):
+            return dt * torch.mean(z, axis=-1).real
+
+        if name == "SETDRK1":  # _uncached.py:71-72
+            self.c1 = mean((torch.exp(lr) - 1) / lr)
+        elif name == "SETDRK2":  # _uncached.py:102-104
+            self.c1 = mean((torch.exp(lr) - 1) / lr)
+            self.c2 = mean((torch.exp(lr) - 1 - lr) / lr**2)
+        elif name == "SETDRK3":  # _uncached.py:135-156
+            self.E2 = torch.exp(0.5 * dt * L)
+            self.c1 = mean((torch.exp(lr / 2) - 1) / lr)
+            self.c2 = mean((torch.exp(lr) - 1) / lr)
+            self.c3 = mean((-4 - lr + torch.exp(lr) * (4 - 3 * lr + lr**2)) / (lr**3))
+            self.c4 = mean((4.0 * (2.0 + lr + torch.exp(lr) * (-2 + lr))) / (lr**3))
+            self.c5 = mean((-4 - 3 * lr - lr**2 + torch.exp(lr) * (4 - lr)) / (lr**3))
+        elif name == "SETDRK4":  # _uncached.py:191-211
+            self.E2 = torch.exp(0.5 * dt * L)
+            self.c1 = mean((torch.exp(lr / 2) - 1) / lr)
+            self.c2 = self.c1
+            self.c3 = self.c1
+            self.c4 = mean((-4 - lr + torch.exp(lr) * (4 - 3 * lr + lr**2)) / (lr**3))
+            self.c5 = mean((2 + lr + torch.exp(lr) * (-2 + lr)) / (lr**3))
+            self.c6 = mean((-4 - 3 * lr - lr**2 + torch.exp(lr) * (4 - lr)) / (lr**3))
+        else:
+            raise ValueError(name)
+
+    def step(self, u):
+        n = self.name
+        if n == "ETDRK0":  # _etdrk.py:23-27
+            return self.E * u
+        if n in ("ETDRK1", "SETDRK1"):  # _etdrk.py:47-51, _setdrk_step.py:5-11
+            return self.E * u + self.c1 * self.N(u)
+        if n in ("ETDRK2", "SETDRK2"):  # _etdrk.py:72-82, _setdrk_step.py:14-25
+            n0 = self.N(u)
+            a = self.E * u + self.c1 * n0
+            na = self.N(a)
+            return a + self.c2 * (na - n0)
+        if n == "SETDRK3":  # _setdrk_step.py:28-52
+            n0 = self.N(u)
+            a = self.E2 * u + self.c1 * n0
+            na = self.N(a)
+            b = self.E * u + self.c2 * (2 * na - n0)
+            nb = self.N(b)
+            return self.E * u + self.c3 * n0 + self.c4 * na + self.c5 * nb
+        # SETDRK4 (_setdrk_step.py:55-82)
+        n0 = self.N(u)
+        a = self.E2 * u + self.c1 * n0
+        na = self.N(a)
+        b = self.E2 * u + self.c2 * na
+        nb = self.N(b)
+        c = self.E2 * a + self.c3 * (2 * nb - n0)
+        nc = self.N(c)
+        return self.E * u + self.c4 * n0 + self.c5 * 2 * (na + nb) + self.c6 * nc
+
+
+def choose_scheme(terms: List[Term], scheme: str) -> str:
+    """_build_integrator 'auto' (_base.py:451-455): ETDRK0 if linear else SETDRK4."""
+    if scheme != "auto":
+        return scheme
+    return "SETDRK4" if any(k in NONLINEAR_KINDS for _, k, _ in terms) else "ETDRK0"
+
+
+def integrate(terms: List[Term], mesh_info, u0, dt, step, scheme="auto", record=None,
+              return_in_fourier=False, dealias_rate=2 / 3, u0_fft=None, n_points=16, radius=1.0):
+    """OperatorLike.integrate (_base.py:676-751).
+
+    ``record`` is None (return the final frame) or a predicate step -> bool; recorded frames are
+    stacked on dim 1 and inverse transformed like AutoRecorder.trajectory (traj_recorder.py:112-125).
+    """
+    x = u0 if u0 is not None else u0_fft
+    mesh = Mesh(mesh_info, device=x.device, dtype=(u0.dtype if u0 is not None else x.real.dtype))
+    n_channel = x.shape[1]
+    check_terms(mesh, terms, n_channel)
+    L = build_linear(mesh, terms, n_channel)
+    closure = build_closure(mesh, terms, dealias_rate)
+    name = choose_scheme(terms, scheme)
+    if name != "ETDRK0" and L is None:  # _base.py:473-478
+        L = torch.tensor([0.0], dtype=mesh.dtype, device=mesh.device)
+    sch = Scheme(name, dt, L, closure, n_points, radius)
+    u_fft = mesh.fft(u0) if u0_fft is None else u0_fft
+    frames = []
+    i = -1
+    for i in range(step):
+        if record is not None and record(i):
+            frames.append(u_fft.clone())
+        u_fft = sch.step(u_fft)
+    if record is not None:
+        if record(i + 1):
+            frames.append(u_fft.clone())
+        if not frames:
+            return None
+        traj = torch.stack(frames, dim=1)
+        if return_in_fourier:
+            return traj
+        return torch.fft.ifftn(traj, dim=tuple(-(j + 1) for j in range(traj.dim() - 3))).real
+    return u_fft if return_in_fourier else mesh.ifft(u_fft).real
+
+
+def apply(terms: List[Term], mesh_info, u, dealias_rate=2 / 3, return_in_fourier=False):
+    """OperatorLike.__call__ (_base.py:753-790) via _build_operator (_base.py:408-439)."""
+    mesh = Mesh(mesh_info, device=u.device, dtype=u.dtype)
+    n_channel = u.shape[1]
+    check_terms(mesh, terms, n_channel)
+    L = build_linear(mesh, terms, n_channel)
+    closure = build_closure(mesh, terms, dealias_rate)
+    u_fft = mesh.fft(u)
+    if closure is None:
+        out = L * u_fft
+    elif L is None:
+        out = closure(u_fft)
+    else:
+        out = L * u_fft + closure(u_fft)
+    return out if return_in_fourier else mesh.ifft(out).real
+
+
+# ----------------------------------------------------------------------------- canonical operators
+def burgers_terms(nu) -> List[Term]:
+    """pde.Burgers (pde.py:13-25): nu*Laplacian() - Convection()."""
+    return [(nu, "lap", None), (-1, "conv", None)]
+
+
+def ks_terms() -> List[Term]:
+    """pde.KuramotoSivashinsky (pde.py:27-39): -Laplacian() - Biharmonic() - Convection()."""
+    return [(-1, "lap", None), (-1, "biharm", None), (-1, "conv", None)]
+
+
+def ns_terms(Re, y=None, drag=-0.1, k=4.0, length_scale=1.0) -> List[Term]:
+    """pde.NavierStokesVorticity + field.kolm_force (pde.py:66-83, field.py:128-148)."""
+    t = [(-1, "vc", None), (1 / Re, "lap", None)]
+    if y is not None:
+        t += [(drag, "unit", None), (-1, "src", k * torch.cos(k * length_scale * y))]
+    return t
+
+
+def rel_l2(a, b):
+    a = torch.as_tensor(a).double()
+    b = torch.as_tensor(b).double()
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))

@@ -1,0 +1,171 @@
+"""Engine integration (OperatorLike.integrate, _base.py:676-751) against the reference's golden
+fixtures and the refpath oracle on the same inputs.  Tolerance: rel-L2 <= 1e-5 in float32."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_cfg1_burgers1d_trajectory(dev):
+    """Burgers(0.01), N=128, u0 = sin(2 pi x)+0.5, dt 0.01, 200 steps, AutoRecorder: (1,201,1,128)."""
+    import torchfsm_amd as tfa
+    from torchfsm_amd import pde
+    from torchfsm_amd.traj_recorder import AutoRecorder
+
+    g = golden("g1_burgers1d.npz")
+    mesh = tfa.MeshGrid([(0, 1, 128)], device=dev)
+    u0 = torch.from_numpy(g["u0_f32"]).to(dev)
+    traj = pde.Burgers(0.01).integrate(u0, mesh=mesh, dt=0.01, step=200, trajectory_recorder=AutoRecorder())
+    assert tuple(traj.shape) == (1, 201, 1, 128)
+    t = traj.cpu().numpy()
+    e32 = _rel(t, g["traj_f32"])
+    e64 = _rel(t, g["traj_f64"])
+    print(f"cfg1 rel-L2 vs ref f32 {e32:.3e} vs ref f64 {e64:.3e}")
+    assert e32 <= TOL
+    assert e64 <= TOL
+
+
+def test_ns64_golden(dev):
+    import torchfsm_amd as tfa
+    from torchfsm_amd import field, pde
+    from torchfsm_amd.integrator import SETDRKIntegrator
+
+    g = golden("g2_ns64.npz")
+    mesh = tfa.MeshGrid([(0, 2 * math.pi, 64)] * 2, device=dev)
+    y = mesh.bc_mesh_grid()[1]
+    op = pde.NavierStokesVorticity(1000, field.kolm_force(y))
+    op.set_integrator(SETDRKIntegrator.SETDRK4)
+    out = op.integrate(torch.from_numpy(g["u0_f32"]).to(dev), mesh=mesh, dt=1e-3, step=20).cpu().numpy()
+    e32, e64 = _rel(out, g["out_f32"]), _rel(out, g["out_f64"])
+    print(f"ns64 rel-L2 vs ref f32 {e32:.3e} vs ref f64 {e64:.3e}")
+    assert e32 <= TOL and e64 <= TOL
+
+
+def test_ns128_100_steps_golden(dev):
+    import torchfsm_amd as tfa
+    from torchfsm_amd import field, pde
+
+    g = golden("g9_ns128.npz")
+    mesh = tfa.MeshGrid([(0, 2 * math.pi, 128)] * 2, device=dev)
+    y = mesh.bc_mesh_grid()[1]
+    op = pde.NavierStokesVorticity(1000, field.kolm_force(y))
+    out = op.integrate(torch.from_numpy(g["u0_f32"]).to(dev), mesh=mesh, dt=1e-3, step=100).cpu().numpy()
+    e32, e64 = _rel(out, g["out_f32"]), _rel(out, g["out_f64"])
+    print(f"ns128 rel-L2 vs ref f32 {e32:.3e} vs ref f64 {e64:.3e}")
+    assert e32 <= TOL and e64 <= TOL
+
+
+def test_ks1d_golden(dev):
+    import torchfsm_amd as tfa
+    from torchfsm_amd import pde
+    from torchfsm_amd.integrator import SETDRKIntegrator
+
+    g = golden("g7_ks1d.npz")
+    mesh = tfa.MeshGrid([(0, 32 * math.pi, 512)], device=dev)
+    op = pde.KuramotoSivashinsky()
+    op.set_integrator(SETDRKIntegrator.SETDRK4)
+    out = op.integrate(torch.from_numpy(g["u0"]).to(dev), mesh=mesh, dt=0.1, step=10).cpu().numpy()
+    e = _rel(out, g["out"])
+    print(f"ks1d rel-L2 {e:.3e}")
+    assert e <= TOL
+
+
+def test_linear_etdrk0_golden(dev):
+    import torchfsm_amd as tfa
+
+    g = golden("g8_linear.npz")
+    mesh = tfa.MeshGrid([(0, 1, 32)] * 2, device=dev)
+    op = 0.01 * tfa.Laplacian()
+    out = op.integrate(torch.from_numpy(g["u0"]).to(dev), mesh=mesh, dt=0.05, step=5).cpu().numpy()
+    e = _rel(out, g["out"])
+    print(f"linear rel-L2 {e:.3e}")
+    assert e <= TOL
+
+
+def test_burgers2d_setdrk2_golden(dev):
+    import torchfsm_amd as tfa
+    from torchfsm_amd import pde
+    from torchfsm_amd.integrator import SETDRKIntegrator
+
+    g = golden("g4_burgers2d_grad.npz")
+    mesh = tfa.MeshGrid([(0, 1, 32)] * 2, device=dev)
+    op = pde.Burgers(0.01)
+    op.set_integrator(SETDRKIntegrator.SETDRK2)
+    out = op.integrate(torch.from_numpy(g["u0_f32"]).to(dev), mesh=mesh, dt=0.01, step=10).cpu().numpy()
+    e32, e64 = _rel(out, g["out_f32"]), _rel(out, g["out_f64"])
+    print(f"burgers2d rel-L2 vs ref f32 {e32:.3e} vs f64 {e64:.3e}")
+    assert e32 <= TOL
+
+
+def test_nonlinear_terms_golden(dev):
+    """One evaluation through Operator.__call__(return_in_fourier=True) (_base.py:753-790)."""
+    import torchfsm_amd as tfa
+
+    g = golden("g6_nonlinear.npz")
+    vc = tfa.VorticityConvection()
+    mesh = tfa.FourierMesh([(0, 2 * math.pi, 64)] * 2, device=dev)
+    out = vc(u=torch.from_numpy(g["vc_in"]).to(dev), mesh=mesh, return_in_fourier=True).cpu().numpy()
+    ref = g["vc_out"]
+    e = float(np.linalg.norm(out - ref) / np.linalg.norm(ref))
+    print(f"vc rel-L2 {e:.3e}")
+    assert e <= TOL
+    for tag, n, d in (("c1", 64, 1), ("c2", 32, 2)):
+        conv = tfa.Convection()
+        mesh = tfa.FourierMesh([(0, 1, n)] * d, device=dev)
+        out = conv(u=torch.from_numpy(g[tag + "_in"]).to(dev), mesh=mesh, return_in_fourier=True).cpu().numpy()
+        ref = g[tag + "_out"]
+        e = float(np.linalg.norm(out - ref) / np.linalg.norm(ref))
+        print(f"{tag} rel-L2 {e:.3e}")
+        assert e <= TOL
+
+
+def test_coefficients_golden(dev):
+    """SETDRK4 coefficients (float64 contour means on the GPU) and the complex64 ETDRK2 closed forms
+    against the reference's tables (_uncached.py:175-211, _etdrk.py:54-70) and numref's phi forms."""
+    from oracle import numref
+    from torchfsm_amd.engine import SpectralPlan
+
+    g = golden("g5_coefs.npz")
+    plan = SpectralPlan([(0, 2 * math.pi, 64)] * 2, 1, 1, nonlinear="vc", nonlinear_coef=-1.0,
+                        linear=[("lap", 1 / 1000), ("unit", -0.1)], device=dev)
+    plan.set_scheme("SETDRK4", 1e-3)
+    H = 33
+    L = g["ns_L"].real.astype(np.float64)
+    exact = numref.coefficients("SETDRK4", 1e-3, L)
+    for name, key in (("E", "ns_E"), ("E2", "ns_E2"), ("c1", "ns_c1"), ("c4", "ns_c4"), ("c5", "ns_c5"), ("c6", "ns_c6")):
+        got = plan.coefficients(name).cpu().numpy()[0, 0, :, :H]
+        ref = g[key].real[0, 0, :, :H]
+        ex = exact[name if name in exact else "c1"][0, 0, :, :H]
+        er, ee = _rel(got, ref), _rel(got, ex)
+        print(f"SETDRK4 {name}: rel vs reference f32 {er:.2e}, vs float64 phi forms {ee:.2e}")
+        assert er < 2e-6 and ee < 2e-7
+    plan2 = SpectralPlan([(0, 1, 128)] * 2, 1, 2, nonlinear="conv", nonlinear_coef=-1.0, linear=[("lap", 0.01)],
+                         device=dev)
+    plan2.set_scheme("ETDRK2", 0.01)
+    H = 65
+    for name, key in (("c1", "b2_c1"), ("c2", "b2_c2")):
+        got = plan2.coefficients(name).cpu().numpy()[0, 0, :, :H]
+        ref = g[key].real[0, 0, :, :H]
+        mism = int((got != ref).sum())
+        print(f"ETDRK2 {name}: {mism} float32 mismatches of {ref.size}")
+        if mism:
+            idx = np.argwhere(got != ref)[:8]
+            Lr = g["b2_L"].real[0, 0, :, :H]
+            Er = g["b2_E"].real[0, 0, :, :H]
+            Eg = plan2.coefficients("E").cpu().numpy()[0, 0, :, :H]
+            for i, j in idx:
+                print(f"  [{i},{j}] L={Lr[i, j]!r} E_ref={Er[i, j]!r} E_got={Eg[i, j]!r} got={got[i, j]!r} ref={ref[i, j]!r}")
+        assert mism <= ref.size // 1000

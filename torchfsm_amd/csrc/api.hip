@@ -1,0 +1,782 @@
+// C ABI of the ETD engine: plans (geometry, symbols, coefficient tables), transforms, stepping,
+// graph capture.  Kernels live in k_*.hip; this file holds host logic only.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+#include "tfsm.h"
+
+using namespace tfsm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    return fail(TFSM_E_OOM, std::string(what) + ": out of device memory (" + hipGetErrorString(e) + ")");
+  return fail(TFSM_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)               \
+  do {                                    \
+    hipError_t _e = (expr);               \
+    if (_e != hipSuccess) return hip_fail(_e, what); \
+  } while (0)
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+size_t roundup(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+// fftfreq order of integer wavenumbers
+int signed_k(int i, int n) { return i < (n + 1) / 2 ? i : i - n; }
+
+// axis wavenumbers exactly as the reference forms them in float32:
+// f = fl32(k) * fl32(1 / (n d)), d = length / n  (torch.fft.fftfreq, mesh.py:185-191)
+// a = fl32(fl32(2 pi) * f)                        ((2j*pi*bf)^1, mesh.py:399-404)
+void axis_table(int n, double length, std::vector<float>& a, std::vector<float>* f_out = nullptr) {
+  const double d = length / n;
+  const float s = (float)(1.0 / (n * d));
+  const float two_pi = (float)(2.0 * M_PI);
+  a.resize(n);
+  if (f_out) f_out->resize(n);
+  for (int i = 0; i < n; ++i) {
+    const float f = (float)signed_k(i, n) * s;
+    a[i] = two_pi * f;
+    if (f_out) (*f_out)[i] = f;
+  }
+}
+
+// low_pass_filter (mesh.py:443-461): keep |f| <= max|f| * rate, in float32; returns max kept |k|
+int dealias_kmax(int n, double length, double rate) {
+  std::vector<float> a, f;
+  axis_table(n, length, a, &f);
+  float fmax = 0.f;
+  for (float v : f) fmax = std::fabs(v) > fmax ? std::fabs(v) : fmax;
+  const float thr = fmax * (float)rate;
+  int kmax = -1;
+  for (int i = 0; i < n; ++i) {
+    const int k = std::abs(signed_k(i, n));
+    const bool masked = std::fabs(f[i]) > thr;
+    if (!masked && k > kmax) kmax = k;
+  }
+  // the kept set must be {|k| <= kmax} for the engine's index-range mask
+  for (int i = 0; i < n; ++i) {
+    const int k = std::abs(signed_k(i, n));
+    const bool masked = std::fabs(f[i]) > thr;
+    if ((k <= kmax) == masked) return -2;
+  }
+  return kmax;
+}
+
+}  // namespace
+
+struct tfsm_plan {
+  int D = 0, n[3] = {1, 1, 1}, B = 0, C = 0;
+  double len[3] = {0, 0, 0};
+  int H = 0, P = 0;
+  size_t modes = 0;  // complex elements of one [N...] block in half-spectrum layout
+  int term = TERM_NONE;
+  float coef_t = 0.f;
+  double rate = 2.0 / 3.0;
+  int kmax[3] = {0, 0, 0};
+  int nlin = 0;
+  int Bc = 1;
+  float2* tw[3] = {nullptr, nullptr, nullptr};
+  float* kv[3] = {nullptr, nullptr, nullptr};
+  size_t kv_bstride = 0;
+  float* L = nullptr;
+  float* coef = nullptr;
+  int scheme = -1;
+  double dt = 0.0;
+  float2* src = nullptr;
+  int srcB = 0, srcC = 0;
+  int V = 16;
+  int PF = 0, NF = 0;
+  size_t off_A1 = 0, off_A2 = 0, off_P = 0, off_G = 0, ws_bytes = 0;
+  std::vector<void*> allocs;
+
+  size_t state_elems() const { return (size_t)B * C * modes; }
+  size_t src_bstride() const { return srcB > 1 ? (size_t)srcC * modes : 0; }
+  size_t src_cstride() const { return srcC > 1 ? modes : 0; }
+  size_t coef_bstride() const { return Bc > 1 ? (size_t)NCOEF * modes : 0; }
+};
+
+struct tfsm_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+template <class T>
+int dev_alloc(tfsm_plan* p, T** ptr, size_t count, const char* what) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, count * sizeof(T) > 0 ? count * sizeof(T) : 16);
+  if (e != hipSuccess) return hip_fail(e, what);
+  p->allocs.push_back(q);
+  *ptr = static_cast<T*>(q);
+  return TFSM_OK;
+}
+
+int build_twiddles(tfsm_plan* p, int d) {
+  const int n = p->n[d];
+  for (int e = 0; e < d; ++e)
+    if (p->n[e] == n) {
+      p->tw[d] = p->tw[e];
+      return TFSM_OK;
+    }
+  std::vector<float2> h(n);
+  for (int m = 0; m < n; ++m) {
+    const double ang = -2.0 * M_PI * (double)m / (double)n;
+    h[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+  int rc = dev_alloc(p, &p->tw[d], (size_t)n, "twiddle table");
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(p->tw[d], h.data(), n * sizeof(float2), hipMemcpyHostToDevice), "twiddle upload");
+  return TFSM_OK;
+}
+
+int check_stream_errors(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, what);
+  return TFSM_OK;
+}
+
+// one nonlinear-evaluation pair of 2D kernels
+int run2d_eval(tfsm_plan* p, float2* U, char* ws, int stage, hipStream_t st) {
+  float2* A1 = (float2*)(ws + p->off_A1);
+  float2* A2 = (float2*)(ws + p->off_A2);
+  float2* Pp = (float2*)(ws + p->off_P);
+  float2* G = (float2*)(ws + p->off_G);
+  if (stage != ST_PRO) {
+    Row2Args r;
+    r.B = p->B;
+    r.N0 = p->n[0];
+    r.N1 = p->n[1];
+    r.H1 = p->H;
+    r.P1 = p->P;
+    r.PF = p->PF;
+    r.kmax1 = p->kmax[1];
+    r.NF = p->NF;
+    r.tw1 = p->tw[1];
+    r.ay = p->kv[1];
+    r.G = G;
+    r.P = Pp;
+    r.b0 = 0;
+    HIP_TRY(launch_row2d(p->term, r, st), "row kernel launch");
+  }
+  Col2Args c;
+  c.B = p->B;
+  c.C = p->C;
+  c.N0 = p->n[0];
+  c.N1 = p->n[1];
+  c.H1 = p->H;
+  c.P1 = p->P;
+  c.PF = p->PF;
+  c.kmax0 = p->kmax[0];
+  c.kmax1 = p->kmax[1];
+  c.NF = p->NF;
+  c.coef_t = p->coef_t;
+  c.scale = (float)(1.0 / ((double)p->n[0] * p->n[1]));
+  c.tw0 = p->tw[0];
+  c.ax = p->kv[0];
+  c.ay = p->kv[1];
+  c.P = Pp;
+  c.U = U;
+  c.A1 = A1;
+  c.A2 = A2;
+  c.coef = p->coef;
+  c.coef_bstride = p->coef_bstride();
+  c.src = p->src;
+  c.src_bstride = p->src_bstride();
+  c.src_cstride = p->src_cstride();
+  c.G = G;
+  c.b0 = 0;
+  HIP_TRY(launch_col2d(p->term, c, p->B, stage, p->V, st), "column kernel launch");
+  return TFSM_OK;
+}
+
+Run1Args run1_args(tfsm_plan* p, float2* U, char* ws) {
+  Run1Args a;
+  a.B = p->B * p->C;
+  a.N = p->n[0];
+  a.H = p->H;
+  a.P = p->P;
+  a.kmax = p->kmax[0];
+  a.coef_t = p->coef_t;
+  a.scale = (float)(1.0 / p->n[0]);
+  a.scheme = p->scheme;
+  a.tw = p->tw[0];
+  a.ax = p->kv[0];
+  a.ax_bstride = p->kv_bstride;
+  a.U = U;
+  a.A1 = (float2*)(ws + p->off_A1);
+  a.A2 = (float2*)(ws + p->off_A2);
+  a.coef = p->coef;
+  a.coef_bstride = p->coef_bstride();
+  a.src = p->src;
+  a.src_bstride = p->src_bstride();
+  a.nsteps = 0;
+  a.nsteps_per = nullptr;
+  a.rec_slot = nullptr;
+  a.traj = nullptr;
+  a.T = 0;
+  a.eval_only = 0;
+  return a;
+}
+
+int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* traj, int T, void* ws, hipStream_t st) {
+  float2* U = (float2*)uh;
+  if (p->scheme == ETDRK0) {
+    HIP_TRY(launch_linear_steps(U, p->coef + CS_E * p->modes, p->coef_bstride(), p->modes, p->C, p->state_elems(),
+                                nsteps, rec_slot, (float2*)traj, T, st),
+            "linear step launch");
+    return TFSM_OK;
+  }
+  if (!ws) return fail(TFSM_E_INVALID, "workspace pointer is null");
+  if (p->D == 1) {
+    Run1Args a = run1_args(p, U, (char*)ws);
+    a.nsteps = nsteps;
+    a.rec_slot = rec_slot;
+    a.traj = (float2*)traj;
+    a.T = T;
+    HIP_TRY(launch_run1d(a, st), "1D stepper launch");
+    return TFSM_OK;
+  }
+  if (p->D == 2) {
+    const int nev = scheme_evals(p->scheme);
+    int rc = run2d_eval(p, U, (char*)ws, ST_PRO, st);
+    if (rc) return rc;
+    const size_t per = (size_t)p->C * p->modes;
+    for (int i = 0; i < nsteps; ++i) {
+      if (rec_slot) HIP_TRY(launch_record(U, (float2*)traj, rec_slot, i, p->B, per, T, st), "record launch");
+      for (int e = 0; e < nev; ++e) {
+        rc = run2d_eval(p, U, (char*)ws, scheme_stage(p->scheme, e), st);
+        if (rc) return rc;
+      }
+    }
+    if (rec_slot) HIP_TRY(launch_record(U, (float2*)traj, rec_slot, nsteps, p->B, per, T, st), "record launch");
+    return TFSM_OK;
+  }
+  return fail(TFSM_E_INVALID, "stepping is not available for this dimension");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tfsm_last_error(void) { return g_err.c_str(); }
+
+int tfsm_version(void) { return (0 << 16) | 1; }
+
+int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* stream, tfsm_plan** out) {
+  if (!geom || !terms || !out) return fail(TFSM_E_INVALID, "null argument");
+  *out = nullptr;
+  const int D = geom->ndim;
+  if (D < 1 || D > 2) return fail(TFSM_E_INVALID, "the engine supports 1D and 2D meshes");
+  if (geom->batch < 1 || geom->channels < 1) return fail(TFSM_E_INVALID, "batch and channels must be >= 1");
+  for (int d = 0; d < D; ++d) {
+    if (!is_pow2(geom->n[d])) return fail(TFSM_E_INVALID, "mesh sizes must be powers of two");
+    if (!(geom->length[d] > 0)) return fail(TFSM_E_INVALID, "mesh lengths must be positive");
+  }
+  if (D == 1 && (geom->n[0] < 32 || geom->n[0] > 4096))
+    return fail(TFSM_E_INVALID, "1D meshes must have 32..4096 points");
+  if (D == 2) {
+    for (int d = 0; d < 2; ++d)
+      if (geom->n[d] < 16 || geom->n[d] > 512) return fail(TFSM_E_INVALID, "2D meshes must have 16..512 points per axis");
+  }
+  const int term = terms->nonlinear;
+  if (term != TERM_NONE && term != TERM_CONV && term != TERM_VC) return fail(TFSM_E_INVALID, "unknown nonlinear term");
+  if (term == TERM_CONV && geom->channels != D)
+    return fail(TFSM_E_INVALID, "convection operator only works for vector field with the same dimension as mesh");
+  if (term == TERM_VC && (D != 2 || geom->channels != 1))
+    return fail(TFSM_E_INVALID, "Only vorticity in 2Dmesh is supported");
+  if (terms->n_linear < 0 || terms->n_linear > 4) return fail(TFSM_E_INVALID, "at most 4 linear terms");
+  if (terms->n_linear > 0 && !terms->linear_coef) return fail(TFSM_E_INVALID, "linear coefficients missing");
+  const int Bc = terms->coef_batch < 1 ? 1 : terms->coef_batch;
+  if (Bc != 1 && Bc != geom->batch) return fail(TFSM_E_INVALID, "coef_batch must be 1 or the batch size");
+  if (geom->length_per_sample && D != 1) return fail(TFSM_E_INVALID, "per-sample lengths are 1D only");
+
+  hipStream_t st = (hipStream_t)stream;
+  tfsm_plan* p = new tfsm_plan();
+  p->D = D;
+  p->B = geom->batch;
+  p->C = geom->channels;
+  for (int d = 0; d < D; ++d) {
+    p->n[d] = geom->n[d];
+    p->len[d] = geom->length[d];
+  }
+  p->H = p->n[D - 1] / 2 + 1;
+  p->P = (int)roundup(p->H, 16);
+  p->modes = (size_t)p->P;
+  for (int d = 0; d < D - 1; ++d) p->modes *= p->n[d];
+  p->term = term;
+  p->coef_t = terms->nonlinear_coef;
+  p->rate = terms->dealias_rate > 0 ? terms->dealias_rate : 2.0 / 3.0;
+  p->nlin = terms->n_linear;
+  p->Bc = Bc;
+  int rc = TFSM_OK;
+  for (int d = 0; d < D; ++d) {
+    p->kmax[d] = dealias_kmax(p->n[d], p->len[d], p->rate);
+    if (p->kmax[d] < -1) {
+      tfsm_plan_destroy(p);
+      return fail(TFSM_E_INVALID, "dealias mask is not an index range");
+    }
+    rc = build_twiddles(p, d);
+    if (rc) break;
+  }
+  // axis wavenumber tables
+  if (!rc) {
+    for (int d = 0; d < D && !rc; ++d) {
+      if (D == 1 && geom->length_per_sample) {
+        std::vector<float> all((size_t)p->B * p->C * p->n[0]), a;
+        for (int b = 0; b < p->B; ++b) {
+          axis_table(p->n[0], geom->length_per_sample[b], a);
+          for (int c = 0; c < p->C; ++c)
+            std::memcpy(&all[((size_t)b * p->C + c) * p->n[0]], a.data(), a.size() * sizeof(float));
+        }
+        rc = dev_alloc(p, &p->kv[0], all.size(), "axis table");
+        if (!rc) {
+          hipError_t e = hipMemcpy(p->kv[0], all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice);
+          if (e != hipSuccess) rc = hip_fail(e, "axis table upload");
+        }
+        p->kv_bstride = (size_t)p->n[0];
+      } else {
+        std::vector<float> a;
+        axis_table(p->n[d], p->len[d], a);
+        rc = dev_alloc(p, &p->kv[d], a.size(), "axis table");
+        if (!rc) {
+          hipError_t e = hipMemcpy(p->kv[d], a.data(), a.size() * sizeof(float), hipMemcpyHostToDevice);
+          if (e != hipSuccess) rc = hip_fail(e, "axis table upload");
+        }
+      }
+    }
+  }
+  // linear symbol L per (coefficient sample, mode)
+  const int Lb = (D == 1 && geom->length_per_sample) ? p->B * p->C : Bc;
+  if (D == 1 && geom->length_per_sample) p->Bc = Lb;
+  if (!rc) rc = dev_alloc(p, &p->L, (size_t)Lb * p->modes, "linear symbol");
+  if (!rc) rc = dev_alloc(p, &p->coef, (size_t)Lb * NCOEF * p->modes, "coefficient table");
+  if (!rc) {
+    hipError_t e = hipMemsetAsync(p->coef, 0, (size_t)Lb * NCOEF * p->modes * sizeof(float), st);
+    if (e != hipSuccess) rc = hip_fail(e, "coefficient table clear");
+  }
+  if (!rc) {
+    LinArgs la;
+    la.D = D;
+    for (int d = 0; d < 3; ++d) {
+      la.n[d] = p->n[d];
+      la.kv[d] = p->kv[d];
+    }
+    la.H = p->H;
+    la.P = p->P;
+    la.modes = p->modes;
+    la.kv_bstride = p->kv_bstride;
+    la.nterm = p->nlin;
+    for (int t = 0; t < 4; ++t) la.kind[t] = t < p->nlin ? terms->linear_kind[t] : 0;
+    la.Bc = Lb;
+    la.L = p->L;
+    float* dcoef = nullptr;
+    std::vector<float> hc((size_t)(p->nlin > 0 ? p->nlin : 1) * Lb, 0.f);
+    for (int t = 0; t < p->nlin; ++t)
+      for (int b = 0; b < Lb; ++b) {
+        const int src_b = (Bc == 1) ? 0 : (D == 1 && geom->length_per_sample ? b / p->C : b);
+        hc[(size_t)t * Lb + b] = terms->linear_coef[(size_t)t * Bc + src_b];
+      }
+    rc = dev_alloc(p, &dcoef, hc.size(), "linear coefficients");
+    if (!rc) {
+      hipError_t e = hipMemcpy(dcoef, hc.data(), hc.size() * sizeof(float), hipMemcpyHostToDevice);
+      if (e != hipSuccess) rc = hip_fail(e, "linear coefficient upload");
+    }
+    la.coef = dcoef;
+    if (!rc) {
+      hipError_t e = launch_linear_symbol(la, st);
+      if (e != hipSuccess) rc = hip_fail(e, "linear symbol launch");
+    }
+  }
+  // workspace layout
+  if (!rc) {
+    const size_t sb = p->state_elems() * sizeof(float2);
+    p->off_A1 = 0;
+    p->off_A2 = roundup(sb, 256);
+    if (D == 2) {
+      p->NF = 4;
+      p->PF = (int)roundup((size_t)p->kmax[1] + 1, 16);
+      p->off_P = p->off_A2 + roundup(sb, 256);
+      p->off_G = p->off_P + roundup(sb, 256);
+      p->ws_bytes = p->off_G + roundup((size_t)p->B * p->NF * p->n[0] * p->PF * sizeof(float2), 256);
+    } else {
+      p->ws_bytes = p->off_A2 + roundup(sb, 256);
+    }
+  }
+  if (rc) {
+    tfsm_plan_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return TFSM_OK;
+}
+
+int tfsm_plan_destroy(tfsm_plan* plan) {
+  if (!plan) return TFSM_OK;
+  for (void* q : plan->allocs) (void)hipFree(q);
+  delete plan;
+  return TFSM_OK;
+}
+
+int64_t tfsm_plan_pitch(const tfsm_plan* plan) { return plan ? plan->P : 0; }
+
+int64_t tfsm_plan_state_elems(const tfsm_plan* plan) { return plan ? (int64_t)plan->state_elems() : 0; }
+
+int64_t tfsm_plan_workspace_bytes(const tfsm_plan* plan) { return plan ? (int64_t)plan->ws_bytes : 0; }
+
+int64_t tfsm_plan_inverse_scratch_bytes(const tfsm_plan* plan, int32_t nbatch) {
+  if (!plan) return 0;
+  if (plan->D == 1) return 0;
+  return (int64_t)((size_t)nbatch * plan->C * plan->modes * sizeof(float2));
+}
+
+int tfsm_plan_set_tile(tfsm_plan* plan, int32_t col_tile) {
+  if (!plan) return fail(TFSM_E_INVALID, "null plan");
+  if (plan->D == 2 && !col2d_supported(plan->n[0], col_tile)) return fail(TFSM_E_INVALID, "unsupported column tile");
+  if (plan->D == 2 && (plan->P % col_tile) != 0) return fail(TFSM_E_INVALID, "column tile must divide the pitch");
+  plan->V = col_tile;
+  return TFSM_OK;
+}
+
+int tfsm_plan_set_scheme(tfsm_plan* plan, int32_t scheme, double dt, int32_t n_points, double radius, void* stream) {
+  if (!plan) return fail(TFSM_E_INVALID, "null plan");
+  if (scheme < ETDRK0 || scheme > SETDRK4) return fail(TFSM_E_INVALID, "unknown scheme");
+  const bool nonlinear = plan->term != TERM_NONE || plan->src != nullptr;
+  if (scheme == ETDRK0 && nonlinear) return fail(TFSM_E_INVALID, "The ETDRK0 integrator only supports linear term");
+  if (scheme != ETDRK0 && plan->term == TERM_NONE)
+    return fail(TFSM_E_INVALID, "term not supported by engine: a nonlinear scheme needs Convection or VorticityConvection");
+  if (n_points < 1) return fail(TFSM_E_INVALID, "n_integration_points must be >= 1");
+  CoefArgs a;
+  a.scheme = scheme;
+  a.dt = dt;
+  a.dt32 = (float)dt;
+  a.hdt32 = (float)(0.5 * dt);
+  a.M = n_points;
+  a.radius = radius;
+  a.modes = plan->modes;
+  a.Bc = plan->Bc;
+  a.L = plan->L;
+  a.coef = plan->coef;
+  HIP_TRY(launch_coefs(a, (hipStream_t)stream), "coefficient launch");
+  plan->scheme = scheme;
+  plan->dt = dt;
+  return TFSM_OK;
+}
+
+int tfsm_plan_coefficients(tfsm_plan* p, int32_t slot, float* out, void* stream) {
+  if (!p || !out) return fail(TFSM_E_INVALID, "null argument");
+  if (slot < 0 || slot >= NCOEF) return fail(TFSM_E_INVALID, "coefficient slot out of range");
+  if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
+  const size_t bytes = p->modes * sizeof(float);
+  for (int b = 0; b < p->Bc; ++b)
+    HIP_TRY(hipMemcpyAsync(out + (size_t)b * p->modes, p->coef + ((size_t)b * NCOEF + slot) * p->modes, bytes,
+                           hipMemcpyDeviceToDevice, (hipStream_t)stream),
+            "coefficient copy");
+  return TFSM_OK;
+}
+
+int tfsm_forward(tfsm_plan* p, const float* u, void* uh, int32_t nbatch, void* stream) {
+  if (!p || !u || !uh) return fail(TFSM_E_INVALID, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  float2* out = (float2*)uh;
+  const size_t blocks = (size_t)nbatch * p->C;
+  const int nl = p->n[p->D - 1];
+  size_t rows = blocks;
+  for (int d = 0; d < p->D - 1; ++d) rows *= p->n[d];
+  HIP_TRY(launch_rows_r2c(nl, u, (size_t)nl, out, (size_t)p->P, (int)rows, 1.0f, p->tw[p->D - 1], st), "r2c launch");
+  if (p->D == 2)
+    HIP_TRY(launch_cols_c2c(p->n[0], -1, out, out, blocks, p->modes, (size_t)p->P, p->H, 1.0f, p->tw[0], st),
+            "column fft launch");
+  return TFSM_OK;
+}
+
+int tfsm_inverse(tfsm_plan* p, const void* uh, float* u, int32_t nbatch, void* scratch, void* stream) {
+  if (!p || !u || !uh) return fail(TFSM_E_INVALID, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t blocks = (size_t)nbatch * p->C;
+  const int nl = p->n[p->D - 1];
+  double total = 1.0;
+  for (int d = 0; d < p->D; ++d) total *= p->n[d];
+  const float scale = (float)(1.0 / total);
+  const float2* in = (const float2*)uh;
+  size_t rows = blocks;
+  for (int d = 0; d < p->D - 1; ++d) rows *= p->n[d];
+  if (p->D == 2) {
+    if (!scratch) return fail(TFSM_E_INVALID, "inverse transform needs scratch");
+    float2* sc = (float2*)scratch;
+    HIP_TRY(launch_cols_c2c(p->n[0], +1, in, sc, blocks, p->modes, (size_t)p->P, p->H, 1.0f, p->tw[0], st),
+            "column ifft launch");
+    in = sc;
+  }
+  HIP_TRY(launch_rows_c2r(nl, in, (size_t)p->P, u, (size_t)nl, (int)rows, scale, p->tw[p->D - 1], st), "c2r launch");
+  return TFSM_OK;
+}
+
+int tfsm_c2c(tfsm_plan* p, const void* in, void* out, int32_t nbatch, int32_t dir, void* stream) {
+  if (!p || !in || !out) return fail(TFSM_E_INVALID, "null argument");
+  if (dir != -1 && dir != 1) return fail(TFSM_E_INVALID, "dir must be -1 or +1");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t blocks = (size_t)nbatch * p->C;
+  const int nl = p->n[p->D - 1];
+  double total = 1.0;
+  for (int d = 0; d < p->D; ++d) total *= p->n[d];
+  const float scale = dir > 0 ? (float)(1.0 / total) : 1.0f;
+  size_t rows = blocks;
+  for (int d = 0; d < p->D - 1; ++d) rows *= p->n[d];
+  HIP_TRY(launch_rows_c2c(nl, dir, (const float2*)in, (size_t)nl, (float2*)out, (size_t)nl, (int)rows, scale,
+                          p->tw[p->D - 1], st),
+          "row c2c launch");
+  if (p->D == 2)
+    HIP_TRY(launch_cols_c2c(p->n[0], dir, (const float2*)out, (float2*)out, blocks, (size_t)p->n[0] * nl, (size_t)nl,
+                            nl, 1.0f, p->tw[0], st),
+            "column c2c launch");
+  return TFSM_OK;
+}
+
+int tfsm_half_to_full(tfsm_plan* p, const void* uh, void* full, int32_t nbatch, void* stream) {
+  if (!p || !uh || !full) return fail(TFSM_E_INVALID, "null argument");
+  ConvArgs a;
+  a.D = p->D;
+  for (int d = 0; d < 3; ++d) a.n[d] = p->n[d];
+  a.H = p->H;
+  a.P = p->P;
+  a.nblk = (size_t)nbatch * p->C;
+  HIP_TRY(launch_half_to_full((const float2*)uh, (float2*)full, a, (hipStream_t)stream), "half_to_full launch");
+  return TFSM_OK;
+}
+
+int tfsm_full_to_half(tfsm_plan* p, const void* full, void* uh, int32_t nbatch, void* stream) {
+  if (!p || !uh || !full) return fail(TFSM_E_INVALID, "null argument");
+  ConvArgs a;
+  a.D = p->D;
+  for (int d = 0; d < 3; ++d) a.n[d] = p->n[d];
+  a.H = p->H;
+  a.P = p->P;
+  a.nblk = (size_t)nbatch * p->C;
+  HIP_TRY(launch_full_to_half((const float2*)full, (float2*)uh, a, (hipStream_t)stream), "full_to_half launch");
+  return TFSM_OK;
+}
+
+int tfsm_plan_add_source(tfsm_plan* p, const float* src, int32_t src_batch, int32_t src_channels, float coef,
+                         void* stream) {
+  if (!p || !src) return fail(TFSM_E_INVALID, "null argument");
+  if (!(src_batch == 1 || src_batch == p->B) || !(src_channels == 1 || src_channels == p->C))
+    return fail(TFSM_E_INVALID, "source batch/channels must be 1 or match the state");
+  if (p->src && (src_batch != p->srcB || src_channels != p->srcC))
+    return fail(TFSM_E_INVALID, "all explicit sources of one operator must have the same shape");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t total = (size_t)src_batch * src_channels * p->modes;
+  float2* spec = nullptr;
+  HIP_TRY(hipMalloc(&spec, total * sizeof(float2)), "source spectrum scratch");
+  // forward transform of the source with a plan-compatible batch of src_batch blocks of src_channels
+  const int saveC = p->C;
+  p->C = src_channels;
+  int rc = tfsm_forward(p, src, spec, src_batch, stream);
+  p->C = saveC;
+  const bool first = p->src == nullptr;
+  if (!rc && first) rc = dev_alloc(p, &p->src, total, "source spectrum");
+  if (!rc) {
+    hipError_t e = launch_accum_source(p->src, spec, coef, total, first ? 1 : 0, st);
+    if (e != hipSuccess) rc = hip_fail(e, "source accumulate launch");
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(spec);
+  if (rc) return rc;
+  p->srcB = src_batch;
+  p->srcC = src_channels;
+  return TFSM_OK;
+}
+
+int tfsm_step(tfsm_plan* p, void* uh, int32_t nsteps, const int32_t* rec_slot, void* traj, int32_t T, void* ws,
+              void* stream) {
+  if (!p || !uh) return fail(TFSM_E_INVALID, "null argument");
+  if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
+  if (nsteps < 0) return fail(TFSM_E_INVALID, "nsteps must be >= 0");
+  if (rec_slot && (!traj || T < 1)) return fail(TFSM_E_INVALID, "recording needs a trajectory buffer");
+  return step_impl(p, uh, nsteps, rec_slot, traj, T, ws, (hipStream_t)stream);
+}
+
+int tfsm_step_varying(tfsm_plan* p, void* uh, const int32_t* nsteps_per_sample, int32_t max_steps, void* ws,
+                      void* stream) {
+  if (!p || !uh || !nsteps_per_sample || !ws) return fail(TFSM_E_INVALID, "null argument");
+  if (p->D != 1 || p->scheme == ETDRK0) return fail(TFSM_E_INVALID, "per-sample step counts need a nonlinear 1D plan");
+  Run1Args a = run1_args(p, (float2*)uh, (char*)ws);
+  a.nsteps = max_steps;
+  a.nsteps_per = nsteps_per_sample;
+  HIP_TRY(launch_run1d(a, (hipStream_t)stream), "1D stepper launch");
+  return TFSM_OK;
+}
+
+int tfsm_apply(tfsm_plan* p, const void* uh, void* out, void* ws, void* stream) {
+  if (!p || !uh || !out) return fail(TFSM_E_INVALID, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const bool have_nl = p->term != TERM_NONE;
+  const bool have_lin = p->nlin > 0;
+  float2* U = (float2*)uh;
+  if (have_nl) {
+    if (!ws) return fail(TFSM_E_INVALID, "workspace pointer is null");
+    if (p->D == 1) {
+      Run1Args a = run1_args(p, U, (char*)ws);
+      a.eval_only = 1;
+      a.nsteps = 1;
+      HIP_TRY(launch_run1d(a, st), "1D evaluation launch");
+    } else {
+      int rc = run2d_eval(p, U, (char*)ws, ST_PRO, st);
+      if (!rc) rc = run2d_eval(p, U, (char*)ws, ST_EVAL, st);
+      if (rc) return rc;
+    }
+  }
+  const float2* Nv = have_nl ? (const float2*)((char*)ws + p->off_A1) : nullptr;
+  HIP_TRY(launch_apply_linear(U, Nv, p->L, p->Bc > 1 ? p->modes : 0, p->modes, p->C, p->state_elems(), have_lin,
+                              have_nl, (float2*)out, st),
+          "apply launch");
+  return TFSM_OK;
+}
+
+int tfsm_solve(tfsm_plan* p, const void* uh, void* out, void* stream) {
+  if (!p || !uh || !out) return fail(TFSM_E_INVALID, "null argument");
+  if (p->nlin == 0) return fail(TFSM_E_INVALID, "solve needs a linear operator");
+  HIP_TRY(launch_apply_linear((const float2*)uh, nullptr, p->L, p->Bc > 1 ? p->modes : 0, p->modes, p->C,
+                              p->state_elems(), 2, 0, (float2*)out, (hipStream_t)stream),
+          "solve launch");
+  return TFSM_OK;
+}
+
+int tfsm_launch_pass(tfsm_plan* p, void* uh, void* ws, int32_t pass_id, int32_t stage, void* stream) {
+  if (!p || !uh || !ws) return fail(TFSM_E_INVALID, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  if (p->D == 2) {
+    if (p->term == TERM_NONE) return fail(TFSM_E_INVALID, "no product term");
+    if (pass_id == 0) {
+      Row2Args r;
+      r.B = p->B;
+      r.N0 = p->n[0];
+      r.N1 = p->n[1];
+      r.H1 = p->H;
+      r.P1 = p->P;
+      r.PF = p->PF;
+      r.kmax1 = p->kmax[1];
+      r.NF = p->NF;
+      r.tw1 = p->tw[1];
+      r.ay = p->kv[1];
+      r.G = (const float2*)((char*)ws + p->off_G);
+      r.P = (float2*)((char*)ws + p->off_P);
+      r.b0 = 0;
+      HIP_TRY(launch_row2d(p->term, r, st), "row kernel launch");
+      return TFSM_OK;
+    }
+    if (pass_id == 1) {
+      if (stage < ST_PRO || stage > ST_EVAL) return fail(TFSM_E_INVALID, "bad stage");
+      if (stage != ST_PRO && p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
+      // the column kernel alone: run2d_eval without its row kernel is the ST_PRO path for stage 0;
+      // for other stages launch it directly on the products already in the workspace
+      const int saved = stage;
+      if (saved == ST_PRO) return run2d_eval(p, (float2*)uh, (char*)ws, ST_PRO, st);
+      Col2Args c;
+      c.B = p->B;
+      c.C = p->C;
+      c.N0 = p->n[0];
+      c.N1 = p->n[1];
+      c.H1 = p->H;
+      c.P1 = p->P;
+      c.PF = p->PF;
+      c.kmax0 = p->kmax[0];
+      c.kmax1 = p->kmax[1];
+      c.NF = p->NF;
+      c.coef_t = p->coef_t;
+      c.scale = (float)(1.0 / ((double)p->n[0] * p->n[1]));
+      c.tw0 = p->tw[0];
+      c.ax = p->kv[0];
+      c.ay = p->kv[1];
+      c.P = (const float2*)((char*)ws + p->off_P);
+      c.U = (float2*)uh;
+      c.A1 = (float2*)((char*)ws + p->off_A1);
+      c.A2 = (float2*)((char*)ws + p->off_A2);
+      c.coef = p->coef;
+      c.coef_bstride = p->coef_bstride();
+      c.src = p->src;
+      c.src_bstride = p->src_bstride();
+      c.src_cstride = p->src_cstride();
+      c.G = (float2*)((char*)ws + p->off_G);
+      c.b0 = 0;
+      HIP_TRY(launch_col2d(p->term, c, p->B, stage, p->V, st), "column kernel launch");
+      return TFSM_OK;
+    }
+  }
+  if (p->D == 1 && pass_id == 2) {
+    if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
+    Run1Args a = run1_args(p, (float2*)uh, (char*)ws);
+    a.nsteps = 1;
+    HIP_TRY(launch_run1d(a, st), "1D stepper launch");
+    return TFSM_OK;
+  }
+  return fail(TFSM_E_INVALID, "unknown pass for this plan");
+}
+
+int tfsm_graph_create(tfsm_plan* p, void* uh, int32_t nsteps, void* ws, void* stream, tfsm_graph** out) {
+  if (!p || !uh || !out) return fail(TFSM_E_INVALID, "null argument");
+  if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
+  *out = nullptr;
+  hipStream_t cap = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "capture stream");
+  // order the capture stream after the caller's stream so that setup work is visible
+  hipEvent_t ev = nullptr;
+  HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "capture event");
+  HIP_TRY(hipEventRecord(ev, (hipStream_t)stream), "capture event record");
+  HIP_TRY(hipStreamWaitEvent(cap, ev, 0), "capture wait");
+  HIP_TRY(hipStreamSynchronize(cap), "capture sync");
+  (void)hipEventDestroy(ev);
+  tfsm_graph* g = new tfsm_graph();
+  hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    delete g;
+    (void)hipStreamDestroy(cap);
+    return hip_fail(e, "begin capture");
+  }
+  int rc = step_impl(p, uh, nsteps, nullptr, nullptr, 0, ws, cap);
+  e = hipStreamEndCapture(cap, &g->graph);
+  if (rc == TFSM_OK && e != hipSuccess) rc = hip_fail(e, "end capture");
+  if (rc == TFSM_OK) {
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) rc = hip_fail(e, "graph instantiate");
+  }
+  (void)hipStreamDestroy(cap);
+  if (rc) {
+    tfsm_graph_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return TFSM_OK;
+}
+
+int tfsm_graph_launch(tfsm_graph* g, void* stream) {
+  if (!g || !g->exec) return fail(TFSM_E_INVALID, "null graph");
+  HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream), "graph launch");
+  return TFSM_OK;
+}
+
+int tfsm_graph_destroy(tfsm_graph* g) {
+  if (!g) return TFSM_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return TFSM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,272 @@
+// Shared definitions of the ETD engine: data layout, stage codes, kernel argument blocks and the
+// per-mode stage combine.
+//
+// Data layout in HBM (all complex values are interleaved float2):
+//   half spectrum  [B][C][N0]...[N_{D-2}][P]  last axis holds k = 0..N_{D-1}/2 (H values), P = pitch
+//                  (H rounded up to a multiple of 16 so that column tiles start on 128-byte lines)
+//   coefficient    [Bc][8][modes] float, one real table per slot (E, E2, c1..c6); Bc = 1 or B
+//   2D fields      [B][NF][N0][PF] float2: inverse-x-transformed fields, ky <= kmax1 only
+//   2D products    [B][NP][N0][P]  float2: forward-y-transformed physical products
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_core.hpp"
+
+namespace tfsm {
+
+enum TermKind : int { TERM_NONE = 0, TERM_CONV = 1, TERM_VC = 2 };
+
+enum SchemeId : int { ETDRK0 = 0, ETDRK1 = 1, ETDRK2 = 2, SETDRK1 = 3, SETDRK2 = 4, SETDRK3 = 5, SETDRK4 = 6 };
+
+// coefficient slots
+enum CoefSlot : int { CS_E = 0, CS_E2 = 1, CS_C1 = 2, CS_C2 = 3, CS_C3 = 4, CS_C4 = 5, CS_C5 = 6, CS_C6 = 7, NCOEF = 8 };
+
+// Stage codes of the fused combine.  s = the spectrum the next nonlinear evaluation needs.
+enum Stage : int {
+  ST_PRO = 0,   // s = U (fields of the current state only)
+  ST_R1_E0,     // U' = E U + c1 N                                   (_etdrk.py:47-51, _setdrk_step.py:5-11)
+  ST_R2_E0,     // A1 = N0;  s = E U + c1 N0                         (_setdrk_step.py:14-25)
+  ST_R2_E1,     // U' = (E U + c1 A1) + c2 (N - A1)
+  ST_R3_E0,     // A1 = N0;  s = E2 U + c1 N0                        (_setdrk_step.py:28-52)
+  ST_R3_E1,     // A2 = Na;  s = E U + c2 (2 Na - A1)
+  ST_R3_E2,     // U' = E U + c3 A1 + c4 A2 + c5 N
+  ST_R4_E0,     // A1 = N0;  s = E2 U + c1 N0                        (_setdrk_step.py:55-82)
+  ST_R4_E1,     // A2 = Na;  s = E2 U + c1 Na          (coef_2 is coef_1, _uncached.py:194)
+  ST_R4_E2,     // a = E2 U + c1 A1; s = E2 a + c1 (2 N - A1); A2 = A2 + N
+  ST_R4_E3,     // U' = ((E U + c4 A1) + (2 c5) A2) + c6 N
+  ST_EVAL,      // A1 = N (one nonlinear evaluation, no time step; Operator.__call__)
+};
+
+// number of nonlinear evaluations per step and the stage code of each
+__host__ __device__ inline int scheme_evals(int scheme) {
+  switch (scheme) {
+    case ETDRK1: case SETDRK1: return 1;
+    case ETDRK2: case SETDRK2: return 2;
+    case SETDRK3: return 3;
+    case SETDRK4: return 4;
+    default: return 0;
+  }
+}
+__host__ __device__ inline int scheme_stage(int scheme, int e) {
+  switch (scheme) {
+    case ETDRK1: case SETDRK1: return ST_R1_E0;
+    case ETDRK2: case SETDRK2: return e == 0 ? ST_R2_E0 : ST_R2_E1;
+    case SETDRK3: return ST_R3_E0 + e;
+    case SETDRK4: return ST_R4_E0 + e;
+    default: return ST_PRO;
+  }
+}
+
+// ---------------------------------------------------------------- non-contracted FP32 arithmetic
+// The reference evaluates every product and sum as a separate float32 op (torch elementwise
+// kernels); the combine below keeps that rounding by forbidding FMA contraction.
+__device__ __forceinline__ float mulr(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ float addr(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ float subr(float a, float b) { return __fsub_rn(a, b); }
+__device__ __forceinline__ float2 rscale(float c, float2 z) { return make_float2(mulr(c, z.x), mulr(c, z.y)); }
+__device__ __forceinline__ float2 radd(float2 a, float2 b) { return make_float2(addr(a.x, b.x), addr(a.y, b.y)); }
+__device__ __forceinline__ float2 rsub(float2 a, float2 b) { return make_float2(subr(a.x, b.x), subr(a.y, b.y)); }
+
+// Per-mode stage combine.  Arrays are indexed by the same element index; coefficient pointer
+// `cf` points at slot 0 of this sample's table, `cstride` = modes per table.  Returns s.
+__device__ __forceinline__ float2 stage_combine(int stage, float2 nval, float2* __restrict__ U,
+                                                float2* __restrict__ A1, float2* __restrict__ A2, size_t ei,
+                                                const float* __restrict__ cf, size_t cstride, size_t ci,
+                                                bool write) {
+  auto C = [&](int slot) { return cf[(size_t)slot * cstride + ci]; };
+  switch (stage) {
+    case ST_R1_E0: {
+      float2 u = U[ei];
+      float2 un = radd(rscale(C(CS_E), u), rscale(C(CS_C1), nval));
+      if (write) U[ei] = un;
+      return un;
+    }
+    case ST_R2_E0: {
+      float2 u = U[ei];
+      if (write) A1[ei] = nval;
+      return radd(rscale(C(CS_E), u), rscale(C(CS_C1), nval));
+    }
+    case ST_R2_E1: {
+      float2 u = U[ei], n0 = A1[ei];
+      float2 a = radd(rscale(C(CS_E), u), rscale(C(CS_C1), n0));
+      float2 un = radd(a, rscale(C(CS_C2), rsub(nval, n0)));
+      if (write) U[ei] = un;
+      return un;
+    }
+    case ST_R3_E0: {
+      float2 u = U[ei];
+      if (write) A1[ei] = nval;
+      return radd(rscale(C(CS_E2), u), rscale(C(CS_C1), nval));
+    }
+    case ST_R3_E1: {
+      float2 u = U[ei], n0 = A1[ei];
+      if (write) A2[ei] = nval;
+      float2 two_na = make_float2(2.0f * nval.x, 2.0f * nval.y);
+      return radd(rscale(C(CS_E), u), rscale(C(CS_C2), rsub(two_na, n0)));
+    }
+    case ST_R3_E2: {
+      float2 u = U[ei], n0 = A1[ei], na = A2[ei];
+      float2 un = radd(radd(radd(rscale(C(CS_E), u), rscale(C(CS_C3), n0)), rscale(C(CS_C4), na)),
+                       rscale(C(CS_C5), nval));
+      if (write) U[ei] = un;
+      return un;
+    }
+    case ST_R4_E0: {
+      float2 u = U[ei];
+      if (write) A1[ei] = nval;
+      return radd(rscale(C(CS_E2), u), rscale(C(CS_C1), nval));
+    }
+    case ST_R4_E1: {
+      float2 u = U[ei];
+      if (write) A2[ei] = nval;
+      return radd(rscale(C(CS_E2), u), rscale(C(CS_C1), nval));
+    }
+    case ST_R4_E2: {
+      float2 u = U[ei], n0 = A1[ei], na = A2[ei];
+      const float e2 = C(CS_E2), c1 = C(CS_C1);
+      float2 a = radd(rscale(e2, u), rscale(c1, n0));
+      float2 two_nb = make_float2(2.0f * nval.x, 2.0f * nval.y);
+      if (write) A2[ei] = radd(na, nval);
+      return radd(rscale(e2, a), rscale(c1, rsub(two_nb, n0)));
+    }
+    case ST_R4_E3: {
+      float2 u = U[ei], n0 = A1[ei], sab = A2[ei];
+      const float c5x2 = 2.0f * C(CS_C5);
+      float2 un = radd(radd(radd(rscale(C(CS_E), u), rscale(C(CS_C4), n0)), rscale(c5x2, sab)),
+                       rscale(C(CS_C6), nval));
+      if (write) U[ei] = un;
+      return un;
+    }
+    case ST_EVAL: {
+      if (write) A1[ei] = nval;
+      return nval;
+    }
+    default:
+      return U[ei];
+  }
+}
+
+// Nonlinear value of one mode: N = coef_t * F + S   (S = sum of coef_s * source spectra)
+__device__ __forceinline__ float2 nl_value(float coef_t, float2 f, const float2* __restrict__ src, size_t si) {
+  float2 n = rscale(coef_t, f);
+  if (src) n = radd(n, src[si]);
+  return n;
+}
+
+// ---------------------------------------------------------------- kernel argument blocks
+struct Col2Args {
+  int B, C, N0, N1, H1, P1, PF;
+  int kmax0, kmax1;
+  int NF;                     // number of field arrays
+  float coef_t;               // coefficient of the product term
+  float scale;                // 1/(N0 N1)
+  const float2* tw0;          // twiddles for N0
+  const float* ax;            // axis-0 wavenumbers (2 pi f), float32
+  const float* ay;            // axis-1 wavenumbers
+  const float2* P;            // products [B][NP][N0][P1]
+  float2* U;                  // state [B][C][N0][P1]
+  float2* A1;
+  float2* A2;
+  const float* coef;          // [Bc][NCOEF][N0*P1]
+  size_t coef_bstride;        // 0 or NCOEF*N0*P1
+  const float2* src;          // [Bs][Cs][N0][P1] or null
+  size_t src_bstride, src_cstride;
+  float2* G;                  // fields [B][NF][N0][PF]
+  int b0;                     // first sample handled by this launch
+};
+
+struct Row2Args {
+  int B, N0, N1, H1, P1, PF, kmax1, NF;
+  const float2* tw1;
+  const float* ay;
+  const float2* G;
+  float2* P;
+  int b0;
+};
+
+struct Run1Args {
+  int B, N, H, P, kmax;
+  float coef_t, scale;
+  int scheme;
+  const float2* tw;
+  const float* ax;            // [Bk][N] axis wavenumbers
+  size_t ax_bstride;
+  float2* U;
+  float2* A1;
+  float2* A2;
+  const float* coef;
+  size_t coef_bstride;
+  const float2* src;
+  size_t src_bstride;
+  int nsteps;
+  const int* nsteps_per;      // optional per-sample step counts (device), null = nsteps for all
+  const int* rec_slot;        // device [nsteps+1]: frame index recorded before step i, or -1
+  float2* traj;               // [B][T][P] recorded half spectra
+  int T;
+  int eval_only;              // 1: one nonlinear evaluation into A1, no stepping
+};
+
+
+enum LinKind : int { LIN_LAP = 1, LIN_BIHARM = 2, LIN_UNIT = 3 };
+
+struct LinArgs {
+  int D;
+  int n[3];
+  int H, P;
+  size_t modes;            // prod(n[:-1]) * P
+  const float* kv[3];      // axis wavenumbers a_d = fl32(fl32(2 pi) * f_d)
+  size_t kv_bstride;       // per-sample axis tables (1D serving); 0 = shared
+  int nterm;
+  int kind[4];
+  const float* coef;       // [nterm][Bc]
+  int Bc;
+  float* L;                // [Bc][modes]
+};
+
+struct CoefArgs {
+  int scheme;
+  double dt;
+  float dt32, hdt32;       // fl32(dt), fl32(0.5 * dt)
+  int M;                   // contour points
+  double radius;
+  size_t modes;
+  int Bc;
+  const float* L;          // [Bc][modes]
+  float* coef;             // [Bc][NCOEF][modes]
+};
+
+struct ConvArgs {
+  int D;
+  int n[3];
+  int H, P;
+  size_t nblk;  // number of [N...] blocks (batch * channels)
+};
+
+// ---------------------------------------------------------------- host launchers (defined in *.hip)
+hipError_t launch_col2d(int term, const Col2Args& a, int nb, int stage, int v, hipStream_t st);
+hipError_t launch_row2d(int term, const Row2Args& a, hipStream_t st);
+bool col2d_supported(int n0, int v);
+bool row2d_supported(int n1);
+hipError_t launch_rows_r2c(int n, const float* in, size_t in_ld, float2* out, size_t out_ld, int nrows, float scale,
+                           const float2* tw, hipStream_t st);
+hipError_t launch_rows_c2r(int n, const float2* in, size_t in_ld, float* out, size_t out_ld, int nrows, float scale,
+                           const float2* tw, hipStream_t st);
+hipError_t launch_rows_c2c(int n, int dir, const float2* in, size_t in_ld, float2* out, size_t out_ld, int nrows,
+                           float scale, const float2* tw, hipStream_t st);
+hipError_t launch_cols_c2c(int n, int dir, const float2* in, float2* out, size_t nouter, size_t outer_stride,
+                           size_t col_stride, int ninner, float scale, const float2* tw, hipStream_t st);
+hipError_t launch_full_to_half(const float2* full, float2* half, const ConvArgs& a, hipStream_t st);
+hipError_t launch_half_to_full(const float2* half, float2* full, const ConvArgs& a, hipStream_t st);
+hipError_t launch_linear_steps(float2* U, const float* E, size_t coef_bstride, size_t modes, int C, size_t total,
+                               int nsteps, const int* rec_slot, float2* traj, int T, hipStream_t st);
+hipError_t launch_record(const float2* U, float2* traj, const int* rec_slot, int step, int B, size_t per_sample,
+                         int T, hipStream_t st);
+hipError_t launch_run1d(const Run1Args& a, hipStream_t st);
+hipError_t launch_linear_symbol(const LinArgs& a, hipStream_t st);
+hipError_t launch_coefs(const CoefArgs& a, hipStream_t st);
+hipError_t launch_accum_source(float2* S, const float2* X, float coef, size_t total, int first, hipStream_t st);
+hipError_t launch_apply_linear(const float2* U, const float2* Nv, const float* L, size_t l_bstride, size_t modes, int C,
+                               size_t total, int have_lin, int have_nl, float2* out, hipStream_t st);
+
+}  // namespace tfsm

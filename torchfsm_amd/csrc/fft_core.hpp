@@ -1,0 +1,216 @@
+// Workgroup-cooperative Stockham FFT building blocks for gfx950 (CDNA4).
+//
+// A transform of length N is carried by NT = N / RMAX threads.  Thread j holds, in registers,
+// the RMAX elements at positions j + m*NT (m = 0..RMAX-1).  Every Stockham stage reads its R inputs
+// at stride N/R and writes its outputs to expand(jb) + r*Ns; with the register layout above the
+// first stage reads straight from registers and the last stage writes straight back to them in
+// natural order, so a transform of S stages needs S-1 LDS exchanges and no bit reversal.
+//
+// Several transforms can share a workgroup in two ways:
+//   * V interleaved transforms (strided columns of an array): position p of transform vi lives at
+//     LDS slot p*(V+PAD) + vi, so the V lanes of one position are contiguous and a wavefront moves
+//     whole 128-byte rows of a column tile (the HBM layout of the tile, no transpose needed);
+//   * G independent groups (contiguous rows), each with its own LDS region.
+//
+// Twiddles come from a table tw[m] = exp(-2 pi i m / N) (computed in double on the host, rounded
+// once), staged in LDS at kernel start.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace tfsm {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+// multiply by +i (DIR>0) or -i (DIR<0)
+template <int DIR>
+__device__ __forceinline__ float2 mul_i(float2 a) {
+  if constexpr (DIR > 0) return make_float2(-a.y, a.x);
+  else return make_float2(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------- in-register DFTs (natural order)
+// DIR = -1: forward (exp(-2 pi i ...)), DIR = +1: inverse (unnormalised).
+template <int DIR>
+__device__ __forceinline__ void dft2(float2* x) {
+  float2 t = x[0];
+  x[0] = cadd(t, x[1]);
+  x[1] = csub(t, x[1]);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft4(float2& x0, float2& x1, float2& x2, float2& x3) {
+  float2 t0 = cadd(x0, x2), t1 = csub(x0, x2);
+  float2 t2 = cadd(x1, x3), t3 = mul_i<DIR>(csub(x1, x3));
+  x0 = cadd(t0, t2);
+  x2 = csub(t0, t2);
+  x1 = cadd(t1, t3);
+  x3 = csub(t1, t3);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft4(float2* x) { dft4<DIR>(x[0], x[1], x[2], x[3]); }
+
+template <int DIR>
+__device__ __forceinline__ void dft8(float2* x) {
+  constexpr float c = 0.70710678118654752440f;
+  float2 e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+  float2 o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+  dft4<DIR>(e0, e1, e2, e3);
+  dft4<DIR>(o0, o1, o2, o3);
+  // W8^1 = (c, DIR c), W8^2 = DIR i, W8^3 = (-c, DIR c)
+  float2 w1 = make_float2(c * (o1.x - DIR * o1.y), c * (o1.y + DIR * o1.x));
+  float2 w2 = mul_i<DIR>(o2);
+  float2 w3 = make_float2(-c * (o3.x + DIR * o3.y), c * (DIR * o3.x - o3.y));
+  x[0] = cadd(e0, o0);
+  x[4] = csub(e0, o0);
+  x[1] = cadd(e1, w1);
+  x[5] = csub(e1, w1);
+  x[2] = cadd(e2, w2);
+  x[6] = csub(e2, w2);
+  x[3] = cadd(e3, w3);
+  x[7] = csub(e3, w3);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft16(float2* x) {
+  float2 e[8], o[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    e[i] = x[2 * i];
+    o[i] = x[2 * i + 1];
+  }
+  dft8<DIR>(e);
+  dft8<DIR>(o);
+  // W16^k = cos(2 pi k/16) + DIR i sin(2 pi k/16)
+  constexpr float C[8] = {1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+                          0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f};
+  constexpr float S[8] = {0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f,
+                          1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    float2 w = make_float2(C[k], DIR * S[k]);
+    float2 t = (k == 0) ? o[0] : cmul(o[k], w);
+    x[k] = cadd(e[k], t);
+    x[k + 8] = csub(e[k], t);
+  }
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void dft(float2* x) {
+  if constexpr (R == 2) dft2<DIR>(x);
+  else if constexpr (R == 4) dft4<DIR>(x);
+  else if constexpr (R == 8) dft8<DIR>(x);
+  else if constexpr (R == 16) dft16<DIR>(x);
+  else static_assert(R == 2 || R == 4 || R == 8 || R == 16, "unsupported radix");
+}
+
+// ---------------------------------------------------------------- radix plans
+// FftPlan<N>: the radix sequence used for a transform of length N, RMAX elements per thread.
+template <int N>
+struct FftPlan;
+template <> struct FftPlan<16> { static constexpr int RMAX = 4; static constexpr int NST = 2; static constexpr int R[4] = {4, 4, 0, 0}; };
+template <> struct FftPlan<32> { static constexpr int RMAX = 8; static constexpr int NST = 2; static constexpr int R[4] = {8, 4, 0, 0}; };
+template <> struct FftPlan<64> { static constexpr int RMAX = 8; static constexpr int NST = 2; static constexpr int R[4] = {8, 8, 0, 0}; };
+template <> struct FftPlan<128> { static constexpr int RMAX = 16; static constexpr int NST = 2; static constexpr int R[4] = {16, 8, 0, 0}; };
+template <> struct FftPlan<256> { static constexpr int RMAX = 16; static constexpr int NST = 2; static constexpr int R[4] = {16, 16, 0, 0}; };
+template <> struct FftPlan<512> { static constexpr int RMAX = 8; static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 8, 0}; };
+template <> struct FftPlan<1024> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 8, 8, 0}; };
+template <> struct FftPlan<2048> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 16, 8, 0}; };
+template <> struct FftPlan<4096> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 16, 16, 0}; };
+
+template <int N>
+struct Fft {
+  static constexpr int RMAX = FftPlan<N>::RMAX;
+  static constexpr int NT = N / RMAX;  // threads per transform
+  static constexpr int NST = FftPlan<N>::NST;
+  static constexpr int radix(int s) { return FftPlan<N>::R[s]; }
+  static constexpr int ns_before(int s) {
+    int ns = 1;
+    for (int i = 0; i < s; ++i) ns *= FftPlan<N>::R[i];
+    return ns;
+  }
+};
+
+// LDS slot of position p of interleaved transform vi (V transforms interleaved).
+template <int V>
+__device__ __forceinline__ int lds_slot(int p, int vi) {
+  if constexpr (V == 1) return p + (p >> 4);
+  else return p * (V + 1) + vi;
+}
+template <int N, int V>
+struct LdsSize {
+  static constexpr int slots = (V == 1) ? (N + N / 16) : N * (V + 1);
+};
+
+// One Stockham stage (radix R, Ns = product of earlier radices).  See header comment.
+template <int N, int S, int DIR, int V>
+__device__ __forceinline__ void fft_stage(float2 (&v)[Fft<N>::RMAX], float2* lds, const float2* tw, int j,
+                                          int vi) {
+  using F = Fft<N>;
+  constexpr int RMAX = F::RMAX;
+  constexpr int NT = F::NT;
+  constexpr int R = F::radix(S);
+  constexpr int NS = F::ns_before(S);
+  constexpr int Q = RMAX / R;
+  constexpr bool FIRST = (S == 0);
+  constexpr bool LAST = (S == F::NST - 1);
+  constexpr int TWS = N / (NS * R);
+  if constexpr (!FIRST) {
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) v[m] = lds[lds_slot<V>(j + m * NT, vi)];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int jb = j + q * NT;
+    float2 x[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = v[q + r * Q];
+    if constexpr (NS > 1) {
+      const int k = jb % NS;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[r * k * TWS];
+        x[r] = (DIR < 0) ? cmul(x[r], w) : cmulc(x[r], w);
+      }
+    }
+    dft<R, DIR>(x);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q + r * Q] = x[r];
+    } else {
+      const int idxd = (jb / NS) * NS * R + (jb % NS);
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[lds_slot<V>(idxd + r * NS, vi)] = x[r];
+    }
+  }
+  if constexpr (!LAST) __syncthreads();
+}
+
+// Full transform: registers (positions j + m*NT) -> registers (same positions, natural order).
+// All threads of the workgroup must call it (it contains barriers).
+template <int N, int DIR, int V>
+__device__ __forceinline__ void fft_wg(float2 (&v)[Fft<N>::RMAX], float2* lds, const float2* tw, int j, int vi) {
+  using F = Fft<N>;
+  fft_stage<N, 0, DIR, V>(v, lds, tw, j, vi);
+  if constexpr (F::NST > 1) fft_stage<N, 1, DIR, V>(v, lds, tw, j, vi);
+  if constexpr (F::NST > 2) fft_stage<N, 2, DIR, V>(v, lds, tw, j, vi);
+  if constexpr (F::NST > 3) fft_stage<N, 3, DIR, V>(v, lds, tw, j, vi);
+}
+
+// Cooperative copy of the twiddle table into LDS (call before the first fft_wg, then barrier).
+template <int N>
+__device__ __forceinline__ void load_twiddles(float2* tw_lds, const float2* tw_glb) {
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw_lds[i] = tw_glb[i];
+}
+
+}  // namespace tfsm

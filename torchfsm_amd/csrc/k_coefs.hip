@@ -1,0 +1,245 @@
+// Setup kernels: linear symbol L per mode (float32, the reference's op order), ETD coefficient
+// tables (SETDRK: float64 contour means; ETDRK: the reference's complex64 closed forms), and the
+// accumulation of explicit source spectra.
+#include "engine.hpp"
+
+namespace tfsm {
+
+
+
+// L = sum_t coef_t * sym_t in generator order (_base.py:339-357), each product and sum rounded in
+// float32: lap = (-(a0 a0)) + (-(a1 a1)) [+ (-(a2 a2))] (mesh.py:406-426), biharm = lap * lap,
+// unit = 1 (_source.py:9-17).  Padding modes (k >= H) get 0.
+__global__ void k_linear_symbol(LinArgs a) {
+  const size_t total = a.modes * a.Bc;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t mode = e % a.modes;
+    const int b = (int)(e / a.modes);
+    const int k = (int)(mode % a.P);
+    size_t r = mode / a.P;
+    int idx[3] = {0, 0, 0};
+    idx[a.D - 1] = k;
+    if (a.D == 3) {
+      idx[1] = (int)(r % a.n[1]);
+      r /= a.n[1];
+    }
+    if (a.D >= 2) idx[0] = (int)r;
+    float out = 0.f;
+    if (k < a.H) {
+      float lap = 0.f;
+      for (int d = 0; d < a.D; ++d) {
+        const float av = a.kv[d][(size_t)b * a.kv_bstride + idx[d]];
+        const float g = -mulr(av, av);
+        lap = (d == 0) ? g : addr(lap, g);
+      }
+      for (int t = 0; t < a.nterm; ++t) {
+        const float c = a.coef[t * a.Bc + b];
+        float sym = 1.f;
+        if (a.kind[t] == LIN_LAP) sym = lap;
+        else if (a.kind[t] == LIN_BIHARM) sym = mulr(lap, lap);
+        const float v = mulr(c, sym);
+        out = (t == 0) ? v : addr(out, v);
+      }
+    }
+    a.L[e] = out;
+  }
+}
+
+// ---------------------------------------------------------------- complex double helpers
+struct cd {
+  double x, y;
+};
+__device__ __forceinline__ cd cd_add(cd a, cd b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cd cd_sub(cd a, cd b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cd cd_mul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ cd cd_scale(cd a, double s) { return {a.x * s, a.y * s}; }
+__device__ __forceinline__ cd cd_div(cd a, cd b) {
+  const double d = b.x * b.x + b.y * b.y;
+  return {(a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d};
+}
+__device__ __forceinline__ cd cd_exp(cd a) {
+  const double e = exp(a.x);
+  double s, c;
+  sincos(a.y, &s, &c);
+  return {e * c, e * s};
+}
+
+
+
+// SETDRK1-4: Kassam-Trefethen contour means in float64 (_uncached.py:34-43, 71-72, 102-104,
+// 135-156, 191-211).  z = fl32(L * fl32(dt)) is the reference's float32 centre; the contour points
+// lr_j = r e^{i theta_j} + z with theta_j = 2 pi (j - 1/2) / M, the mean is taken in double and
+// rounded once.  E = exp(z), E2 = exp(fl32(L * fl32(dt/2))) in double, rounded once.
+__global__ void k_setdrk_coefs(CoefArgs a) {
+  const size_t total = a.modes * a.Bc;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t mode = e % a.modes;
+    const size_t b = e / a.modes;
+    const float l = a.L[e];
+    const double z = (double)mulr(l, a.dt32);
+    const double z2 = (double)mulr(l, a.hdt32);
+    float* out = a.coef + b * (size_t)NCOEF * a.modes + mode;
+    out[CS_E * a.modes] = (float)exp(z);
+    if (a.scheme == SETDRK3 || a.scheme == SETDRK4) out[CS_E2 * a.modes] = (float)exp(z2);
+    cd s1 = {0, 0}, s2 = {0, 0}, s3 = {0, 0}, s4 = {0, 0}, s5 = {0, 0};
+    for (int jj = 1; jj <= a.M; ++jj) {
+      const double th = 2.0 * M_PI * (jj - 0.5) / a.M;
+      cd lr = {a.radius * cos(th) + z, a.radius * sin(th)};
+      const cd one = {1.0, 0.0};
+      const cd el = cd_exp(lr);
+      const cd lr2 = cd_mul(lr, lr);
+      const cd lr3 = cd_mul(lr2, lr);
+      switch (a.scheme) {
+        case SETDRK1:
+          s1 = cd_add(s1, cd_div(cd_sub(el, one), lr));
+          break;
+        case SETDRK2:
+          s1 = cd_add(s1, cd_div(cd_sub(el, one), lr));
+          s2 = cd_add(s2, cd_div(cd_sub(cd_sub(el, one), lr), lr2));
+          break;
+        case SETDRK3: {
+          const cd eh = cd_exp(cd_scale(lr, 0.5));
+          s1 = cd_add(s1, cd_div(cd_sub(eh, one), lr));
+          s2 = cd_add(s2, cd_div(cd_sub(el, one), lr));
+          // (-4 - lr + e^lr (4 - 3 lr + lr^2)) / lr^3
+          cd t = cd_add(cd_sub({4.0, 0.0}, cd_scale(lr, 3.0)), lr2);
+          s3 = cd_add(s3, cd_div(cd_add(cd_sub({-4.0, 0.0}, lr), cd_mul(el, t)), lr3));
+          // 4 (2 + lr + e^lr (-2 + lr)) / lr^3
+          t = cd_add({-2.0, 0.0}, lr);
+          s4 = cd_add(s4, cd_div(cd_scale(cd_add(cd_add({2.0, 0.0}, lr), cd_mul(el, t)), 4.0), lr3));
+          // (-4 - 3 lr - lr^2 + e^lr (4 - lr)) / lr^3
+          t = cd_sub({4.0, 0.0}, lr);
+          s5 = cd_add(s5, cd_div(cd_add(cd_sub(cd_sub({-4.0, 0.0}, cd_scale(lr, 3.0)), lr2), cd_mul(el, t)), lr3));
+          break;
+        }
+        case SETDRK4: {
+          const cd eh = cd_exp(cd_scale(lr, 0.5));
+          s1 = cd_add(s1, cd_div(cd_sub(eh, one), lr));
+          cd t = cd_add(cd_sub({4.0, 0.0}, cd_scale(lr, 3.0)), lr2);
+          s2 = cd_add(s2, cd_div(cd_add(cd_sub({-4.0, 0.0}, lr), cd_mul(el, t)), lr3));
+          t = cd_add({-2.0, 0.0}, lr);
+          s3 = cd_add(s3, cd_div(cd_add(cd_add({2.0, 0.0}, lr), cd_mul(el, t)), lr3));
+          t = cd_sub({4.0, 0.0}, lr);
+          s4 = cd_add(s4, cd_div(cd_add(cd_sub(cd_sub({-4.0, 0.0}, cd_scale(lr, 3.0)), lr2), cd_mul(el, t)), lr3));
+          break;
+        }
+        default:
+          break;
+      }
+    }
+    const double w = a.dt / a.M;
+    switch (a.scheme) {
+      case SETDRK1:
+        out[CS_C1 * a.modes] = (float)(w * s1.x);
+        break;
+      case SETDRK2:
+        out[CS_C1 * a.modes] = (float)(w * s1.x);
+        out[CS_C2 * a.modes] = (float)(w * s2.x);
+        break;
+      case SETDRK3:
+        out[CS_C1 * a.modes] = (float)(w * s1.x);
+        out[CS_C2 * a.modes] = (float)(w * s2.x);
+        out[CS_C3 * a.modes] = (float)(w * s3.x);
+        out[CS_C4 * a.modes] = (float)(w * s4.x);
+        out[CS_C5 * a.modes] = (float)(w * s5.x);
+        break;
+      case SETDRK4:
+        out[CS_C1 * a.modes] = (float)(w * s1.x);
+        out[CS_C4 * a.modes] = (float)(w * s2.x);
+        out[CS_C5 * a.modes] = (float)(w * s3.x);
+        out[CS_C6 * a.modes] = (float)(w * s4.x);
+        break;
+      default:
+        break;
+    }
+  }
+}
+
+// ETDRK0/1/2: the reference's complex64 closed forms (_etdrk.py:21, 43-45, 66-70) with a real L:
+// E = exp(fl32(dt L)) (double exp, rounded once: the reference's float32 exp is correctly rounded
+// on these inputs), c1 = where(L == 0, dt, (E - 1) * (1 / L)), c2 = where(L == 0, dt/2,
+// ((E - 1) - L dt) * (1 / (L^2 dt))).  complex64 division by a value with zero imaginary part is a
+// multiplication by the reciprocal in torch, kept here; every op is rounded in float32, so the
+// float32 cancellation of c2 at small |dt L| is reproduced, not repaired.
+__global__ void k_etdrk_coefs(CoefArgs a) {
+  const size_t total = a.modes * a.Bc;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t mode = e % a.modes;
+    const size_t b = e / a.modes;
+    const float l = a.L[e];
+    const float ldt = mulr(l, a.dt32);
+    const float E = (float)exp((double)ldt);
+    float* out = a.coef + b * (size_t)NCOEF * a.modes + mode;
+    out[CS_E * a.modes] = E;
+    if (a.scheme == ETDRK0) continue;
+    const float em1 = subr(E, 1.0f);
+    out[CS_C1 * a.modes] = (l == 0.f) ? a.dt32 : mulr(em1, __fdiv_rn(1.0f, l));
+    if (a.scheme == ETDRK2) {
+      const float num = subr(em1, ldt);
+      const float den = mulr(mulr(l, l), a.dt32);
+      out[CS_C2 * a.modes] = (l == 0.f) ? (float)(a.dt / 2) : mulr(num, __fdiv_rn(1.0f, den));
+    }
+  }
+}
+
+// S <- S + coef * X  (first source: S = coef * X); X and S are half spectra of the same shape.
+__global__ void k_accum_source(float2* __restrict__ S, const float2* __restrict__ X, float coef, size_t total,
+                               int first) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const float2 v = rscale(coef, X[e]);
+    S[e] = first ? v : radd(S[e], v);
+  }
+}
+
+// out = L * U + N   (Operator.__call__ via _build_operator, _base.py:408-439); L per mode, real.
+// have_lin == 2: out = where(L == 0, 1, 1 / L) * U   (LinearOperator.solve, _base.py:250-258).
+__global__ void k_apply_linear(const float2* __restrict__ U, const float2* __restrict__ Nv, const float* __restrict__ L,
+                               size_t l_bstride, size_t modes, int C, size_t total, int have_lin, int have_nl,
+                               float2* __restrict__ out) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t mode = e % modes;
+    const size_t b = e / modes / C;
+    float2 r = make_float2(0.f, 0.f);
+    if (have_lin == 2) {  // solve: where(L == 0, 1, 1 / L) * U
+      const float l = L[b * l_bstride + mode];
+      out[e] = rscale(l == 0.f ? 1.0f : __fdiv_rn(1.0f, l), U[e]);
+      continue;
+    }
+    if (have_lin) r = rscale(L[b * l_bstride + mode], U[e]);
+    if (have_nl) r = have_lin ? radd(r, Nv[e]) : Nv[e];
+    out[e] = r;
+  }
+}
+
+static unsigned grid_for(size_t total) {
+  size_t b = (total + 255) / 256;
+  if (b > 65535u * 4) b = 65535u * 4;
+  return (unsigned)(b ? b : 1);
+}
+
+hipError_t launch_linear_symbol(const LinArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_linear_symbol, dim3(grid_for(a.modes * a.Bc)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_coefs(const CoefArgs& a, hipStream_t st) {
+  if (a.scheme >= SETDRK1)
+    hipLaunchKernelGGL(k_setdrk_coefs, dim3(grid_for(a.modes * a.Bc)), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_etdrk_coefs, dim3(grid_for(a.modes * a.Bc)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_accum_source(float2* S, const float2* X, float coef, size_t total, int first, hipStream_t st) {
+  hipLaunchKernelGGL(k_accum_source, dim3(grid_for(total)), dim3(256), 0, st, S, X, coef, total, first);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_linear(const float2* U, const float2* Nv, const float* L, size_t l_bstride, size_t modes, int C,
+                               size_t total, int have_lin, int have_nl, float2* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_apply_linear, dim3(grid_for(total)), dim3(256), 0, st, U, Nv, L, l_bstride, modes, C, total,
+                     have_lin, have_nl, out);
+  return hipGetLastError();
+}
+
+}  // namespace tfsm

@@ -1,0 +1,316 @@
+"""Python handle of a compiled engine plan (tfsm_plan) and its device buffers.
+
+A plan is one operator on one mesh for one batch size: wavenumber and twiddle tables, the linear
+symbol L, ETD coefficient tables, accumulated explicit-source spectra, and the workspace layout of
+the fused step kernels.  All device memory for states and workspaces comes from the torch caching
+allocator; pointers and the current torch stream are passed through the C ABI.
+"""
+
+import ctypes
+from typing import Optional, Sequence, Tuple
+
+import torch
+
+from . import _lib
+from ._lib import check, lib
+
+SCHEME_IDS = {
+    "ETDRK0": _lib.ETDRK0,
+    "ETDRK1": _lib.ETDRK1,
+    "ETDRK2": _lib.ETDRK2,
+    "SETDRK1": _lib.SETDRK1,
+    "SETDRK2": _lib.SETDRK2,
+    "SETDRK3": _lib.SETDRK3,
+    "SETDRK4": _lib.SETDRK4,
+}
+
+
+def _stream(device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _ptr(t: Optional[torch.Tensor]) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr() if t is not None else 0)
+
+
+def require_gpu_tensor(t: torch.Tensor, what="input"):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{what} must be a torch.Tensor")
+    if t.device.type != "cuda":
+        raise RuntimeError(
+            f"{what} is on {t.device}: the MI355X engine runs on the GPU only (move the tensor and mesh to 'cuda')"
+        )
+
+
+class SpectralPlan:
+    """tfsm_plan wrapper.
+
+    Args:
+        mesh_info: sequence of (start, end, n) per axis.
+        batch, channels: B and C of the state.
+        nonlinear: None, "conv" or "vc" (the dealiased product term) and its coefficient.
+        linear: list of (kind, coef) with kind in {"lap", "biharm", "unit"}; coef a float or a
+            per-sample sequence of length `batch`.
+        device: CUDA device.
+        dealias_rate: 2/3 by default.
+        lengths_per_sample: 1D only, per-sample domain lengths (serving of mixed 1D requests).
+    """
+
+    KIND = {"lap": _lib.LIN_LAP, "biharm": _lib.LIN_BIHARM, "unit": _lib.LIN_UNIT}
+    TERM = {None: _lib.TERM_NONE, "conv": _lib.TERM_CONV, "vc": _lib.TERM_VC}
+
+    def __init__(self, mesh_info, batch, channels, nonlinear=None, nonlinear_coef=0.0, linear=(), device="cuda",
+                 dealias_rate=2 / 3, lengths_per_sample: Optional[Sequence[float]] = None):
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise RuntimeError("the MI355X engine needs a CUDA (HIP) device")
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        self.mesh_info = [tuple(m) for m in mesh_info]
+        self.ndim = len(self.mesh_info)
+        self.n = tuple(int(m[2]) for m in self.mesh_info)
+        self.batch = int(batch)
+        self.channels = int(channels)
+        self.nonlinear = nonlinear
+        geom = _lib.Geometry()
+        geom.ndim = self.ndim
+        for d, (s, e, n) in enumerate(self.mesh_info):
+            geom.n[d] = int(n)
+            geom.length[d] = float(e - s)
+        geom.batch = self.batch
+        geom.channels = self.channels
+        self._lens = None
+        if lengths_per_sample is not None:
+            self._lens = (ctypes.c_double * self.batch)(*[float(x) for x in lengths_per_sample])
+            geom.length_per_sample = ctypes.cast(self._lens, ctypes.POINTER(ctypes.c_double))
+        terms = _lib.Terms()
+        terms.nonlinear = self.TERM[nonlinear]
+        terms.nonlinear_coef = float(nonlinear_coef)
+        terms.dealias_rate = float(dealias_rate)
+        lin = list(linear)
+        if len(lin) > 4:
+            raise ValueError("term not supported by engine: more than 4 linear terms")
+        terms.n_linear = len(lin)
+        per_sample = any(not isinstance(c, (int, float)) for _, c in lin)
+        cb = self.batch if per_sample else 1
+        flat = []
+        for t, (kind, c) in enumerate(lin):
+            if kind not in self.KIND:
+                raise ValueError(f"term not supported by engine: linear kind {kind}")
+            terms.linear_kind[t] = self.KIND[kind]
+            if isinstance(c, (int, float)):
+                flat += [float(c)] * cb
+            else:
+                vals = [float(v) for v in c]
+                if len(vals) == 1:
+                    vals = vals * cb
+                if len(vals) != cb:
+                    raise ValueError("per-sample linear coefficients must match the batch size")
+                flat += vals
+        self._coefs = (ctypes.c_float * max(1, len(flat)))(*flat)
+        terms.linear_coef = ctypes.cast(self._coefs, ctypes.POINTER(ctypes.c_float))
+        terms.coef_batch = cb
+        self._coef_batch = cb if lengths_per_sample is None else self.batch
+        self._plan = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_plan_create(ctypes.byref(geom), ctypes.byref(terms), _stream(self.device),
+                                       ctypes.byref(self._plan)), "plan")
+        self.pitch = int(lib.tfsm_plan_pitch(self._plan))
+        self.state_elems = int(lib.tfsm_plan_state_elems(self._plan))
+        self.ws_bytes = int(lib.tfsm_plan_workspace_bytes(self._plan))
+        self.spec_shape = (self.batch, self.channels) + self.n[:-1] + (self.pitch,)
+        self._ws = None
+        self.scheme = None
+        self.dt = None
+        self.tile = 16
+
+    # ------------------------------------------------------------------ lifecycle
+    def close(self):
+        if getattr(self, "_plan", None) is not None and self._plan.value:
+            lib.tfsm_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._plan
+
+    def workspace(self) -> torch.Tensor:
+        if self._ws is None:
+            self._ws = torch.zeros(max(self.ws_bytes, 256), dtype=torch.uint8, device=self.device)
+        return self._ws
+
+    def set_tile(self, tile: int):
+        check(lib.tfsm_plan_set_tile(self._plan, int(tile)), "tile")
+        self.tile = int(tile)
+
+    def set_scheme(self, scheme: str, dt: float, n_points: int = 16, radius: float = 1.0):
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_plan_set_scheme(self._plan, SCHEME_IDS[scheme], float(dt), int(n_points), float(radius),
+                                           _stream(self.device)), "scheme")
+        self.scheme = scheme
+        self.dt = dt
+
+    COEF_SLOTS = {"E": 0, "E2": 1, "c1": 2, "c2": 3, "c3": 4, "c4": 5, "c5": 6, "c6": 7}
+
+    def coefficients(self, name: str) -> torch.Tensor:
+        """Coefficient table (E, E2, c1..c6) as float32 [coef_batch, C?=1, N..., P] (valid k < N/2+1)."""
+        nb = self.batch if self._coef_batch > 1 else 1
+        out = torch.empty((nb, 1) + self.n[:-1] + (self.pitch,), dtype=torch.float32, device=self.device)
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_plan_coefficients(self._plan, self.COEF_SLOTS[name], _ptr(out), _stream(self.device)),
+                  "coefficients")
+        return out
+
+    def add_source(self, src: torch.Tensor, coef: float):
+        """Explicit source coef * fftn(src); src is (Bs, Cs, N...) with Bs in {1, B}, Cs in {1, C}."""
+        if src.dim() != self.ndim + 2:
+            raise ValueError(f"explicit source must have shape (B, C, N...), got {tuple(src.shape)}")
+        s = src.detach().to(device=self.device, dtype=torch.float32).contiguous()
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_plan_add_source(self._plan, _ptr(s), int(s.shape[0]), int(s.shape[1]), float(coef),
+                                           _stream(self.device)), "source")
+            torch.cuda.current_stream(self.device).synchronize()
+
+    # ------------------------------------------------------------------ buffers
+    def empty_spectrum(self, nbatch=None) -> torch.Tensor:
+        nb = self.batch if nbatch is None else nbatch
+        return torch.zeros((nb, self.channels) + self.n[:-1] + (self.pitch,), dtype=torch.complex64,
+                           device=self.device)
+
+    # ------------------------------------------------------------------ transforms
+    def forward(self, u: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Real (nb, C, N...) float32 -> half spectrum (nb, C, ..., P)."""
+        nb = u.shape[0]
+        u = u.to(torch.float32).contiguous()
+        out = self.empty_spectrum(nb) if out is None else out
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_forward(self._plan, _ptr(u), _ptr(out), int(nb), _stream(self.device)), "forward")
+        return out
+
+    def inverse(self, uh: torch.Tensor, nbatch: Optional[int] = None) -> torch.Tensor:
+        """Half spectrum (nb, C, ..., P) -> real (nb, C, N...) = ifftn(.).real."""
+        nb = uh.shape[0] if nbatch is None else nbatch
+        out = torch.empty((nb, self.channels) + self.n, dtype=torch.float32, device=self.device)
+        sb = int(lib.tfsm_plan_inverse_scratch_bytes(self._plan, int(nb)))
+        scratch = torch.empty(max(sb, 8), dtype=torch.uint8, device=self.device)
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_inverse(self._plan, _ptr(uh), _ptr(out), int(nb), _ptr(scratch), _stream(self.device)),
+                  "inverse")
+        return out
+
+    def half_to_full(self, uh: torch.Tensor, nbatch: Optional[int] = None) -> torch.Tensor:
+        nb = uh.shape[0] if nbatch is None else nbatch
+        out = torch.empty((nb, self.channels) + self.n, dtype=torch.complex64, device=self.device)
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_half_to_full(self._plan, _ptr(uh), _ptr(out), int(nb), _stream(self.device)), "half_to_full")
+        return out
+
+    def full_to_half(self, full: torch.Tensor) -> torch.Tensor:
+        nb = full.shape[0]
+        full = full.to(torch.complex64).contiguous()
+        out = self.empty_spectrum(nb)
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_full_to_half(self._plan, _ptr(full), _ptr(out), int(nb), _stream(self.device)), "full_to_half")
+        return out
+
+    def c2c(self, x: torch.Tensor, direction: int) -> torch.Tensor:
+        """Complex (nb, C, N...) -> complex, over the last D axes (direction -1 forward, +1 inverse)."""
+        nb = x.shape[0]
+        x = x.to(torch.complex64).contiguous()
+        out = torch.empty_like(x)
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_c2c(self._plan, _ptr(x), _ptr(out), int(nb), int(direction), _stream(self.device)), "c2c")
+        return out
+
+    # ------------------------------------------------------------------ stepping
+    def step(self, uh: torch.Tensor, nsteps: int, rec_slot: Optional[torch.Tensor] = None,
+             traj: Optional[torch.Tensor] = None):
+        """Advance uh in place by nsteps (optionally recording frames into traj [B][T][C][...][P])."""
+        if self.scheme is None:
+            raise RuntimeError("set_scheme must be called before stepping")
+        T = int(traj.shape[1]) if traj is not None else 0
+        ws = self.workspace()
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_step(self._plan, _ptr(uh), int(nsteps), _ptr(rec_slot), _ptr(traj), T, _ptr(ws),
+                                _stream(self.device)), "step")
+        return uh
+
+    def step_varying(self, uh: torch.Tensor, nsteps_per_sample: torch.Tensor, max_steps: int):
+        ws = self.workspace()
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_step_varying(self._plan, _ptr(uh), _ptr(nsteps_per_sample), int(max_steps), _ptr(ws),
+                                        _stream(self.device)), "step_varying")
+        return uh
+
+    def apply(self, uh: torch.Tensor) -> torch.Tensor:
+        """L uh + N(uh) (half spectra)."""
+        out = torch.empty_like(uh)
+        ws = self.workspace()
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_apply(self._plan, _ptr(uh), _ptr(out), _ptr(ws), _stream(self.device)), "apply")
+        return out
+
+    def solve(self, uh: torch.Tensor) -> torch.Tensor:
+        """uh / L with 1/0 := 1 (half spectra)."""
+        out = torch.empty_like(uh)
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_solve(self._plan, _ptr(uh), _ptr(out), _stream(self.device)), "solve")
+        return out
+
+    def launch_pass(self, uh: torch.Tensor, pass_id: int, stage: int = 0):
+        """One row (0) / column (1, stage code) / 1D step (2) pass, for kernel timing."""
+        ws = self.workspace()
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_launch_pass(self._plan, _ptr(uh), _ptr(ws), int(pass_id), int(stage),
+                                       _stream(self.device)), "launch_pass")
+
+    def graph(self, uh: torch.Tensor, nsteps: int) -> "StepGraph":
+        return StepGraph(self, uh, nsteps)
+
+
+class StepGraph:
+    """nsteps of tfsm_step captured into a HIP graph on fixed state/workspace buffers."""
+
+    def __init__(self, plan: SpectralPlan, uh: torch.Tensor, nsteps: int):
+        self.plan = plan
+        self.uh = uh
+        self.nsteps = nsteps
+        self._g = ctypes.c_void_p()
+        ws = plan.workspace()
+        with torch.cuda.device(plan.device):
+            check(lib.tfsm_graph_create(plan.handle, _ptr(uh), int(nsteps), _ptr(ws), _stream(plan.device),
+                                        ctypes.byref(self._g)), "graph")
+
+    def launch(self):
+        with torch.cuda.device(self.plan.device):
+            check(lib.tfsm_graph_launch(self._g, _stream(self.plan.device)), "graph launch")
+
+    def close(self):
+        if self._g.value:
+            lib.tfsm_graph_destroy(self._g)
+            self._g = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_TRANSFORM_PLANS = {}
+
+
+def transform_plan(mesh_info, device) -> SpectralPlan:
+    """Geometry-only plan (no operator) used by FourierMesh.fft / .ifft; cached per mesh and device."""
+    key = (tuple(tuple(float(x) if i < 2 else int(x) for i, x in enumerate(m)) for m in mesh_info), str(device))
+    p = _TRANSFORM_PLANS.get(key)
+    if p is None:
+        p = SpectralPlan(mesh_info, 1, 1, device=device)
+        _TRANSFORM_PLANS[key] = p
+    return p

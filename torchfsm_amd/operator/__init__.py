@@ -1,0 +1,31 @@
+"""Operators with the reference's names (torchfsm.operator)."""
+
+from ._base import (
+    CoreGenerator,
+    ExplicitSource,
+    LinearCoef,
+    LinearOperator,
+    NonlinearFunc,
+    NonlinearOperator,
+    Operator,
+    OperatorLike,
+    check_value_with_mesh,
+)
+from ._cores import Biharmonic, Convection, ImplicitSource, Laplacian, VorticityConvection
+
+__all__ = [
+    "Operator",
+    "OperatorLike",
+    "LinearOperator",
+    "NonlinearOperator",
+    "LinearCoef",
+    "NonlinearFunc",
+    "CoreGenerator",
+    "ExplicitSource",
+    "ImplicitSource",
+    "Laplacian",
+    "Biharmonic",
+    "Convection",
+    "VorticityConvection",
+    "check_value_with_mesh",
+]

@@ -1,0 +1,575 @@
+"""Operator algebra and the integrate driver with the reference's interface
+(operator/_base.py:16-1028), compiled onto the MI355X engine.
+
+register_mesh turns the generator list into an engine term descriptor: linear symbols
+{Laplacian, Biharmonic, ImplicitSource()} with their coefficients, at most one dealiased product
+term {Convection, VorticityConvection} and any number of ExplicitSource spectra.  integrate and
+__call__ run entirely in libtfsm; a term outside that set raises ValueError (there is no torch
+fallback).
+"""
+
+import os
+from abc import ABC, abstractmethod
+from typing import Callable, List, Literal, Optional, Sequence, Tuple, Union
+
+import torch
+from torch import Tensor
+from tqdm.auto import tqdm
+
+from ..engine import SpectralPlan, require_gpu_tensor
+from ..integrator import ETDRKIntegrator, RKIntegrator, SETDRKIntegrator
+from ..mesh import FourierMesh, MeshGrid
+from ..traj_recorder import AutoRecorder, _TrajRecorder
+from ..utils import default
+
+
+class LinearCoef(ABC):
+    """A linear symbol generator core (_base.py:16-55)."""
+
+    _engine_kind: Optional[str] = None
+
+    @abstractmethod
+    def __call__(self, f_mesh: FourierMesh, n_channel: int) -> Tensor:
+        raise NotImplementedError
+
+    def nonlinear_like(self, u_fft, f_mesh, u=None):
+        return self(f_mesh, u_fft.shape[1]) * u_fft
+
+
+class NonlinearFunc(ABC):
+    """A nonlinear term core (_base.py:58-109)."""
+
+    _engine_kind: Optional[str] = None
+
+    def __init__(self, dealiasing_swtich: bool = True) -> None:
+        self._dealiasing_swtich = dealiasing_swtich
+
+    @abstractmethod
+    def __call__(self, u_fft, f_mesh: FourierMesh, u=None):
+        raise NotImplementedError
+
+    def spatial_value(self, u_fft, f_mesh: FourierMesh, u=None):
+        return f_mesh.ifft(self(u_fft, f_mesh, u)).real
+
+
+class CoreGenerator(ABC):
+    """Builds a core for a mesh and channel count (_base.py:112-131)."""
+
+    @abstractmethod
+    def __call__(self, f_mesh: FourierMesh, n_channel: int) -> Union[LinearCoef, NonlinearFunc]:
+        raise NotImplementedError
+
+
+GeneratorLike = Union[CoreGenerator, Callable[[FourierMesh, int], Union[LinearCoef, NonlinearFunc]]]
+
+
+def check_value_with_mesh(u, mesh):
+    """_base.py:144-167."""
+    if isinstance(mesh, (FourierMesh, MeshGrid)):
+        mesh = mesh.mesh_info
+    n_dim = len(mesh)
+    value_shape = u.shape
+    if len(value_shape) - 2 != n_dim:
+        raise ValueError(f"the value shape {value_shape} is not compatible with mesh dim {n_dim}")
+    for i in range(n_dim):
+        if value_shape[i + 2] != mesh[i][2]:
+            raise ValueError(f"Expect to have {mesh[i][2]} points in dim {i} but got {value_shape[i+2]}")
+
+
+class _MutableMixIn:
+    """+, -, scalar * and / on operators (_base.py:170-207)."""
+
+    def __radd__(self, other):
+        return self + other
+
+    def __iadd__(self, other):
+        return self + other
+
+    def __sub__(self, other):
+        try:
+            return self + (-1 * other)
+        except Exception:
+            return NotImplemented
+
+    def __rsub__(self, other):
+        try:
+            return other + (-1 * self)
+        except Exception:
+            return NotImplemented
+
+    def __isub__(self, other):
+        return self - other
+
+    def __rmul__(self, other):
+        return self * other
+
+    def __imul__(self, other):
+        return self * other
+
+    def __truediv__(self, other):
+        try:
+            return self * (1 / other)
+        except Exception:
+            return NotImplemented
+
+
+def _coef_value(coef, batch):
+    """A python number, or a tensor broadcast over the batch (per-sample coefficients)."""
+    if isinstance(coef, (int, float)):
+        return float(coef)
+    if isinstance(coef, Tensor):
+        if coef.is_complex():
+            raise ValueError("term not supported by engine: complex coefficient")
+        flat = coef.detach().reshape(-1).to("cpu", torch.float64)
+        if flat.numel() == 1:
+            return float(flat[0])
+        if coef.dim() >= 1 and coef.shape[0] == batch and flat.numel() == batch:
+            return [float(v) for v in flat]
+        raise ValueError(f"term not supported by engine: coefficient of shape {tuple(coef.shape)}")
+    if isinstance(coef, complex):
+        raise ValueError("term not supported by engine: complex coefficient")
+    return float(coef)
+
+
+class OperatorLike(_MutableMixIn):
+    """Base of all operators (_base.py:284-805)."""
+
+    def __init__(self, operator_generators=None, coefs: Optional[List] = None) -> None:
+        super().__init__()
+        self.operator_generators = default(operator_generators, [])
+        if not isinstance(self.operator_generators, list):
+            self.operator_generators = [self.operator_generators]
+        self.coefs = default(coefs, [1] * len(self.operator_generators))
+        self._state_dict = {
+            "f_mesh": None,
+            "n_channel": None,
+            "linear_coef": None,
+            "nonlinear_func": None,
+            "operator": None,
+            "integrator": None,
+            "invert_linear_coef": None,
+            "engine": None,
+        }
+        self._nonlinear_funcs = []
+        self._linear_cores = []
+        self._de_aliasing_rate = 2 / 3
+        self._value_mesh_check_func = lambda dim_value, dim_mesh: True
+        self._integrator = "auto"
+        self._integrator_config = {}
+        self._is_etdrk_integrator = True
+
+    # ------------------------------------------------------------------ composition
+    @property
+    def is_linear(self) -> bool:
+        assert self._state_dict["f_mesh"] is not None, "Mesh should be registered before checking if the operator is linear"
+        return len(self._nonlinear_funcs) == 0 and len(self._linear_cores) > 0
+
+    def register_mesh(self, mesh, n_channel: int, device=None, dtype=None):
+        """Split generators into linear symbols and nonlinear terms (_base.py:581-624)."""
+        if isinstance(mesh, FourierMesh):
+            f_mesh = mesh
+            if device is not None or dtype is not None:
+                f_mesh.to(device=device, dtype=dtype)
+        else:
+            f_mesh = FourierMesh(mesh, device=device, dtype=dtype)
+        for key in self._state_dict:
+            self._state_dict[key] = None
+        self._state_dict.update({"f_mesh": f_mesh, "n_channel": n_channel, "engine": {}})
+        linear, nonlinear = [], []
+        for coef, generator in zip(self.coefs, self.operator_generators):
+            op = generator(f_mesh, n_channel)
+            if isinstance(op, LinearCoef):
+                linear.append((coef, op))
+            elif isinstance(op, NonlinearFunc):
+                nonlinear.append((coef, op))
+            else:
+                raise ValueError(f"Operator {op} is not supported")
+        self._linear_cores = linear
+        self._nonlinear_funcs = nonlinear
+        if nonlinear:
+            f_mesh.set_default_rel_freq_threshold(self._de_aliasing_rate)
+
+    def register_additional_check(self, func: Callable[[int, int], bool]):
+        self._value_mesh_check_func = func
+
+    def add_generator(self, generator, coef=1):
+        self.operator_generators.append(generator)
+        self.coefs.append(coef)
+
+    def set_integrator(self, integrator, **integrator_config):
+        """_base.py:646-674."""
+        if isinstance(integrator, str):
+            assert integrator == "auto", (
+                "The integrator should be 'auto' or an instance of ETDRKIntegrator, SETDRKIntegrator or RKIntegrator"
+            )
+        else:
+            assert isinstance(integrator, (ETDRKIntegrator, SETDRKIntegrator, RKIntegrator)), (
+                "The integrator should be 'auto' or an instance of ETDRKIntegrator, SETDRKIntegrator or RKIntegrator"
+            )
+        self._integrator = integrator
+        self._integrator_config = integrator_config
+        self._state_dict["integrator"] = None
+
+    # ------------------------------------------------------------------ engine compilation
+    def _engine_descriptor(self, batch):
+        lin = []
+        for coef, core in self._linear_cores:
+            kind = getattr(core, "_engine_kind", None)
+            if kind not in ("lap", "biharm", "unit"):
+                raise ValueError(f"term not supported by engine: linear core {type(core).__name__}")
+            lin.append((kind, _coef_value(coef, batch)))
+        prod, prod_coef, sources = None, 0.0, []
+        for coef, core in self._nonlinear_funcs:
+            kind = getattr(core, "_engine_kind", None)
+            if kind == "src":
+                c = _coef_value(coef, batch)
+                if isinstance(c, list):
+                    raise ValueError("term not supported by engine: per-sample coefficient on an explicit source")
+                sources.append((c, core._source_spatial))
+            elif kind in ("conv", "vc"):
+                if prod is not None:
+                    raise ValueError("term not supported by engine: more than one product term")
+                if not core._dealiasing_swtich:
+                    raise ValueError("term not supported by engine: product term without dealiasing")
+                c = _coef_value(coef, batch)
+                if isinstance(c, list):
+                    raise ValueError("term not supported by engine: per-sample coefficient on a product term")
+                prod, prod_coef = kind, c
+            else:
+                raise ValueError(f"term not supported by engine: nonlinear core {type(core).__name__}")
+        if sources and prod is None:
+            raise ValueError("term not supported by engine: explicit sources need a product term")
+        return lin, prod, prod_coef, sources
+
+    def _engine(self, batch: int, device) -> SpectralPlan:
+        cache = self._state_dict.get("engine")
+        if cache is None:
+            cache = {}
+            self._state_dict["engine"] = cache
+        key = (int(batch), str(device))
+        plan = cache.get(key)
+        if plan is None:
+            lin, prod, prod_coef, sources = self._engine_descriptor(batch)
+            f_mesh = self._state_dict["f_mesh"]
+            try:
+                plan = SpectralPlan(f_mesh.mesh_info, batch, self._state_dict["n_channel"], nonlinear=prod,
+                                    nonlinear_coef=prod_coef, linear=lin, device=device,
+                                    dealias_rate=self._de_aliasing_rate)
+                for c, src in sources:
+                    plan.add_source(src, c)
+            except torch.cuda.OutOfMemoryError as e:
+                raise RuntimeError(os.linesep.join([
+                    "Cuda out of memory when building the integrator.",
+                    "Original error message: {}".format(str(e)),
+                    "Please try to use a smaller mesh or a low-order integrator.",
+                ]))
+            cache[key] = plan
+        return plan
+
+    def _scheme(self):
+        """_build_integrator (_base.py:441-526): 'auto' = ETDRK0 if linear else SETDRK4."""
+        solver = self._integrator
+        if solver == "auto":
+            solver = ETDRKIntegrator.ETDRK0 if self.is_linear else SETDRKIntegrator.SETDRK4
+        if isinstance(solver, RKIntegrator):
+            raise ValueError("term not supported by engine: explicit Runge-Kutta integrators are not ETD schemes")
+        self._is_etdrk_integrator = True
+        if solver == ETDRKIntegrator.ETDRK0:
+            assert self.is_linear, "The ETDRK0 integrator only supports linear term"
+        cfg = dict(self._integrator_config)
+        n_points = int(cfg.pop("n_integration_points", 16))
+        radius = float(cfg.pop("integration_radius", 1.0))
+        cfg.pop("cpu_cached", None)
+        if cfg:
+            raise ValueError(f"unknown integrator options {sorted(cfg)}")
+        return solver.value, n_points, radius
+
+    def _prepare(self, batch, device, dt):
+        plan = self._engine(batch, device)
+        name, n_points, radius = self._scheme()
+        want = (name, float(dt), n_points, radius)
+        if getattr(plan, "_scheme_key", None) != want:
+            plan.set_scheme(name, dt, n_points, radius)
+            plan._scheme_key = want
+        self._state_dict["integrator"] = plan
+        return plan
+
+    def _pre_check(self, u=None, u_fft=None, mesh=None) -> Tuple[FourierMesh, int]:
+        """_base.py:528-579."""
+        if u_fft is None and u is None:
+            raise ValueError("Either u or u_fft should be given")
+        if u_fft is not None and u is not None:
+            assert u.shape == u_fft.shape, "The shape of u and u_fft should be the same"
+        assert mesh is not None, "Mesh should be given"
+        value_device = u.device if u is not None else u_fft.device
+        value_dtype = u.dtype if u is not None else u_fft.dtype
+        if not isinstance(mesh, FourierMesh):
+            if not isinstance(mesh, MeshGrid):
+                mesh = FourierMesh(mesh, device=value_device, dtype=value_dtype)
+            else:
+                mesh = FourierMesh(mesh)
+        n_channel = u.shape[1] if u is not None else u_fft.shape[1]
+        value_shape = u.shape if u is not None else u_fft.shape
+        assert len(value_shape) == mesh.n_dim + 2, (
+            f"the value shape {value_shape} is not compatible with mesh dim {mesh.n_dim}"
+        )
+        for i in range(mesh.n_dim):
+            assert value_shape[i + 2] == mesh.mesh_info[i][2], (
+                f"Expect to have {mesh.mesh_info[i][2]} points in dim {i} but got {value_shape[i+2]}"
+            )
+        assert value_device == mesh.device, (
+            "The device of mesh {} and the device of value {} are not the same".format(mesh.device, value_device)
+        )
+        assert self._value_mesh_check_func(len(value_shape) - 2, mesh.n_dim), "Value and mesh do not match the requirement"
+        return mesh, n_channel
+
+    def _to_half(self, plan, u_0, u_0_fft):
+        if u_0_fft is None:
+            require_gpu_tensor(u_0, "u_0")
+            if u_0.dtype != torch.float32:
+                raise TypeError(f"the MI355X engine computes in float32; got u_0 of dtype {u_0.dtype}")
+            return plan.forward(u_0)
+        require_gpu_tensor(u_0_fft, "u_0_fft")
+        if u_0_fft.dtype not in (torch.complex64,):
+            raise TypeError(f"the MI355X engine computes in complex64; got u_0_fft of dtype {u_0_fft.dtype}")
+        return plan.full_to_half(u_0_fft)
+
+    # ------------------------------------------------------------------ integrate
+    def integrate(self, u_0=None, u_0_fft=None, dt: float = 1, step: int = 1, mesh=None, progressive: bool = False,
+                  trajectory_recorder: Optional[_TrajRecorder] = None, return_in_fourier: bool = False):
+        """OperatorLike.integrate (_base.py:676-751) on the engine."""
+        if self._state_dict["f_mesh"] is None or mesh is not None:
+            mesh, n_channel = self._pre_check(u=u_0, u_fft=u_0_fft, mesh=mesh)
+            self.register_mesh(mesh, n_channel)
+        else:
+            self._pre_check(u=u_0, u_fft=u_0_fft, mesh=self._state_dict["f_mesh"])
+        x = u_0 if u_0 is not None else u_0_fft
+        require_gpu_tensor(x, "u_0" if u_0 is not None else "u_0_fft")
+        step = int(step)
+        if trajectory_recorder is not None and step <= 0:
+            raise UnboundLocalError("cannot access local variable 'i' where it is not associated with a value")
+        plan = self._prepare(x.shape[0], x.device, dt)
+        try:
+            uh = self._to_half(plan, u_0, u_0_fft)
+            if trajectory_recorder is None:
+                self._run_steps(plan, uh, step, progressive)
+            elif isinstance(trajectory_recorder, AutoRecorder):
+                sched = [bool(trajectory_recorder.control_func(i)) for i in range(step + 1)]
+                slots, t = [], 0
+                for s in sched:
+                    slots.append(t if s else -1)
+                    t += int(s)
+                if t == 0:
+                    self._run_steps(plan, uh, step, progressive)
+                else:
+                    rec = torch.tensor(slots, dtype=torch.int32, device=plan.device)
+                    traj = torch.zeros((plan.batch, t) + tuple(plan.spec_shape[1:]), dtype=torch.complex64,
+                                       device=plan.device)
+                    plan.step(uh, step, rec, traj)
+                    trajectory_recorder._take_engine_frames(plan, traj)
+            else:
+                # generic recorder protocol: full complex frames handed to record() at recorded steps
+                sched = [bool(trajectory_recorder.control_func(i)) for i in range(step + 1)]
+                done = 0
+                for i in range(step + 1):
+                    if sched[i]:
+                        self._run_steps(plan, uh, i - done, False)
+                        done = i
+                        trajectory_recorder.record(i, plan.half_to_full(uh))
+                self._run_steps(plan, uh, step - done, False)
+        except torch.cuda.OutOfMemoryError as e:
+            raise RuntimeError(os.linesep.join([
+                "Cuda out of memory when integrating the operator.",
+                "Original error message: {}".format(str(e)),
+                "Please try to use a smaller mesh or a low-order integrator.",
+            ]))
+        if trajectory_recorder is not None:
+            trajectory_recorder.return_in_fourier = return_in_fourier
+            return trajectory_recorder.trajectory
+        if return_in_fourier:
+            return plan.half_to_full(uh)
+        return plan.inverse(uh)
+
+    @staticmethod
+    def _run_steps(plan, uh, step, progressive):
+        if step <= 0:
+            return
+        if not progressive:
+            plan.step(uh, step)
+            return
+        chunk = max(1, step // 100)
+        with tqdm(total=step, desc="Integrating") as bar:
+            done = 0
+            while done < step:
+                k = min(chunk, step - done)
+                plan.step(uh, k)
+                torch.cuda.current_stream(plan.device).synchronize()
+                done += k
+                bar.update(k)
+
+    def __call__(self, u=None, u_fft=None, mesh=None, return_in_fourier=False):
+        """L u^ + N(u^) (_base.py:753-790) on the engine."""
+        if self._state_dict["f_mesh"] is None or mesh is not None:
+            mesh, n_channel = self._pre_check(u, u_fft, mesh)
+            self.register_mesh(mesh, n_channel)
+        else:
+            self._pre_check(u=u, u_fft=u_fft, mesh=self._state_dict["f_mesh"])
+        if not self._linear_cores and not self._nonlinear_funcs:
+            raise ValueError("Both linear coefficient and nonlinear function are None. Cannot build operator.")
+        x = u if u is not None else u_fft
+        require_gpu_tensor(x, "u" if u is not None else "u_fft")
+        plan = self._engine(x.shape[0], x.device)
+        uh = self._to_half(plan, u, u_fft)
+        out = plan.apply(uh)
+        if return_in_fourier:
+            return plan.half_to_full(out)
+        return plan.inverse(out)
+
+    def to(self, device=None, dtype=None):
+        if self._state_dict is not None and self._state_dict["f_mesh"] is not None:
+            self._state_dict["f_mesh"].to(device=device, dtype=dtype)
+            self.register_mesh(self._state_dict["f_mesh"], self._state_dict["n_channel"])
+
+
+class _InverseSolveMixin:
+    """LinearOperator.solve (_base.py:209-262): x^ = b^ / L with 1/0 := 1."""
+
+    def solve(self, b=None, b_fft=None, mesh=None, n_channel=None, return_in_fourier=False):
+        if not (mesh is not None and n_channel is not None):
+            assert self._state_dict["f_mesh"] is not None, "Mesh and n_channel should be given when calling solve"
+        if not (mesh is None and n_channel is None):
+            mesh = self._state_dict["f_mesh"] if mesh is None else mesh
+            n_channel = self._state_dict["n_channel"] if n_channel is None else n_channel
+            self.register_mesh(mesh, n_channel)
+        x = b if b is not None else b_fft
+        require_gpu_tensor(x, "b" if b is not None else "b_fft")
+        plan = self._engine(x.shape[0], x.device)
+        uh = self._to_half(plan, b, b_fft)
+        out = plan.solve(uh)
+        if return_in_fourier:
+            return plan.half_to_full(out)
+        return plan.inverse(out)
+
+
+class _DeAliasMixin:
+    def set_de_aliasing_rate(self, de_aliasing_rate: float):
+        """_base.py:265-281."""
+        self._de_aliasing_rate = de_aliasing_rate
+        self._state_dict = {k: None for k in self._state_dict}
+
+
+class Operator(OperatorLike, _DeAliasMixin):
+    """Linear and nonlinear terms (_base.py:807-850)."""
+
+    def __init__(self, operator_generators=None, coefs: Optional[List] = None) -> None:
+        super().__init__(operator_generators, coefs)
+
+    def __add__(self, other):
+        if isinstance(other, OperatorLike):
+            return Operator(self.operator_generators + other.operator_generators, self.coefs + other.coefs)
+        if isinstance(other, Tensor):
+            return Operator(self.operator_generators + [lambda f_mesh, n_channel: _ExplicitSourceCore(other)],
+                            self.coefs + [1])
+        return NotImplemented
+
+    def __mul__(self, other):
+        if isinstance(other, OperatorLike):
+            return NotImplemented
+        return Operator(self.operator_generators, [coef * other for coef in self.coefs])
+
+    def __neg__(self):
+        return Operator(self.operator_generators, [-1 * coef for coef in self.coefs])
+
+
+class LinearOperator(OperatorLike, _InverseSolveMixin):
+    """Linear terms only (_base.py:853-919)."""
+
+    def __init__(self, linear_coef=None, coefs: Optional[List] = None) -> None:
+        if not isinstance(linear_coef, list):
+            linear_coef = [linear_coef]
+        super().__init__(
+            operator_generators=[
+                (c if not isinstance(c, LinearCoef) else (lambda f_mesh, n_channel, _c=c: _c)) for c in linear_coef
+            ],
+            coefs=coefs,
+        )
+
+    @property
+    def is_linear(self):
+        return True
+
+    def __add__(self, other):
+        if isinstance(other, LinearOperator):
+            return LinearOperator(self.operator_generators + other.operator_generators, self.coefs + other.coefs)
+        if isinstance(other, OperatorLike):
+            return Operator(self.operator_generators + other.operator_generators, self.coefs + other.coefs)
+        if isinstance(other, Tensor):
+            return Operator(self.operator_generators + [lambda f_mesh, n_channel: _ExplicitSourceCore(other)],
+                            self.coefs + [1])
+        return NotImplemented
+
+    def __mul__(self, other):
+        if isinstance(other, OperatorLike):
+            return NotImplemented
+        return LinearOperator(self.operator_generators, [coef * other for coef in self.coefs])
+
+    def __neg__(self):
+        return LinearOperator(self.operator_generators, [-1 * coef for coef in self.coefs])
+
+
+class NonlinearOperator(OperatorLike, _DeAliasMixin):
+    """Nonlinear terms only (_base.py:922-988)."""
+
+    def __init__(self, nonlinear_func=None, coefs: Optional[List] = None) -> None:
+        if not isinstance(nonlinear_func, list):
+            nonlinear_func = [nonlinear_func]
+        super().__init__(
+            operator_generators=[
+                (f if not isinstance(f, NonlinearFunc) else (lambda f_mesh, n_channel, _f=f: _f)) for f in nonlinear_func
+            ],
+            coefs=coefs,
+        )
+
+    @property
+    def is_linear(self):
+        return False
+
+    def __add__(self, other):
+        if isinstance(other, NonlinearOperator):
+            return NonlinearOperator(self.operator_generators + other.operator_generators, self.coefs + other.coefs)
+        if isinstance(other, OperatorLike):
+            return Operator(self.operator_generators + other.operator_generators, self.coefs + other.coefs)
+        if isinstance(other, Tensor):
+            return Operator(self.operator_generators + [lambda f_mesh, n_channel: _ExplicitSourceCore(other)],
+                            self.coefs + [1])
+        return NotImplemented
+
+    def __mul__(self, other):
+        if isinstance(other, OperatorLike):
+            return NotImplemented
+        return NonlinearOperator(self.operator_generators, [coef * other for coef in self.coefs])
+
+    def __neg__(self):
+        return NonlinearOperator(self.operator_generators, [-1 * coef for coef in self.coefs])
+
+
+class _ExplicitSourceCore(NonlinearFunc):
+    """Constant source spectrum (_base.py:994-1015); the engine transforms the spatial source on
+    the device when the operator is compiled."""
+
+    _engine_kind = "src"
+
+    def __init__(self, source: Tensor) -> None:
+        super().__init__(dealiasing_swtich=False)
+        self._source_spatial = source
+
+    def __call__(self, u_fft, f_mesh: FourierMesh, u=None):
+        src = self._source_spatial.to(f_mesh.device)
+        return f_mesh.fft(src)
+
+
+class ExplicitSource(NonlinearOperator):
+    """Explicit source term (_base.py:1018-1028)."""
+
+    def __init__(self, source: Tensor) -> None:
+        super().__init__(_ExplicitSourceCore(source))

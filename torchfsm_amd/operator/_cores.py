@@ -1,0 +1,138 @@
+"""Operator cores the engine compiles (generic/_laplacian.py, _biharmonic.py, _convection.py,
+_source.py, dedicated/_navier_stokes.py:18-71).
+
+Each core carries an ``_engine_kind`` that register_mesh maps onto the engine's term
+descriptor.  LinearCoef cores still return the reference's symbol tensors when called directly.
+NonlinearFunc cores evaluated directly (outside an operator) run through libtfsm with the mask
+switched off, as the reference's core call does not dealias by itself.
+"""
+
+from typing import Callable, Optional
+
+import torch
+
+from ..mesh import FourierMesh
+from ._base import CoreGenerator, LinearCoef, LinearOperator, NonlinearFunc, NonlinearOperator, Operator
+
+
+class _LaplacianCore(LinearCoef):
+    _engine_kind = "lap"
+
+    def __call__(self, f_mesh: FourierMesh, n_channel: int):
+        return torch.cat([f_mesh.laplacian()] * n_channel, dim=1)
+
+
+class Laplacian(LinearOperator):
+    """nabla^2 of each channel (_laplacian.py:17-26)."""
+
+    def __init__(self) -> None:
+        super().__init__(_LaplacianCore())
+
+
+class _BiharmonicCore(LinearCoef):
+    _engine_kind = "biharm"
+
+    def __call__(self, f_mesh: FourierMesh, n_channel: int):
+        return torch.cat([f_mesh.laplacian() * f_mesh.laplacian()] * n_channel, dim=1)
+
+
+class Biharmonic(LinearOperator):
+    """nabla^4 of each channel (_biharmonic.py:20-29)."""
+
+    def __init__(self) -> None:
+        super().__init__(_BiharmonicCore())
+
+
+class _ImplicitUnitSourceCore(LinearCoef):
+    _engine_kind = "unit"
+
+    def __call__(self, f_mesh: FourierMesh, n_channel: int):
+        return torch.ones_like(f_mesh.bf_x)
+
+
+class _ImplicitFuncSourceCore(NonlinearFunc):
+    """f(u) source with an arbitrary python function: not an engine term."""
+
+    def __init__(self, source_func: Callable[[torch.Tensor], torch.Tensor], non_linear: bool = True) -> None:
+        super().__init__(non_linear)
+        self.source_func = source_func
+
+    def __call__(self, u_fft, f_mesh: FourierMesh, u=None):
+        if u is None:
+            u = f_mesh.ifft(u_fft).real
+        return f_mesh.fft(self.source_func(u))
+
+
+class ImplicitSource(Operator):
+    """Implicit source (_source.py:44-74): unit form f(u) = u (a linear symbol), or a function."""
+
+    def __init__(self, source_func: Optional[Callable[[torch.Tensor], torch.Tensor]] = None,
+                 non_linear: bool = True) -> None:
+        if source_func is None:
+            generator = lambda f_mesh, n_channel: _ImplicitUnitSourceCore()  # noqa: E731
+        else:
+            generator = lambda f_mesh, n_channel: _ImplicitFuncSourceCore(source_func, non_linear)  # noqa: E731
+        super().__init__(generator)
+
+
+def _core_direct(kind, u_fft, f_mesh: FourierMesh):
+    """Evaluate a product core on its raw input (no dealiasing) through the engine."""
+    from ..engine import SpectralPlan, require_gpu_tensor
+
+    require_gpu_tensor(u_fft, "u_fft")
+    plan = SpectralPlan(f_mesh.mesh_info, u_fft.shape[0], u_fft.shape[1], nonlinear=kind, nonlinear_coef=1.0,
+                        device=u_fft.device, dealias_rate=2.0)
+    out = plan.apply(plan.full_to_half(u_fft))
+    return plan.half_to_full(out)
+
+
+class _ConvectionCore(NonlinearFunc):
+    """(u . grad) u_c (_convection.py:9-48)."""
+
+    _engine_kind = "conv"
+
+    def __init__(self):
+        super().__init__()
+
+    def __call__(self, u_fft, f_mesh: FourierMesh, u=None):
+        return _core_direct("conv", u_fft, f_mesh)
+
+
+class _ConvectionGenerator(CoreGenerator):
+    def __call__(self, f_mesh: FourierMesh, n_channel: int):
+        if f_mesh.n_dim != n_channel:
+            raise ValueError("convection operator only works for vector field with the same dimension as mesh")
+        return _ConvectionCore()
+
+
+class Convection(NonlinearOperator):
+    """u . grad u for a vector field with C == D (_convection.py:67-76)."""
+
+    def __init__(self) -> None:
+        super().__init__(_ConvectionGenerator())
+
+
+class _VorticityConvectionCore(NonlinearFunc):
+    """(u . grad) omega with u = (d_y psi, -d_x psi), psi = -lap^-1 omega (_navier_stokes.py:18-46)."""
+
+    _engine_kind = "vc"
+
+    def __init__(self) -> None:
+        super().__init__()
+
+    def __call__(self, u_fft, f_mesh: FourierMesh, u=None):
+        return _core_direct("vc", u_fft, f_mesh)
+
+
+class _VorticityConvectionGenerator(CoreGenerator):
+    def __call__(self, f_mesh: FourierMesh, n_channel: int) -> NonlinearFunc:
+        if f_mesh.n_dim != 2 or n_channel != 1:
+            raise ValueError("Only vorticity in 2Dmesh is supported")
+        return _VorticityConvectionCore()
+
+
+class VorticityConvection(NonlinearOperator):
+    """Vorticity advection in 2D (_navier_stokes.py:62-71)."""
+
+    def __init__(self) -> None:
+        super().__init__(_VorticityConvectionGenerator())

@@ -1,0 +1,25 @@
+"""Equation factories of the reference (pde.py:13-83) that the engine integrates."""
+
+from typing import Optional
+
+from .operator import Biharmonic, Convection, Laplacian, Operator, VorticityConvection
+
+
+def Burgers(nu) -> Operator:
+    """u_t = -u . grad u + nu lap u  (pde.py:13-25)."""
+    return nu * Laplacian() - Convection()
+
+
+def KuramotoSivashinsky() -> Operator:
+    """phi_t = -phi_xx - phi_xxxx - phi phi_x, 1D only (pde.py:27-39)."""
+    ks_eqn = -Laplacian() - Biharmonic() - Convection()
+    ks_eqn.register_additional_check(lambda dim_value, dim_mesh: dim_value == 1 and dim_mesh == 1)
+    return ks_eqn
+
+
+def NavierStokesVorticity(Re: float, force: Optional[Operator] = None) -> Operator:
+    """w_t + (u . grad) w = lap w / Re + curl f  (pde.py:66-83)."""
+    ns_vorticity = -VorticityConvection() + 1 / Re * Laplacian()
+    if force is not None:
+        ns_vorticity += force
+    return ns_vorticity
