@@ -82,7 +82,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--torch-steps", type=int, default=20)
     ap.add_argument("--no-baselines", action="store_true", help="engine leg only (used under the profiler)")
-    ap.add_argument("--tile", type=int, default=0, help="column tile (8/16); 0 = autotune")
+    ap.add_argument("--tile", type=int, default=0, help="column tile (8/16); 0 = autotune the launch shape")
+    ap.add_argument("--col-var", type=int, default=0)
+    ap.add_argument("--row-var", type=int, default=0)
+    ap.add_argument("--row-wg", type=int, default=256)
+    ap.add_argument("--sub-batch", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run 3 plain 256 MiB device copies (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X and this build has no multi-GPU path "
@@ -95,27 +101,25 @@ def main():
     from torchfsm_amd.workloads import ns_initial_vorticity
 
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("[bench] copy probe: 3 x 256 MiB device-to-device copies done")
     log(f"[bench] NS {N}^2 B={B} SETDRK4 dt={DT}: building input and plan")
     w0 = ns_initial_vorticity(N, B, seed=0)
     op, plan, uh, y = build_engine(dev, w0)
     if args.tile:
-        plan.set_tile(args.tile)
+        plan.set_tuning(args.tile, args.col_var, args.row_var, args.row_wg, args.sub_batch)
     else:
-        best = None
-        for tile in (16, 8):
-            plan.set_tile(tile)
-            u = uh.clone()
-            plan.step(u, 3)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            plan.step(u, 20)
-            torch.cuda.synchronize()
-            t = time.perf_counter() - t0
-            log(f"[bench] autotune column tile {tile}: {t / 20 * 1e3:.3f} ms/step")
-            if best is None or t < best[0]:
-                best = (t, tile)
-        plan.set_tile(best[1])
-    log(f"[bench] column tile {plan.tile}; warmup {args.warmup} steps")
+        from torchfsm_amd.autotune import autotune
+
+        autotune(plan, uh, steps=6, log=log)
+    log(f"[bench] launch shape (col tile, col var, row var, row wg, sub-batch) = {plan.tuning}; "
+        f"warmup {args.warmup} steps")
     plan.step(uh, args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -150,7 +154,9 @@ def main():
             "scheme": "SETDRK4",
             "dealias": "2/3",
             "forcing": "kolm_force(y): -0.1 w - 4 cos(4y)",
-            "column_tile": plan.tile,
+            "launch_shape": {"col_tile": plan.tuning[0], "col_radix_variant": plan.tuning[1],
+                             "row_radix_variant": plan.tuning[2], "row_workgroup": plan.tuning[3],
+                             "sub_batch": plan.tuning[4]},
             "parallelism": "single GPU, batch-parallel",
         },
     }

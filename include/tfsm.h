@@ -100,8 +100,14 @@ int64_t tfsm_plan_workspace_bytes(const tfsm_plan* plan);
 /* Bytes of scratch tfsm_inverse needs for nbatch blocks of [C][N...]. */
 int64_t tfsm_plan_inverse_scratch_bytes(const tfsm_plan* plan, int32_t nbatch);
 
-/* Column tile width of the fused 2D column kernel (8 or 16); set by the autotuner. */
-int tfsm_plan_set_tile(tfsm_plan* plan, int32_t col_tile);
+/* Launch shape of the fused 2D kernels (chosen per shape class by the autotuner): column tile
+ * width (8 or 16 columns), radix-plan variant of the column and row transforms (0 or 1: variant 1
+ * carries a transform with half as many threads), row-kernel workgroup size (64 or 256 threads),
+ * and the sub-batch of samples stepped together through all steps of a tfsm_step call (0 = the
+ * whole batch; smaller sub-batches keep their working set in the last-level cache).
+ * (No reference counterpart: launch configuration of the engine's kernels.) */
+int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int32_t row_var, int32_t row_wg,
+                         int32_t sub_batch);
 
 /* ETD coefficients on the device: SETDRK1-4 as float64 contour means of n_points points on a
  * circle of radius `radius`; ETDRK0-2 with the reference's complex64 closed forms.

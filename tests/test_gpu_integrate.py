@@ -169,3 +169,70 @@ def test_coefficients_golden(dev):
             for i, j in idx:
                 print(f"  [{i},{j}] L={Lr[i, j]!r} E_ref={Er[i, j]!r} E_got={Eg[i, j]!r} got={got[i, j]!r} ref={ref[i, j]!r}")
         assert mism <= ref.size // 1000
+
+
+def test_every_launch_shape_is_exact(dev):
+    """Every autotuner candidate (column tile, radix variants, row workgroup) gives the same
+    physics: NS 128^2 golden (100 steps) and 2-channel Burgers 64^2 against refpath."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import autotune, field, pde
+    from torchfsm_amd.workloads import fourier_modes_field
+
+    g = golden("g9_ns128.npz")
+    mesh = tfa.MeshGrid([(0, 2 * math.pi, 128)] * 2, device=dev)
+    y = mesh.bc_mesh_grid()[1]
+    op = pde.NavierStokesVorticity(1000, field.kolm_force(y))
+    op.register_mesh(tfa.FourierMesh(mesh), 1)
+    plan = op._prepare(1, dev, 1e-3)
+    u0 = torch.from_numpy(g["u0_f32"]).to(dev)
+    cands = autotune.candidates(plan)
+    assert len(cands) == 16
+    cands = cands + [(16, 0, 0, 256, 8)]
+    for cand in cands:
+        plan.set_tuning(*cand)
+        uh = plan.forward(u0)
+        plan.step(uh, 100)
+        out = plan.inverse(uh).cpu().numpy()
+        e = _rel(out, g["out_f32"])
+        print(f"ns128 tuning {cand}: rel-L2 {e:.3e}")
+        assert e <= TOL
+
+    u = fourier_modes_field(64, 2, 2, 2, seed=5)
+    ref = refpath.integrate(refpath.burgers_terms(0.01), [(0, 1, 64)] * 2, u, 0.005, 20, scheme="SETDRK4")
+    op = pde.Burgers(0.01)
+    op.register_mesh(tfa.FourierMesh([(0, 1, 64)] * 2, device=dev), 2)
+    plan = op._prepare(2, dev, 0.005)
+    for cand in autotune.candidates(plan):
+        plan.set_tuning(*cand)
+        uh = plan.forward(u.to(dev))
+        plan.step(uh, 20)
+        e = _rel(plan.inverse(uh).cpu().numpy(), ref.numpy())
+        print(f"burgers64 tuning {cand}: rel-L2 {e:.3e}")
+        assert e <= TOL
+
+
+def test_sub_batch_blocking_is_bit_identical(dev):
+    """Temporal blocking (sub-batches stepped through all steps in turn) changes the launch order
+    only: states and recorded frames are bit-identical to whole-batch stepping."""
+    import torchfsm_amd as tfa
+    from torchfsm_amd import field, pde
+    from torchfsm_amd.workloads import ns_initial_vorticity
+
+    mesh = tfa.MeshGrid([(0, 2 * math.pi, 64)] * 2, device=dev)
+    y = mesh.bc_mesh_grid()[1]
+    op = pde.NavierStokesVorticity(1000, field.kolm_force(y))
+    op.register_mesh(tfa.FourierMesh(mesh), 1)
+    plan = op._prepare(16, dev, 1e-3)
+    w0 = ns_initial_vorticity(64, 16, seed=3).to(dev)
+    outs = []
+    for sb in (0, 8, 24):
+        plan.set_tuning(16, 0, 0, 256, sb)
+        uh = plan.forward(w0)
+        rec = torch.tensor([0, -1, 1, -1, 2, -1, 3], dtype=torch.int32, device=dev)
+        traj = torch.zeros((16, 4) + tuple(plan.spec_shape[1:]), dtype=torch.complex64, device=dev)
+        plan.step(uh, 6, rec, traj)
+        outs.append((uh.clone(), traj.clone()))
+    for uh, traj in outs[1:]:
+        assert torch.equal(uh, outs[0][0])
+        assert torch.equal(traj, outs[0][1])

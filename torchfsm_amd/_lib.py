@@ -58,7 +58,7 @@ SIGNATURES = {
     "tfsm_plan_state_elems": (I64, [VP]),
     "tfsm_plan_workspace_bytes": (I64, [VP]),
     "tfsm_plan_inverse_scratch_bytes": (I64, [VP, I32]),
-    "tfsm_plan_set_tile": (ctypes.c_int, [VP, I32]),
+    "tfsm_plan_set_tuning": (ctypes.c_int, [VP, I32, I32, I32, I32, I32]),
     "tfsm_plan_set_scheme": (ctypes.c_int, [VP, I32, ctypes.c_double, I32, ctypes.c_double, VP]),
     "tfsm_plan_coefficients": (ctypes.c_int, [VP, I32, VP, VP]),
     "tfsm_plan_add_source": (ctypes.c_int, [VP, VP, I32, I32, ctypes.c_float, VP]),

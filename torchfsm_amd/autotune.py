@@ -1,0 +1,93 @@
+"""Autotuner for the launch shape of the fused 2D step kernels.
+
+The search space per shape class (grid, batch, channels, term): column tile width {8, 16} x
+column radix-plan variant {0, 1} x row radix-plan variant {0, 1} x row workgroup size {64, 256},
+then, for the best launch shape, the temporal-blocking sub-batch (samples stepped together).
+Candidates are timed on the plan's own buffers (HIP events around a few full steps on the
+current stream) and the fastest is kept for the process; results can be persisted as JSON.
+Choices are fixed per shape class, never per batch occupancy, so results do not depend on the mix.
+"""
+
+import itertools
+import json
+import os
+from typing import Dict, Optional, Tuple
+
+import torch
+
+_CACHE: Dict[str, Tuple[int, int, int, int]] = {}
+
+
+def shape_key(plan) -> str:
+    return f"{plan.ndim}d_{'x'.join(map(str, plan.n))}_b{plan.batch}_c{plan.channels}_{plan.nonlinear}_{plan.scheme}"
+
+
+def candidates(plan):
+    if plan.ndim != 2:
+        return [(16, 0, 0, 256, 0)]
+    out = []
+    for tile, cv, rv, wg in itertools.product((16, 8), (0, 1), (0, 1), (256, 64)):
+        try:
+            plan.set_tuning(tile, cv, rv, wg, 0)
+        except ValueError:
+            continue
+        out.append((tile, cv, rv, wg, 0))
+    return out
+
+
+def sub_batches(plan):
+    b = plan.batch
+    return [s for s in (0, 8, 16, 24, 32) if s == 0 or (s < b and s % 8 == 0)]
+
+
+def time_steps(plan, uh, steps=6, warmup=2) -> float:
+    """Milliseconds per step of plan.step on a scratch copy of uh."""
+    u = uh.clone()
+    plan.step(u, warmup)
+    stream = torch.cuda.current_stream(plan.device)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    plan.step(u, steps)
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def autotune(plan, uh, steps=6, log=None, cache_file: Optional[str] = None):
+    """Pick and apply the fastest launch shape for this plan; returns (tuning, {tuning: ms})."""
+    key = shape_key(plan)
+    if cache_file and os.path.exists(cache_file) and key not in _CACHE:
+        try:
+            with open(cache_file) as f:
+                _CACHE.update({k: tuple(v) for k, v in json.load(f).items()})
+        except (OSError, ValueError):
+            pass
+    if key in _CACHE:
+        plan.set_tuning(*_CACHE[key])
+        return _CACHE[key], {}
+    results = {}
+    for cand in candidates(plan):
+        plan.set_tuning(*cand)
+        results[cand] = time_steps(plan, uh, steps)
+        if log:
+            log(f"[autotune] {key} {cand}: {results[cand]:.4f} ms/step")
+    shape = min(results, key=results.get)
+    for sb in sub_batches(plan):
+        cand = shape[:4] + (sb,)
+        if cand in results:
+            continue
+        plan.set_tuning(*cand)
+        results[cand] = time_steps(plan, uh, steps)
+        if log:
+            log(f"[autotune] {key} {cand}: {results[cand]:.4f} ms/step")
+    best = min(results, key=results.get)
+    plan.set_tuning(*best)
+    _CACHE[key] = best
+    if cache_file:
+        try:
+            with open(cache_file, "w") as f:
+                json.dump({k: list(v) for k, v in _CACHE.items()}, f)
+        except OSError:
+            pass
+    return best, results

@@ -91,16 +91,21 @@ struct tfsm_plan {
   int kmax[3] = {0, 0, 0};
   int nlin = 0;
   int Bc = 1;
-  float2* tw[3] = {nullptr, nullptr, nullptr};
+  float2* tw[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};  // [axis][radix variant]
   float* kv[3] = {nullptr, nullptr, nullptr};
   size_t kv_bstride = 0;
   float* L = nullptr;
   float* coef = nullptr;
+  float4* cpk = nullptr;  // packed per-mode coefficients of the current scheme
   int scheme = -1;
   double dt = 0.0;
   float2* src = nullptr;
   int srcB = 0, srcC = 0;
-  int V = 16;
+  int V = 16;        // column tile of the fused 2D column kernel
+  int col_var = 0;   // radix-plan variant of the column transforms
+  int row_var = 0;   // radix-plan variant of the row transforms
+  int row_wg = 256;  // row-kernel workgroup size
+  int sub_batch = 0; // samples stepped together through all steps (0 = whole batch)
   int PF = 0, NF = 0;
   size_t off_A1 = 0, off_A2 = 0, off_P = 0, off_G = 0, ws_bytes = 0;
   std::vector<void*> allocs;
@@ -128,21 +133,56 @@ int dev_alloc(tfsm_plan* p, T** ptr, size_t count, const char* what) {
   return TFSM_OK;
 }
 
-int build_twiddles(tfsm_plan* p, int d) {
-  const int n = p->n[d];
-  for (int e = 0; e < d; ++e)
-    if (p->n[e] == n) {
-      p->tw[d] = p->tw[e];
-      return TFSM_OK;
-    }
-  std::vector<float2> h(n);
-  for (int m = 0; m < n; ++m) {
-    const double ang = -2.0 * M_PI * (double)m / (double)n;
-    h[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+// per-stage twiddle tables in the layout of Fft<N, VAR>::tw_offset: W^(r k), W = exp(-2 pi i/(Ns R))
+template <int N, int VAR>
+void twiddle_table_t(std::vector<float2>& h) {
+  using F = Fft<N, VAR>;
+  h.assign(F::tw_size(), make_float2(1.f, 0.f));
+  for (int s = 1; s < F::NST; ++s) {
+    const int R = F::radix(s), ns = F::ns_before(s), off = F::tw_offset(s);
+    for (int r = 0; r < R; ++r)
+      for (int k = 0; k < ns; ++k) {
+        const double ang = -2.0 * M_PI * (double)r * (double)k / ((double)ns * R);
+        h[off + r * ns + k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+      }
   }
-  int rc = dev_alloc(p, &p->tw[d], (size_t)n, "twiddle table");
-  if (rc) return rc;
-  HIP_TRY(hipMemcpy(p->tw[d], h.data(), n * sizeof(float2), hipMemcpyHostToDevice), "twiddle upload");
+}
+
+bool twiddle_table(int n, int var, std::vector<float2>& h) {
+  if (var == 0) {
+    switch (n) {
+      case 16: twiddle_table_t<16, 0>(h); return true;
+      case 32: twiddle_table_t<32, 0>(h); return true;
+      case 64: twiddle_table_t<64, 0>(h); return true;
+      case 128: twiddle_table_t<128, 0>(h); return true;
+      case 256: twiddle_table_t<256, 0>(h); return true;
+      case 512: twiddle_table_t<512, 0>(h); return true;
+      case 1024: twiddle_table_t<1024, 0>(h); return true;
+      case 2048: twiddle_table_t<2048, 0>(h); return true;
+      case 4096: twiddle_table_t<4096, 0>(h); return true;
+      default: return false;
+    }
+  }
+  switch (n) {
+    case 64: twiddle_table_t<64, 1>(h); return true;
+    case 128: twiddle_table_t<128, 1>(h); return true;
+    case 256: twiddle_table_t<256, 1>(h); return true;
+    case 512: twiddle_table_t<512, 1>(h); return true;
+    default: return false;
+  }
+}
+
+int build_twiddles(tfsm_plan* p, int d) {
+  for (int var = 0; var < 2; ++var) {
+    std::vector<float2> h;
+    if (!twiddle_table(p->n[d], var, h)) {
+      p->tw[d][var] = nullptr;
+      continue;
+    }
+    int rc = dev_alloc(p, &p->tw[d][var], h.size(), "twiddle table");
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(p->tw[d][var], h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice), "twiddle upload");
+  }
   return TFSM_OK;
 }
 
@@ -152,29 +192,25 @@ int check_stream_errors(const char* what) {
   return TFSM_OK;
 }
 
-// one nonlinear-evaluation pair of 2D kernels
-int run2d_eval(tfsm_plan* p, float2* U, char* ws, int stage, hipStream_t st) {
-  float2* A1 = (float2*)(ws + p->off_A1);
-  float2* A2 = (float2*)(ws + p->off_A2);
-  float2* Pp = (float2*)(ws + p->off_P);
-  float2* G = (float2*)(ws + p->off_G);
-  if (stage != ST_PRO) {
-    Row2Args r;
-    r.B = p->B;
-    r.N0 = p->n[0];
-    r.N1 = p->n[1];
-    r.H1 = p->H;
-    r.P1 = p->P;
-    r.PF = p->PF;
-    r.kmax1 = p->kmax[1];
-    r.NF = p->NF;
-    r.tw1 = p->tw[1];
-    r.ay = p->kv[1];
-    r.G = G;
-    r.P = Pp;
-    r.b0 = 0;
-    HIP_TRY(launch_row2d(p->term, r, st), "row kernel launch");
-  }
+Row2Args row_args(tfsm_plan* p, char* ws, int b0, int nb) {
+  Row2Args r;
+  r.B = nb;
+  r.N0 = p->n[0];
+  r.N1 = p->n[1];
+  r.H1 = p->H;
+  r.P1 = p->P;
+  r.PF = p->PF;
+  r.kmax1 = p->kmax[1];
+  r.NF = p->NF;
+  r.tw1 = p->tw[1][p->row_var];
+  r.ay = p->kv[1];
+  r.G = (const float2*)(ws + p->off_G);
+  r.P = (float2*)(ws + p->off_P);
+  r.b0 = b0;
+  return r;
+}
+
+Col2Args col_args(tfsm_plan* p, float2* U, char* ws) {
   Col2Args c;
   c.B = p->B;
   c.C = p->C;
@@ -188,22 +224,45 @@ int run2d_eval(tfsm_plan* p, float2* U, char* ws, int stage, hipStream_t st) {
   c.NF = p->NF;
   c.coef_t = p->coef_t;
   c.scale = (float)(1.0 / ((double)p->n[0] * p->n[1]));
-  c.tw0 = p->tw[0];
+  c.tw0 = p->tw[0][p->col_var];
   c.ax = p->kv[0];
   c.ay = p->kv[1];
-  c.P = Pp;
+  c.P = (const float2*)(ws + p->off_P);
   c.U = U;
-  c.A1 = A1;
-  c.A2 = A2;
+  c.A1 = (float2*)(ws + p->off_A1);
+  c.A2 = (float2*)(ws + p->off_A2);
   c.coef = p->coef;
   c.coef_bstride = p->coef_bstride();
+  c.cpk = p->cpk;
+  c.cpk_bstride = p->Bc > 1 ? 2 * p->modes : 0;
   c.src = p->src;
   c.src_bstride = p->src_bstride();
   c.src_cstride = p->src_cstride();
-  c.G = G;
+  c.G = (float2*)(ws + p->off_G);
   c.b0 = 0;
-  HIP_TRY(launch_col2d(p->term, c, p->B, stage, p->V, st), "column kernel launch");
+  c.nb = p->B;
+  return c;
+}
+
+int launch_row(tfsm_plan* p, char* ws, int b0, int nb, hipStream_t st) {
+  HIP_TRY(launch_row2d(p->term, row_args(p, ws, b0, nb), p->row_var, p->row_wg, st), "row kernel launch");
   return TFSM_OK;
+}
+
+int launch_col(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hipStream_t st) {
+  Col2Args c = col_args(p, U, ws);
+  c.b0 = b0;
+  HIP_TRY(launch_col2d(p->term, c, nb, stage, p->V, p->col_var, st), "column kernel launch");
+  return TFSM_OK;
+}
+
+// one nonlinear-evaluation pair of 2D kernels for samples [b0, b0 + nb)
+int run2d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hipStream_t st) {
+  if (stage != ST_PRO) {
+    int rc = launch_row(p, ws, b0, nb, st);
+    if (rc) return rc;
+  }
+  return launch_col(p, U, ws, stage, b0, nb, st);
 }
 
 Run1Args run1_args(tfsm_plan* p, float2* U, char* ws) {
@@ -216,7 +275,7 @@ Run1Args run1_args(tfsm_plan* p, float2* U, char* ws) {
   a.coef_t = p->coef_t;
   a.scale = (float)(1.0 / p->n[0]);
   a.scheme = p->scheme;
-  a.tw = p->tw[0];
+  a.tw = p->tw[0][0];
   a.ax = p->kv[0];
   a.ax_bstride = p->kv_bstride;
   a.U = U;
@@ -254,18 +313,27 @@ int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* tra
     return TFSM_OK;
   }
   if (p->D == 2) {
+    // Temporal blocking: a sub-batch of samples goes through all nsteps before the next one, so
+    // its state, auxiliary stages, products and fields stay resident in the last-level cache
+    // across the 2 * evals launches of every step.  Samples are independent: same results.
     const int nev = scheme_evals(p->scheme);
-    int rc = run2d_eval(p, U, (char*)ws, ST_PRO, st);
-    if (rc) return rc;
+    const int sb = (p->sub_batch > 0 && p->sub_batch < p->B) ? p->sub_batch : p->B;
     const size_t per = (size_t)p->C * p->modes;
-    for (int i = 0; i < nsteps; ++i) {
-      if (rec_slot) HIP_TRY(launch_record(U, (float2*)traj, rec_slot, i, p->B, per, T, st), "record launch");
-      for (int e = 0; e < nev; ++e) {
-        rc = run2d_eval(p, U, (char*)ws, scheme_stage(p->scheme, e), st);
-        if (rc) return rc;
+    for (int b0 = 0; b0 < p->B; b0 += sb) {
+      const int nb = (p->B - b0) < sb ? (p->B - b0) : sb;
+      float2* Ub = U + (size_t)b0 * per;
+      float2* Tb = traj ? (float2*)traj + (size_t)b0 * T * per : nullptr;
+      int rc = run2d_eval(p, U, (char*)ws, ST_PRO, b0, nb, st);
+      if (rc) return rc;
+      for (int i = 0; i < nsteps; ++i) {
+        if (rec_slot) HIP_TRY(launch_record(Ub, Tb, rec_slot, i, nb, per, T, st), "record launch");
+        for (int e = 0; e < nev; ++e) {
+          rc = run2d_eval(p, U, (char*)ws, scheme_stage(p->scheme, e), b0, nb, st);
+          if (rc) return rc;
+        }
       }
+      if (rec_slot) HIP_TRY(launch_record(Ub, Tb, rec_slot, nsteps, nb, per, T, st), "record launch");
     }
-    if (rec_slot) HIP_TRY(launch_record(U, (float2*)traj, rec_slot, nsteps, p->B, per, T, st), "record launch");
     return TFSM_OK;
   }
   return fail(TFSM_E_INVALID, "stepping is not available for this dimension");
@@ -367,6 +435,7 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   if (D == 1 && geom->length_per_sample) p->Bc = Lb;
   if (!rc) rc = dev_alloc(p, &p->L, (size_t)Lb * p->modes, "linear symbol");
   if (!rc) rc = dev_alloc(p, &p->coef, (size_t)Lb * NCOEF * p->modes, "coefficient table");
+  if (!rc) rc = dev_alloc(p, &p->cpk, (size_t)Lb * 2 * p->modes, "packed coefficient table");
   if (!rc) {
     hipError_t e = hipMemsetAsync(p->coef, 0, (size_t)Lb * NCOEF * p->modes * sizeof(float), st);
     if (e != hipSuccess) rc = hip_fail(e, "coefficient table clear");
@@ -446,11 +515,19 @@ int64_t tfsm_plan_inverse_scratch_bytes(const tfsm_plan* plan, int32_t nbatch) {
   return (int64_t)((size_t)nbatch * plan->C * plan->modes * sizeof(float2));
 }
 
-int tfsm_plan_set_tile(tfsm_plan* plan, int32_t col_tile) {
+int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int32_t row_var, int32_t row_wg,
+                         int32_t sub_batch) {
   if (!plan) return fail(TFSM_E_INVALID, "null plan");
-  if (plan->D == 2 && !col2d_supported(plan->n[0], col_tile)) return fail(TFSM_E_INVALID, "unsupported column tile");
-  if (plan->D == 2 && (plan->P % col_tile) != 0) return fail(TFSM_E_INVALID, "column tile must divide the pitch");
+  if (plan->D == 2) {
+    if (!col2d_supported(plan->n[0], col_tile, col_var)) return fail(TFSM_E_INVALID, "unsupported column tuning");
+    if ((plan->P % col_tile) != 0) return fail(TFSM_E_INVALID, "column tile must divide the pitch");
+    if (!row2d_supported(plan->n[1], row_var, row_wg)) return fail(TFSM_E_INVALID, "unsupported row tuning");
+  }
   plan->V = col_tile;
+  plan->col_var = col_var;
+  plan->row_var = row_var;
+  plan->row_wg = row_wg;
+  plan->sub_batch = sub_batch < 0 ? 0 : sub_batch;
   return TFSM_OK;
 }
 
@@ -473,7 +550,9 @@ int tfsm_plan_set_scheme(tfsm_plan* plan, int32_t scheme, double dt, int32_t n_p
   a.Bc = plan->Bc;
   a.L = plan->L;
   a.coef = plan->coef;
+  a.cpk = plan->cpk;
   HIP_TRY(launch_coefs(a, (hipStream_t)stream), "coefficient launch");
+  HIP_TRY(launch_pack_coefs(a, (hipStream_t)stream), "coefficient pack launch");
   plan->scheme = scheme;
   plan->dt = dt;
   return TFSM_OK;
@@ -499,9 +578,9 @@ int tfsm_forward(tfsm_plan* p, const float* u, void* uh, int32_t nbatch, void* s
   const int nl = p->n[p->D - 1];
   size_t rows = blocks;
   for (int d = 0; d < p->D - 1; ++d) rows *= p->n[d];
-  HIP_TRY(launch_rows_r2c(nl, u, (size_t)nl, out, (size_t)p->P, (int)rows, 1.0f, p->tw[p->D - 1], st), "r2c launch");
+  HIP_TRY(launch_rows_r2c(nl, u, (size_t)nl, out, (size_t)p->P, (int)rows, 1.0f, p->tw[p->D - 1][0], st), "r2c launch");
   if (p->D == 2)
-    HIP_TRY(launch_cols_c2c(p->n[0], -1, out, out, blocks, p->modes, (size_t)p->P, p->H, 1.0f, p->tw[0], st),
+    HIP_TRY(launch_cols_c2c(p->n[0], -1, out, out, blocks, p->modes, (size_t)p->P, p->H, 1.0f, p->tw[0][0], st),
             "column fft launch");
   return TFSM_OK;
 }
@@ -520,11 +599,11 @@ int tfsm_inverse(tfsm_plan* p, const void* uh, float* u, int32_t nbatch, void* s
   if (p->D == 2) {
     if (!scratch) return fail(TFSM_E_INVALID, "inverse transform needs scratch");
     float2* sc = (float2*)scratch;
-    HIP_TRY(launch_cols_c2c(p->n[0], +1, in, sc, blocks, p->modes, (size_t)p->P, p->H, 1.0f, p->tw[0], st),
+    HIP_TRY(launch_cols_c2c(p->n[0], +1, in, sc, blocks, p->modes, (size_t)p->P, p->H, 1.0f, p->tw[0][0], st),
             "column ifft launch");
     in = sc;
   }
-  HIP_TRY(launch_rows_c2r(nl, in, (size_t)p->P, u, (size_t)nl, (int)rows, scale, p->tw[p->D - 1], st), "c2r launch");
+  HIP_TRY(launch_rows_c2r(nl, in, (size_t)p->P, u, (size_t)nl, (int)rows, scale, p->tw[p->D - 1][0], st), "c2r launch");
   return TFSM_OK;
 }
 
@@ -540,11 +619,11 @@ int tfsm_c2c(tfsm_plan* p, const void* in, void* out, int32_t nbatch, int32_t di
   size_t rows = blocks;
   for (int d = 0; d < p->D - 1; ++d) rows *= p->n[d];
   HIP_TRY(launch_rows_c2c(nl, dir, (const float2*)in, (size_t)nl, (float2*)out, (size_t)nl, (int)rows, scale,
-                          p->tw[p->D - 1], st),
+                          p->tw[p->D - 1][0], st),
           "row c2c launch");
   if (p->D == 2)
     HIP_TRY(launch_cols_c2c(p->n[0], dir, (const float2*)out, (float2*)out, blocks, (size_t)p->n[0] * nl, (size_t)nl,
-                            nl, 1.0f, p->tw[0], st),
+                            nl, 1.0f, p->tw[0][0], st),
             "column c2c launch");
   return TFSM_OK;
 }
@@ -637,8 +716,8 @@ int tfsm_apply(tfsm_plan* p, const void* uh, void* out, void* ws, void* stream) 
       a.nsteps = 1;
       HIP_TRY(launch_run1d(a, st), "1D evaluation launch");
     } else {
-      int rc = run2d_eval(p, U, (char*)ws, ST_PRO, st);
-      if (!rc) rc = run2d_eval(p, U, (char*)ws, ST_EVAL, st);
+      int rc = run2d_eval(p, U, (char*)ws, ST_PRO, 0, p->B, st);
+      if (!rc) rc = run2d_eval(p, U, (char*)ws, ST_EVAL, 0, p->B, st);
       if (rc) return rc;
     }
   }
@@ -663,60 +742,11 @@ int tfsm_launch_pass(tfsm_plan* p, void* uh, void* ws, int32_t pass_id, int32_t 
   hipStream_t st = (hipStream_t)stream;
   if (p->D == 2) {
     if (p->term == TERM_NONE) return fail(TFSM_E_INVALID, "no product term");
-    if (pass_id == 0) {
-      Row2Args r;
-      r.B = p->B;
-      r.N0 = p->n[0];
-      r.N1 = p->n[1];
-      r.H1 = p->H;
-      r.P1 = p->P;
-      r.PF = p->PF;
-      r.kmax1 = p->kmax[1];
-      r.NF = p->NF;
-      r.tw1 = p->tw[1];
-      r.ay = p->kv[1];
-      r.G = (const float2*)((char*)ws + p->off_G);
-      r.P = (float2*)((char*)ws + p->off_P);
-      r.b0 = 0;
-      HIP_TRY(launch_row2d(p->term, r, st), "row kernel launch");
-      return TFSM_OK;
-    }
+    if (pass_id == 0) return launch_row(p, (char*)ws, 0, p->B, st);
     if (pass_id == 1) {
       if (stage < ST_PRO || stage > ST_EVAL) return fail(TFSM_E_INVALID, "bad stage");
       if (stage != ST_PRO && p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
-      // the column kernel alone: run2d_eval without its row kernel is the ST_PRO path for stage 0;
-      // for other stages launch it directly on the products already in the workspace
-      const int saved = stage;
-      if (saved == ST_PRO) return run2d_eval(p, (float2*)uh, (char*)ws, ST_PRO, st);
-      Col2Args c;
-      c.B = p->B;
-      c.C = p->C;
-      c.N0 = p->n[0];
-      c.N1 = p->n[1];
-      c.H1 = p->H;
-      c.P1 = p->P;
-      c.PF = p->PF;
-      c.kmax0 = p->kmax[0];
-      c.kmax1 = p->kmax[1];
-      c.NF = p->NF;
-      c.coef_t = p->coef_t;
-      c.scale = (float)(1.0 / ((double)p->n[0] * p->n[1]));
-      c.tw0 = p->tw[0];
-      c.ax = p->kv[0];
-      c.ay = p->kv[1];
-      c.P = (const float2*)((char*)ws + p->off_P);
-      c.U = (float2*)uh;
-      c.A1 = (float2*)((char*)ws + p->off_A1);
-      c.A2 = (float2*)((char*)ws + p->off_A2);
-      c.coef = p->coef;
-      c.coef_bstride = p->coef_bstride();
-      c.src = p->src;
-      c.src_bstride = p->src_bstride();
-      c.src_cstride = p->src_cstride();
-      c.G = (float2*)((char*)ws + p->off_G);
-      c.b0 = 0;
-      HIP_TRY(launch_col2d(p->term, c, p->B, stage, p->V, st), "column kernel launch");
-      return TFSM_OK;
+      return launch_col(p, (float2*)uh, (char*)ws, stage, 0, p->B, st);
     }
   }
   if (p->D == 1 && pass_id == 2) {

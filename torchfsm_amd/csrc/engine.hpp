@@ -61,9 +61,20 @@ __host__ __device__ inline int scheme_stage(int scheme, int e) {
 // ---------------------------------------------------------------- non-contracted FP32 arithmetic
 // The reference evaluates every product and sum as a separate float32 op (torch elementwise
 // kernels); the combine below keeps that rounding by forbidding FMA contraction.
-__device__ __forceinline__ float mulr(float a, float b) { return __fmul_rn(a, b); }
-__device__ __forceinline__ float addr(float a, float b) { return __fadd_rn(a, b); }
-__device__ __forceinline__ float subr(float a, float b) { return __fsub_rn(a, b); }
+// (HIP's __fmul_rn / __fadd_rn are plain operators and would still be fused into FMAs under the
+// default -ffp-contract=fast; the pragma removes the contract flag from these operations.)
+__device__ __forceinline__ float mulr(float a, float b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ float addr(float a, float b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+__device__ __forceinline__ float subr(float a, float b) {
+#pragma clang fp contract(off)
+  return a - b;
+}
 __device__ __forceinline__ float2 rscale(float c, float2 z) { return make_float2(mulr(c, z.x), mulr(c, z.y)); }
 __device__ __forceinline__ float2 radd(float2 a, float2 b) { return make_float2(addr(a.x, b.x), addr(a.y, b.y)); }
 __device__ __forceinline__ float2 rsub(float2 a, float2 b) { return make_float2(subr(a.x, b.x), subr(a.y, b.y)); }
@@ -147,6 +158,87 @@ __device__ __forceinline__ float2 stage_combine(int stage, float2 nval, float2* 
   }
 }
 
+// ---------------------------------------------------------------- packed coefficients
+// Per mode, the coefficients a scheme's stages need are packed into 8 floats (two float4 halves)
+// so that each stage reads them with one 16-byte load:
+//   ETDRK1/SETDRK1: [E, c1, -, - | -]          ETDRK2/SETDRK2: [E, c1, c2, - | -]
+//   SETDRK3:        [E2, c1, E, c2 | E, c3, c4, c5]
+//   SETDRK4:        [E2, c1, -, - | E, c4, c5, c6]
+__host__ __device__ inline int stage_coef_half(int stage) {
+  return (stage == ST_R3_E2 || stage == ST_R4_E3) ? 1 : 0;
+}
+__host__ __device__ inline bool stage_reads_a1(int stage) {
+  return stage == ST_R2_E1 || stage == ST_R3_E1 || stage == ST_R3_E2 || stage == ST_R4_E2 || stage == ST_R4_E3;
+}
+__host__ __device__ inline bool stage_reads_a2(int stage) {
+  return stage == ST_R3_E2 || stage == ST_R4_E2 || stage == ST_R4_E3;
+}
+// 0: the stage writes U, 1: A1, 2: A2
+__host__ __device__ inline int stage_target(int stage) {
+  switch (stage) {
+    case ST_R2_E0: case ST_R3_E0: case ST_R4_E0: case ST_EVAL: return 1;
+    case ST_R3_E1: case ST_R4_E1: case ST_R4_E2: return 2;
+    default: return 0;
+  }
+}
+
+// Stage combine on values already in registers (same arithmetic and rounding as stage_combine).
+// Returns s; *w is the value the stage stores to its target array (stage_target).
+__device__ __forceinline__ float2 stage_combine_v(int stage, float2 nval, float2 u, float2 a1, float2 a2, float4 c,
+                                                  float2* w) {
+  switch (stage) {
+    case ST_R1_E0: {  // c = [E, c1, ...]
+      float2 un = radd(rscale(c.x, u), rscale(c.y, nval));
+      *w = un;
+      return un;
+    }
+    case ST_R2_E0:  // [E, c1, c2]
+      *w = nval;
+      return radd(rscale(c.x, u), rscale(c.y, nval));
+    case ST_R2_E1: {
+      float2 a = radd(rscale(c.x, u), rscale(c.y, a1));
+      float2 un = radd(a, rscale(c.z, rsub(nval, a1)));
+      *w = un;
+      return un;
+    }
+    case ST_R3_E0:  // half 0: [E2, c1, E, c2]
+      *w = nval;
+      return radd(rscale(c.x, u), rscale(c.y, nval));
+    case ST_R3_E1: {
+      *w = nval;
+      float2 two_na = make_float2(2.0f * nval.x, 2.0f * nval.y);
+      return radd(rscale(c.z, u), rscale(c.w, rsub(two_na, a1)));
+    }
+    case ST_R3_E2: {  // half 1: [E, c3, c4, c5]
+      float2 un = radd(radd(radd(rscale(c.x, u), rscale(c.y, a1)), rscale(c.z, a2)), rscale(c.w, nval));
+      *w = un;
+      return un;
+    }
+    case ST_R4_E0:  // half 0: [E2, c1]
+    case ST_R4_E1:
+      *w = nval;
+      return radd(rscale(c.x, u), rscale(c.y, nval));
+    case ST_R4_E2: {
+      float2 a = radd(rscale(c.x, u), rscale(c.y, a1));
+      float2 two_nb = make_float2(2.0f * nval.x, 2.0f * nval.y);
+      *w = radd(a2, nval);
+      return radd(rscale(c.x, a), rscale(c.y, rsub(two_nb, a1)));
+    }
+    case ST_R4_E3: {  // half 1: [E, c4, c5, c6]
+      const float c5x2 = 2.0f * c.z;
+      float2 un = radd(radd(radd(rscale(c.x, u), rscale(c.y, a1)), rscale(c5x2, a2)), rscale(c.w, nval));
+      *w = un;
+      return un;
+    }
+    case ST_EVAL:
+      *w = nval;
+      return nval;
+    default:
+      *w = u;
+      return u;
+  }
+}
+
 // Nonlinear value of one mode: N = coef_t * F + S   (S = sum of coef_s * source spectra)
 __device__ __forceinline__ float2 nl_value(float coef_t, float2 f, const float2* __restrict__ src, size_t si) {
   float2 n = rscale(coef_t, f);
@@ -170,10 +262,13 @@ struct Col2Args {
   float2* A2;
   const float* coef;          // [Bc][NCOEF][N0*P1]
   size_t coef_bstride;        // 0 or NCOEF*N0*P1
+  const float4* cpk;          // packed coefficients [Bc][N0*P1][2]
+  size_t cpk_bstride;         // 0 or 2*N0*P1 (float4 units)
   const float2* src;          // [Bs][Cs][N0][P1] or null
   size_t src_bstride, src_cstride;
   float2* G;                  // fields [B][NF][N0][PF]
   int b0;                     // first sample handled by this launch
+  int nb;                     // samples handled by this launch (set by the launcher)
 };
 
 struct Row2Args {
@@ -225,6 +320,7 @@ struct LinArgs {
 };
 
 struct CoefArgs {
+  float4* cpk;             // packed coefficients [Bc][modes][2], written after the slot tables
   int scheme;
   double dt;
   float dt32, hdt32;       // fl32(dt), fl32(0.5 * dt)
@@ -244,10 +340,10 @@ struct ConvArgs {
 };
 
 // ---------------------------------------------------------------- host launchers (defined in *.hip)
-hipError_t launch_col2d(int term, const Col2Args& a, int nb, int stage, int v, hipStream_t st);
-hipError_t launch_row2d(int term, const Row2Args& a, hipStream_t st);
-bool col2d_supported(int n0, int v);
-bool row2d_supported(int n1);
+hipError_t launch_col2d(int term, const Col2Args& a, int nb, int stage, int v, int var, hipStream_t st);
+hipError_t launch_row2d(int term, const Row2Args& a, int var, int wgt, hipStream_t st);
+bool col2d_supported(int n0, int v, int var);
+bool row2d_supported(int n1, int var, int wgt);
 hipError_t launch_rows_r2c(int n, const float* in, size_t in_ld, float2* out, size_t out_ld, int nrows, float scale,
                            const float2* tw, hipStream_t st);
 hipError_t launch_rows_c2r(int n, const float2* in, size_t in_ld, float* out, size_t out_ld, int nrows, float scale,
@@ -265,6 +361,7 @@ hipError_t launch_record(const float2* U, float2* traj, const int* rec_slot, int
 hipError_t launch_run1d(const Run1Args& a, hipStream_t st);
 hipError_t launch_linear_symbol(const LinArgs& a, hipStream_t st);
 hipError_t launch_coefs(const CoefArgs& a, hipStream_t st);
+hipError_t launch_pack_coefs(const CoefArgs& a, hipStream_t st);
 hipError_t launch_accum_source(float2* S, const float2* X, float coef, size_t total, int first, hipStream_t st);
 hipError_t launch_apply_linear(const float2* U, const float2* Nv, const float* L, size_t l_bstride, size_t modes, int C,
                                size_t total, int have_lin, int have_nl, float2* out, hipStream_t st);

@@ -12,7 +12,7 @@
 //     whole 128-byte rows of a column tile (the HBM layout of the tile, no transpose needed);
 //   * G independent groups (contiguous rows), each with its own LDS region.
 //
-// Twiddles come from a table tw[m] = exp(-2 pi i m / N) (computed in double on the host, rounded
+// Twiddles come from per-stage tables (Fft<N>::tw_offset, computed in double on the host, rounded
 // once), staged in LDS at kernel start.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -115,29 +115,45 @@ __device__ __forceinline__ void dft(float2* x) {
 
 // ---------------------------------------------------------------- radix plans
 // FftPlan<N>: the radix sequence used for a transform of length N, RMAX elements per thread.
-template <int N>
+template <int N, int VAR = 0>
 struct FftPlan;
-template <> struct FftPlan<16> { static constexpr int RMAX = 4; static constexpr int NST = 2; static constexpr int R[4] = {4, 4, 0, 0}; };
-template <> struct FftPlan<32> { static constexpr int RMAX = 8; static constexpr int NST = 2; static constexpr int R[4] = {8, 4, 0, 0}; };
-template <> struct FftPlan<64> { static constexpr int RMAX = 8; static constexpr int NST = 2; static constexpr int R[4] = {8, 8, 0, 0}; };
-template <> struct FftPlan<128> { static constexpr int RMAX = 16; static constexpr int NST = 2; static constexpr int R[4] = {16, 8, 0, 0}; };
-template <> struct FftPlan<256> { static constexpr int RMAX = 16; static constexpr int NST = 2; static constexpr int R[4] = {16, 16, 0, 0}; };
-template <> struct FftPlan<512> { static constexpr int RMAX = 8; static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 8, 0}; };
-template <> struct FftPlan<1024> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 8, 8, 0}; };
-template <> struct FftPlan<2048> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 16, 8, 0}; };
-template <> struct FftPlan<4096> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 16, 16, 0}; };
+// VAR 0: default plans
+template <> struct FftPlan<16, 0> { static constexpr int RMAX = 4; static constexpr int NST = 2; static constexpr int R[4] = {4, 4, 0, 0}; };
+template <> struct FftPlan<32, 0> { static constexpr int RMAX = 8; static constexpr int NST = 2; static constexpr int R[4] = {8, 4, 0, 0}; };
+template <> struct FftPlan<64, 0> { static constexpr int RMAX = 8; static constexpr int NST = 2; static constexpr int R[4] = {8, 8, 0, 0}; };
+template <> struct FftPlan<128, 0> { static constexpr int RMAX = 16; static constexpr int NST = 2; static constexpr int R[4] = {16, 8, 0, 0}; };
+template <> struct FftPlan<256, 0> { static constexpr int RMAX = 16; static constexpr int NST = 2; static constexpr int R[4] = {16, 16, 0, 0}; };
+template <> struct FftPlan<512, 0> { static constexpr int RMAX = 8; static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 8, 0}; };
+template <> struct FftPlan<1024, 0> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 8, 8, 0}; };
+template <> struct FftPlan<2048, 0> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 16, 8, 0}; };
+template <> struct FftPlan<4096, 0> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 16, 16, 0}; };
+// VAR 1: twice the elements per thread (half the threads per transform), same radix sequence
+// where possible; lets a column tile of the same width run with half the workgroup size.
+template <> struct FftPlan<64, 1> { static constexpr int RMAX = 16; static constexpr int NST = 2; static constexpr int R[4] = {8, 8, 0, 0}; };
+template <> struct FftPlan<128, 1> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {8, 4, 4, 0}; };
+template <> struct FftPlan<256, 1> { static constexpr int RMAX = 8; static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 4, 0}; };
+template <> struct FftPlan<512, 1> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 8, 0}; };
 
-template <int N>
+template <int N, int VAR = 0>
 struct Fft {
-  static constexpr int RMAX = FftPlan<N>::RMAX;
+  static constexpr int RMAX = FftPlan<N, VAR>::RMAX;
   static constexpr int NT = N / RMAX;  // threads per transform
-  static constexpr int NST = FftPlan<N>::NST;
-  static constexpr int radix(int s) { return FftPlan<N>::R[s]; }
+  static constexpr int NST = FftPlan<N, VAR>::NST;
+  static constexpr int radix(int s) { return FftPlan<N, VAR>::R[s]; }
   static constexpr int ns_before(int s) {
     int ns = 1;
-    for (int i = 0; i < s; ++i) ns *= FftPlan<N>::R[i];
+    for (int i = 0; i < s; ++i) ns *= FftPlan<N, VAR>::R[i];
     return ns;
   }
+  // Twiddle table layout: stage s >= 1 holds W^(r k), r < R_s, k < Ns_s, at tw_offset(s) + r*Ns + k
+  // with W = exp(-2 pi i / (Ns R)).  Lanes of one stage read consecutive k (or broadcast), so the
+  // table reads are free of LDS bank conflicts.
+  static constexpr int tw_offset(int s) {
+    int off = 0;
+    for (int i = 1; i < s; ++i) off += radix(i) * ns_before(i);
+    return off;
+  }
+  static constexpr int tw_size() { return tw_offset(NST) > 0 ? tw_offset(NST) : 1; }
 };
 
 // LDS slot of position p of interleaved transform vi (V transforms interleaved).
@@ -152,10 +168,10 @@ struct LdsSize {
 };
 
 // One Stockham stage (radix R, Ns = product of earlier radices).  See header comment.
-template <int N, int S, int DIR, int V>
-__device__ __forceinline__ void fft_stage(float2 (&v)[Fft<N>::RMAX], float2* lds, const float2* tw, int j,
+template <int N, int S, int DIR, int V, int VAR = 0>
+__device__ __forceinline__ void fft_stage(float2 (&v)[(Fft<N, VAR>::RMAX)], float2* lds, const float2* tw, int j,
                                           int vi) {
-  using F = Fft<N>;
+  using F = Fft<N, VAR>;
   constexpr int RMAX = F::RMAX;
   constexpr int NT = F::NT;
   constexpr int R = F::radix(S);
@@ -163,7 +179,7 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[Fft<N>::RMAX], float2* lds
   constexpr int Q = RMAX / R;
   constexpr bool FIRST = (S == 0);
   constexpr bool LAST = (S == F::NST - 1);
-  constexpr int TWS = N / (NS * R);
+  constexpr int TWO = F::tw_offset(S);
   if constexpr (!FIRST) {
 #pragma unroll
     for (int m = 0; m < RMAX; ++m) v[m] = lds[lds_slot<V>(j + m * NT, vi)];
@@ -179,7 +195,7 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[Fft<N>::RMAX], float2* lds
       const int k = jb % NS;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        float2 w = tw[r * k * TWS];
+        float2 w = tw[TWO + r * NS + k];
         x[r] = (DIR < 0) ? cmul(x[r], w) : cmulc(x[r], w);
       }
     }
@@ -198,19 +214,20 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[Fft<N>::RMAX], float2* lds
 
 // Full transform: registers (positions j + m*NT) -> registers (same positions, natural order).
 // All threads of the workgroup must call it (it contains barriers).
-template <int N, int DIR, int V>
-__device__ __forceinline__ void fft_wg(float2 (&v)[Fft<N>::RMAX], float2* lds, const float2* tw, int j, int vi) {
-  using F = Fft<N>;
-  fft_stage<N, 0, DIR, V>(v, lds, tw, j, vi);
-  if constexpr (F::NST > 1) fft_stage<N, 1, DIR, V>(v, lds, tw, j, vi);
-  if constexpr (F::NST > 2) fft_stage<N, 2, DIR, V>(v, lds, tw, j, vi);
-  if constexpr (F::NST > 3) fft_stage<N, 3, DIR, V>(v, lds, tw, j, vi);
+template <int N, int DIR, int V, int VAR = 0>
+__device__ __forceinline__ void fft_wg(float2 (&v)[(Fft<N, VAR>::RMAX)], float2* lds, const float2* tw, int j,
+                                       int vi) {
+  using F = Fft<N, VAR>;
+  fft_stage<N, 0, DIR, V, VAR>(v, lds, tw, j, vi);
+  if constexpr (F::NST > 1) fft_stage<N, 1, DIR, V, VAR>(v, lds, tw, j, vi);
+  if constexpr (F::NST > 2) fft_stage<N, 2, DIR, V, VAR>(v, lds, tw, j, vi);
+  if constexpr (F::NST > 3) fft_stage<N, 3, DIR, V, VAR>(v, lds, tw, j, vi);
 }
 
 // Cooperative copy of the twiddle table into LDS (call before the first fft_wg, then barrier).
-template <int N>
+template <int N, int VAR = 0>
 __device__ __forceinline__ void load_twiddles(float2* tw_lds, const float2* tw_glb) {
-  for (int i = threadIdx.x; i < N; i += blockDim.x) tw_lds[i] = tw_glb[i];
+  for (int i = threadIdx.x; i < Fft<N, VAR>::tw_size(); i += blockDim.x) tw_lds[i] = tw_glb[i];
 }
 
 }  // namespace tfsm

@@ -182,6 +182,34 @@ __global__ void k_etdrk_coefs(CoefArgs a) {
   }
 }
 
+// Pack the slot tables into the per-mode layout read by the fused column kernel (see engine.hpp).
+__global__ void k_pack_coefs(CoefArgs a) {
+  const size_t total = a.modes * a.Bc;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t mode = e % a.modes;
+    const size_t b = e / a.modes;
+    const float* t = a.coef + b * (size_t)NCOEF * a.modes + mode;
+    auto S = [&](int slot) { return t[(size_t)slot * a.modes]; };
+    float4 h0 = make_float4(0.f, 0.f, 0.f, 0.f), h1 = h0;
+    switch (a.scheme) {
+      case ETDRK0: h0 = make_float4(S(CS_E), 0.f, 0.f, 0.f); break;
+      case ETDRK1: case SETDRK1: h0 = make_float4(S(CS_E), S(CS_C1), 0.f, 0.f); break;
+      case ETDRK2: case SETDRK2: h0 = make_float4(S(CS_E), S(CS_C1), S(CS_C2), 0.f); break;
+      case SETDRK3:
+        h0 = make_float4(S(CS_E2), S(CS_C1), S(CS_E), S(CS_C2));
+        h1 = make_float4(S(CS_E), S(CS_C3), S(CS_C4), S(CS_C5));
+        break;
+      case SETDRK4:
+        h0 = make_float4(S(CS_E2), S(CS_C1), 0.f, 0.f);
+        h1 = make_float4(S(CS_E), S(CS_C4), S(CS_C5), S(CS_C6));
+        break;
+      default: break;
+    }
+    a.cpk[e * 2] = h0;
+    a.cpk[e * 2 + 1] = h1;
+  }
+}
+
 // S <- S + coef * X  (first source: S = coef * X); X and S are half spectra of the same shape.
 __global__ void k_accum_source(float2* __restrict__ S, const float2* __restrict__ X, float coef, size_t total,
                                int first) {
@@ -227,6 +255,11 @@ hipError_t launch_coefs(const CoefArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_setdrk_coefs, dim3(grid_for(a.modes * a.Bc)), dim3(256), 0, st, a);
   else
     hipLaunchKernelGGL(k_etdrk_coefs, dim3(grid_for(a.modes * a.Bc)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_coefs(const CoefArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_pack_coefs, dim3(grid_for(a.modes * a.Bc)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

@@ -38,21 +38,38 @@ struct Term2<TERM_CONV> {
   static constexpr int NC = 2, NF = 4;
 };
 
-template <int N0, int V, int TERM>
-__global__ void __launch_bounds__(Fft<N0>::NT* V) k2_col(Col2Args a, int stage) {
-  using F = Fft<N0>;
+template <int N0, int V, int TERM, int VAR>
+__global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int stage) {
+  using F = Fft<N0, VAR>;
   constexpr int RMAX = F::RMAX, NT = F::NT;
   constexpr int NC = Term2<TERM>::NC, NF = Term2<TERM>::NF;
-  __shared__ float2 s_tw[N0];
+  __shared__ float2 s_tw[Fft<N0, VAR>::tw_size()];
   __shared__ float2 s_buf[LdsSize<N0, V>::slots];
-  load_twiddles<N0>(s_tw, a.tw0);
+  load_twiddles<N0, VAR>(s_tw, a.tw0);
   __syncthreads();
   const int vi = threadIdx.x % V, j = threadIdx.x / V;
-  const int b = a.b0 + blockIdx.y;
-  const int ky = blockIdx.x * V + vi;
+  // Block -> (tile, sample).  Workgroups are dispatched round-robin over the 8 XCDs (block v on
+  // XCD v % 8); with nb % 8 == 0 each XCD owns a group of nb/8 samples and walks the tiles in
+  // order, so a coefficient tile is fetched once into that XCD's L2 and reused by its samples, and
+  // the heavy tiles (those with kept columns, ky <= kmax1) are dispatched before the light ones.
+  const int ntiles = (a.H1 + V - 1) / V;
+  int tile, bl;
+  if ((a.nb & 7) == 0) {
+    const int x = blockIdx.x & 7, sidx = blockIdx.x >> 3, g = a.nb >> 3;
+    tile = sidx / g;
+    bl = x * g + sidx % g;
+  } else {
+    tile = blockIdx.x % ntiles;
+    bl = blockIdx.x / ntiles;
+  }
+  const int b = a.b0 + bl;
+  const int ky = tile * V + vi;
   const bool colv = ky < a.H1;
   const size_t plane = (size_t)N0 * a.P1;
-  const float* cf = a.coef + (size_t)b * a.coef_bstride;
+  const float4* cpk = a.cpk + (size_t)b * a.cpk_bstride;
+  const int chalf = stage_coef_half(stage);
+  const bool rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
+  const int tgt = stage_target(stage);
   float2 s[NC][RMAX];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -66,21 +83,31 @@ __global__ void __launch_bounds__(Fft<N0>::NT* V) k2_col(Col2Args a, int stage) 
 #pragma unroll
       for (int m = 0; m < RMAX; ++m)
         v[m] = colv ? a.P[base + (size_t)(j + m * NT) * a.P1 + ky] : make_float2(0.f, 0.f);
-      fft_wg<N0, -1, V>(v, s_buf, s_tw, j, vi);
+      fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
+      // combine: every store of this tile is deferred until all loads have been issued, so the
+      // loads need not wait behind stores to arrays the compiler cannot prove disjoint
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride : nullptr;
+      float2* dst = tgt == 0 ? a.U : (tgt == 1 ? a.A1 : a.A2);
+      float2 wv[RMAX];
 #pragma unroll
       for (int m = 0; m < RMAX; ++m) {
         const size_t ci = (size_t)(j + m * NT) * a.P1 + ky;
-        if (colv) {
-          float2 nv = nl_value(a.coef_t, v[m], src, ci);
-          s[c][m] = stage_combine(stage, nv, a.U, a.A1, a.A2, base + ci, cf, plane, ci, true);
-        } else {
-          s[c][m] = make_float2(0.f, 0.f);
-        }
+        const float2 z = make_float2(0.f, 0.f);
+        const float2 uu = colv ? a.U[base + ci] : z;
+        const float2 q1 = (colv && rd1) ? a.A1[base + ci] : z;
+        const float2 q2 = (colv && rd2) ? a.A2[base + ci] : z;
+        const float4 cc = colv ? cpk[ci * 2 + chalf] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float2 nv = rscale(a.coef_t, v[m]);
+        if (src && colv) nv = radd(nv, src[ci]);
+        s[c][m] = colv ? stage_combine_v(stage, nv, uu, q1, q2, cc, &wv[m]) : z;
+      }
+      if (colv) {
+#pragma unroll
+        for (int m = 0; m < RMAX; ++m) dst[base + (size_t)(j + m * NT) * a.P1 + ky] = wv[m];
       }
     }
   }
-  if (blockIdx.x * V > a.kmax1) return;  // uniform per workgroup: no kept column in this tile
+  if (tile * V > a.kmax1) return;  // uniform per workgroup: no kept column in this tile
   const float ay = colv ? a.ay[ky] : 0.f;
   (void)ay;
 #pragma unroll
@@ -113,7 +140,7 @@ __global__ void __launch_bounds__(Fft<N0>::NT* V) k2_col(Col2Args a, int stage) 
       }
       v[m] = val;
     }
-    fft_wg<N0, +1, V>(v, s_buf, s_tw, j, vi);
+    fft_wg<N0, +1, V, VAR>(v, s_buf, s_tw, j, vi);
     if (ky < a.PF) {
       float2* g = a.G + (((size_t)b * NF + f) * N0) * a.PF + ky;
 #pragma unroll
@@ -122,15 +149,15 @@ __global__ void __launch_bounds__(Fft<N0>::NT* V) k2_col(Col2Args a, int stage) 
   }
 }
 
-template <int N1, int ROWS, int TERM>
-__global__ void __launch_bounds__(Fft<N1>::NT* ROWS) k2_row(Row2Args a) {
-  using F = Fft<N1>;
+template <int N1, int ROWS, int TERM, int VAR>
+__global__ void __launch_bounds__((Fft<N1, VAR>::NT * ROWS)) k2_row(Row2Args a) {
+  using F = Fft<N1, VAR>;
   constexpr int RMAX = F::RMAX, NT = F::NT;
   constexpr int SL = LdsSize<N1, 1>::slots;
   constexpr int NF = Term2<TERM>::NF;
-  __shared__ float2 s_tw[N1];
+  __shared__ float2 s_tw[Fft<N1, VAR>::tw_size()];
   __shared__ float2 s_buf[ROWS * SL];
-  load_twiddles<N1>(s_tw, a.tw1);
+  load_twiddles<N1, VAR>(s_tw, a.tw1);
   __syncthreads();
   const int g = threadIdx.x / NT, j = threadIdx.x % NT;
   const int row = blockIdx.x * ROWS + g;
@@ -166,12 +193,12 @@ __global__ void __launch_bounds__(Fft<N1>::NT* ROWS) k2_row(Row2Args a) {
       Bv = make_float2(-ay * f3.y, ay * f3.x);
       z2[m] = pack_pair(A, Bv, cj, edge);
     }
-    fft_wg<N1, +1, 1>(z1, lds, s_tw, j, 0);
-    fft_wg<N1, +1, 1>(z2, lds, s_tw, j, 0);
+    fft_wg<N1, +1, 1, VAR>(z1, lds, s_tw, j, 0);
+    fft_wg<N1, +1, 1, VAR>(z2, lds, s_tw, j, 0);
     float2 v[RMAX];
 #pragma unroll
     for (int m = 0; m < RMAX; ++m) v[m] = make_float2(z1[m].x * z2[m].x + z1[m].y * z2[m].y, 0.f);
-    fft_wg<N1, -1, 1>(v, lds, s_tw, j, 0);
+    fft_wg<N1, -1, 1, VAR>(v, lds, s_tw, j, 0);
     if (rv) {
       float2* p = a.P + ((size_t)b * a.N0 + x) * a.P1;
 #pragma unroll
@@ -202,9 +229,9 @@ __global__ void __launch_bounds__(Fft<N1>::NT* ROWS) k2_row(Row2Args a) {
       z2[m] = pack_pair(make_float2(-f1.y, f1.x), make_float2(-ay * f0.y, ay * f0.x), cj, edge);  // d0u0, d1u0
       z3[m] = pack_pair(make_float2(-f3.y, f3.x), make_float2(-ay * f2.y, ay * f2.x), cj, edge);  // d0u1, d1u1
     }
-    fft_wg<N1, +1, 1>(z1, lds, s_tw, j, 0);
-    fft_wg<N1, +1, 1>(z2, lds, s_tw, j, 0);
-    fft_wg<N1, +1, 1>(z3, lds, s_tw, j, 0);
+    fft_wg<N1, +1, 1, VAR>(z1, lds, s_tw, j, 0);
+    fft_wg<N1, +1, 1, VAR>(z2, lds, s_tw, j, 0);
+    fft_wg<N1, +1, 1, VAR>(z3, lds, s_tw, j, 0);
     float2 v[RMAX];
 #pragma unroll
     for (int m = 0; m < RMAX; ++m) {
@@ -212,7 +239,7 @@ __global__ void __launch_bounds__(Fft<N1>::NT* ROWS) k2_row(Row2Args a) {
       const float p1 = z1[m].x * z3[m].x + z1[m].y * z3[m].y;
       v[m] = make_float2(p0, p1);
     }
-    fft_wg<N1, -1, 1>(v, lds, s_tw, j, 0);
+    fft_wg<N1, -1, 1, VAR>(v, lds, s_tw, j, 0);
     // unpack two real spectra: P0 = (Z[k] + conj Z[N-k]) / 2, P1 = (Z[k] - conj Z[N-k]) / (2i)
 #pragma unroll
     for (int m = 0; m < RMAX; ++m) lds[lds_slot<1>(j + m * NT, 0)] = v[m];
@@ -236,60 +263,73 @@ __global__ void __launch_bounds__(Fft<N1>::NT* ROWS) k2_row(Row2Args a) {
 }
 
 // ---------------------------------------------------------------- host launchers
-template <int N0, int V, int TERM>
+// Tuning knobs (chosen per shape class by the autotuner): column tile V in {8, 16}, radix-plan
+// variant of the column and row transforms (0: N/RMAX threads per transform; 1: half as many),
+// and the row-kernel workgroup size (64 or 256 threads).
+template <int N0, int V, int TERM, int VAR>
 static hipError_t launch_col_t(const Col2Args& a, int nb, int stage, hipStream_t st) {
   const int tiles = (a.H1 + V - 1) / V;
-  dim3 grid(tiles, nb);
-  hipLaunchKernelGGL((k2_col<N0, V, TERM>), grid, dim3(Fft<N0>::NT * V), 0, st, a, stage);
+  Col2Args b = a;
+  b.nb = nb;
+  hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR>), dim3(tiles * nb), dim3(Fft<N0, VAR>::NT * V), 0, st, b, stage);
   return hipGetLastError();
 }
 
-template <int N1, int TERM>
+template <int N1, int TERM, int VAR, int WGT>
 static hipError_t launch_row_t(const Row2Args& a, hipStream_t st) {
-  constexpr int NT = Fft<N1>::NT;
-  constexpr int ROWS = (NT >= 256) ? 1 : 256 / NT;
+  constexpr int NT = Fft<N1, VAR>::NT;
+  constexpr int ROWS = (NT >= WGT) ? 1 : WGT / NT;
   const int rows = a.B * a.N0;
-  dim3 grid((rows + ROWS - 1) / ROWS);
-  hipLaunchKernelGGL((k2_row<N1, ROWS, TERM>), grid, dim3(NT * ROWS), 0, st, a);
+  hipLaunchKernelGGL((k2_row<N1, ROWS, TERM, VAR>), dim3((rows + ROWS - 1) / ROWS), dim3(NT * ROWS), 0, st, a);
   return hipGetLastError();
 }
 
 template <int TERM>
-static hipError_t launch_col_n(const Col2Args& a, int nb, int stage, int v, hipStream_t st) {
-#define TFSM_COL(N, VV) \
-  if (a.N0 == N && v == VV) return launch_col_t<N, VV, TERM>(a, nb, stage, st);
-  TFSM_COL(16, 8) TFSM_COL(16, 16) TFSM_COL(32, 8) TFSM_COL(32, 16) TFSM_COL(64, 8) TFSM_COL(64, 16) TFSM_COL(128, 8) TFSM_COL(128, 16) TFSM_COL(256, 8) TFSM_COL(256, 16)
-  TFSM_COL(512, 8) TFSM_COL(512, 16)
+static hipError_t launch_col_n(const Col2Args& a, int nb, int stage, int v, int var, hipStream_t st) {
+#define TFSM_COL(N, VV, VR) \
+  if (a.N0 == N && v == VV && var == VR) return launch_col_t<N, VV, TERM, VR>(a, nb, stage, st);
+  TFSM_COL(16, 8, 0) TFSM_COL(16, 16, 0) TFSM_COL(32, 8, 0) TFSM_COL(32, 16, 0)
+  TFSM_COL(64, 8, 0) TFSM_COL(64, 16, 0) TFSM_COL(64, 8, 1) TFSM_COL(64, 16, 1)
+  TFSM_COL(128, 8, 0) TFSM_COL(128, 16, 0) TFSM_COL(128, 8, 1) TFSM_COL(128, 16, 1)
+  TFSM_COL(256, 8, 0) TFSM_COL(256, 16, 0) TFSM_COL(256, 8, 1) TFSM_COL(256, 16, 1)
+  TFSM_COL(512, 8, 0) TFSM_COL(512, 16, 0) TFSM_COL(512, 8, 1) TFSM_COL(512, 16, 1)
 #undef TFSM_COL
   return hipErrorInvalidValue;
 }
 
 template <int TERM>
-static hipError_t launch_row_n(const Row2Args& a, hipStream_t st) {
-  switch (a.N1) {
-    case 16: return launch_row_t<16, TERM>(a, st);
-    case 32: return launch_row_t<32, TERM>(a, st);
-    case 64: return launch_row_t<64, TERM>(a, st);
-    case 128: return launch_row_t<128, TERM>(a, st);
-    case 256: return launch_row_t<256, TERM>(a, st);
-    case 512: return launch_row_t<512, TERM>(a, st);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-hipError_t launch_col2d(int term, const Col2Args& a, int nb, int stage, int v, hipStream_t st) {
-  if (term == TERM_VC) return launch_col_n<TERM_VC>(a, nb, stage, v, st);
-  if (term == TERM_CONV) return launch_col_n<TERM_CONV>(a, nb, stage, v, st);
+static hipError_t launch_row_n(const Row2Args& a, int var, int wgt, hipStream_t st) {
+#define TFSM_ROW(N, VR, W) \
+  if (a.N1 == N && var == VR && wgt == W) return launch_row_t<N, TERM, VR, W>(a, st);
+  TFSM_ROW(16, 0, 256) TFSM_ROW(32, 0, 256)
+  TFSM_ROW(64, 0, 256) TFSM_ROW(64, 1, 256) TFSM_ROW(64, 0, 64) TFSM_ROW(64, 1, 64)
+  TFSM_ROW(128, 0, 256) TFSM_ROW(128, 1, 256) TFSM_ROW(128, 0, 64) TFSM_ROW(128, 1, 64)
+  TFSM_ROW(256, 0, 256) TFSM_ROW(256, 1, 256) TFSM_ROW(256, 0, 64) TFSM_ROW(256, 1, 64)
+  TFSM_ROW(512, 0, 256) TFSM_ROW(512, 1, 256) TFSM_ROW(512, 0, 64) TFSM_ROW(512, 1, 64)
+#undef TFSM_ROW
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_row2d(int term, const Row2Args& a, hipStream_t st) {
-  if (term == TERM_VC) return launch_row_n<TERM_VC>(a, st);
-  if (term == TERM_CONV) return launch_row_n<TERM_CONV>(a, st);
+hipError_t launch_col2d(int term, const Col2Args& a, int nb, int stage, int v, int var, hipStream_t st) {
+  if (term == TERM_VC) return launch_col_n<TERM_VC>(a, nb, stage, v, var, st);
+  if (term == TERM_CONV) return launch_col_n<TERM_CONV>(a, nb, stage, v, var, st);
   return hipErrorInvalidValue;
 }
 
-bool col2d_supported(int n0, int v) { return (n0 == 16 || n0 == 32 || n0 == 64 || n0 == 128 || n0 == 256 || n0 == 512) && (v == 8 || v == 16); }
-bool row2d_supported(int n1) { return n1 == 16 || n1 == 32 || n1 == 64 || n1 == 128 || n1 == 256 || n1 == 512; }
+hipError_t launch_row2d(int term, const Row2Args& a, int var, int wgt, hipStream_t st) {
+  if (term == TERM_VC) return launch_row_n<TERM_VC>(a, var, wgt, st);
+  if (term == TERM_CONV) return launch_row_n<TERM_CONV>(a, var, wgt, st);
+  return hipErrorInvalidValue;
+}
+
+bool col2d_supported(int n0, int v, int var) {
+  if (!(v == 8 || v == 16)) return false;
+  if (n0 == 16 || n0 == 32) return var == 0;
+  return (n0 == 64 || n0 == 128 || n0 == 256 || n0 == 512) && (var == 0 || var == 1);
+}
+bool row2d_supported(int n1, int var, int wgt) {
+  if (n1 == 16 || n1 == 32) return var == 0 && wgt == 256;
+  return (n1 == 64 || n1 == 128 || n1 == 256 || n1 == 512) && (var == 0 || var == 1) && (wgt == 64 || wgt == 256);
+}
 
 }  // namespace tfsm

@@ -13,7 +13,7 @@ __global__ void __launch_bounds__(Fft<N>::NT* ROWS)
                float scale, const float2* tw) {
   using F = Fft<N>;
   constexpr int RMAX = F::RMAX, NT = F::NT, SL = LdsSize<N, 1>::slots;
-  __shared__ float2 s_tw[N];
+  __shared__ float2 s_tw[Fft<N>::tw_size()];
   __shared__ float2 s_buf[ROWS * SL];
   load_twiddles<N>(s_tw, tw);
   __syncthreads();
@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(Fft<N>::NT* ROWS)
                float scale, const float2* tw) {
   using F = Fft<N>;
   constexpr int RMAX = F::RMAX, NT = F::NT, SL = LdsSize<N, 1>::slots;
-  __shared__ float2 s_tw[N];
+  __shared__ float2 s_tw[Fft<N>::tw_size()];
   __shared__ float2 s_buf[ROWS * SL];
   load_twiddles<N>(s_tw, tw);
   __syncthreads();
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(Fft<N>::NT* ROWS)
                const float2* tw) {
   using F = Fft<N>;
   constexpr int RMAX = F::RMAX, NT = F::NT, SL = LdsSize<N, 1>::slots;
-  __shared__ float2 s_tw[N];
+  __shared__ float2 s_tw[Fft<N>::tw_size()];
   __shared__ float2 s_buf[ROWS * SL];
   load_twiddles<N>(s_tw, tw);
   __syncthreads();
@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(Fft<N>::NT* V)
                const float2* tw) {
   using F = Fft<N>;
   constexpr int RMAX = F::RMAX, NT = F::NT;
-  __shared__ float2 s_tw[N];
+  __shared__ float2 s_tw[Fft<N>::tw_size()];
   __shared__ float2 s_buf[LdsSize<N, V>::slots];
   load_twiddles<N>(s_tw, tw);
   __syncthreads();
@@ -239,7 +239,7 @@ template <int N, int V>
 __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
   using F = Fft<N>;
   constexpr int RMAX = F::RMAX, NT = F::NT, H = N / 2 + 1;
-  __shared__ float2 s_tw[N];
+  __shared__ float2 s_tw[Fft<N>::tw_size()];
   __shared__ float2 s_buf[LdsSize<N, V>::slots];
   __shared__ float2 s_stage[V * H];
   load_twiddles<N>(s_tw, a.tw);

@@ -122,7 +122,7 @@ class SpectralPlan:
         self._ws = None
         self.scheme = None
         self.dt = None
-        self.tile = 16
+        self.tuning = (16, 0, 0, 256, 0)
 
     # ------------------------------------------------------------------ lifecycle
     def close(self):
@@ -145,9 +145,12 @@ class SpectralPlan:
             self._ws = torch.zeros(max(self.ws_bytes, 256), dtype=torch.uint8, device=self.device)
         return self._ws
 
-    def set_tile(self, tile: int):
-        check(lib.tfsm_plan_set_tile(self._plan, int(tile)), "tile")
-        self.tile = int(tile)
+    def set_tuning(self, col_tile=16, col_var=0, row_var=0, row_wg=256, sub_batch=0):
+        """Launch shape of the fused 2D kernels and the temporal-blocking sub-batch (see
+        tfsm_plan_set_tuning)."""
+        check(lib.tfsm_plan_set_tuning(self._plan, int(col_tile), int(col_var), int(row_var), int(row_wg),
+                                       int(sub_batch)), "tuning")
+        self.tuning = (int(col_tile), int(col_var), int(row_var), int(row_wg), int(sub_batch))
 
     def set_scheme(self, scheme: str, dt: float, n_points: int = 16, radius: float = 1.0):
         with torch.cuda.device(self.device):
