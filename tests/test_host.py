@@ -1,0 +1,190 @@
+"""Host-side logic (operator algebra, engine descriptors, recorder schedules, error behaviour) and
+the C ABI boundary (library loads, exports every declared symbol, rejects bad plans).  CPU only:
+no compute call reaches the GPU."""
+
+import ctypes
+import math
+import os
+import re
+
+import pytest
+import torch
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    from torchfsm_amd import _lib
+
+    header = open(os.path.join(ROOT, "include", "tfsm.h")).read()
+    declared = set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(tfsm_\w+)\s*\(", header, re.M))
+    assert "tfsm_step" in declared and "tfsm_plan_create" in declared
+    assert declared == set(_lib.SIGNATURES), declared ^ set(_lib.SIGNATURES)
+    for name in declared:
+        assert hasattr(_lib.lib, name)
+    assert _lib.lib.tfsm_version() >= 1
+
+
+@pytest.mark.parametrize("ndim,n,batch,ch,term,err", [
+    (4, (16, 16, 16), 1, 1, 0, "1D and 2D"),
+    (2, (100, 100, 1), 1, 1, 0, "powers of two"),
+    (1, (16, 1, 1), 1, 1, 0, "32..4096"),
+    (2, (64, 64, 1), 1, 1, 1, "convection operator only works"),
+    (1, (64, 1, 1), 1, 1, 2, "Only vorticity in 2Dmesh"),
+    (2, (64, 64, 1), 0, 1, 0, "batch and channels"),
+])
+def test_plan_create_rejects_bad_configurations(ndim, n, batch, ch, term, err):
+    from torchfsm_amd import _lib
+
+    g = _lib.Geometry()
+    g.ndim = ndim
+    for d in range(3):
+        g.n[d] = n[d]
+        g.length[d] = 1.0
+    g.batch = batch
+    g.channels = ch
+    t = _lib.Terms()
+    t.nonlinear = term
+    t.dealias_rate = 2 / 3
+    t.n_linear = 0
+    plan = ctypes.c_void_p()
+    rc = _lib.lib.tfsm_plan_create(ctypes.byref(g), ctypes.byref(t), None, ctypes.byref(plan))
+    assert rc == _lib.TFSM_E_INVALID
+    assert err in _lib.lib.tfsm_last_error().decode()
+    with pytest.raises(ValueError, match=re.escape(err)):
+        _lib.check(rc)
+
+
+def test_operator_algebra_matches_reference_lists():
+    """Generator/coefficient lists as the reference builds them (_base.py:170-207, 826-988)."""
+    from torchfsm_amd import field, pde
+    from torchfsm_amd.operator import ExplicitSource, Laplacian, LinearOperator, NonlinearOperator, Operator
+
+    b = pde.Burgers(0.01)
+    assert isinstance(b, Operator) and b.coefs == [0.01, -1]
+    ks = pde.KuramotoSivashinsky()
+    assert ks.coefs == [-1, -1, -1]
+    y = torch.zeros(1, 1, 8, 8)
+    ns = pde.NavierStokesVorticity(1000, field.kolm_force(y))
+    assert ns.coefs == [-1, 0.001, -0.1, -1]
+    lap = Laplacian()
+    assert isinstance(2 * lap, LinearOperator) and (2 * lap).coefs == [2]
+    assert isinstance(-lap, LinearOperator) and (-lap).coefs == [-1]
+    assert isinstance(lap / 4, LinearOperator) and (lap / 4).coefs == [0.25]
+    assert isinstance(lap - lap, LinearOperator) and (lap - lap).coefs == [1, -1]
+    assert isinstance(ExplicitSource(y) * 3, NonlinearOperator)
+    s = lap + y
+    assert isinstance(s, Operator) and s.coefs == [1, 1]
+
+
+def test_register_mesh_compiles_engine_descriptor():
+    import torchfsm_amd as tfa
+    from torchfsm_amd import field, pde
+
+    y = torch.zeros(1, 1, 16, 16)
+    ns = pde.NavierStokesVorticity(1000, field.kolm_force(y))
+    ns.register_mesh(tfa.FourierMesh([(0, 2 * math.pi, 16)] * 2), 1)
+    lin, prod, pc, srcs = ns._engine_descriptor(4)
+    assert lin == [("lap", 0.001), ("unit", -0.1)]
+    assert prod == "vc" and pc == -1.0
+    assert len(srcs) == 1 and srcs[0][0] == -1.0
+    ks = pde.KuramotoSivashinsky()
+    ks.register_mesh(tfa.FourierMesh([(0, 32 * math.pi, 64)]), 1)
+    assert ks._engine_descriptor(1)[:3] == ([("lap", -1.0), ("biharm", -1.0)], "conv", -1.0)
+    # per-sample viscosity (batched parameters): a [B,1,1] tensor coefficient
+    nu = torch.tensor([0.01, 0.02, 0.03]).reshape(3, 1, 1)
+    b = pde.Burgers(nu)
+    b.register_mesh(tfa.FourierMesh([(0, 1, 32)]), 1)
+    lin, prod, pc, _ = b._engine_descriptor(3)
+    assert prod == "conv" and pc == -1.0
+    assert lin[0][0] == "lap" and lin[0][1] == pytest.approx([0.01, 0.02, 0.03])
+
+
+def test_generator_errors_as_reference():
+    import torchfsm_amd as tfa
+
+    with pytest.raises(ValueError, match="same dimension as mesh"):
+        tfa.Convection().register_mesh(tfa.FourierMesh([(0, 1, 16)] * 2), 1)
+    with pytest.raises(ValueError, match="Only vorticity in 2Dmesh"):
+        tfa.VorticityConvection().register_mesh(tfa.FourierMesh([(0, 1, 16)]), 1)
+
+
+def test_integrate_needs_gpu_tensors_and_float32():
+    import torchfsm_amd as tfa
+    from torchfsm_amd import pde
+
+    with pytest.raises(RuntimeError, match="GPU only"):
+        pde.Burgers(0.01).integrate(torch.zeros(1, 1, 64), mesh=[(0, 1, 64)], dt=0.01, step=1)
+    with pytest.raises(ValueError, match="Either u or u_fft"):
+        pde.Burgers(0.01).integrate(mesh=[(0, 1, 64)], dt=0.01, step=1)
+    with pytest.raises(AssertionError, match="Expect to have 64 points"):
+        pde.Burgers(0.01).integrate(torch.zeros(1, 1, 32), mesh=[(0, 1, 64)], dt=0.01, step=1)
+
+
+def test_set_integrator_validation():
+    from torchfsm_amd import pde
+    from torchfsm_amd.integrator import RKIntegrator, SETDRKIntegrator
+
+    op = pde.Burgers(0.01)
+    with pytest.raises(AssertionError):
+        op.set_integrator("rk4")
+    op.set_integrator(SETDRKIntegrator.SETDRK2, n_integration_points=32)
+    assert op._integrator_config == {"n_integration_points": 32}
+    op.set_integrator(RKIntegrator.RK4)
+    import torchfsm_amd as tfa
+
+    op.register_mesh(tfa.FourierMesh([(0, 1, 32)]), 1)
+    with pytest.raises(ValueError, match="Runge-Kutta"):
+        op._scheme()
+
+
+def test_auto_scheme_selection():
+    import torchfsm_amd as tfa
+    from torchfsm_amd import pde
+
+    op = pde.Burgers(0.01)
+    op.register_mesh(tfa.FourierMesh([(0, 1, 32)]), 1)
+    assert op._scheme()[0] == "SETDRK4"
+    lin = 0.1 * tfa.Laplacian()
+    lin.register_mesh(tfa.FourierMesh([(0, 1, 32)]), 1)
+    assert lin._scheme()[0] == "ETDRK0"
+
+
+def test_interval_controller_and_recorder_schedule():
+    from torchfsm_amd.traj_recorder import AutoRecorder, IntervalController
+
+    c = IntervalController(interval=3, start=2)
+    assert [s for s in range(12) if c(s)] == [2, 5, 8, 11]
+    r = AutoRecorder(IntervalController(2), include_initial_state=False)
+    assert [s for s in range(7) if r.control_func(s)] == [2, 4, 6]
+    r = AutoRecorder()
+    assert all(r.control_func(s) for s in range(5))
+
+
+def test_mesh_grid_and_fourier_mesh_symbols():
+    """MeshGrid coordinates (start ignored) and FourierMesh symbols as the reference forms them."""
+    import torchfsm_amd as tfa
+    from oracle import refpath as R
+
+    mg = tfa.MeshGrid([(1.0, 3.0, 8)])
+    assert torch.allclose(mg.x, 2.0 * torch.arange(8) / 8)
+    assert tfa.mesh.mesh_shape(mg, 2, 3) == (2, 3, 8)
+    fm = tfa.FourierMesh([(0, 2 * math.pi, 16)] * 2)
+    ref = R.Mesh([(0, 2 * math.pi, 16)] * 2)
+    assert torch.equal(fm.laplacian(), ref.lap())
+    assert torch.equal(fm.invert_laplacian(), ref.inv_lap())
+    assert torch.equal(fm.low_pass_filter(2 / 3), ref.mask(2 / 3))
+    assert torch.equal(fm.nabla_vector(1), ref.nabla_vector(1))
+
+
+def test_workloads_are_deterministic():
+    from torchfsm_amd.workloads import fourier_modes_field, ns_initial_vorticity
+
+    a = ns_initial_vorticity(32, 2, seed=4)
+    b = ns_initial_vorticity(32, 2, seed=4)
+    assert torch.equal(a, b)
+    assert a.dtype == torch.float32
+    rms = a.double().pow(2).mean(dim=(-1, -2)).sqrt()
+    assert torch.allclose(rms, torch.ones_like(rms), atol=1e-6)
+    assert abs(float(a.double().mean())) < 1e-6
+    assert torch.equal(fourier_modes_field(16, 2, 2, 1, seed=1), fourier_modes_field(16, 2, 2, 1, seed=1))

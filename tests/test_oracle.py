@@ -1,0 +1,227 @@
+"""The oracle, pinned: refpath against the reference's own golden vectors (tests/golden, written by
+importing the reference), numref against refpath in float64, and numref against closed forms,
+brute force, invariants and convergence orders.  CPU only."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import golden
+from oracle import numref
+from oracle import refpath as R
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def kolm_y(n, dtype):
+    return (2 * math.pi * torch.arange(0, n, dtype=dtype) / n).reshape(1, 1, 1, n).repeat(1, 1, n, 1)
+
+
+# ---------------------------------------------------------------- refpath == reference (golden)
+@pytest.mark.parametrize("name,dt", [("f32", torch.float32), ("f64", torch.float64)])
+def test_refpath_burgers1d_trajectory(name, dt):
+    g = golden("g1_burgers1d.npz")
+    u0 = torch.from_numpy(g["u0_" + name])
+    t = R.integrate(R.burgers_terms(0.01), [(0, 1, 128)], u0, 0.01, 200, record=lambda s: True)
+    assert t.shape == (1, 201, 1, 128)
+    assert rel(t.numpy(), g["traj_" + name]) < 1e-6
+
+
+@pytest.mark.parametrize("name,dt", [("f32", torch.float32), ("f64", torch.float64)])
+def test_refpath_ns64(name, dt):
+    g = golden("g2_ns64.npz")
+    o = R.integrate(R.ns_terms(1000, kolm_y(64, dt)), [(0, 2 * math.pi, 64)] * 2, torch.from_numpy(g["u0_" + name]),
+                    1e-3, 20, scheme="SETDRK4")
+    assert rel(o.numpy(), g["out_" + name]) < 1e-6
+
+
+def test_refpath_ns128_100_steps():
+    g = golden("g9_ns128.npz")
+    o = R.integrate(R.ns_terms(1000, kolm_y(128, torch.float32)), [(0, 2 * math.pi, 128)] * 2,
+                    torch.from_numpy(g["u0_f32"]), 1e-3, 100)
+    assert rel(o.numpy(), g["out_f32"]) < 1e-6
+
+
+@pytest.mark.parametrize("scheme", ["etdrk2", "setdrk2"])
+def test_refpath_burgers3d(scheme):
+    g = golden("g3_burgers3d.npz")
+    for name in ("f32", "f64"):
+        o = R.integrate(R.burgers_terms(0.01), [(0, 1, 16)] * 3, torch.from_numpy(g["u0_" + name]), 5e-3, 10,
+                        scheme=scheme.upper())
+        assert rel(o.numpy(), g[f"out_{scheme}_{name}"]) < 1e-6
+
+
+def test_refpath_burgers2d_autograd():
+    g = golden("g4_burgers2d_grad.npz")
+    for name, dt in (("f32", torch.float32), ("f64", torch.float64)):
+        nu = torch.tensor(0.01, dtype=dt, requires_grad=True)
+        u = torch.from_numpy(g["u0_" + name]).clone().requires_grad_(True)
+        o = R.integrate(R.burgers_terms(nu), [(0, 1, 32)] * 2, u, 0.01, 10, scheme="SETDRK2")
+        loss = ((o - torch.from_numpy(g["target_" + name])) ** 2).mean()
+        loss.backward()
+        assert rel(o.detach().numpy(), g["out_" + name]) < 1e-6
+        assert rel(u.grad.numpy(), g["grad_u0_" + name]) < 1e-6
+        assert abs(float(nu.grad) - float(g["grad_nu_" + name])) < 1e-5 * abs(float(g["grad_nu_" + name]))
+
+
+def test_refpath_coefficients():
+    g = golden("g5_coefs.npz")
+    mesh = R.Mesh([(0, 2 * math.pi, 64)] * 2)
+    terms = R.ns_terms(1000, kolm_y(64, torch.float32))
+    L = R.build_linear(mesh, terms, 1)
+    assert np.array_equal(L.numpy(), g["ns_L"])
+    sch = R.Scheme("SETDRK4", 1e-3, L, None)
+    for k, key in (("E", "ns_E"), ("E2", "ns_E2"), ("c1", "ns_c1"), ("c4", "ns_c4"), ("c5", "ns_c5"), ("c6", "ns_c6")):
+        assert rel(getattr(sch, k).numpy(), g[key]) < 1e-6
+    mesh = R.Mesh([(0, 1, 128)] * 2)
+    L = R.build_linear(mesh, R.burgers_terms(0.01), 2)
+    sch = R.Scheme("ETDRK2", 0.01, L, None)
+    assert np.array_equal(sch.c1.numpy(), g["b2_c1"]) and np.array_equal(sch.c2.numpy(), g["b2_c2"])
+
+
+def test_refpath_nonlinear_terms():
+    g = golden("g6_nonlinear.npz")
+    o = R.apply([(1, "vc", None)], [(0, 2 * math.pi, 64)] * 2, torch.from_numpy(g["vc_in"]), return_in_fourier=True)
+    assert rel(o.numpy(), g["vc_out"]) < 1e-6
+    for tag, n, d in (("c1", 64, 1), ("c2", 32, 2), ("c3", 16, 3)):
+        o = R.apply([(1, "conv", None)], [(0, 1, n)] * d, torch.from_numpy(g[tag + "_in"]), return_in_fourier=True)
+        assert rel(o.numpy(), g[tag + "_out"]) < 1e-6
+
+
+def test_refpath_ks_and_linear():
+    g = golden("g7_ks1d.npz")
+    o = R.integrate(R.ks_terms(), [(0, 32 * math.pi, 512)], torch.from_numpy(g["u0"]), 0.1, 10, scheme="SETDRK4")
+    assert rel(o.numpy(), g["out"]) < 1e-6
+    g = golden("g8_linear.npz")
+    o = R.integrate([(0.01, "lap", None)], [(0, 1, 32)] * 2, torch.from_numpy(g["u0"]), 0.05, 5)
+    assert rel(o.numpy(), g["out"]) < 1e-6
+
+
+# ---------------------------------------------------------------- numref == refpath in float64
+def test_numref_matches_refpath_f64():
+    g = golden("g1_burgers1d.npz")
+    t = numref.integrate([(0, 1, 128)], [(0.01, "lap", None), (-1, "conv", None)], g["u0_f64"], 0.01, 200,
+                         record_every=1)
+    assert rel(t, g["traj_f64"]) < 1e-12
+    g = golden("g2_ns64.npz")
+    y = np.broadcast_to(2 * np.pi * np.arange(64) / 64, (1, 1, 64, 64))
+    terms = [(-1, "vc", None), (1 / 1000, "lap", None), (-0.1, "unit", None), (-1, "src", 4 * np.cos(4 * y))]
+    o = numref.integrate([(0, 2 * np.pi, 64)] * 2, terms, g["u0_f64"], 1e-3, 20, "SETDRK4")
+    assert rel(o, g["out_f64"]) < 1e-12
+    g = golden("g3_burgers3d.npz")
+    for s in ("etdrk2", "setdrk2"):
+        o = numref.integrate([(0, 1, 16)] * 3, [(0.01, "lap", None), (-1, "conv", None)], g["u0_f64"], 5e-3, 10,
+                             s.upper())
+        assert rel(o, g[f"out_{s}_f64"]) < 1e-11
+
+
+# ---------------------------------------------------------------- closed forms, brute force, invariants
+def test_fft_brute_force():
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(32) + 1j * rng.standard_normal(32)
+    assert np.abs(numref.dft_brute(x) - np.fft.fft(x)).max() < 1e-12
+    # torch.fft (refpath's transform) against the brute-force DFT
+    assert np.abs(torch.fft.fft(torch.from_numpy(x)).numpy() - numref.dft_brute(x)).max() < 1e-12
+
+
+def test_phi_functions_closed_forms():
+    z = np.array([1e-8, -1e-3, 0.3 + 0.2j, -0.49, -0.51, 2.0, -30.0, 5j])
+    for order, ref in ((1, lambda z: (np.exp(z) - 1) / z), (2, lambda z: (np.exp(z) - 1 - z) / z**2),
+                       (3, lambda z: (np.exp(z) - 1 - z - z**2 / 2) / z**3)):
+        got = numref.phi(z, order)
+        big = np.abs(z) > 1e-2
+        assert np.allclose(got[big], ref(z[big]), rtol=1e-12, atol=0)
+        assert abs(got[0] - 1 / math.factorial(order)) < 1e-8
+    # the contour mean of the reference (16 points, radius 1) equals the phi forms in float64
+    L = torch.tensor([0.0, -1e-3, -0.5, -3.0, -40.0, 0.2], dtype=torch.float64)
+    sch = R.Scheme("SETDRK4", 1.0, L, None)
+    ex = numref.coefficients("SETDRK4", 1.0, L.numpy())
+    for k in ("c1", "c4", "c5", "c6"):
+        assert np.allclose(getattr(sch, k).numpy(), ex[k], rtol=1e-10, atol=1e-13)
+
+
+def test_dealias_mask_table():
+    for n, kept in ((64, 21), (128, 42), (256, 85), (512, 170), (1024, 341), (2048, 682), (4096, 1365)):
+        m = R.Mesh([(0, 2 * math.pi, n)]).mask(2 / 3).reshape(-1).numpy()
+        k = np.abs(np.fft.fftfreq(n, 1.0 / n))
+        assert (m == (k <= kept)).all(), n
+        assert np.array_equal(numref.Box([(0, 2 * math.pi, n)]).mask().reshape(-1), (k[: n // 2 + 1] <= kept))
+
+
+def test_vorticity_convection_single_mode_vanishes():
+    n = 32
+    x = np.arange(n) * 2 * np.pi / n
+    w = np.cos(3 * x)[:, None] * np.sin(2 * x)[None, :]
+    box = numref.Box([(0, 2 * np.pi, n)] * 2)
+    out = numref.vorticity_convection(box, box.rfft(w[None, None]))
+    assert np.abs(out).max() < 1e-10
+    o = R.apply([(1, "vc", None)], [(0, 2 * math.pi, n)] * 2, torch.from_numpy(w[None, None]))
+    assert o.abs().max() < 1e-10
+
+
+def test_convection_known_answer():
+    """(u.grad)u for u = (sin x, cos y): ((sin x cos x), (-cos y sin y))."""
+    n = 32
+    x = np.arange(n) * 2 * np.pi / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    u = np.stack([np.sin(X), np.cos(Y)])[None]
+    ref = np.stack([np.sin(X) * np.cos(X), -np.cos(Y) * np.sin(Y)])[None]
+    box = numref.Box([(0, 2 * np.pi, n)] * 2)
+    got = box.irfft(numref.convection(box, box.rfft(u)))
+    assert np.abs(got - ref).max() < 1e-12
+    got = R.apply([(1, "conv", None)], [(0, 2 * math.pi, n)] * 2, torch.from_numpy(u)).numpy()
+    assert np.abs(got - ref).max() < 1e-12
+
+
+def test_burgers_mean_conserved():
+    g = golden("g1_burgers1d.npz")
+    t = g["traj_f64"][0, :, 0, :]
+    assert np.abs(t.mean(axis=1) - t[0].mean()).max() < 1e-12
+
+
+def test_heat_equation_exact():
+    """ETDRK0 on nu*lap: every Fourier mode decays exactly as exp(-nu k^2 t)."""
+    n, nu, dt, steps = 32, 0.01, 0.05, 7
+    x = np.arange(n) / n
+    u0 = np.cos(2 * np.pi * 3 * x)[None, None]
+    o = numref.integrate([(0, 1, n)], [(nu, "lap", None)], u0, dt, steps)
+    ex = u0 * np.exp(-nu * (2 * np.pi * 3) ** 2 * dt * steps)
+    assert np.abs(o - ex).max() < 1e-12
+    o2 = R.integrate([(nu, "lap", None)], [(0, 1, n)], torch.from_numpy(u0), dt, steps).numpy()
+    assert np.abs(o2 - ex).max() < 1e-12
+
+
+def test_constant_forcing_exact_for_etd1():
+    """u' = L u + s with constant s: ETD1 is exact, u(t) = e^{Lt} u0 + t phi1(Lt) s."""
+    n, nu, dt = 16, 0.05, 0.1
+    x = np.arange(n) / n
+    src = np.cos(2 * np.pi * 2 * x)[None, None]
+    u0 = np.sin(2 * np.pi * x)[None, None]
+    terms = [(nu, "lap", None), (1.0, "src", src)]
+    o = numref.integrate([(0, 1, n)], terms, u0, dt, 5, "SETDRK1")
+    lam = -nu * (2 * np.pi * 2) ** 2
+    t = 5 * dt
+    ex = np.exp(-nu * (2 * np.pi) ** 2 * t) * u0 + t * (np.exp(lam * t) - 1) / (lam * t) * src
+    assert np.abs(o - ex).max() < 1e-12
+
+
+@pytest.mark.parametrize("scheme,order", [("SETDRK2", 2), ("SETDRK4", 4), ("SETDRK3", 3)])
+def test_convergence_order(scheme, order):
+    """Observed order of the stage formulas on 1D viscous Burgers (numref, float64)."""
+    n = 64
+    x = np.arange(n) / n
+    u0 = (np.sin(2 * np.pi * x) + 0.5)[None, None]
+    terms = [(0.02, "lap", None), (-1, "conv", None)]
+    T = 0.2
+    ref = numref.integrate([(0, 1, n)], terms, u0, T / 400, 400, "SETDRK4")
+    errs = []
+    for m in (10, 20, 40):
+        errs.append(np.abs(numref.integrate([(0, 1, n)], terms, u0, T / m, m, scheme) - ref).max())
+    rates = [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    assert abs(rates[-1] - order) < 0.5, rates
