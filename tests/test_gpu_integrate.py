@@ -110,6 +110,42 @@ def test_burgers2d_setdrk2_golden(dev):
     assert e32 <= TOL
 
 
+@pytest.mark.parametrize("scheme", ["ETDRK2", "SETDRK2"])
+def test_burgers3d_golden(dev, scheme):
+    """3D vector Burgers(0.01), 16^3 on [0,1)^3, C=3, dt 5e-3, 10 steps (reference fixture)."""
+    import torchfsm_amd as tfa
+    from torchfsm_amd import pde
+    from torchfsm_amd.integrator import ETDRKIntegrator, SETDRKIntegrator
+
+    g = golden("g3_burgers3d.npz")
+    mesh = tfa.MeshGrid([(0, 1, 16)] * 3, device=dev)
+    op = pde.Burgers(0.01)
+    op.set_integrator(ETDRKIntegrator.ETDRK2 if scheme == "ETDRK2" else SETDRKIntegrator.SETDRK2)
+    out = op.integrate(torch.from_numpy(g["u0_f32"]).to(dev), mesh=mesh, dt=5e-3, step=10).cpu().numpy()
+    tag = scheme.lower()
+    e32, e64 = _rel(out, g[f"out_{tag}_f32"]), _rel(out, g[f"out_{tag}_f64"])
+    print(f"burgers3d {scheme} rel-L2 vs ref f32 {e32:.3e} vs ref f64 {e64:.3e}")
+    assert e32 <= TOL
+
+
+@pytest.mark.parametrize("n,batch,steps", [((32, 32, 32), 2, 10), ((64, 32, 16), 2, 10), ((128, 128, 128), 1, 3)])
+def test_burgers3d_setdrk4_vs_oracle(dev, n, batch, steps):
+    """3D vector Burgers, SETDRK4 (the auto-selected scheme), against refpath on the same input."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import pde
+    from torchfsm_amd.workloads import fourier_modes_field
+
+    mesh_info = [(0, 1, n[0]), (0, 1, n[1]), (0, 1, n[2])]
+    u = fourier_modes_field(n, 3, 3, batch, seed=9)
+    dt = 2e-3
+    ref = refpath.integrate(refpath.burgers_terms(0.01), mesh_info, u, dt, steps, scheme="SETDRK4")
+    out = pde.Burgers(0.01).integrate(u.to(dev), mesh=tfa.MeshGrid(mesh_info, device=dev), dt=dt, step=steps)
+    e = _rel(out.cpu().numpy(), ref.numpy())
+    print(f"burgers3d {n} B={batch} SETDRK4 {steps} steps rel-L2 vs refpath {e:.3e}")
+    assert e <= TOL
+
+
 def test_nonlinear_terms_golden(dev):
     """One evaluation through Operator.__call__(return_in_fourier=True) (_base.py:753-790)."""
     import torchfsm_amd as tfa
@@ -122,7 +158,7 @@ def test_nonlinear_terms_golden(dev):
     e = float(np.linalg.norm(out - ref) / np.linalg.norm(ref))
     print(f"vc rel-L2 {e:.3e}")
     assert e <= TOL
-    for tag, n, d in (("c1", 64, 1), ("c2", 32, 2)):
+    for tag, n, d in (("c1", 64, 1), ("c2", 32, 2), ("c3", 16, 3)):
         conv = tfa.Convection()
         mesh = tfa.FourierMesh([(0, 1, n)] * d, device=dev)
         out = conv(u=torch.from_numpy(g[tag + "_in"]).to(dev), mesh=mesh, return_in_fourier=True).cpu().numpy()

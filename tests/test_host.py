@@ -26,7 +26,9 @@ def test_library_exports_every_declared_symbol():
 
 
 @pytest.mark.parametrize("ndim,n,batch,ch,term,err", [
-    (4, (16, 16, 16), 1, 1, 0, "1D and 2D"),
+    (4, (16, 16, 16), 1, 1, 0, "1D, 2D and 3D"),
+    (3, (256, 16, 16), 1, 3, 1, "16..128 points"),
+    (3, (16, 16, 16), 1, 2, 1, "convection operator only works"),
     (2, (100, 100, 1), 1, 1, 0, "powers of two"),
     (1, (16, 1, 1), 1, 1, 0, "32..4096"),
     (2, (64, 64, 1), 1, 1, 1, "convection operator only works"),

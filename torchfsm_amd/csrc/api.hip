@@ -91,7 +91,7 @@ struct tfsm_plan {
   int kmax[3] = {0, 0, 0};
   int nlin = 0;
   int Bc = 1;
-  float2* tw[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};  // [axis][radix variant]
+  float2* tw[3][3] = {};  // [axis][radix-plan variant]
   float* kv[3] = {nullptr, nullptr, nullptr};
   size_t kv_bstride = 0;
   float* L = nullptr;
@@ -107,7 +107,8 @@ struct tfsm_plan {
   int row_wg = 256;  // row-kernel workgroup size
   int sub_batch = 0; // samples stepped together through all steps (0 = whole batch)
   int PF = 0, NF = 0;
-  size_t off_A1 = 0, off_A2 = 0, off_P = 0, off_G = 0, ws_bytes = 0;
+  int K1 = 0;  // 3D: kept ky values (2 kmax1 + 1)
+  size_t off_A1 = 0, off_A2 = 0, off_P = 0, off_G = 0, off_G2 = 0, ws_bytes = 0;
   std::vector<void*> allocs;
 
   size_t state_elems() const { return (size_t)B * C * modes; }
@@ -163,6 +164,13 @@ bool twiddle_table(int n, int var, std::vector<float2>& h) {
       default: return false;
     }
   }
+  if (var == 2) {
+    if (n == 128) {
+      twiddle_table_t<128, 2>(h);
+      return true;
+    }
+    return false;
+  }
   switch (n) {
     case 64: twiddle_table_t<64, 1>(h); return true;
     case 128: twiddle_table_t<128, 1>(h); return true;
@@ -173,7 +181,7 @@ bool twiddle_table(int n, int var, std::vector<float2>& h) {
 }
 
 int build_twiddles(tfsm_plan* p, int d) {
-  for (int var = 0; var < 2; ++var) {
+  for (int var = 0; var < 3; ++var) {
     std::vector<float2> h;
     if (!twiddle_table(p->n[d], var, h)) {
       p->tw[d][var] = nullptr;
@@ -265,6 +273,86 @@ int run2d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hip
   return launch_col(p, U, ws, stage, b0, nb, st);
 }
 
+// ---------------------------------------------------------------- 3D evaluation
+Col3Args col3_args(tfsm_plan* p, float2* U, char* ws, int b0, int nb) {
+  Col3Args c;
+  c.B = p->B;
+  c.N0 = p->n[0];
+  c.N1 = p->n[1];
+  c.N2 = p->n[2];
+  c.H2 = p->H;
+  c.P2 = p->P;
+  c.PF2 = p->PF;
+  c.K1 = p->K1;
+  c.kmax0 = p->kmax[0];
+  c.kmax1 = p->kmax[1];
+  c.kmax2 = p->kmax[2];
+  c.coef_t = p->coef_t;
+  c.scale = (float)(1.0 / ((double)p->n[0] * p->n[1] * p->n[2]));
+  c.tw0 = p->tw[0][var3(p->n[0])];
+  c.ax = p->kv[0];
+  c.P = (const float2*)(ws + p->off_P);
+  c.U = U;
+  c.A1 = (float2*)(ws + p->off_A1);
+  c.A2 = (float2*)(ws + p->off_A2);
+  c.cpk = p->cpk;
+  c.cpk_bstride = p->Bc > 1 ? 2 * p->modes : 0;
+  c.src = p->src;
+  c.src_bstride = p->src_bstride();
+  c.src_cstride = p->src_cstride();
+  c.G1 = (float2*)(ws + p->off_G);
+  c.b0 = b0;
+  c.nb = nb;
+  return c;
+}
+
+// one nonlinear evaluation of the 3D convection term for samples [b0, b0 + nb):
+// z rows (products) -> y columns forward -> x columns (combine, fields) -> y columns inverse
+int run3d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hipStream_t st) {
+  const int N0 = p->n[0], N1 = p->n[1];
+  float2* Pp = (float2*)(ws + p->off_P);
+  if (stage != ST_PRO) {
+    Row3Args r;
+    r.N0 = N0;
+    r.N1 = N1;
+    r.N2 = p->n[2];
+    r.P2 = p->P;
+    r.PF2 = p->PF;
+    r.kmax2 = p->kmax[2];
+    r.tw2 = p->tw[2][var3(p->n[2])];
+    r.az = p->kv[2];
+    r.G2 = (const float2*)(ws + p->off_G2);
+    r.P = Pp;
+    r.b0 = b0;
+    r.nb = nb;
+    HIP_TRY(launch_row3d(r, st), "3D row kernel launch");
+    float2* Pb = Pp + (size_t)b0 * 3 * p->modes;
+    HIP_TRY(launch_cols_c2c(N1, -1, Pb, Pb, (size_t)nb * 3 * N0, (size_t)N1 * p->P, (size_t)p->P, p->H, 1.0f,
+                            p->tw[1][0], st),
+            "3D y-forward launch");
+  }
+  HIP_TRY(launch_col3d(col3_args(p, U, ws, b0, nb), nb, stage, st), "3D column kernel launch");
+  Yinv3Args y;
+  y.N0 = N0;
+  y.N1 = N1;
+  y.PF2 = p->PF;
+  y.K1 = p->K1;
+  y.kmax1 = p->kmax[1];
+  y.kmax2 = p->kmax[2];
+  y.tw1 = p->tw[1][var3(N1)];
+  y.ay = p->kv[1];
+  y.G1 = (const float2*)(ws + p->off_G);
+  y.G2 = (float2*)(ws + p->off_G2);
+  y.b0 = b0;
+  y.nb = nb;
+  HIP_TRY(launch_yinv3d(y, st), "3D y-inverse launch");
+  return TFSM_OK;
+}
+
+int run_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hipStream_t st) {
+  return p->D == 3 ? run3d_eval(p, U, ws, stage, b0, nb, st) : run2d_eval(p, U, ws, stage, b0, nb, st);
+}
+
 Run1Args run1_args(tfsm_plan* p, float2* U, char* ws) {
   Run1Args a;
   a.B = p->B * p->C;
@@ -312,7 +400,7 @@ int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* tra
     HIP_TRY(launch_run1d(a, st), "1D stepper launch");
     return TFSM_OK;
   }
-  if (p->D == 2) {
+  if (p->D >= 2) {
     // Temporal blocking: a sub-batch of samples goes through all nsteps before the next one, so
     // its state, auxiliary stages, products and fields stay resident in the last-level cache
     // across the 2 * evals launches of every step.  Samples are independent: same results.
@@ -323,12 +411,12 @@ int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* tra
       const int nb = (p->B - b0) < sb ? (p->B - b0) : sb;
       float2* Ub = U + (size_t)b0 * per;
       float2* Tb = traj ? (float2*)traj + (size_t)b0 * T * per : nullptr;
-      int rc = run2d_eval(p, U, (char*)ws, ST_PRO, b0, nb, st);
+      int rc = run_eval(p, U, (char*)ws, ST_PRO, b0, nb, st);
       if (rc) return rc;
       for (int i = 0; i < nsteps; ++i) {
         if (rec_slot) HIP_TRY(launch_record(Ub, Tb, rec_slot, i, nb, per, T, st), "record launch");
         for (int e = 0; e < nev; ++e) {
-          rc = run2d_eval(p, U, (char*)ws, scheme_stage(p->scheme, e), b0, nb, st);
+          rc = run_eval(p, U, (char*)ws, scheme_stage(p->scheme, e), b0, nb, st);
           if (rc) return rc;
         }
       }
@@ -351,7 +439,7 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   if (!geom || !terms || !out) return fail(TFSM_E_INVALID, "null argument");
   *out = nullptr;
   const int D = geom->ndim;
-  if (D < 1 || D > 2) return fail(TFSM_E_INVALID, "the engine supports 1D and 2D meshes");
+  if (D < 1 || D > 3) return fail(TFSM_E_INVALID, "the engine supports 1D, 2D and 3D meshes");
   if (geom->batch < 1 || geom->channels < 1) return fail(TFSM_E_INVALID, "batch and channels must be >= 1");
   for (int d = 0; d < D; ++d) {
     if (!is_pow2(geom->n[d])) return fail(TFSM_E_INVALID, "mesh sizes must be powers of two");
@@ -363,12 +451,18 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     for (int d = 0; d < 2; ++d)
       if (geom->n[d] < 16 || geom->n[d] > 512) return fail(TFSM_E_INVALID, "2D meshes must have 16..512 points per axis");
   }
+  if (D == 3) {
+    for (int d = 0; d < 3; ++d)
+      if (geom->n[d] < 16 || geom->n[d] > 128) return fail(TFSM_E_INVALID, "3D meshes must have 16..128 points per axis");
+  }
   const int term = terms->nonlinear;
   if (term != TERM_NONE && term != TERM_CONV && term != TERM_VC) return fail(TFSM_E_INVALID, "unknown nonlinear term");
   if (term == TERM_CONV && geom->channels != D)
     return fail(TFSM_E_INVALID, "convection operator only works for vector field with the same dimension as mesh");
   if (term == TERM_VC && (D != 2 || geom->channels != 1))
     return fail(TFSM_E_INVALID, "Only vorticity in 2Dmesh is supported");
+  if (term == TERM_CONV && D == 3 && !step3d_supported(geom->n[0], geom->n[1], geom->n[2]))
+    return fail(TFSM_E_INVALID, "3D convection needs 16, 32, 64 or 128 points per axis");
   if (terms->n_linear < 0 || terms->n_linear > 4) return fail(TFSM_E_INVALID, "at most 4 linear terms");
   if (terms->n_linear > 0 && !terms->linear_coef) return fail(TFSM_E_INVALID, "linear coefficients missing");
   const int Bc = terms->coef_batch < 1 ? 1 : terms->coef_batch;
@@ -478,7 +572,15 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     const size_t sb = p->state_elems() * sizeof(float2);
     p->off_A1 = 0;
     p->off_A2 = roundup(sb, 256);
-    if (D == 2) {
+    if (D == 3) {
+      // products P [B][3][N0][N1][P2]; fields G1 [B][6][N0][K1][PF2]; G2 [B][9][N0][N1][PF2]
+      p->PF = (int)roundup((size_t)p->kmax[2] + 1, 16);
+      p->K1 = 2 * p->kmax[1] + 1;
+      p->off_P = p->off_A2 + roundup(sb, 256);
+      p->off_G = p->off_P + roundup(sb, 256);
+      p->off_G2 = p->off_G + roundup((size_t)p->B * 6 * p->n[0] * p->K1 * p->PF * sizeof(float2), 256);
+      p->ws_bytes = p->off_G2 + roundup((size_t)p->B * 9 * p->n[0] * p->n[1] * p->PF * sizeof(float2), 256);
+    } else if (D == 2) {
       p->NF = 4;
       p->PF = (int)roundup((size_t)p->kmax[1] + 1, 16);
       p->off_P = p->off_A2 + roundup(sb, 256);
@@ -582,6 +684,13 @@ int tfsm_forward(tfsm_plan* p, const float* u, void* uh, int32_t nbatch, void* s
   if (p->D == 2)
     HIP_TRY(launch_cols_c2c(p->n[0], -1, out, out, blocks, p->modes, (size_t)p->P, p->H, 1.0f, p->tw[0][0], st),
             "column fft launch");
+  if (p->D == 3) {
+    const size_t plane = (size_t)p->n[1] * p->P;
+    HIP_TRY(launch_cols_c2c(p->n[1], -1, out, out, blocks * p->n[0], plane, (size_t)p->P, p->H, 1.0f, p->tw[1][0], st),
+            "y fft launch");
+    HIP_TRY(launch_cols_c2c(p->n[0], -1, out, out, blocks, p->modes, plane, (int)plane, 1.0f, p->tw[0][0], st),
+            "x fft launch");
+  }
   return TFSM_OK;
 }
 
@@ -601,6 +710,16 @@ int tfsm_inverse(tfsm_plan* p, const void* uh, float* u, int32_t nbatch, void* s
     float2* sc = (float2*)scratch;
     HIP_TRY(launch_cols_c2c(p->n[0], +1, in, sc, blocks, p->modes, (size_t)p->P, p->H, 1.0f, p->tw[0][0], st),
             "column ifft launch");
+    in = sc;
+  }
+  if (p->D == 3) {
+    if (!scratch) return fail(TFSM_E_INVALID, "inverse transform needs scratch");
+    float2* sc = (float2*)scratch;
+    const size_t plane = (size_t)p->n[1] * p->P;
+    HIP_TRY(launch_cols_c2c(p->n[0], +1, in, sc, blocks, p->modes, plane, (int)plane, 1.0f, p->tw[0][0], st),
+            "x ifft launch");
+    HIP_TRY(launch_cols_c2c(p->n[1], +1, sc, sc, blocks * p->n[0], plane, (size_t)p->P, p->H, 1.0f, p->tw[1][0], st),
+            "y ifft launch");
     in = sc;
   }
   HIP_TRY(launch_rows_c2r(nl, in, (size_t)p->P, u, (size_t)nl, (int)rows, scale, p->tw[p->D - 1][0], st), "c2r launch");
@@ -625,6 +744,15 @@ int tfsm_c2c(tfsm_plan* p, const void* in, void* out, int32_t nbatch, int32_t di
     HIP_TRY(launch_cols_c2c(p->n[0], dir, (const float2*)out, (float2*)out, blocks, (size_t)p->n[0] * nl, (size_t)nl,
                             nl, 1.0f, p->tw[0][0], st),
             "column c2c launch");
+  if (p->D == 3) {
+    const size_t plane = (size_t)p->n[1] * nl;
+    HIP_TRY(launch_cols_c2c(p->n[1], dir, (const float2*)out, (float2*)out, blocks * p->n[0], plane, (size_t)nl, nl,
+                            1.0f, p->tw[1][0], st),
+            "y c2c launch");
+    HIP_TRY(launch_cols_c2c(p->n[0], dir, (const float2*)out, (float2*)out, blocks, (size_t)p->n[0] * plane, plane,
+                            (int)plane, 1.0f, p->tw[0][0], st),
+            "x c2c launch");
+  }
   return TFSM_OK;
 }
 
@@ -716,8 +844,8 @@ int tfsm_apply(tfsm_plan* p, const void* uh, void* out, void* ws, void* stream) 
       a.nsteps = 1;
       HIP_TRY(launch_run1d(a, st), "1D evaluation launch");
     } else {
-      int rc = run2d_eval(p, U, (char*)ws, ST_PRO, 0, p->B, st);
-      if (!rc) rc = run2d_eval(p, U, (char*)ws, ST_EVAL, 0, p->B, st);
+      int rc = run_eval(p, U, (char*)ws, ST_PRO, 0, p->B, st);
+      if (!rc) rc = run_eval(p, U, (char*)ws, ST_EVAL, 0, p->B, st);
       if (rc) return rc;
     }
   }
@@ -748,6 +876,13 @@ int tfsm_launch_pass(tfsm_plan* p, void* uh, void* ws, int32_t pass_id, int32_t 
       if (stage != ST_PRO && p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
       return launch_col(p, (float2*)uh, (char*)ws, stage, 0, p->B, st);
     }
+  }
+  if (p->D == 3) {
+    if (p->term == TERM_NONE) return fail(TFSM_E_INVALID, "no product term");
+    if (stage < ST_PRO || stage > ST_EVAL) return fail(TFSM_E_INVALID, "bad stage");
+    if (stage != ST_PRO && p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
+    // one whole evaluation (4 kernels) per call; pass ids 0/1 are not split for 3D
+    if (pass_id == 3) return run3d_eval(p, (float2*)uh, (char*)ws, stage, 0, p->B, st);
   }
   if (p->D == 1 && pass_id == 2) {
     if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");

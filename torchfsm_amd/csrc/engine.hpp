@@ -280,6 +280,42 @@ struct Row2Args {
   int b0;
 };
 
+struct Col3Args {
+  int B, N0, N1, N2, H2, P2, PF2, K1;
+  int kmax0, kmax1, kmax2;
+  float coef_t, scale;
+  const float2* tw0;
+  const float* ax;
+  const float2* P;            // y-forward products [B][3][N0][N1][P2]
+  float2* U;
+  float2* A1;
+  float2* A2;
+  const float4* cpk;
+  size_t cpk_bstride;
+  const float2* src;
+  size_t src_bstride, src_cstride;
+  float2* G1;                 // [B][6][N0][K1][PF2]
+  int b0, nb;
+};
+
+struct Yinv3Args {
+  int N0, N1, PF2, K1, kmax1, kmax2;
+  const float2* tw1;
+  const float* ay;
+  const float2* G1;
+  float2* G2;                 // [B][9][N0][N1][PF2]
+  int b0, nb;
+};
+
+struct Row3Args {
+  int N0, N1, N2, P2, PF2, kmax2;
+  const float2* tw2;
+  const float* az;
+  const float2* G2;
+  float2* P;                  // products [B][3][N0][N1][P2]
+  int b0, nb;
+};
+
 struct Run1Args {
   int B, N, H, P, kmax;
   float coef_t, scale;
@@ -359,6 +395,12 @@ hipError_t launch_linear_steps(float2* U, const float* E, size_t coef_bstride, s
 hipError_t launch_record(const float2* U, float2* traj, const int* rec_slot, int step, int B, size_t per_sample,
                          int T, hipStream_t st);
 hipError_t launch_run1d(const Run1Args& a, hipStream_t st);
+hipError_t launch_col3d(const Col3Args& a, int nb, int stage, hipStream_t st);
+hipError_t launch_yinv3d(const Yinv3Args& a, hipStream_t st);
+hipError_t launch_row3d(const Row3Args& a, hipStream_t st);
+bool step3d_supported(int n0, int n1, int n2);
+// radix-plan variant used by the 3D kernels for an axis of length n
+inline int var3(int n) { return n == 128 ? 2 : 0; }
 hipError_t launch_linear_symbol(const LinArgs& a, hipStream_t st);
 hipError_t launch_coefs(const CoefArgs& a, hipStream_t st);
 hipError_t launch_pack_coefs(const CoefArgs& a, hipStream_t st);

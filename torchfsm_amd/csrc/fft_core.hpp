@@ -133,6 +133,8 @@ template <> struct FftPlan<64, 1> { static constexpr int RMAX = 16; static const
 template <> struct FftPlan<128, 1> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {8, 4, 4, 0}; };
 template <> struct FftPlan<256, 1> { static constexpr int RMAX = 8; static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 4, 0}; };
 template <> struct FftPlan<512, 1> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 8, 0}; };
+// VAR 2: 8 elements per thread where VAR 0 holds 16 (lower register use in the 3D kernels)
+template <> struct FftPlan<128, 2> { static constexpr int RMAX = 8; static constexpr int NST = 3; static constexpr int R[4] = {8, 4, 4, 0}; };
 
 template <int N, int VAR = 0>
 struct Fft {

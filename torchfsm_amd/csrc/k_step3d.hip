@@ -1,0 +1,337 @@
+// 3D fused ETD evaluation kernels for Convection with C = D = 3 (3D vector Burgers).
+//
+// One nonlinear evaluation = four passes over the half spectrum [B][3][N0][N1][P2]:
+//   k3_row   z rows: 12 C2R (packed in pairs) of the y-inverse fields -> the three products
+//            p_c = u_0 d_x u_c + u_1 d_y u_c + u_2 d_z u_c -> R2C (p0 + i p1 packed, p2 alone)
+//   cols_c2c y columns, forward, in place on the products (generic strided FFT pass)
+//   k3_col   x columns: forward FFT of the products -> N^ -> stage combine -> the dealiased fields
+//            u^_c and a_x u^_c of the next stage -> inverse x FFT (kept (ky, kz) only) -> G1
+//   k3_yinv  y columns: from G1 the nine fields u^_c, a_y u^_c, a_x u^_c -> inverse y FFT -> G2
+// Reference path replaced: _ConvectionCore (_convection.py:18-46) inside the closure
+// (_base.py:375-403) and the ETDRK2 / SETDRK2 combines (_etdrk.py:72-82, _setdrk_step.py:14-25).
+//
+// Field layouts (kz <= kmax2 only, pitch PF2; ky compressed to the K1 = 2 kmax1 + 1 kept values):
+//   G1 [B][6][N0][K1][PF2]   f = 2c: u^_c,  f = 2c + 1: a_x u^_c          (x inverse done)
+//   G2 [B][9][N0][N1][PF2]   f = 3c: u_c,   3c + 1: a_y u_c,  3c + 2: a_x u_c  (x, y inverse done)
+#include "engine.hpp"
+
+namespace tfsm {
+
+__device__ __forceinline__ float2 pack_pair3(float2 A, float2 Bv, bool cj, bool edge) {
+  if (edge) {
+    A.y = 0.f;
+    Bv.y = 0.f;
+  }
+  return cj ? make_float2(A.x + Bv.y, Bv.x - A.y) : make_float2(A.x - Bv.y, A.y + Bv.x);
+}
+
+// kept ky -> compressed index, or -1
+__device__ __forceinline__ int ky_compress(int ky, int N1, int kmax1) {
+  if (ky <= kmax1) return ky;
+  if (ky >= N1 - kmax1) return ky - (N1 - 2 * kmax1 - 1);
+  return -1;
+}
+
+// ---------------------------------------------------------------- x columns (fused combine)
+template <int N0, int V, int VAR>
+__global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int stage) {
+  using F = Fft<N0, VAR>;
+  constexpr int RMAX = F::RMAX, NT = F::NT;
+  __shared__ float2 s_tw[Fft<N0, VAR>::tw_size()];
+  __shared__ float2 s_buf[LdsSize<N0, V>::slots];
+  load_twiddles<N0, VAR>(s_tw, a.tw0);
+  __syncthreads();
+  const int vi = threadIdx.x % V, j = threadIdx.x / V;
+  // tiles (ky, kz-tile) go round-robin to the XCDs (block v on XCD v % 8); the nb samples of one
+  // tile run back to back on the same XCD, so each coefficient tile is fetched once per launch
+  const int nzt = a.P2 / V;
+  const int ntiles = a.N1 * nzt;
+  const int x8 = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int tile = x8 + 8 * (slot / a.nb);
+  const int bl = slot % a.nb;
+  if (tile >= ntiles) return;
+  const int b = a.b0 + bl;
+  const int ky = tile / nzt;
+  const int kz = (tile % nzt) * V + vi;
+  const bool colv = kz < a.H2;
+  const size_t inner = (size_t)a.N1 * a.P2;  // x stride
+  const size_t plane = (size_t)N0 * inner;   // modes
+  const size_t coff = (size_t)ky * a.P2 + kz;
+  const float4* cpk = a.cpk + (size_t)b * a.cpk_bstride;
+  const int chalf = stage_coef_half(stage);
+  const bool rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
+  const int tgt = stage_target(stage);
+  float2 s[3][RMAX];
+  if (stage == ST_PRO) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t base = ((size_t)b * 3 + c) * plane + coff;
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) s[c][m] = colv ? a.U[base + (size_t)(j + m * NT) * inner] : make_float2(0.f, 0.f);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t base = ((size_t)b * 3 + c) * plane + coff;
+      float2 v[RMAX];
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) v[m] = colv ? a.P[base + (size_t)(j + m * NT) * inner] : make_float2(0.f, 0.f);
+      fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
+      const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride + coff : nullptr;
+      float2* dst = tgt == 0 ? a.U : (tgt == 1 ? a.A1 : a.A2);
+      float2 wv[RMAX];
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) {
+        const size_t xo = (size_t)(j + m * NT) * inner;
+        const float2 z = make_float2(0.f, 0.f);
+        const float2 uu = colv ? a.U[base + xo] : z;
+        const float2 q1 = (colv && rd1) ? a.A1[base + xo] : z;
+        const float2 q2 = (colv && rd2) ? a.A2[base + xo] : z;
+        const float4 cc = colv ? cpk[(coff + xo) * 2 + chalf] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float2 nv = rscale(a.coef_t, v[m]);
+        if (src && colv) nv = radd(nv, src[xo]);
+        s[c][m] = colv ? stage_combine_v(stage, nv, uu, q1, q2, cc, &wv[m]) : z;
+      }
+      if (colv) {
+#pragma unroll
+        for (int m = 0; m < RMAX; ++m) dst[base + (size_t)(j + m * NT) * inner] = wv[m];
+      }
+    }
+  }
+  const int kyc = ky_compress(ky, a.N1, a.kmax1);
+  if (kyc < 0 || (tile % nzt) * V > a.kmax2) return;  // uniform: no kept column in this tile
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    float2 v[RMAX];
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) {
+      const int kx = j + m * NT;
+      const int ks = kx <= N0 / 2 ? kx : N0 - kx;
+      const bool keep = colv && kz <= a.kmax2 && ks <= a.kmax0;
+      float2 val = make_float2(0.f, 0.f);
+      if (keep) {
+        const float2 u = s[f >> 1][m];
+        if (f & 1) {
+          const float ax = a.ax[kx];
+          val = make_float2(mulr(ax, u.x), mulr(ax, u.y));
+        } else {
+          val = u;
+        }
+        val = make_float2(val.x * a.scale, val.y * a.scale);
+      }
+      v[m] = val;
+    }
+    fft_wg<N0, +1, V, VAR>(v, s_buf, s_tw, j, vi);
+    if (kz < a.PF2) {
+      float2* g = a.G1 + ((((size_t)b * 6 + f) * N0) * a.K1 + kyc) * a.PF2 + kz;
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) g[(size_t)(j + m * NT) * a.K1 * a.PF2] = v[m];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- y columns (inverse, 6 -> 9 fields)
+template <int N1, int V, int VAR>
+__global__ void __launch_bounds__((Fft<N1, VAR>::NT * V)) k3_yinv(Yinv3Args a) {
+  using F = Fft<N1, VAR>;
+  constexpr int RMAX = F::RMAX, NT = F::NT;
+  __shared__ float2 s_tw[Fft<N1, VAR>::tw_size()];
+  __shared__ float2 s_buf[LdsSize<N1, V>::slots];
+  load_twiddles<N1, VAR>(s_tw, a.tw1);
+  __syncthreads();
+  const int vi = threadIdx.x % V, j = threadIdx.x / V;
+  const int nzt = a.PF2 / V;
+  const int kz = (blockIdx.x % nzt) * V + vi;
+  const int x = (blockIdx.x / nzt) % a.N0;
+  const int b = a.b0 + blockIdx.x / (nzt * a.N0);
+  const bool kv = kz <= a.kmax2;
+#pragma unroll 1
+  for (int c = 0; c < 3; ++c) {
+    const float2* g0 = a.G1 + ((((size_t)b * 6 + 2 * c) * a.N0 + x) * a.K1) * a.PF2 + kz;
+    const float2* g1 = a.G1 + ((((size_t)b * 6 + 2 * c + 1) * a.N0 + x) * a.K1) * a.PF2 + kz;
+    float2 f0[RMAX], f1[RMAX];
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) {
+      const int ky = j + m * NT;
+      const int kc = ky_compress(ky, N1, a.kmax1);
+      const bool ld = kv && kc >= 0;
+      f0[m] = ld ? g0[(size_t)kc * a.PF2] : make_float2(0.f, 0.f);
+      f1[m] = ld ? g1[(size_t)kc * a.PF2] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float2 v[RMAX];
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) {
+        if (k == 0) v[m] = f0[m];
+        else if (k == 2) v[m] = f1[m];
+        else {
+          const float ay = a.ay[j + m * NT];
+          v[m] = make_float2(mulr(ay, f0[m].x), mulr(ay, f0[m].y));
+        }
+      }
+      fft_wg<N1, +1, V, VAR>(v, s_buf, s_tw, j, vi);
+      if (kz < a.PF2) {
+        float2* g = a.G2 + ((((size_t)b * 9 + 3 * c + k) * a.N0 + x) * N1) * a.PF2 + kz;
+#pragma unroll
+        for (int m = 0; m < RMAX; ++m) g[(size_t)(j + m * NT) * a.PF2] = v[m];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- z rows (products)
+template <int N2, int ROWS, int VAR>
+__global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) {
+  using F = Fft<N2, VAR>;
+  constexpr int RMAX = F::RMAX, NT = F::NT;
+  constexpr int SL = LdsSize<N2, 1>::slots;
+  __shared__ float2 s_tw[Fft<N2, VAR>::tw_size()];
+  __shared__ float2 s_buf[ROWS * SL];
+  load_twiddles<N2, VAR>(s_tw, a.tw2);
+  __syncthreads();
+  const int g = threadIdx.x / NT, j = threadIdx.x % NT;
+  const int row = blockIdx.x * ROWS + g;
+  const size_t nrows = (size_t)a.nb * a.N0 * a.N1;
+  const bool rv = (size_t)row < nrows;
+  const int bl = rv ? row / (a.N0 * a.N1) : 0;
+  const int xy = rv ? row % (a.N0 * a.N1) : 0;
+  const int b = a.b0 + bl;
+  float2* lds = s_buf + g * SL;
+  const size_t fs = (size_t)a.N0 * a.N1 * a.PF2;  // one G2 field of one sample
+  const float2* gb = a.G2 + (size_t)b * 9 * fs + (size_t)xy * a.PF2;
+  // packed inverse transform of two fields: A = ma(fa), B = mb(fb) where the multiplier is
+  // 0: 1, 1: i, 2: i a_z  (u_c = C2R(G[3c]), d_y u_c = C2R(i G[3c+1]), d_x u_c = C2R(i G[3c+2]),
+  // d_z u_c = C2R(i a_z G[3c]))
+  auto inv_pair = [&](float2 (&z)[RMAX], int fa, int ma, int fb, int mb) {
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) {
+      const int pos = j + m * NT;
+      const bool cj = pos > N2 / 2;
+      const int k = cj ? N2 - pos : pos;
+      const bool edge = (k == 0) || (k == N2 / 2);
+      float2 A = make_float2(0.f, 0.f), Bv = A;
+      if (rv && k <= a.kmax2) {
+        const float az = a.az[k];
+        float2 x1 = gb[(size_t)fa * fs + k], x2 = gb[(size_t)fb * fs + k];
+        A = ma == 0 ? x1 : (ma == 1 ? make_float2(-x1.y, x1.x) : make_float2(-az * x1.y, az * x1.x));
+        Bv = mb == 0 ? x2 : (mb == 1 ? make_float2(-x2.y, x2.x) : make_float2(-az * x2.y, az * x2.x));
+      }
+      z[m] = pack_pair3(A, Bv, cj, edge);
+    }
+    fft_wg<N2, +1, 1, VAR>(z, lds, s_tw, j, 0);
+  };
+  float u0[RMAX], u1[RMAX], u2[RMAX], p0[RMAX], p1[RMAX], p2[RMAX];
+  float2 z[RMAX];
+  inv_pair(z, 0, 0, 3, 0);  // (u0, u1)
+#pragma unroll
+  for (int m = 0; m < RMAX; ++m) {
+    u0[m] = z[m].x;
+    u1[m] = z[m].y;
+  }
+  inv_pair(z, 6, 0, 2, 1);  // (u2, d_x u0)
+#pragma unroll
+  for (int m = 0; m < RMAX; ++m) {
+    u2[m] = z[m].x;
+    p0[m] = u0[m] * z[m].y;
+  }
+  inv_pair(z, 1, 1, 0, 2);  // (d_y u0, d_z u0)
+#pragma unroll
+  for (int m = 0; m < RMAX; ++m) p0[m] = (p0[m] + u1[m] * z[m].x) + u2[m] * z[m].y;
+  inv_pair(z, 5, 1, 4, 1);  // (d_x u1, d_y u1)
+#pragma unroll
+  for (int m = 0; m < RMAX; ++m) p1[m] = u0[m] * z[m].x + u1[m] * z[m].y;
+  inv_pair(z, 3, 2, 8, 1);  // (d_z u1, d_x u2)
+#pragma unroll
+  for (int m = 0; m < RMAX; ++m) {
+    p1[m] = p1[m] + u2[m] * z[m].x;
+    p2[m] = u0[m] * z[m].y;
+  }
+  inv_pair(z, 7, 1, 6, 2);  // (d_y u2, d_z u2)
+#pragma unroll
+  for (int m = 0; m < RMAX; ++m) p2[m] = (p2[m] + u1[m] * z[m].x) + u2[m] * z[m].y;
+  // forward: (p0 + i p1) packed, then p2
+#pragma unroll
+  for (int m = 0; m < RMAX; ++m) z[m] = make_float2(p0[m], p1[m]);
+  fft_wg<N2, -1, 1, VAR>(z, lds, s_tw, j, 0);
+#pragma unroll
+  for (int m = 0; m < RMAX; ++m) lds[lds_slot<1>(j + m * NT, 0)] = z[m];
+  __syncthreads();
+  const size_t cs = (size_t)a.N0 * a.N1 * a.P2;  // one channel of the products
+  float2* pb = a.P + (size_t)b * 3 * cs + (size_t)xy * a.P2;
+  if (rv) {
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) {
+      const int pos = j + m * NT;
+      if (pos <= N2 / 2) {
+        const float2 zk = z[m];
+        const float2 zm = lds[lds_slot<1>((N2 - pos) & (N2 - 1), 0)];
+        pb[pos] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);
+        pb[cs + pos] = make_float2(0.5f * d.y, -0.5f * d.x);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < RMAX; ++m) z[m] = make_float2(p2[m], 0.f);
+  fft_wg<N2, -1, 1, VAR>(z, lds, s_tw, j, 0);
+  if (rv) {
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) {
+      const int pos = j + m * NT;
+      if (pos <= N2 / 2) pb[2 * cs + pos] = z[m];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- host launchers
+#define TFSM_SWITCH3(n, ...)                           \
+  switch (n) {                                         \
+    case 16: { constexpr int NN = 16; __VA_ARGS__ }    \
+    case 32: { constexpr int NN = 32; __VA_ARGS__ }    \
+    case 64: { constexpr int NN = 64; __VA_ARGS__ }    \
+    case 128: { constexpr int NN = 128; __VA_ARGS__ }  \
+    default: return hipErrorInvalidValue;              \
+  }
+
+hipError_t launch_col3d(const Col3Args& a, int nb, int stage, hipStream_t st) {
+  TFSM_SWITCH3(a.N0, {
+    constexpr int V = 16;
+    constexpr int VR = NN == 128 ? 2 : 0;
+    const int ntiles = a.N1 * (a.P2 / V);
+    const int slots = (ntiles + 7) / 8;
+    Col3Args b = a;
+    b.nb = nb;
+    hipLaunchKernelGGL((k3_col<NN, V, VR>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b, stage);
+    return hipGetLastError();
+  })
+}
+
+hipError_t launch_yinv3d(const Yinv3Args& a, hipStream_t st) {
+  TFSM_SWITCH3(a.N1, {
+    constexpr int V = 16;
+    constexpr int VR = NN == 128 ? 2 : 0;
+    const int nzt = a.PF2 / V;
+    hipLaunchKernelGGL((k3_yinv<NN, V, VR>), dim3(nzt * a.N0 * a.nb), dim3(Fft<NN, VR>::NT * V), 0, st, a);
+    return hipGetLastError();
+  })
+}
+
+hipError_t launch_row3d(const Row3Args& a, hipStream_t st) {
+  TFSM_SWITCH3(a.N2, {
+    constexpr int VR = NN == 128 ? 2 : 0;
+    constexpr int NT = Fft<NN, VR>::NT;
+    constexpr int R = NT >= 256 ? 1 : 256 / NT;
+    const size_t rows = (size_t)a.nb * a.N0 * a.N1;
+    hipLaunchKernelGGL((k3_row<NN, R, VR>), dim3((unsigned)((rows + R - 1) / R)), dim3(NT * R), 0, st, a);
+    return hipGetLastError();
+  })
+}
+
+bool step3d_supported(int n0, int n1, int n2) {
+  auto ok = [](int n) { return n == 16 || n == 32 || n == 64 || n == 128; };
+  return ok(n0) && ok(n1) && ok(n2);
+}
+
+}  // namespace tfsm

@@ -387,14 +387,22 @@ hipError_t launch_cols_c2c(int n, int dir, const float2* in, float2* out, size_t
   if (n > 512) return hipErrorInvalidValue;
   TFSM_SWITCH_N(n, {
     constexpr int V = (NN > 512) ? 1 : ((Fft<NN>::NT * 16 <= 1024) ? 16 : 8);
-    dim3 grid((ninner + V - 1) / V, (unsigned)nouter);
-    if (dir < 0)
-      hipLaunchKernelGGL((k_cols_c2c<NN, V, -1>), grid, dim3(Fft<NN>::NT * V), 0, st, in, out, outer_stride,
-                         col_stride, ninner, scale, tw);
-    else
-      hipLaunchKernelGGL((k_cols_c2c<NN, V, 1>), grid, dim3(Fft<NN>::NT * V), 0, st, in, out, outer_stride,
-                         col_stride, ninner, scale, tw);
-    return hipGetLastError();
+    // grid.y is limited to 65535: launch the outer blocks in chunks
+    for (size_t o0 = 0; o0 < nouter; o0 += 65535) {
+      const size_t no = (nouter - o0) < 65535 ? (nouter - o0) : 65535;
+      dim3 grid((ninner + V - 1) / V, (unsigned)no);
+      const float2* ib = in + o0 * outer_stride;
+      float2* ob = out + o0 * outer_stride;
+      if (dir < 0)
+        hipLaunchKernelGGL((k_cols_c2c<NN, V, -1>), grid, dim3(Fft<NN>::NT * V), 0, st, ib, ob, outer_stride,
+                           col_stride, ninner, scale, tw);
+      else
+        hipLaunchKernelGGL((k_cols_c2c<NN, V, 1>), grid, dim3(Fft<NN>::NT * V), 0, st, ib, ob, outer_stride,
+                           col_stride, ninner, scale, tw);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
   })
 }
 

@@ -24,11 +24,15 @@ def ns_initial_vorticity(n: int, batch: int, seed: int = 0, kmax: float = 32.0, 
 def fourier_modes_field(n: int, dims: int, channels: int, batch: int, seed: int = 0, nmodes: int = 8, kabs: int = 4,
                         length: float = 1.0, dtype=torch.float32):
     """Per (batch, channel): sum of nmodes cosines a cos(2 pi k.x / length + phi) with integer k,
-    1 <= |k| <= kabs (rejection sampled), a ~ U(-1, 1), phi ~ U(0, 2 pi)."""
+    1 <= |k| <= kabs (rejection sampled), a ~ U(-1, 1), phi ~ U(0, 2 pi).  `n` is the points per
+    axis, or a tuple of `dims` sizes for a non-cubic box."""
     g = torch.Generator().manual_seed(seed)
-    xs = [torch.arange(n, dtype=torch.float64) * length / n for _ in range(dims)]
+    ns = tuple(n) if isinstance(n, (tuple, list)) else (n,) * dims
+    if len(ns) != dims:
+        raise ValueError("one size per axis expected")
+    xs = [torch.arange(nd, dtype=torch.float64) * length / nd for nd in ns]
     grids = torch.meshgrid(*xs, indexing="ij")
-    out = torch.zeros(batch, channels, *([n] * dims), dtype=torch.float64)
+    out = torch.zeros(batch, channels, *ns, dtype=torch.float64)
     for b in range(batch):
         for c in range(channels):
             m = 0
