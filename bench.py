@@ -1,14 +1,18 @@
 #!/usr/bin/env python
-"""Benchmark: 2D incompressible Navier-Stokes (vorticity form) with Kolmogorov forcing on a 512^2
-grid, batch 64, nu = 1e-3 (Re = 1000), SETDRK4, dt = 1e-3 -- the throughput workload of the ETD
-engine, in DOF*steps/s (DOF = B * C * 512^2).
+"""Benchmark of the ETD engine.  Prints ONE JSON line on stdout; progress goes to stderr.
 
-One step = one SETDRK4 step of the whole batch (4 nonlinear evaluations: 4 row + 4 column
-kernels), state resident in HBM.  Prints ONE JSON line on stdout; progress goes to stderr.
+Workloads (--workload):
+  ns2d       (default) 2D incompressible Navier-Stokes, vorticity form, Kolmogorov forcing,
+             512^2 grid, batch 64, nu = 1e-3 (Re = 1000), SETDRK4, dt = 1e-3.  One step = one
+             SETDRK4 step of the whole batch (4 nonlinear evaluations: 4 row + 4 column kernels).
+  burgers3d  3D vector Burgers(0.01), 128^3 grid, 3 channels, batch 8, SETDRK2, dt = 5e-3.  One
+             step = 2 evaluations, each a z-row, y-forward, x-column and y-inverse kernel.
+The metric is DOF*steps/s with DOF = B * C * prod(N), state resident in HBM.
 
 Legs:
   engine       K timed steps after W warm-up steps (torch.cuda.synchronize on both sides)
-  kernels      HIP-event timing of the row and column kernels on the engine's stream -> roofline
+  kernels      HIP-event timing of each kernel of the step on the engine's stream, launched with
+               the same sub-batch shape as the step -> roofline of the dominant kernel
   cpu_baseline the oracle's restatement of the reference path on the host cores (bounded sample)
   torch_gpu    the same restatement on the GPU (torch.fft / rocFFT), the "torch path"
 """
@@ -23,65 +27,227 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-N, B, RE, DT = 512, 64, 1000.0, 1e-3
 HBM_PEAK_GBS = 8000.0
 METRIC = "DOF·steps/s, 2D NS 512² b64 vs torch path; p50/p99 latency, mixed load"
+
+# stage codes of the fused kernels (engine.hpp Stage)
+ST_PRO, ST_R2_E0, ST_R2_E1, ST_R4_E0, ST_R4_E1, ST_R4_E2, ST_R4_E3 = 0, 2, 3, 7, 8, 9, 10
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_engine(dev, w0):
+class NS2D:
+    name = "ns2d_kolmogorov_512x512_b64_setdrk4"
+    N, B, C, RE, DT, SCHEME = 512, 64, 1, 1000.0, 1e-3, "SETDRK4"
+    default_steps = 1000
+    traffic_file = "k2_col_ns512_b64_traffic.txt"
+
+    def __init__(self, dev):
+        import torch
+
+        import torchfsm_amd as tfa
+        from torchfsm_amd import field, pde
+        from torchfsm_amd.integrator import SETDRKIntegrator
+        from torchfsm_amd.workloads import ns_initial_vorticity
+
+        self.dev = dev
+        self.u0 = ns_initial_vorticity(self.N, self.B, seed=0)
+        mesh = tfa.MeshGrid([(0, 2 * math.pi, self.N)] * 2, device=dev)
+        self.y = mesh.bc_mesh_grid()[1]
+        op = pde.NavierStokesVorticity(self.RE, field.kolm_force(self.y))
+        op.set_integrator(SETDRKIntegrator.SETDRK4)
+        op.register_mesh(tfa.FourierMesh(mesh), 1)
+        self.plan = op._prepare(self.B, dev, self.DT)
+        self.uh = self.plan.forward(self.u0.to(dev))
+        torch.cuda.synchronize()
+        self.dof = self.B * self.C * self.N * self.N
+
+    def config(self):
+        return {"workload": self.name, "grid": [self.N, self.N], "batch": self.B, "nu": 1.0 / self.RE,
+                "dt": self.DT, "scheme": self.SCHEME, "dealias": "2/3",
+                "forcing": "kolm_force(y): -0.1 w - 4 cos(4y)"}
+
+    data = "synthetic: seeded Gaussian random vorticity, E(k)~k^-3 on 1<=|k|<=32, unit RMS"
+
+    def kernels(self, nb):
+        """(name, pass id, stage, algorithmic bytes per launch, launches per step).  A = one
+        half-spectrum field of the launch's nb samples; rho = kept fraction of ky (2/3 rule)."""
+        H = self.N // 2 + 1
+        A = nb * self.N * H * 8
+        rho = (self.N // 3 + 1) / H
+        per = self.B // nb
+        row = ("k2_row<512,4,VC,*>", 0, ST_R4_E0, (4 * rho + 1) * A, 4 * per)
+        cols = [(f"k2_col<512,tile,VC,*> e{i}", 1, st, (r + 4 * rho) * A, per)
+                for i, (st, r) in enumerate([(ST_R4_E0, 3), (ST_R4_E1, 3), (ST_R4_E2, 5), (ST_R4_E3, 5)])]
+        return [row] + cols
+
+    def prime(self):
+        """Fill the product/field buffers so single passes see a valid state."""
+        self.plan.launch_pass(self.uh.clone(), 1, ST_PRO)
+
+    def cpu_baseline(self, args):
+        import torch
+
+        from oracle import refpath
+
+        terms = refpath.ns_terms(self.RE, self.y.cpu())
+        mesh_info = [(0, 2 * math.pi, self.N)] * 2
+        mesh_c = refpath.Mesh(mesh_info, device="cpu")
+        L = refpath.build_linear(mesh_c, terms, 1)
+        sch = refpath.Scheme("SETDRK4", self.DT, L, refpath.build_closure(mesh_c, terms))
+        uc = sch.step(mesh_c.fft(self.u0))
+        t0 = time.perf_counter()
+        for _ in range(args.cpu_steps):
+            uc = sch.step(uc)
+        t = time.perf_counter() - t0
+        return self.dof * args.cpu_steps / t, (
+            f"oracle refpath (torch CPU, MKL FFT={torch.backends.mkl.is_available()}), full workload "
+            f"B={self.B} 512^2, 1 warm-up + {args.cpu_steps} timed SETDRK4 steps ({t:.1f} s)")
+
+    def torch_gpu(self, steps):
+        from oracle import refpath
+
+        mesh_info = [(0, 2 * math.pi, self.N)] * 2
+        mesh_g = refpath.Mesh(mesh_info, device=self.dev)
+        terms = refpath.ns_terms(self.RE, self.y)
+        sch = refpath.Scheme("SETDRK4", self.DT, refpath.build_linear(mesh_g, terms, 1),
+                             refpath.build_closure(mesh_g, terms))
+        return sch, mesh_g.fft(self.u0.to(self.dev)), self.dof
+
+
+class Burgers3D:
+    name = "burgers3d_128cubed_c3_b8_setdrk2"
+    N, B, C, NU, DT, SCHEME = 128, 8, 3, 0.01, 5e-3, "SETDRK2"
+    default_steps = 200
+    traffic_file = "k3_col_burgers128_b8_traffic.txt"
+    data = "synthetic: per (b, c) 8 seeded cosines, integer |k| in [1, 4], a ~ U(-1,1), phase ~ U(0, 2pi)"
+
+    def __init__(self, dev):
+        import torch
+
+        import torchfsm_amd as tfa
+        from torchfsm_amd import pde
+        from torchfsm_amd.integrator import SETDRKIntegrator
+        from torchfsm_amd.workloads import fourier_modes_field
+
+        self.dev = dev
+        self.u0 = fourier_modes_field(self.N, 3, 3, self.B, seed=0)
+        self.mesh_info = [(0, 1, self.N)] * 3
+        op = pde.Burgers(self.NU)
+        op.set_integrator(SETDRKIntegrator.SETDRK2)
+        op.register_mesh(tfa.FourierMesh(self.mesh_info, device=dev), 3)
+        self.plan = op._prepare(self.B, dev, self.DT)
+        self.uh = self.plan.forward(self.u0.to(dev))
+        torch.cuda.synchronize()
+        self.dof = self.B * self.C * self.N ** 3
+
+    def config(self):
+        return {"workload": self.name, "grid": [self.N] * 3, "channels": 3, "batch": self.B, "nu": self.NU,
+                "dt": self.DT, "scheme": self.SCHEME, "dealias": "2/3"}
+
+    def kernels(self, nb):
+        """A1 = one half-spectrum channel of the launch's nb samples; rz = kept kz fraction,
+        ryz = kept (ky, kz) fraction.  Fields G2 hold kept kz only, G1 kept (ky, kz) only."""
+        N = self.N
+        H = N // 2 + 1
+        A1 = nb * N * N * H * 8
+        k = N // 3
+        rz = (k + 1) / H
+        ryz = (2 * k + 1) / N * rz
+        per = self.B // nb
+        return [
+            ("k3_row<128,16,2> (z rows: 12 C2R, products, 3 R2C)", 4, ST_R2_E0, (9 * rz + 3) * A1, 2 * per),
+            ("k_cols_c2c<128,*,-1> (y forward of the products)", 5, ST_R2_E0, 6 * A1, 2 * per),
+            ("k3_col<128,16,2> e0 (x fwd + combine + 6 x inverse)", 6, ST_R2_E0, (9 + 6 * ryz) * A1, per),
+            ("k3_col<128,16,2> e1", 6, ST_R2_E1, (12 + 6 * ryz) * A1, per),
+            ("k3_yinv<128,16,2> (6 -> 9 fields, y inverse)", 7, ST_R2_E0, (6 * ryz + 9 * rz) * A1, 2 * per),
+        ]
+
+    def prime(self):
+        self.plan.launch_pass(self.uh.clone(), 3, ST_PRO)
+
+    def cpu_baseline(self, args):
+        import torch
+
+        from oracle import refpath
+
+        # bounded sample: one sample of the batch, 1 warm-up + cpu_steps timed steps
+        terms = refpath.burgers_terms(self.NU)
+        mesh_c = refpath.Mesh(self.mesh_info, device="cpu")
+        sch = refpath.Scheme("SETDRK2", self.DT, refpath.build_linear(mesh_c, terms, 3),
+                             refpath.build_closure(mesh_c, terms))
+        uc = sch.step(mesh_c.fft(self.u0[:1]))
+        t0 = time.perf_counter()
+        for _ in range(args.cpu_steps):
+            uc = sch.step(uc)
+        t = time.perf_counter() - t0
+        dof1 = self.C * self.N ** 3
+        return dof1 * args.cpu_steps / t, (
+            f"oracle refpath (torch CPU, MKL FFT={torch.backends.mkl.is_available()}), 1 of the {self.B} "
+            f"samples at 128^3 x 3 channels, 1 warm-up + {args.cpu_steps} timed SETDRK2 steps ({t:.1f} s)")
+
+    def torch_gpu(self, steps):
+        from oracle import refpath
+
+        mesh_g = refpath.Mesh(self.mesh_info, device=self.dev)
+        terms = refpath.burgers_terms(self.NU)
+        sch = refpath.Scheme("SETDRK2", self.DT, refpath.build_linear(mesh_g, terms, 3),
+                             refpath.build_closure(mesh_g, terms))
+        return sch, mesh_g.fft(self.u0.to(self.dev)), self.dof
+
+
+WORKLOADS = {"ns2d": NS2D, "burgers3d": Burgers3D}
+
+
+def time_kernel(plan, uh, pass_id, stage, reps=20):
+    """Average duration (ms) of one pass, from HIP events recorded on the stream the engine
+    launches on (torch's current stream)."""
     import torch
 
-    import torchfsm_amd as tfa
-    from torchfsm_amd import field, pde
-    from torchfsm_amd.integrator import SETDRKIntegrator
-
-    mesh = tfa.MeshGrid([(0, 2 * math.pi, N)] * 2, device=dev)
-    y = mesh.bc_mesh_grid()[1]
-    op = pde.NavierStokesVorticity(RE, field.kolm_force(y))
-    op.set_integrator(SETDRKIntegrator.SETDRK4)
-    op.register_mesh(tfa.FourierMesh(mesh), 1)
-    plan = op._prepare(B, dev, DT)
-    uh = plan.forward(w0.to(dev))
-    torch.cuda.synchronize()
-    return op, plan, uh, y
-
-
-def time_kernels(plan, uh, reps=20):
-    """Average duration (ms) of the row kernel and of the column kernel per SETDRK4 stage, from HIP
-    events recorded on the stream the kernels are launched on."""
-    import torch
-
-    u = uh.clone()
     stream = torch.cuda.current_stream()
-    out = {}
-    plan.launch_pass(u, 1, 0)  # fields of the state
-    plan.launch_pass(u, 0)
-    for name, pid, stage in [("row", 0, 0), ("col_e0", 1, 7), ("col_e1", 1, 8), ("col_e2", 1, 9), ("col_e3", 1, 10)]:
-        plan.launch_pass(u, pid, stage)
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(reps):
-            plan.launch_pass(u, pid, stage)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        out[name] = e0.elapsed_time(e1) / reps
-    return out
+    plan.launch_pass(uh, pass_id, stage)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        plan.launch_pass(uh, pass_id, stage)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def read_traffic(fname, tuning):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a plain-copy calibration), or None when
+    the summary is absent or was taken at another launch shape."""
+    path = os.path.join(HERE, "profiles", fname)
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    if vals.get("launch_shape") != ",".join(map(str, tuning)):
+        return None
+    try:
+        return float(vals["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="ns2d")
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (0 = the workload's default)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--torch-steps", type=int, default=20)
-    ap.add_argument("--no-baselines", action="store_true", help="engine leg only (used under the profiler)")
+    ap.add_argument("--no-baselines", action="store_true", help="engine and kernel legs only")
     ap.add_argument("--tile", type=int, default=0, help="column tile (8/16); 0 = autotune the launch shape")
     ap.add_argument("--col-var", type=int, default=0)
     ap.add_argument("--row-var", type=int, default=0)
@@ -92,13 +258,11 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X and this build has no multi-GPU path "
-              f"(--gpus {args.gpus} requested). The batch shards across GPUs with no communication; see DESIGN.md.",
+              f"(--gpus {args.gpus} requested). The batch shards across GPUs with no communication.",
               file=sys.stderr)
         sys.exit(2)
 
     import torch
-
-    from torchfsm_amd.workloads import ns_initial_vorticity
 
     dev = torch.device("cuda:0")
     if args.copy_probe:
@@ -109,9 +273,11 @@ def main():
         torch.cuda.synchronize()
         del src, dst
         log("[bench] copy probe: 3 x 256 MiB device-to-device copies done")
-    log(f"[bench] NS {N}^2 B={B} SETDRK4 dt={DT}: building input and plan")
-    w0 = ns_initial_vorticity(N, B, seed=0)
-    op, plan, uh, y = build_engine(dev, w0)
+    W = WORKLOADS[args.workload]
+    steps = args.steps or W.default_steps
+    log(f"[bench] {W.name}: building input and plan")
+    w = W(dev)
+    plan, uh = w.plan, w.uh
     if args.tile:
         plan.set_tuning(args.tile, args.col_var, args.row_var, args.row_wg, args.sub_batch)
     else:
@@ -123,119 +289,95 @@ def main():
     plan.step(uh, args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    plan.step(uh, args.steps)
+    plan.step(uh, steps)
     torch.cuda.synchronize()
     t_eng = time.perf_counter() - t0
-    dof = B * 1 * N * N
-    value = dof * args.steps / t_eng
-    ms_step = t_eng / args.steps * 1e3
+    value = w.dof * steps / t_eng
+    ms_step = t_eng / steps * 1e3
     finite = bool(torch.isfinite(plan.inverse(uh)).all().item())
     log(f"[bench] engine: {ms_step:.4f} ms/step, {value:.4e} DOF*steps/s, finite={finite}")
 
+    cfg = w.config()
+    cfg["launch_shape"] = {"col_tile": plan.tuning[0], "col_radix_variant": plan.tuning[1],
+                           "row_radix_variant": plan.tuning[2], "row_workgroup": plan.tuning[3],
+                           "sub_batch": plan.tuning[4]}
+    cfg["parallelism"] = "single GPU, batch-parallel"
     result = {
         "metric": METRIC,
         "value": value,
         "unit": "DOF·steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: seeded Gaussian random vorticity, E(k)~k^-3 on 1<=|k|<=32, unit RMS",
-        "config": {
-            "workload": "ns2d_kolmogorov_512x512_b64_setdrk4",
-            "grid": [N, N],
-            "batch": B,
-            "nu": 1.0 / RE,
-            "dt": DT,
-            "scheme": "SETDRK4",
-            "dealias": "2/3",
-            "forcing": "kolm_force(y): -0.1 w - 4 cos(4y)",
-            "launch_shape": {"col_tile": plan.tuning[0], "col_radix_variant": plan.tuning[1],
-                             "row_radix_variant": plan.tuning[2], "row_workgroup": plan.tuning[3],
-                             "sub_batch": plan.tuning[4]},
-            "parallelism": "single GPU, batch-parallel",
-        },
+        "data": w.data,
+        "config": cfg,
     }
-    if args.no_baselines:
-        print(json.dumps(result), flush=True)
-        return
 
     # ---------------------------------------------------------------- kernel roofline
-    kt = time_kernels(plan, uh)
-    H = N // 2 + 1
-    kmax = plan_kmax = N // 3  # |k| <= floor(N/3) kept (2/3 rule)
-    A = B * N * H * 8  # bytes of one half-spectrum field over the batch
-    rho = (kmax + 1) / H
-    row_bytes = (4 * rho + 1) * A
-    col_bytes = {"col_e0": 3 * A, "col_e1": 3 * A, "col_e2": 5 * A, "col_e3": 5 * A}
-    col_bytes = {k: v + 4 * rho * A for k, v in col_bytes.items()}
-    t_row_step = 4 * kt["row"]
-    t_col_step = sum(kt[k] for k in col_bytes)
-    step_bytes = 4 * row_bytes + sum(col_bytes.values())
-    if t_col_step >= t_row_step:
-        dom = ("k2_col<512,tile,VC> (column: fwd x-FFT + SETDRK4 combine + 4 inverse x-FFTs)",
-               sum(col_bytes.values()) / 4, t_col_step / 4)
-    else:
-        dom = ("k2_row<512,4,VC> (row: 2 packed C2R + product + R2C)", row_bytes, kt["row"])
-    achieved = dom[1] / (dom[2] * 1e-3) / 1e9
+    sb = plan.tuning[4]
+    nb = sb if 0 < sb < w.B else w.B
+    w.prime()
+    u = uh.clone()
+    ks = w.kernels(nb)
+    kt = {}
+    for name, pid, stage, _, _ in ks:
+        kt[name] = time_kernel(plan, u, pid, stage)
+    # dominant kernel family = the largest share of the step's kernel time
+    fam = {}
+    for name, pid, stage, nbytes, per_step in ks:
+        f = name.split(" e")[0] if " e" in name else name
+        t, b, n = fam.get(f, (0.0, 0.0, 0))
+        fam[f] = (t + kt[name] * per_step, b + nbytes * per_step, n + per_step)
+    dom = max(fam, key=lambda f: fam[f][0])
+    t_dom, b_dom, n_dom = fam[dom]
+    avg_ms = t_dom / n_dom
+    avg_bytes = b_dom / n_dom
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    step_bytes = sum(nbytes * per_step for _, _, _, nbytes, per_step in ks)
+    traffic = read_traffic(W.traffic_file, plan.tuning)
     result["roofline"] = {
         "bound": "hbm",
-        "kernel": dom[0],
+        "kernel": dom,
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": avg_bytes,
+        "avg_launch_ms": avg_ms,
+        "launch_samples": nb,
         "kernel_ms": {k: round(v, 5) for k, v in kt.items()},
         "step_bytes_model": step_bytes,
         "step_frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
     }
-    log(f"[bench] kernels (ms): {kt}; dominant {dom[0]} {achieved:.0f} GB/s")
+    log(f"[bench] kernels (ms): {kt}; dominant {dom}: {avg_bytes / 1e6:.1f} MB per launch in {avg_ms:.4f} ms "
+        f"= {achieved:.0f} GB/s; traffic {traffic}")
+    if args.no_baselines:
+        print(json.dumps(result), flush=True)
+        return
 
     # ---------------------------------------------------------------- baselines (oracle restatement)
-    from oracle import refpath
-
-    terms_cpu = refpath.ns_terms(RE, y.cpu())
     threads = min(16, os.cpu_count() or 1)
     torch.set_num_threads(threads)
-    log(f"[bench] cpu baseline: refpath float32 on {threads} threads, B={B}, {args.cpu_steps} steps")
-    mesh_info = [(0, 2 * math.pi, N)] * 2
-    mesh_c = refpath.Mesh(mesh_info, device="cpu")
-    L = refpath.build_linear(mesh_c, terms_cpu, 1)
-    sch = refpath.Scheme("SETDRK4", DT, L, refpath.build_closure(mesh_c, terms_cpu))
-    uc = mesh_c.fft(w0)
-    uc = sch.step(uc)
-    t0 = time.perf_counter()
-    for _ in range(args.cpu_steps):
-        uc = sch.step(uc)
-    t_cpu = time.perf_counter() - t0
-    cpu_val = dof * args.cpu_steps / t_cpu
-    result["cpu_baseline"] = {
-        "value": cpu_val,
-        "unit": "DOF·steps/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"oracle refpath (torch CPU, MKL FFT={torch.backends.mkl.is_available()}), full workload "
-                  f"B={B} 512^2, 1 warm-up + {args.cpu_steps} timed SETDRK4 steps ({t_cpu:.1f} s)",
-    }
-    log(f"[bench] cpu baseline {cpu_val:.3e} DOF*steps/s ({t_cpu:.1f} s)")
+    log(f"[bench] cpu baseline: refpath float32 on {threads} threads, {args.cpu_steps} steps")
+    cpu_val, sample = w.cpu_baseline(args)
+    result["cpu_baseline"] = {"value": cpu_val, "unit": "DOF·steps/s", "cores": threads, "kind": "port",
+                              "sample": sample}
+    log(f"[bench] cpu baseline {cpu_val:.3e} DOF*steps/s")
 
     log(f"[bench] torch-GPU baseline: refpath on {dev}, {args.torch_steps} steps")
-    mesh_g = refpath.Mesh(mesh_info, device=dev)
-    terms_g = refpath.ns_terms(RE, y)
-    Lg = refpath.build_linear(mesh_g, terms_g, 1)
-    schg = refpath.Scheme("SETDRK4", DT, Lg, refpath.build_closure(mesh_g, terms_g))
-    ug = mesh_g.fft(w0.to(dev))
+    sch, ug, dof = w.torch_gpu(args.torch_steps)
     for _ in range(2):
-        ug = schg.step(ug)
+        ug = sch.step(ug)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.torch_steps):
-        ug = schg.step(ug)
+        ug = sch.step(ug)
     torch.cuda.synchronize()
     t_tg = time.perf_counter() - t0
     tg_val = dof * args.torch_steps / t_tg
@@ -244,7 +386,7 @@ def main():
         "unit": "DOF·steps/s",
         "ms_per_step": t_tg / args.torch_steps * 1e3,
         "kind": "port",
-        "sample": f"oracle refpath (full complex torch.fft, reference op order) on the GPU, B={B}, "
+        "sample": f"oracle refpath (full complex torch.fft, reference op order) on the GPU, whole batch, "
                   f"2 warm-up + {args.torch_steps} timed steps",
         "engine_speedup": value / tg_val,
     }

@@ -169,10 +169,13 @@ int tfsm_apply(tfsm_plan* plan, const void* uh, void* out, void* ws, void* strea
  * (_base.py:217-262). */
 int tfsm_solve(tfsm_plan* plan, const void* uh, void* out, void* stream);
 
-/* Launch one pass of the fused step on the current state/workspace, for profiling: pass 0 = the
+/* Launch one pass of the fused step on the current state/workspace, for profiling, over the first
+ * sub-batch (tfsm_plan_set_tuning; the whole batch when none is set), exactly as tfsm_step
+ * launches it: pass 0 = the
  * row kernel (2D), pass 1 = the column kernel with stage code `stage` (0 prologue, 7..10 the four
  * SETDRK4 evaluations), pass 2 = the 1D persistent stepper for one step, pass 3 = one whole 3D
- * evaluation (z-row, y-forward, x-column and y-inverse kernels) with stage code `stage`.  (No reference
+ * evaluation (z-row, y-forward, x-column and y-inverse kernels) with stage code `stage`, passes
+ * 4..7 = one of those four kernels alone.  (No reference
  * counterpart: instrumentation of the engine's own kernels.) */
 int tfsm_launch_pass(tfsm_plan* plan, void* uh, void* ws, int32_t pass_id, int32_t stage, void* stream);
 

@@ -308,10 +308,11 @@ Col3Args col3_args(tfsm_plan* p, float2* U, char* ws, int b0, int nb) {
 
 // one nonlinear evaluation of the 3D convection term for samples [b0, b0 + nb):
 // z rows (products) -> y columns forward -> x columns (combine, fields) -> y columns inverse
-int run3d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hipStream_t st) {
+// `parts` selects kernels (bit 0 z rows, 1 y forward, 2 x columns, 3 y inverse) for profiling.
+int run3d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hipStream_t st, int parts = 15) {
   const int N0 = p->n[0], N1 = p->n[1];
   float2* Pp = (float2*)(ws + p->off_P);
-  if (stage != ST_PRO) {
+  if (stage != ST_PRO && (parts & 1)) {
     Row3Args r;
     r.N0 = N0;
     r.N1 = N1;
@@ -326,12 +327,15 @@ int run3d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hip
     r.b0 = b0;
     r.nb = nb;
     HIP_TRY(launch_row3d(r, st), "3D row kernel launch");
+  }
+  if (stage != ST_PRO && (parts & 2)) {
     float2* Pb = Pp + (size_t)b0 * 3 * p->modes;
     HIP_TRY(launch_cols_c2c(N1, -1, Pb, Pb, (size_t)nb * 3 * N0, (size_t)N1 * p->P, (size_t)p->P, p->H, 1.0f,
                             p->tw[1][0], st),
             "3D y-forward launch");
   }
-  HIP_TRY(launch_col3d(col3_args(p, U, ws, b0, nb), nb, stage, st), "3D column kernel launch");
+  if (parts & 4) HIP_TRY(launch_col3d(col3_args(p, U, ws, b0, nb), nb, stage, st), "3D column kernel launch");
+  if (!(parts & 8)) return TFSM_OK;
   Yinv3Args y;
   y.N0 = N0;
   y.N1 = N1;
@@ -868,13 +872,15 @@ int tfsm_solve(tfsm_plan* p, const void* uh, void* out, void* stream) {
 int tfsm_launch_pass(tfsm_plan* p, void* uh, void* ws, int32_t pass_id, int32_t stage, void* stream) {
   if (!p || !uh || !ws) return fail(TFSM_E_INVALID, "null argument");
   hipStream_t st = (hipStream_t)stream;
+  // the launches cover the first sub-batch, exactly as tfsm_step launches them
+  const int nb = (p->sub_batch > 0 && p->sub_batch < p->B) ? p->sub_batch : p->B;
   if (p->D == 2) {
     if (p->term == TERM_NONE) return fail(TFSM_E_INVALID, "no product term");
-    if (pass_id == 0) return launch_row(p, (char*)ws, 0, p->B, st);
+    if (pass_id == 0) return launch_row(p, (char*)ws, 0, nb, st);
     if (pass_id == 1) {
       if (stage < ST_PRO || stage > ST_EVAL) return fail(TFSM_E_INVALID, "bad stage");
       if (stage != ST_PRO && p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
-      return launch_col(p, (float2*)uh, (char*)ws, stage, 0, p->B, st);
+      return launch_col(p, (float2*)uh, (char*)ws, stage, 0, nb, st);
     }
   }
   if (p->D == 3) {
@@ -882,7 +888,9 @@ int tfsm_launch_pass(tfsm_plan* p, void* uh, void* ws, int32_t pass_id, int32_t 
     if (stage < ST_PRO || stage > ST_EVAL) return fail(TFSM_E_INVALID, "bad stage");
     if (stage != ST_PRO && p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
     // one whole evaluation (4 kernels) per call; pass ids 0/1 are not split for 3D
-    if (pass_id == 3) return run3d_eval(p, (float2*)uh, (char*)ws, stage, 0, p->B, st);
+    if (pass_id == 3) return run3d_eval(p, (float2*)uh, (char*)ws, stage, 0, nb, st);
+    if (pass_id >= 4 && pass_id <= 7)
+      return run3d_eval(p, (float2*)uh, (char*)ws, stage, 0, nb, st, 1 << (pass_id - 4));
   }
   if (p->D == 1 && pass_id == 2) {
     if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
