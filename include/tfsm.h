@@ -115,6 +115,15 @@ int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int
  * (_etdrk.py:15-70) and UnCachedSETDRK1-4.__init__ (_uncached.py:20-211). */
 int tfsm_plan_set_scheme(tfsm_plan* plan, int32_t scheme, double dt, int32_t n_points, double radius, void* stream);
 
+/* Re-target sample b of a plan created with per-sample coefficients (coef_batch = batch): new
+ * linear coefficients (host float[n_linear], in generator order) and, for 1D plans created with
+ * length_per_sample, a new domain length (> 0; <= 0 keeps it).  Recomputes that sample's axis
+ * table, L and, when a scheme is set, its ETD coefficients, asynchronously on the stream; other
+ * samples are untouched.  Used by the serving loop to admit a request into a free slot.
+ * Replaces register_mesh + _build_integrator (_base.py:441-526, 581-624) for one sample of a
+ * batched-parameter operator. */
+int tfsm_plan_set_sample(tfsm_plan* plan, int32_t b, const float* linear_coef, double length, void* stream);
+
 /* Copy coefficient table `slot` (0 E, 1 E2, 2..7 coef_1..coef_6) into out: float32
  * [coef_batch][modes] with modes = prod(N[:-1]) * P.  Inspection of the integrator's
  * _exp_term / _half_exp_term / _coef_i attributes (_etdrk.py:21-70, _uncached.py:30-211). */
@@ -178,6 +187,26 @@ int tfsm_solve(tfsm_plan* plan, const void* uh, void* out, void* stream);
  * 4..7 = one of those four kernels alone.  (No reference
  * counterpart: instrumentation of the engine's own kernels.) */
 int tfsm_launch_pass(tfsm_plan* plan, void* uh, void* ws, int32_t pass_id, int32_t stage, void* stream);
+
+/* Bytes of the adjoint workspace tfsm_adjoint_step needs (stage arrays, transposed-Jacobian
+ * fields and products). */
+int64_t tfsm_plan_adjoint_workspace_bytes(const tfsm_plan* plan);
+
+/* Prepare the adjoint of the plan's current scheme (ETDRK1/2, SETDRK1/2; Convection in 1D/2D):
+ * float64 tables of dE/dL, dc1/dL, dc2/dL and, when grad_term >= 0, the symbol dL/dtheta of linear
+ * term grad_term (its coefficient theta receives a gradient).  Call again after tfsm_plan_set_scheme.
+ * Replaces what torch autograd records through integrator.step (_etdrk.py:72-82,
+ * _setdrk_step.py:14-25) and the coefficient constructors (_etdrk.py:54-70, _uncached.py:91-104). */
+int tfsm_adjoint_prepare(tfsm_plan* plan, int32_t grad_term, void* stream);
+
+/* One reverse step: given the state u before a step and the adjoint ubar_next of the state after
+ * it, write the adjoint ubar of u (real inner product over the full spectrum; half spectra) and,
+ * if grad is non-null (device float64 [batch]), add each sample's derivative of the loss with
+ * respect to the prepared term's coefficient.  Recomputes N(u), the stage a and N(a) from u.
+ * ws: tfsm_plan_workspace_bytes, wsa: tfsm_plan_adjoint_workspace_bytes.
+ * Replaces torch autograd's backward through one integrator.step call. */
+int tfsm_adjoint_step(tfsm_plan* plan, void* u, const void* ubar_next, void* ubar, double* grad, void* ws, void* wsa,
+                      void* stream);
 
 /* Capture nsteps of tfsm_step (no recording) on uh/ws into a HIP graph for repeated launch.
  * (No reference counterpart: the reference launches ~116 kernels per step from Python.) */

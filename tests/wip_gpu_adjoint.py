@@ -1,0 +1,157 @@
+"""Hand-written adjoint of the ETD step (torchfsm_amd/adjoint.py, tfsm_adjoint_step) against the
+reference's autograd gradients (golden fixture) and float64 central finite differences of the
+oracle.  Tolerances: relative error <= 2e-4 for every gradient (float32 adjoint vs float64 truth)."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+TOL = 2e-4
+
+
+def _rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_adjoint_vs_reference_autograd_golden(dev):
+    """2D Burgers(nu) 32^2, C=2, B=2, SETDRK2, dt 0.01, 10 steps; loss = MSE to a target.  The
+    reference's float64 autograd gradients w.r.t. u0 and nu (tests/golden/g4)."""
+    import torchfsm_amd as tfa
+    from torchfsm_amd import pde
+    from torchfsm_amd.integrator import SETDRKIntegrator
+
+    g = golden("g4_burgers2d_grad.npz")
+    nu = torch.tensor(0.01, requires_grad=True)
+    u = torch.from_numpy(g["u0_f32"]).to(dev).requires_grad_(True)
+    op = pde.Burgers(nu)
+    op.set_integrator(SETDRKIntegrator.SETDRK2)
+    res = op.integrate(u, mesh=tfa.MeshGrid([(0, 1, 32)] * 2, device=dev), dt=0.01, step=10)
+    loss = ((res - torch.from_numpy(g["target_f32"]).to(dev)) ** 2).mean()
+    loss.backward()
+    e_out = _rel(res.detach().cpu().numpy(), g["out_f32"])
+    e_u = _rel(u.grad.cpu().numpy(), g["grad_u0_f64"])
+    e_nu = abs(float(nu.grad) - float(g["grad_nu_f64"])) / abs(float(g["grad_nu_f64"]))
+    e_u32 = _rel(u.grad.cpu().numpy(), g["grad_u0_f32"])
+    print(f"golden: out {e_out:.2e}; grad u0 vs ref f64 {e_u:.2e} (vs ref f32 {e_u32:.2e}); grad nu "
+          f"{float(nu.grad):.6f} vs {float(g['grad_nu_f64']):.6f} rel {e_nu:.2e}")
+    assert e_out <= 1e-5
+    assert e_u <= TOL and e_nu <= TOL
+
+
+def _loss64(u0, target, nu, scheme, steps, dt, n):
+    from oracle import refpath
+
+    out = refpath.integrate(refpath.burgers_terms(nu), [(0, 1, n)] * 2, u0, dt, steps, scheme=scheme)
+    return float(((out - target) ** 2).mean())
+
+
+@pytest.mark.parametrize("scheme,n,batch,steps", [("ETDRK2", 128, 16, 100), ("SETDRK2", 64, 4, 40)])
+def test_adjoint_vs_fp64_finite_differences(dev, scheme, n, batch, steps):
+    """2D Burgers(nu=0.01), C=2, dt 0.01; checkpoint every 10 steps.  d loss / d nu and the
+    directional derivatives along 3 seeded unit directions of u0, against float64 central finite
+    differences (eps = 1e-6) of the oracle run in float64 on the GPU."""
+    import torchfsm_amd as tfa
+    from torchfsm_amd import pde
+    from torchfsm_amd.integrator import ETDRKIntegrator, SETDRKIntegrator
+    from torchfsm_amd.workloads import fourier_modes_field
+
+    dt, eps = 0.01, 1e-6
+    u0 = fourier_modes_field(n, 2, 2, batch, seed=21).to(dev)
+    target = fourier_modes_field(n, 2, 2, batch, seed=22).to(dev)
+    nu = torch.tensor(0.01, device=dev, requires_grad=True)
+    u = u0.clone().requires_grad_(True)
+    op = pde.Burgers(nu)
+    op.set_integrator(ETDRKIntegrator.ETDRK2 if scheme == "ETDRK2" else SETDRKIntegrator.SETDRK2)
+    res = op.integrate(u, mesh=tfa.MeshGrid([(0, 1, n)] * 2, device=dev), dt=dt, step=steps)
+    loss = ((res - target) ** 2).mean()
+    loss.backward()
+    gnu = float(nu.grad)
+    gu = u.grad.double()
+
+    u64, t64 = u0.double(), target.double()
+    fd_nu = (_loss64(u64, t64, 0.01 + eps, scheme, steps, dt, n) - _loss64(u64, t64, 0.01 - eps, scheme, steps, dt, n)) / (2 * eps)
+    e_nu = abs(gnu - fd_nu) / abs(fd_nu)
+    print(f"{scheme} {n}^2 B={batch} {steps} steps: loss {float(loss):.6f}; d/dnu adjoint {gnu:.6e} FD {fd_nu:.6e} rel {e_nu:.2e}")
+    assert e_nu <= TOL
+    gen = torch.Generator().manual_seed(23)
+    for i in range(3):
+        d = torch.randn(u0.shape, generator=gen, dtype=torch.float64).to(dev)
+        d = d / d.norm()
+        fd = (_loss64(u64 + eps * d, t64, 0.01, scheme, steps, dt, n)
+              - _loss64(u64 - eps * d, t64, 0.01, scheme, steps, dt, n)) / (2 * eps)
+        ad = float((gu * d).sum())
+        e = abs(ad - fd) / max(abs(fd), 1e-30)
+        print(f"  direction {i}: adjoint {ad:.6e} FD {fd:.6e} rel {e:.2e}")
+        assert e <= TOL
+
+
+def test_adjoint_vs_oracle_autograd_u0(dev):
+    """ETDRK2 (config of the differentiation workload, smaller): the oracle's own autograd in float64
+    on the GPU gives the u0 gradient (its nu gradient is NaN: torch.where in _etdrk.py:54-70)."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import pde
+    from torchfsm_amd.integrator import ETDRKIntegrator
+    from torchfsm_amd.workloads import fourier_modes_field
+
+    n, batch, steps, dt = 64, 2, 30, 0.01
+    u0 = fourier_modes_field(n, 2, 2, batch, seed=31).to(dev)
+    target = fourier_modes_field(n, 2, 2, batch, seed=32).to(dev)
+    u = u0.clone().requires_grad_(True)
+    op = pde.Burgers(0.01)
+    op.set_integrator(ETDRKIntegrator.ETDRK2)
+    op.adjoint_checkpoint_every = 7
+    res = op.integrate(u, mesh=tfa.MeshGrid([(0, 1, n)] * 2, device=dev), dt=dt, step=steps)
+    ((res - target) ** 2).mean().backward()
+    nu = torch.tensor(0.01, dtype=torch.float64, device=dev, requires_grad=True)
+    u64 = u0.double().requires_grad_(True)
+    out = refpath.integrate(refpath.burgers_terms(nu), [(0, 1, n)] * 2, u64, dt, steps, scheme="ETDRK2")
+    ((out - target.double()) ** 2).mean().backward()
+    e = _rel(u.grad.cpu().numpy(), u64.grad.cpu().numpy())
+    print(f"ETDRK2 u0 grad vs oracle autograd f64: {e:.2e}; oracle nu grad {float(nu.grad)}")
+    assert e <= TOL
+
+
+def test_adjoint_1d_and_errors(dev):
+    """1D Burgers SETDRK1/ETDRK1 gradient against FD of the oracle; unsupported cases raise."""
+    import torchfsm_amd as tfa
+    from torchfsm_amd import pde
+    from torchfsm_amd.integrator import ETDRKIntegrator, SETDRKIntegrator
+
+    n, steps, dt, eps = 128, 20, 0.005, 1e-6
+    x = torch.arange(n, dtype=torch.float64) / n
+    u0 = (torch.sin(2 * math.pi * x) + 0.5).reshape(1, 1, n).repeat(3, 1, 1)
+    u0[1] *= 0.7
+    u0[2] += 0.2 * torch.cos(6 * math.pi * x)
+    target = torch.cos(2 * math.pi * x).reshape(1, 1, n).repeat(3, 1, 1)
+    for scheme in (ETDRKIntegrator.ETDRK1, SETDRKIntegrator.SETDRK1):
+        nu = torch.tensor(0.02, requires_grad=True)
+        u = u0.float().to(dev).requires_grad_(True)
+        op = pde.Burgers(nu)
+        op.set_integrator(scheme)
+        res = op.integrate(u, mesh=tfa.MeshGrid([(0, 1, n)], device=dev), dt=dt, step=steps)
+        ((res - target.float().to(dev)) ** 2).mean().backward()
+        from oracle import refpath
+
+        def L(nuv, uu):
+            o = refpath.integrate(refpath.burgers_terms(nuv), [(0, 1, n)], uu.to(dev), dt, steps, scheme=scheme.name)
+            return float(((o - target.to(dev)) ** 2).mean())
+
+        fd = (L(0.02 + eps, u0) - L(0.02 - eps, u0)) / (2 * eps)
+        e = abs(float(nu.grad) - fd) / abs(fd)
+        d = torch.randn(u0.shape, generator=torch.Generator().manual_seed(5), dtype=torch.float64)
+        d /= d.norm()
+        fdu = (L(0.02, u0 + eps * d) - L(0.02, u0 - eps * d)) / (2 * eps)
+        eu = abs(float((u.grad.double().cpu() * d).sum()) - fdu) / abs(fdu)
+        print(f"1D {scheme.name}: d/dnu rel {e:.2e}, directional rel {eu:.2e}")
+        assert e <= TOL and eu <= TOL
+    with pytest.raises(ValueError, match="gradients need one of"):
+        op = pde.Burgers(torch.tensor(0.01, requires_grad=True))
+        op.integrate(u0.float().to(dev), mesh=tfa.MeshGrid([(0, 1, n)], device=dev), dt=dt, step=2)

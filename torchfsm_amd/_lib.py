@@ -61,6 +61,7 @@ SIGNATURES = {
     "tfsm_plan_set_tuning": (ctypes.c_int, [VP, I32, I32, I32, I32, I32]),
     "tfsm_plan_set_scheme": (ctypes.c_int, [VP, I32, ctypes.c_double, I32, ctypes.c_double, VP]),
     "tfsm_plan_coefficients": (ctypes.c_int, [VP, I32, VP, VP]),
+    "tfsm_plan_set_sample": (ctypes.c_int, [VP, I32, VP, ctypes.c_double, VP]),
     "tfsm_plan_add_source": (ctypes.c_int, [VP, VP, I32, I32, ctypes.c_float, VP]),
     "tfsm_forward": (ctypes.c_int, [VP, VP, VP, I32, VP]),
     "tfsm_inverse": (ctypes.c_int, [VP, VP, VP, I32, VP, VP]),
@@ -72,6 +73,9 @@ SIGNATURES = {
     "tfsm_apply": (ctypes.c_int, [VP, VP, VP, VP, VP]),
     "tfsm_solve": (ctypes.c_int, [VP, VP, VP, VP]),
     "tfsm_launch_pass": (ctypes.c_int, [VP, VP, VP, I32, I32, VP]),
+    "tfsm_plan_adjoint_workspace_bytes": (I64, [VP]),
+    "tfsm_adjoint_prepare": (ctypes.c_int, [VP, I32, VP]),
+    "tfsm_adjoint_step": (ctypes.c_int, [VP, VP, VP, VP, VP, VP, VP, VP]),
     "tfsm_graph_create": (ctypes.c_int, [VP, VP, I32, VP, VP, ctypes.POINTER(VP)]),
     "tfsm_graph_launch": (ctypes.c_int, [VP, VP]),
     "tfsm_graph_destroy": (ctypes.c_int, [VP]),

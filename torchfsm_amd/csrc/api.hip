@@ -90,6 +90,8 @@ struct tfsm_plan {
   double rate = 2.0 / 3.0;
   int kmax[3] = {0, 0, 0};
   int nlin = 0;
+  int lin_kind[4] = {0, 0, 0, 0};
+  bool per_len = false;  // 1D plan with per-sample lengths (axis tables per row)
   int Bc = 1;
   float2* tw[3][3] = {};  // [axis][radix-plan variant]
   float* kv[3] = {nullptr, nullptr, nullptr};
@@ -99,6 +101,8 @@ struct tfsm_plan {
   float4* cpk = nullptr;  // packed per-mode coefficients of the current scheme
   int scheme = -1;
   double dt = 0.0;
+  int n_points = 16;
+  double radius = 1.0;
   float2* src = nullptr;
   int srcB = 0, srcC = 0;
   int V = 16;        // column tile of the fused 2D column kernel
@@ -110,6 +114,11 @@ struct tfsm_plan {
   int K1 = 0;  // 3D: kept ky values (2 kmax1 + 1)
   size_t off_A1 = 0, off_A2 = 0, off_P = 0, off_G = 0, off_G2 = 0, ws_bytes = 0;
   std::vector<void*> allocs;
+  // adjoint (tfsm_adjoint_prepare)
+  float* dcoef = nullptr;  // [Bc][3][modes]
+  float* sym = nullptr;    // [modes]
+  int adj_term = -2;       // linear term whose coefficient gets a gradient (-1 none; -2 not prepared)
+  int adj_scheme = -1;
 
   size_t state_elems() const { return (size_t)B * C * modes; }
   size_t src_bstride() const { return srcB > 1 ? (size_t)srcC * modes : 0; }
@@ -431,6 +440,118 @@ int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* tra
   return fail(TFSM_E_INVALID, "stepping is not available for this dimension");
 }
 
+// ---------------------------------------------------------------- adjoint helpers
+AdjArgs adj_args(tfsm_plan* p) {
+  AdjArgs a;
+  a.D = p->D;
+  a.C = p->C;
+  a.B = p->B;
+  a.npts = 1;
+  for (int d = 0; d < 3; ++d) {
+    a.n[d] = p->n[d];
+    a.kmax[d] = p->kmax[d];
+    a.kv[d] = p->kv[d];
+    if (d < p->D) a.npts *= p->n[d];
+  }
+  a.H = p->H;
+  a.P = p->P;
+  a.modes = p->modes;
+  a.coef_t = p->coef_t;
+  return a;
+}
+
+struct AdjLayout {
+  size_t Aa, N0s, T1, Jv, Ab, Nb, gpart, F, Fs, X, Q, total;
+};
+
+AdjLayout adj_layout(const tfsm_plan* p) {
+  const int nfa = 2 * p->C + p->C * p->D, nq = p->C + p->C * p->D;
+  size_t npts = 1;
+  for (int d = 0; d < p->D; ++d) npts *= p->n[d];
+  const size_t S = roundup(p->state_elems() * sizeof(float2), 256);
+  AdjLayout l;
+  size_t o = 0;
+  l.Aa = o; o += S;
+  l.N0s = o; o += S;
+  l.T1 = o; o += S;
+  l.Jv = o; o += S;
+  l.Ab = o; o += S;
+  l.Nb = o; o += S;
+  l.gpart = o; o += roundup(p->state_elems() * sizeof(float), 256);
+  l.F = o; o += roundup((size_t)p->B * nfa * p->modes * sizeof(float2), 256);
+  l.Fs = o; o += roundup((size_t)p->B * (nfa > nq ? nfa : nq) * p->modes * sizeof(float2), 256);
+  l.X = o; o += roundup((size_t)p->B * nfa * npts * sizeof(float), 256);
+  l.Q = o; o += roundup((size_t)p->B * nq * npts * sizeof(float), 256);
+  l.total = o;
+  return l;
+}
+
+// W = J_N(U)^T V (Convection, 1D / 2D)
+int adj_jt(tfsm_plan* p, const float2* U, const float2* V, float2* W, char* wsa, hipStream_t st) {
+  const AdjLayout l = adj_layout(p);
+  const AdjArgs a = adj_args(p);
+  const int nfa = 2 * p->C + p->C * p->D, nq = p->C + p->C * p->D;
+  float2* F = (float2*)(wsa + l.F);
+  float2* Fs = (float2*)(wsa + l.Fs);
+  float* X = (float*)(wsa + l.X);
+  float* Q = (float*)(wsa + l.Q);
+  HIP_TRY(launch_adj_fields(a, U, V, F, st), "adjoint fields launch");
+  const int nl = p->n[p->D - 1];
+  const float scale = (float)(1.0 / (double)a.npts);
+  if (p->D == 1) {
+    HIP_TRY(launch_rows_c2r(nl, F, (size_t)p->P, X, (size_t)nl, p->B * nfa, scale, p->tw[0][0], st), "adjoint c2r");
+    HIP_TRY(launch_adj_products(a, X, Q, st), "adjoint products launch");
+    HIP_TRY(launch_rows_r2c(nl, Q, (size_t)nl, Fs, (size_t)p->P, p->B * nq, 1.0f, p->tw[0][0], st), "adjoint r2c");
+  } else {
+    HIP_TRY(launch_cols_c2c(p->n[0], +1, F, Fs, (size_t)p->B * nfa, p->modes, (size_t)p->P, p->H, 1.0f, p->tw[0][0], st),
+            "adjoint column ifft");
+    HIP_TRY(launch_rows_c2r(nl, Fs, (size_t)p->P, X, (size_t)nl, p->B * nfa * p->n[0], scale, p->tw[1][0], st),
+            "adjoint c2r");
+    HIP_TRY(launch_adj_products(a, X, Q, st), "adjoint products launch");
+    HIP_TRY(launch_rows_r2c(nl, Q, (size_t)nl, Fs, (size_t)p->P, p->B * nq * p->n[0], 1.0f, p->tw[1][0], st),
+            "adjoint r2c");
+    HIP_TRY(launch_cols_c2c(p->n[0], -1, Fs, Fs, (size_t)p->B * nq, p->modes, (size_t)p->P, p->H, 1.0f, p->tw[0][0], st),
+            "adjoint column fft");
+  }
+  HIP_TRY(launch_adj_combine(a, Fs, W, st), "adjoint combine launch");
+  return TFSM_OK;
+}
+
+// N(U) -> workspace A1
+int eval_nonlinear(tfsm_plan* p, float2* U, char* ws, hipStream_t st) {
+  if (p->D == 1) {
+    Run1Args r = run1_args(p, U, ws);
+    r.eval_only = 1;
+    r.nsteps = 1;
+    HIP_TRY(launch_run1d(r, st), "1D evaluation launch");
+    return TFSM_OK;
+  }
+  int rc = run_eval(p, U, ws, ST_PRO, 0, p->B, st);
+  if (!rc) rc = run_eval(p, U, ws, ST_EVAL, 0, p->B, st);
+  return rc;
+}
+
+AdjStageArgs stage_args(tfsm_plan* p, int op) {
+  AdjStageArgs s;
+  s.op = op;
+  s.two_stage = (p->scheme == ETDRK2 || p->scheme == SETDRK2) ? 1 : 0;
+  s.B = p->B;
+  s.C = p->C;
+  s.P = p->P;
+  s.H = p->H;
+  s.nlast = p->n[p->D - 1];
+  s.modes = p->modes;
+  s.coef = p->coef;
+  s.coef_bstride = p->coef_bstride();
+  s.dcoef = p->dcoef;
+  s.dcoef_bstride = p->Bc > 1 ? 3 * p->modes : 0;
+  s.sym = p->sym;
+  s.X0 = s.X1 = s.X2 = s.X3 = nullptr;
+  s.Y0 = s.Y1 = nullptr;
+  s.gpart = nullptr;
+  return s;
+}
+
 }  // namespace
 
 extern "C" {
@@ -490,6 +611,8 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   p->coef_t = terms->nonlinear_coef;
   p->rate = terms->dealias_rate > 0 ? terms->dealias_rate : 2.0 / 3.0;
   p->nlin = terms->n_linear;
+  for (int t = 0; t < p->nlin; ++t) p->lin_kind[t] = terms->linear_kind[t];
+  p->per_len = D == 1 && geom->length_per_sample;
   p->Bc = Bc;
   int rc = TFSM_OK;
   for (int d = 0; d < D; ++d) {
@@ -566,6 +689,7 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
       if (e != hipSuccess) rc = hip_fail(e, "linear coefficient upload");
     }
     la.coef = dcoef;
+    for (int t = 0; t < 4; ++t) la.cval[t] = 0.f;
     if (!rc) {
       hipError_t e = launch_linear_symbol(la, st);
       if (e != hipSuccess) rc = hip_fail(e, "linear symbol launch");
@@ -661,6 +785,64 @@ int tfsm_plan_set_scheme(tfsm_plan* plan, int32_t scheme, double dt, int32_t n_p
   HIP_TRY(launch_pack_coefs(a, (hipStream_t)stream), "coefficient pack launch");
   plan->scheme = scheme;
   plan->dt = dt;
+  plan->n_points = n_points;
+  plan->radius = radius;
+  return TFSM_OK;
+}
+
+int tfsm_plan_set_sample(tfsm_plan* p, int32_t b, const float* linear_coef, double length, void* stream) {
+  if (!p) return fail(TFSM_E_INVALID, "null plan");
+  if (b < 0 || b >= p->B) return fail(TFSM_E_INVALID, "sample index out of range");
+  const int rows_per = p->per_len ? p->C : 1;
+  if (p->Bc != p->B * rows_per) return fail(TFSM_E_INVALID, "the plan has shared coefficients (coef_batch 1)");
+  if (length > 0 && !p->per_len) return fail(TFSM_E_INVALID, "per-sample lengths need a 1D plan created with them");
+  hipStream_t st = (hipStream_t)stream;
+  for (int r = b * rows_per; r < (b + 1) * rows_per; ++r) {
+    if (length > 0) {
+      const int n = p->n[0];
+      const double d = length / n;
+      HIP_TRY(launch_axis_row(p->kv[0] + (size_t)r * p->kv_bstride, n, (float)(1.0 / (n * d)), st), "axis row launch");
+    }
+    if (linear_coef || length > 0) {
+      if (!linear_coef) return fail(TFSM_E_INVALID, "a new length needs the sample's linear coefficients");
+      LinArgs la;
+      la.D = p->D;
+      for (int d = 0; d < 3; ++d) {
+        la.n[d] = p->n[d];
+        la.kv[d] = p->kv[d];
+      }
+      if (p->per_len) la.kv[0] = p->kv[0] + (size_t)r * p->kv_bstride;
+      la.H = p->H;
+      la.P = p->P;
+      la.modes = p->modes;
+      la.kv_bstride = 0;
+      la.nterm = p->nlin;
+      for (int t = 0; t < 4; ++t) {
+        la.kind[t] = t < p->nlin ? p->lin_kind[t] : 0;
+        la.cval[t] = t < p->nlin ? linear_coef[t] : 0.f;
+      }
+      la.coef = nullptr;
+      la.Bc = 1;
+      la.L = p->L + (size_t)r * p->modes;
+      HIP_TRY(launch_linear_symbol(la, st), "linear symbol launch");
+    }
+    if (p->scheme >= 0) {
+      CoefArgs a;
+      a.scheme = p->scheme;
+      a.dt = p->dt;
+      a.dt32 = (float)p->dt;
+      a.hdt32 = (float)(0.5 * p->dt);
+      a.M = p->n_points;
+      a.radius = p->radius;
+      a.modes = p->modes;
+      a.Bc = 1;
+      a.L = p->L + (size_t)r * p->modes;
+      a.coef = p->coef + (size_t)r * NCOEF * p->modes;
+      a.cpk = p->cpk + (size_t)r * 2 * p->modes;
+      HIP_TRY(launch_coefs(a, st), "coefficient launch");
+      HIP_TRY(launch_pack_coefs(a, st), "coefficient pack launch");
+    }
+  }
   return TFSM_OK;
 }
 
@@ -900,6 +1082,103 @@ int tfsm_launch_pass(tfsm_plan* p, void* uh, void* ws, int32_t pass_id, int32_t 
     return TFSM_OK;
   }
   return fail(TFSM_E_INVALID, "unknown pass for this plan");
+}
+
+int64_t tfsm_plan_adjoint_workspace_bytes(const tfsm_plan* plan) {
+  return plan ? (int64_t)adj_layout(plan).total : 0;
+}
+
+int tfsm_adjoint_prepare(tfsm_plan* p, int32_t grad_term, void* stream) {
+  if (!p) return fail(TFSM_E_INVALID, "null plan");
+  if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
+  if (p->scheme != ETDRK1 && p->scheme != ETDRK2 && p->scheme != SETDRK1 && p->scheme != SETDRK2)
+    return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint covers ETDRK1/2 and SETDRK1/2");
+  if (p->term != TERM_CONV || p->D > 2)
+    return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint covers Convection in 1D and 2D");
+  if (p->src) return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint does not cover explicit sources");
+  if (p->per_len) return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint needs shared lengths");
+  if (grad_term < -1 || grad_term >= p->nlin) return fail(TFSM_E_INVALID, "grad_term out of range");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = TFSM_OK;
+  if (!p->dcoef) rc = dev_alloc(p, &p->dcoef, (size_t)p->Bc * 3 * p->modes, "adjoint coefficient table");
+  if (!rc && !p->sym) rc = dev_alloc(p, &p->sym, p->modes, "adjoint symbol");
+  if (rc) return rc;
+  HIP_TRY(launch_adj_dcoefs(p->L, p->modes, p->Bc, p->dt, p->dcoef, st), "adjoint coefficient launch");
+  if (grad_term >= 0) HIP_TRY(launch_adj_sym(adj_args(p), p->lin_kind[grad_term], p->sym, st), "adjoint symbol launch");
+  p->adj_term = grad_term;
+  p->adj_scheme = p->scheme;
+  return TFSM_OK;
+}
+
+int tfsm_adjoint_step(tfsm_plan* p, void* u, const void* ubar_next, void* ubar, double* grad, void* ws, void* wsa,
+                      void* stream) {
+  if (!p || !u || !ubar_next || !ubar || !ws || !wsa) return fail(TFSM_E_INVALID, "null argument");
+  if (p->adj_term < -1 || p->adj_scheme != p->scheme) return fail(TFSM_E_STATE, "tfsm_adjoint_prepare not called");
+  if (ubar == ubar_next || ubar == u) return fail(TFSM_E_INVALID, "ubar must not alias the inputs");
+  hipStream_t st = (hipStream_t)stream;
+  char* W = (char*)ws;
+  char* A = (char*)wsa;
+  const AdjLayout l = adj_layout(p);
+  float2* U = (float2*)u;
+  const float2* Ubn = (const float2*)ubar_next;
+  float2* Ubo = (float2*)ubar;
+  float2* A1 = (float2*)(W + p->off_A1);
+  float2* Aa = (float2*)(A + l.Aa);
+  float2* N0s = (float2*)(A + l.N0s);
+  float2* T1 = (float2*)(A + l.T1);
+  float2* Jv = (float2*)(A + l.Jv);
+  float2* Ab = (float2*)(A + l.Ab);
+  float2* Nb = (float2*)(A + l.Nb);
+  float* gp = (grad && p->adj_term >= 0) ? (float*)(A + l.gpart) : nullptr;
+  const size_t sbytes = p->state_elems() * sizeof(float2);
+  int rc = eval_nonlinear(p, U, W, st);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(N0s, A1, sbytes, hipMemcpyDeviceToDevice, st), "adjoint copy");
+  AdjStageArgs s = stage_args(p, ADJ_PRE);
+  if (s.two_stage) {
+    s.op = ADJ_STAGE_A;
+    s.X0 = U;
+    s.X1 = N0s;
+    s.Y0 = Aa;
+    HIP_TRY(launch_adj_stage(s, st), "adjoint stage launch");
+    rc = eval_nonlinear(p, Aa, W, st);  // Na -> A1
+    if (rc) return rc;
+    s = stage_args(p, ADJ_PRE);
+    s.X0 = Ubn;
+    s.Y0 = T1;
+    HIP_TRY(launch_adj_stage(s, st), "adjoint stage launch");
+    rc = adj_jt(p, Aa, T1, Jv, A, st);
+    if (rc) return rc;
+    s = stage_args(p, ADJ_MID);
+    s.X0 = Ubn;
+    s.X1 = Jv;
+    s.X2 = A1;
+    s.X3 = N0s;
+    s.Y0 = Ab;
+    s.Y1 = Nb;
+    s.gpart = gp;
+    HIP_TRY(launch_adj_stage(s, st), "adjoint stage launch");
+    rc = adj_jt(p, U, Nb, Jv, A, st);
+    if (rc) return rc;
+    s = stage_args(p, ADJ_POST);
+    s.X0 = Ab;
+  } else {
+    s.X0 = Ubn;
+    s.Y0 = T1;
+    HIP_TRY(launch_adj_stage(s, st), "adjoint stage launch");
+    rc = adj_jt(p, U, T1, Jv, A, st);
+    if (rc) return rc;
+    s = stage_args(p, ADJ_POST);
+    s.X0 = Ubn;
+  }
+  s.X1 = Jv;
+  s.X2 = U;
+  s.X3 = N0s;
+  s.Y0 = Ubo;
+  s.gpart = gp;
+  HIP_TRY(launch_adj_stage(s, st), "adjoint stage launch");
+  if (gp) HIP_TRY(launch_adj_reduce(gp, (size_t)p->C * p->modes, p->B, grad, st), "adjoint reduce launch");
+  return TFSM_OK;
 }
 
 int tfsm_graph_create(tfsm_plan* p, void* uh, int32_t nsteps, void* ws, void* stream, tfsm_graph** out) {

@@ -350,7 +350,8 @@ struct LinArgs {
   size_t kv_bstride;       // per-sample axis tables (1D serving); 0 = shared
   int nterm;
   int kind[4];
-  const float* coef;       // [nterm][Bc]
+  const float* coef;       // [nterm][Bc]; null: use cval (one row, coefficients by value)
+  float cval[4];
   int Bc;
   float* L;                // [Bc][modes]
 };
@@ -402,9 +403,47 @@ bool step3d_supported(int n0, int n1, int n2);
 // radix-plan variant used by the 3D kernels for an axis of length n
 inline int var3(int n) { return n == 128 ? 2 : 0; }
 hipError_t launch_linear_symbol(const LinArgs& a, hipStream_t st);
+hipError_t launch_axis_row(float* kv, int n, float s, hipStream_t st);
 hipError_t launch_coefs(const CoefArgs& a, hipStream_t st);
 hipError_t launch_pack_coefs(const CoefArgs& a, hipStream_t st);
 hipError_t launch_accum_source(float2* S, const float2* X, float coef, size_t total, int first, hipStream_t st);
+// ---------------------------------------------------------------- adjoint (k_adjoint.hip)
+struct AdjArgs {
+  int D, C, B;
+  int n[3];
+  int H, P;
+  size_t modes;             // prod(n[:-1]) * P
+  size_t npts;              // prod(n)
+  int kmax[3];
+  const float* kv[3];
+  float coef_t;
+};
+enum AdjOp : int { ADJ_STAGE_A = 0, ADJ_PRE = 1, ADJ_MID = 2, ADJ_POST = 3 };
+struct AdjStageArgs {
+  int op, two_stage;
+  int B, C, P, H, nlast;
+  size_t modes;
+  const float* coef;        // [Bc][NCOEF][modes]
+  size_t coef_bstride;
+  const float* dcoef;       // [Bc][3][modes]: dE/dL, dc1/dL, dc2/dL
+  size_t dcoef_bstride;
+  const float* sym;         // dL/dtheta [modes]
+  const float2* X0;
+  const float2* X1;
+  const float2* X2;
+  const float2* X3;
+  float2* Y0;
+  float2* Y1;
+  float* gpart;             // [B][C][modes] per-mode gradient terms, or null
+};
+hipError_t launch_adj_fields(const AdjArgs& a, const float2* U, const float2* V, float2* F, hipStream_t st);
+hipError_t launch_adj_products(const AdjArgs& a, const float* X, float* Q, hipStream_t st);
+hipError_t launch_adj_combine(const AdjArgs& a, const float2* R, float2* W, hipStream_t st);
+hipError_t launch_adj_stage(const AdjStageArgs& a, hipStream_t st);
+hipError_t launch_adj_reduce(const float* gpart, size_t per, int B, double* out, hipStream_t st);
+hipError_t launch_adj_dcoefs(const float* L, size_t modes, int Bc, double dt, float* dcoef, hipStream_t st);
+hipError_t launch_adj_sym(const AdjArgs& a, int kind, float* S, hipStream_t st);
+
 hipError_t launch_apply_linear(const float2* U, const float2* Nv, const float* L, size_t l_bstride, size_t modes, int C,
                                size_t total, int have_lin, int have_nl, float2* out, hipStream_t st);
 

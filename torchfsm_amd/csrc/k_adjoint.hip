@@ -1,0 +1,284 @@
+// Adjoint of the ETD step for the Convection term (1D and 2D, C = D).
+//
+// Forward (per step, 2-stage schemes ETDRK2 / SETDRK2, _etdrk.py:72-82, _setdrk_step.py:14-25):
+//   N0 = N(u),  a = E u + c1 N0,  Na = N(a),  u' = a + c2 (Na - N0)
+// with N(u)_c = coef F[ sum_i u_i d_i u_c ], u = Re IFFT(M u^), M the 2/3 box mask
+// (_convection.py:18-46 inside the closure _base.py:375-403).
+//
+// Adjoints are taken with respect to the real inner product <x, y> = Re sum_k conj(x_k) y_k over
+// the full spectrum (half spectrum: columns k = 0 and k = N/2 weigh 1, the others 2).  With
+// p = Re IFFT(v^) (normalised inverse), the transposed Jacobian of N at u is
+//   (J_N(u)^T v^)_c = coef M F[ sum_j p_j d_c u_j - sum_i d_i (p_c u_i) ]
+// and one adjoint step maps the adjoint u'bar of u' to the adjoint of u:
+//   t1 = c2 u'bar,  abar = u'bar + J_N(a)^T t1,  N0bar = c1 abar - t1,  ubar = E abar + J_N(u)^T N0bar
+// The gradient with respect to the coefficient theta of one linear term (L = ... + theta S + ...)
+// collects  <abar, dE u + dc1 N0> + <u'bar, dc2 (Na - N0)>,  d. = (d./dL) S.
+// 1-stage schemes (ETDRK1 / SETDRK1): u' = E u + c1 N0, ubar = E u'bar + J_N(u)^T (c1 u'bar),
+// gradient <u'bar, dE u + dc1 N0>.
+#include "engine.hpp"
+
+namespace tfsm {
+
+__device__ __forceinline__ bool adj_keep(int D, const int* idx, const int* n, const int* kmax) {
+  for (int d = 0; d < D; ++d) {
+    const int i = idx[d];
+    const int k = (d == D - 1) ? i : (i < (n[d] + 1) / 2 ? i : n[d] - i);
+    if (k > kmax[d]) return false;
+  }
+  return true;
+}
+
+__device__ __forceinline__ void adj_index(const AdjArgs& a, size_t mode, int* idx) {
+  idx[a.D - 1] = (int)(mode % a.P);
+  size_t r = mode / a.P;
+  if (a.D == 3) {
+    idx[1] = (int)(r % a.n[1]);
+    r /= a.n[1];
+  }
+  if (a.D >= 2) idx[0] = (int)r;
+}
+
+// F [B][NFA][modes]: v_c, M u_c, i a_i M u_c  (NFA = 2C + C D)
+__global__ void k_adj_fields(AdjArgs a, const float2* __restrict__ U, const float2* __restrict__ V,
+                             float2* __restrict__ F) {
+  const int C = a.C, D = a.D, nfa = 2 * C + C * D;
+  const size_t total = (size_t)a.B * a.modes;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t mode = e % a.modes;
+    const size_t b = e / a.modes;
+    int idx[3] = {0, 0, 0};
+    adj_index(a, mode, idx);
+    const bool valid = idx[D - 1] < a.H;
+    const bool keep = valid && adj_keep(D, idx, a.n, a.kmax);
+    float av[3] = {0.f, 0.f, 0.f};
+    for (int d = 0; d < D; ++d) av[d] = a.kv[d][idx[d]];
+    float2* fo = F + b * nfa * a.modes + mode;
+    for (int c = 0; c < C; ++c) {
+      const size_t src = (b * C + c) * a.modes + mode;
+      const float2 v = valid ? V[src] : make_float2(0.f, 0.f);
+      const float2 u = keep ? U[src] : make_float2(0.f, 0.f);
+      fo[(size_t)c * a.modes] = v;
+      fo[(size_t)(C + c) * a.modes] = u;
+      for (int d = 0; d < D; ++d) fo[(size_t)(2 * C + c * D + d) * a.modes] = make_float2(-av[d] * u.y, av[d] * u.x);
+    }
+  }
+}
+
+// X [B][NFA][npts] physical (p_c, u_j, d_i u_c) -> Q [B][C + C D][npts]:
+//   Q_i = sum_c p_c d_i u_c,  Q_{C + c D + j} = p_c u_j
+__global__ void k_adj_products(AdjArgs a, const float* __restrict__ X, float* __restrict__ Q) {
+  const int C = a.C, D = a.D, nfa = 2 * C + C * D, nq = C + C * D;
+  const size_t total = (size_t)a.B * a.npts;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t pt = e % a.npts;
+    const size_t b = e / a.npts;
+    const float* x = X + b * nfa * a.npts + pt;
+    float* q = Q + b * nq * a.npts + pt;
+    float p[3], u[3];
+    for (int c = 0; c < C; ++c) {
+      p[c] = x[(size_t)c * a.npts];
+      u[c] = x[(size_t)(C + c) * a.npts];
+    }
+    for (int i = 0; i < D; ++i) {
+      float r = 0.f;
+      for (int c = 0; c < C; ++c) r += p[c] * x[(size_t)(2 * C + c * D + i) * a.npts];
+      q[(size_t)i * a.npts] = r;
+    }
+    for (int c = 0; c < C; ++c)
+      for (int j = 0; j < D; ++j) q[(size_t)(C + c * D + j) * a.npts] = p[c] * u[j];
+  }
+}
+
+// W_c = coef M (R_c - i sum_j a_j S_{c,j}),  R [B][C + C D][modes] spectra of Q
+__global__ void k_adj_combine(AdjArgs a, const float2* __restrict__ R, float2* __restrict__ W) {
+  const int C = a.C, D = a.D, nq = C + C * D;
+  const size_t total = (size_t)a.B * a.modes;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t mode = e % a.modes;
+    const size_t b = e / a.modes;
+    int idx[3] = {0, 0, 0};
+    adj_index(a, mode, idx);
+    const bool keep = idx[D - 1] < a.H && adj_keep(D, idx, a.n, a.kmax);
+    float av[3] = {0.f, 0.f, 0.f};
+    for (int d = 0; d < D; ++d) av[d] = a.kv[d][idx[d]];
+    const float2* r = R + b * nq * a.modes + mode;
+    for (int c = 0; c < C; ++c) {
+      float2 w = make_float2(0.f, 0.f);
+      if (keep) {
+        w = r[(size_t)c * a.modes];
+        for (int j = 0; j < D; ++j) {
+          const float2 s = r[(size_t)(C + c * D + j) * a.modes];
+          w.x += av[j] * s.y;
+          w.y -= av[j] * s.x;
+        }
+        w = make_float2(a.coef_t * w.x, a.coef_t * w.y);
+      }
+      W[(b * C + c) * a.modes + mode] = w;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- stage algebra
+__device__ __forceinline__ float2 cscale(float s, float2 x) { return make_float2(s * x.x, s * x.y); }
+__device__ __forceinline__ float rdot(float2 x, float2 y) { return x.x * y.x + x.y * y.y; }
+
+__global__ void k_adj_stage(AdjStageArgs a) {
+  const size_t total = (size_t)a.B * a.C * a.modes;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t mode = e % a.modes;
+    const size_t bc = e / a.modes;
+    const size_t b = bc / a.C;
+    const int k = (int)(mode % a.P);
+    const bool valid = k < a.H;
+    const float w = !valid ? 0.f : ((k == 0 || 2 * k == a.nlast) ? 1.f : 2.f);
+    const float* cf = a.coef + b * a.coef_bstride + mode;
+    const float E = cf[CS_E * a.modes];
+    const float c1 = cf[CS_C1 * a.modes];
+    const float c2 = cf[CS_C2 * a.modes];
+    const float2 z = make_float2(0.f, 0.f);
+    const float2 x0 = valid ? a.X0[e] : z;
+    switch (a.op) {
+      case ADJ_STAGE_A:  // a = E u + c1 N0
+        a.Y0[e] = valid ? cadd(cscale(E, x0), cscale(c1, a.X1[e])) : z;
+        break;
+      case ADJ_PRE:  // t = c u'bar (c = c2 for 2-stage, c1 for 1-stage)
+        a.Y0[e] = cscale(a.two_stage ? c2 : c1, x0);
+        break;
+      case ADJ_MID: {  // abar = u'bar + J1; N0bar = c1 abar - c2 u'bar; g = <u'bar, dc2 (Na - N0)>
+        const float2 ab = valid ? cadd(x0, a.X1[e]) : z;
+        a.Y0[e] = ab;
+        a.Y1[e] = valid ? csub(cscale(c1, ab), cscale(c2, x0)) : z;
+        float g = 0.f;
+        if (valid && a.gpart) {
+          const float* dc = a.dcoef + b * a.dcoef_bstride + mode;
+          const float s = a.sym[mode];
+          g = w * rdot(x0, cscale(dc[2 * a.modes] * s, csub(a.X2[e], a.X3[e])));
+        }
+        if (a.gpart) a.gpart[e] = g;
+        break;
+      }
+      case ADJ_POST: {  // ubar = E abar + J0; g += <abar, dE u + dc1 N0>
+        a.Y0[e] = valid ? cadd(cscale(E, x0), a.X1[e]) : z;
+        if (a.gpart) {
+          float g = 0.f;
+          if (valid) {
+            const float* dc = a.dcoef + b * a.dcoef_bstride + mode;
+            const float s = a.sym[mode];
+            const float2 d = cadd(cscale(dc[0] * s, a.X2[e]), cscale(dc[a.modes] * s, a.X3[e]));
+            g = w * rdot(x0, d);
+          }
+          a.gpart[e] = a.two_stage ? a.gpart[e] + g : g;
+        }
+        break;
+      }
+    }
+  }
+}
+
+// per-sample sum of gpart [B][per] in a fixed order (float64), added to out[b]
+__global__ void k_adj_reduce(const float* __restrict__ gpart, size_t per, double* __restrict__ out) {
+  __shared__ double s[256];
+  const size_t b = blockIdx.x;
+  double acc = 0.0;
+  for (size_t i = threadIdx.x; i < per; i += blockDim.x) acc += (double)gpart[b * per + i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[b] += s[0];
+}
+
+// d(E, c1, c2)/dL of the exact phi functions in float64: E = e^z, c1 = dt phi1(z), c2 = dt phi2(z),
+// z = L dt.  (The SETDRK contour means of _uncached.py:34-104 equal these to ~1e-14.)
+__device__ void phi_derivs(double z, double* dphi1, double* dphi2) {
+  if (fabs(z) < 0.5) {
+    // phi1' = sum_{n>=1} n z^(n-1) / (n+1)!,  phi2' = sum_{n>=1} n z^(n-1) / (n+2)!
+    double s1 = 0.0, s2 = 0.0, zp = 1.0, f1 = 2.0, f2 = 6.0;  // f1 = (n+1)!, f2 = (n+2)! at n = 1
+    for (int n = 1; n <= 22; ++n) {
+      s1 += n * zp / f1;
+      s2 += n * zp / f2;
+      zp *= z;
+      f1 *= (n + 2);
+      f2 *= (n + 3);
+    }
+    *dphi1 = s1;
+    *dphi2 = s2;
+  } else {
+    const double E = exp(z);
+    const double phi1 = (E - 1.0) / z;
+    const double phi2 = (E - 1.0 - z) / (z * z);
+    *dphi1 = (E - phi1) / z;
+    *dphi2 = (phi1 - 2.0 * phi2) / z;
+  }
+}
+
+__global__ void k_adj_dcoefs(const float* __restrict__ L, size_t modes, int Bc, double dt, float* __restrict__ dcoef) {
+  const size_t total = modes * Bc;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t mode = e % modes;
+    const size_t b = e / modes;
+    const double z = (double)L[e] * dt;
+    double d1, d2;
+    phi_derivs(z, &d1, &d2);
+    float* o = dcoef + b * 3 * modes + mode;
+    o[0] = (float)(dt * exp(z));
+    o[modes] = (float)(dt * dt * d1);
+    o[2 * modes] = (float)(dt * dt * d2);
+  }
+}
+
+// S = d L / d theta for linear term kind (Laplacian / Biharmonic / unit), float32
+__global__ void k_adj_sym(AdjArgs a, int kind, float* __restrict__ S) {
+  for (size_t mode = blockIdx.x * (size_t)blockDim.x + threadIdx.x; mode < a.modes;
+       mode += (size_t)gridDim.x * blockDim.x) {
+    int idx[3] = {0, 0, 0};
+    adj_index(a, mode, idx);
+    float lap = 0.f;
+    for (int d = 0; d < a.D; ++d) {
+      const float av = a.kv[d][idx[d]];
+      lap -= av * av;
+    }
+    float s = 1.f;
+    if (kind == LIN_LAP) s = lap;
+    else if (kind == LIN_BIHARM) s = lap * lap;
+    S[mode] = idx[a.D - 1] < a.H ? s : 0.f;
+  }
+}
+
+static unsigned grid_of(size_t n) {
+  size_t g = (n + 255) / 256;
+  return (unsigned)(g > 65536 ? 65536 : (g < 1 ? 1 : g));
+}
+
+hipError_t launch_adj_fields(const AdjArgs& a, const float2* U, const float2* V, float2* F, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_fields, dim3(grid_of((size_t)a.B * a.modes)), dim3(256), 0, st, a, U, V, F);
+  return hipGetLastError();
+}
+hipError_t launch_adj_products(const AdjArgs& a, const float* X, float* Q, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_products, dim3(grid_of((size_t)a.B * a.npts)), dim3(256), 0, st, a, X, Q);
+  return hipGetLastError();
+}
+hipError_t launch_adj_combine(const AdjArgs& a, const float2* R, float2* W, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_combine, dim3(grid_of((size_t)a.B * a.modes)), dim3(256), 0, st, a, R, W);
+  return hipGetLastError();
+}
+hipError_t launch_adj_stage(const AdjStageArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_stage, dim3(grid_of((size_t)a.B * a.C * a.modes)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+hipError_t launch_adj_reduce(const float* gpart, size_t per, int B, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_reduce, dim3(B), dim3(256), 0, st, gpart, per, out);
+  return hipGetLastError();
+}
+hipError_t launch_adj_dcoefs(const float* L, size_t modes, int Bc, double dt, float* dcoef, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_dcoefs, dim3(grid_of(modes * Bc)), dim3(256), 0, st, L, modes, Bc, dt, dcoef);
+  return hipGetLastError();
+}
+hipError_t launch_adj_sym(const AdjArgs& a, int kind, float* S, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_sym, dim3(grid_of(a.modes)), dim3(256), 0, st, a, kind, S);
+  return hipGetLastError();
+}
+
+}  // namespace tfsm

@@ -33,7 +33,7 @@ __global__ void k_linear_symbol(LinArgs a) {
         lap = (d == 0) ? g : addr(lap, g);
       }
       for (int t = 0; t < a.nterm; ++t) {
-        const float c = a.coef[t * a.Bc + b];
+        const float c = a.coef ? a.coef[t * a.Bc + b] : a.cval[t];
         float sym = 1.f;
         if (a.kind[t] == LIN_LAP) sym = lap;
         else if (a.kind[t] == LIN_BIHARM) sym = mulr(lap, lap);
@@ -247,6 +247,21 @@ static unsigned grid_for(size_t total) {
 
 hipError_t launch_linear_symbol(const LinArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(k_linear_symbol, dim3(grid_for(a.modes * a.Bc)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// axis wavenumbers of one axis table row: a_i = fl32(2 pi) * (fl32(k_i) * s), s = fl32(1 / (n d)),
+// the same float32 operations as the host table (fftfreq, mesh.py:185-191; grad, mesh.py:399-404)
+__global__ void k_axis_row(float* kv, int n, float s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = i < (n + 1) / 2 ? i : i - n;
+  const float f = (float)k * s;
+  kv[i] = (float)(2.0 * M_PI) * f;
+}
+
+hipError_t launch_axis_row(float* kv, int n, float s, hipStream_t st) {
+  hipLaunchKernelGGL(k_axis_row, dim3((n + 255) / 256), dim3(256), 0, st, kv, n, s);
   return hipGetLastError();
 }
 

@@ -159,6 +159,14 @@ class SpectralPlan:
         self.scheme = scheme
         self.dt = dt
 
+    def set_sample(self, b: int, linear_coefs: Sequence[float], length: float = 0.0):
+        """Re-target slot b of a per-sample plan: its linear coefficients (generator order) and,
+        for 1D plans with per-sample lengths, its domain length (tfsm_plan_set_sample)."""
+        vals = (ctypes.c_float * max(1, len(linear_coefs)))(*[float(c) for c in linear_coefs])
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_plan_set_sample(self._plan, int(b), ctypes.cast(vals, ctypes.c_void_p), float(length),
+                                           _stream(self.device)), "set_sample")
+
     COEF_SLOTS = {"E": 0, "E2": 1, "c1": 2, "c2": 3, "c3": 4, "c4": 5, "c5": 6, "c6": 7}
 
     def coefficients(self, name: str) -> torch.Tensor:

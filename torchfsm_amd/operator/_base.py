@@ -348,6 +348,8 @@ class OperatorLike(_MutableMixIn):
         step = int(step)
         if trajectory_recorder is not None and step <= 0:
             raise UnboundLocalError("cannot access local variable 'i' where it is not associated with a value")
+        if torch.is_grad_enabled() and self._grad_requested(u_0, u_0_fft):
+            return self._integrate_with_adjoint(u_0, u_0_fft, dt, step, trajectory_recorder, return_in_fourier)
         plan = self._prepare(x.shape[0], x.device, dt)
         try:
             uh = self._to_half(plan, u_0, u_0_fft)
@@ -389,6 +391,40 @@ class OperatorLike(_MutableMixIn):
         if return_in_fourier:
             return plan.half_to_full(uh)
         return plan.inverse(uh)
+
+    def _grad_requested(self, u_0, u_0_fft) -> bool:
+        for t in (u_0, u_0_fft):
+            if isinstance(t, Tensor) and t.requires_grad:
+                return True
+        for coef, _ in list(self._linear_cores) + list(self._nonlinear_funcs):
+            if isinstance(coef, Tensor) and coef.requires_grad:
+                return True
+        return False
+
+    def _integrate_with_adjoint(self, u_0, u_0_fft, dt, step, trajectory_recorder, return_in_fourier):
+        """integrate() under autograd: the engine's hand-written adjoint with checkpoints every
+        `adjoint_checkpoint_every` steps (default 10) replaces autograd through the step loop."""
+        from ..adjoint import ADJOINT_SCHEMES, IntegrateAdjoint
+
+        if trajectory_recorder is not None or return_in_fourier or u_0_fft is not None or u_0 is None:
+            raise ValueError("term not supported by engine: gradients are available for integrate(u_0, ...) returning "
+                             "the final field (no trajectory recorder, u_0_fft or return_in_fourier)")
+        name = self._scheme()[0]
+        if name not in ADJOINT_SCHEMES:
+            raise ValueError(f"term not supported by engine: gradients need one of {ADJOINT_SCHEMES} "
+                             f"(the operator uses {name}); call set_integrator")
+        for coef, _ in self._nonlinear_funcs:
+            if isinstance(coef, Tensor) and coef.requires_grad:
+                raise ValueError("term not supported by engine: gradient with respect to a nonlinear coefficient")
+        grad_terms = [i for i, (coef, _) in enumerate(self._linear_cores)
+                      if isinstance(coef, Tensor) and coef.requires_grad]
+        if len(grad_terms) > 1:
+            raise ValueError("term not supported by engine: gradients with respect to more than one linear coefficient")
+        gt = grad_terms[0] if grad_terms else -1
+        theta = self._linear_cores[gt][0] if gt >= 0 else torch.zeros((), device=u_0.device)
+        plan = self._prepare(u_0.shape[0], u_0.device, dt)
+        every = int(getattr(self, "adjoint_checkpoint_every", 10))
+        return IntegrateAdjoint.apply(u_0.to(torch.float32), theta, plan, int(step), max(1, every), gt)
 
     @staticmethod
     def _run_steps(plan, uh, step, progressive):
