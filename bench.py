@@ -5,8 +5,11 @@ Workloads (--workload):
   ns2d       (default) 2D incompressible Navier-Stokes, vorticity form, Kolmogorov forcing,
              512^2 grid, batch 64, nu = 1e-3 (Re = 1000), SETDRK4, dt = 1e-3.  One step = one
              SETDRK4 step of the whole batch (4 nonlinear evaluations: 4 row + 4 column kernels).
-  burgers3d  3D vector Burgers(0.01), 128^3 grid, 3 channels, batch 8, SETDRK2, dt = 5e-3.  One
-             step = 2 evaluations, each a z-row, y-forward, x-column and y-inverse kernel.
+  burgers3d  3D vector Burgers(0.01), 128^3 grid, 3 channels, batch 8, ETDRK2, dt = 5e-3.  One
+             step = 2 evaluations, each a z-row, y-forward, x-column and y-inverse kernel.  At
+             these settings the reference itself diverges for 4 of the 8 samples (advective CFL
+             above the ETDRK2 limit); the engine tracks it step for step (tests), so the output is
+             not finite after ~20 steps.  Throughput does not depend on the values.
 The metric is DOF*steps/s with DOF = B * C * prod(N), state resident in HBM.
 
 Legs:
@@ -118,8 +121,8 @@ class NS2D:
 
 
 class Burgers3D:
-    name = "burgers3d_128cubed_c3_b8_setdrk2"
-    N, B, C, NU, DT, SCHEME = 128, 8, 3, 0.01, 5e-3, "SETDRK2"
+    name = "burgers3d_128cubed_c3_b8_etdrk2"
+    N, B, C, NU, DT, SCHEME = 128, 8, 3, 0.01, 5e-3, "ETDRK2"
     default_steps = 200
     traffic_file = "k3_col_burgers128_b8_traffic.txt"
     data = "synthetic: per (b, c) 8 seeded cosines, integer |k| in [1, 4], a ~ U(-1,1), phase ~ U(0, 2pi)"
@@ -129,14 +132,14 @@ class Burgers3D:
 
         import torchfsm_amd as tfa
         from torchfsm_amd import pde
-        from torchfsm_amd.integrator import SETDRKIntegrator
+        from torchfsm_amd.integrator import ETDRKIntegrator
         from torchfsm_amd.workloads import fourier_modes_field
 
         self.dev = dev
         self.u0 = fourier_modes_field(self.N, 3, 3, self.B, seed=0)
         self.mesh_info = [(0, 1, self.N)] * 3
         op = pde.Burgers(self.NU)
-        op.set_integrator(SETDRKIntegrator.SETDRK2)
+        op.set_integrator(ETDRKIntegrator.ETDRK2)
         op.register_mesh(tfa.FourierMesh(self.mesh_info, device=dev), 3)
         self.plan = op._prepare(self.B, dev, self.DT)
         self.uh = self.plan.forward(self.u0.to(dev))
@@ -176,7 +179,7 @@ class Burgers3D:
         # bounded sample: one sample of the batch, 1 warm-up + cpu_steps timed steps
         terms = refpath.burgers_terms(self.NU)
         mesh_c = refpath.Mesh(self.mesh_info, device="cpu")
-        sch = refpath.Scheme("SETDRK2", self.DT, refpath.build_linear(mesh_c, terms, 3),
+        sch = refpath.Scheme(self.SCHEME, self.DT, refpath.build_linear(mesh_c, terms, 3),
                              refpath.build_closure(mesh_c, terms))
         uc = sch.step(mesh_c.fft(self.u0[:1]))
         t0 = time.perf_counter()
@@ -186,19 +189,78 @@ class Burgers3D:
         dof1 = self.C * self.N ** 3
         return dof1 * args.cpu_steps / t, (
             f"oracle refpath (torch CPU, MKL FFT={torch.backends.mkl.is_available()}), 1 of the {self.B} "
-            f"samples at 128^3 x 3 channels, 1 warm-up + {args.cpu_steps} timed SETDRK2 steps ({t:.1f} s)")
+            f"samples at 128^3 x 3 channels, 1 warm-up + {args.cpu_steps} timed {self.SCHEME} steps ({t:.1f} s)")
 
     def torch_gpu(self, steps):
         from oracle import refpath
 
         mesh_g = refpath.Mesh(self.mesh_info, device=self.dev)
         terms = refpath.burgers_terms(self.NU)
-        sch = refpath.Scheme("SETDRK2", self.DT, refpath.build_linear(mesh_g, terms, 3),
+        sch = refpath.Scheme(self.SCHEME, self.DT, refpath.build_linear(mesh_g, terms, 3),
                              refpath.build_closure(mesh_g, terms))
         return sch, mesh_g.fft(self.u0.to(self.dev)), self.dof
 
 
 WORKLOADS = {"ns2d": NS2D, "burgers3d": Burgers3D}
+
+SERVE_SIZES = {"ks1d": (512, 1024, 2048, 4096), "ns2d": (64, 128, 256, 512), "burgers3d": (64,)}
+
+
+def serving_leg(dev, n_requests, rate, seed=0, plain_sample=12):
+    """Mixed request stream through the continuous-batching server, then a sample of the same
+    requests solved one by one with plain integrate() calls (the per-request baseline)."""
+    import torch
+
+    from torchfsm_amd import serving
+
+    reqs = serving.synthetic_requests(n_requests, rate, seed=seed, sizes=SERVE_SIZES, steps=(100, 2000))
+    srv = serving.Server(dev)
+    t0 = time.perf_counter()
+    srv.prewarm(sorted({(r.kind, r.n) for r in reqs}))
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = srv.run(reqs)
+    t_run = time.perf_counter() - t0
+    work = {r.rid: serving.SPECS[r.kind][1] * r.n ** serving.SPECS[r.kind][0] * r.steps for r in reqs}
+    lat = serving.latency_summary(res)
+    out = {
+        "requests": len(res), "arrival_rate_per_s": rate, "makespan_s": t_run, "setup_s": t_setup,
+        "dof_steps_per_s": sum(work.values()) / t_run, "requests_per_s": len(res) / t_run,
+        "p50_latency_ms": lat["p50_ms"], "p99_latency_ms": lat["p99_ms"], "max_latency_ms": lat["max_ms"],
+        "all_finite": all(bool(torch.isfinite(r.u).all()) for r in res),
+        "mix": "KS 1D N in {512..4096}, L ~ U[32pi, 128pi], dt 0.1 SETDRK4 | NS 2D N in {64..512}, nu ~ logU(1e-3, "
+               "5e-2), dt 1e-3 SETDRK4 | 3D Burgers 64^3, nu ~ logU(1e-2, 5e-2), dt 2e-3 SETDRK2; steps ~ U{100..2000}",
+    }
+    srv.close()
+    # plain per-request solves (B = 1 plans through the public API), on a sample of the requests
+    from torchfsm_amd import field, pde
+    from torchfsm_amd.integrator import SETDRKIntegrator
+    from torchfsm_amd.mesh import MeshGrid
+
+    sample = reqs[:plain_sample]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    wsum = 0
+    for r in sample:
+        dims, ch, dt, scheme, length = serving.SPECS[r.kind]
+        if r.kind == "ks1d":
+            op, mesh = pde.KuramotoSivashinsky(), MeshGrid([(0, r.param, r.n)], device=dev)
+        elif r.kind == "ns2d":
+            mesh = MeshGrid([(0, length, r.n)] * 2, device=dev)
+            op = pde.NavierStokesVorticity(r.param, field.kolm_force(mesh.bc_mesh_grid()[1]))
+        else:
+            mesh = MeshGrid([(0, length, r.n)] * 3, device=dev)
+            op = pde.Burgers(r.param)
+        op.set_integrator(getattr(SETDRKIntegrator, scheme))
+        op.integrate(r.u0.to(dev), mesh=mesh, dt=dt, step=r.steps)
+        wsum += work[r.rid]
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    out["plain_per_request"] = {"requests": len(sample), "seconds": t_plain, "dof_steps_per_s": wsum / t_plain,
+                                "note": "first requests of the same stream, solved one at a time with "
+                                        "integrate() (plan setup included, as a per-request solve pays it)"}
+    out["throughput_vs_plain"] = out["dof_steps_per_s"] / (wsum / t_plain)
+    return out
 
 
 def time_kernel(plan, uh, pass_id, stage, reps=20):
@@ -253,6 +315,8 @@ def main():
     ap.add_argument("--row-var", type=int, default=0)
     ap.add_argument("--row-wg", type=int, default=256)
     ap.add_argument("--sub-batch", type=int, default=0)
+    ap.add_argument("--serve-requests", type=int, default=240, help="requests of the serving leg (0 = skip it)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate of the serving leg (1/s)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run 3 plain 256 MiB device copies (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -391,6 +455,15 @@ def main():
         "engine_speedup": value / tg_val,
     }
     log(f"[bench] torch-GPU baseline {tg_val:.3e} DOF*steps/s; engine speedup {value / tg_val:.2f}x")
+    if args.serve_requests > 0 and args.workload == "ns2d":
+        del sch, ug, w, plan, uh, u
+        torch.cuda.empty_cache()
+        log(f"[bench] serving: {args.serve_requests} requests at {args.serve_rate}/s")
+        result["serving"] = serving_leg(dev, args.serve_requests, args.serve_rate)
+        sv = result["serving"]
+        log(f"[bench] serving: {sv['requests']} requests in {sv['makespan_s']:.2f} s, {sv['dof_steps_per_s']:.3e} "
+            f"DOF*steps/s, p50 {sv['p50_latency_ms']:.1f} ms, p99 {sv['p99_latency_ms']:.1f} ms; "
+            f"{sv['throughput_vs_plain']:.2f}x plain per-request solves")
     print(json.dumps(result), flush=True)
 
 

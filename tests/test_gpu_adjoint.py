@@ -63,8 +63,9 @@ def test_adjoint_vs_fp64_finite_differences(dev, scheme, n, batch, steps):
     from torchfsm_amd.workloads import fourier_modes_field
 
     dt, eps = 0.01, 1e-6
-    u0 = fourier_modes_field(n, 2, 2, batch, seed=21).to(dev)
-    target = fourier_modes_field(n, 2, 2, batch, seed=22).to(dev)
+    # amplitude 0.1 x (8 cosines, |k| <= 4): max |u| < 1, advective CFL below 1 at dt 0.01
+    u0 = (0.1 * fourier_modes_field(n, 2, 2, batch, seed=21)).to(dev)
+    target = (0.1 * fourier_modes_field(n, 2, 2, batch, seed=22)).to(dev)
     nu = torch.tensor(0.01, device=dev, requires_grad=True)
     u = u0.clone().requires_grad_(True)
     op = pde.Burgers(nu)
@@ -78,7 +79,7 @@ def test_adjoint_vs_fp64_finite_differences(dev, scheme, n, batch, steps):
     u64, t64 = u0.double(), target.double()
     fd_nu = (_loss64(u64, t64, 0.01 + eps, scheme, steps, dt, n) - _loss64(u64, t64, 0.01 - eps, scheme, steps, dt, n)) / (2 * eps)
     e_nu = abs(gnu - fd_nu) / abs(fd_nu)
-    print(f"{scheme} {n}^2 B={batch} {steps} steps: loss {float(loss):.6f}; d/dnu adjoint {gnu:.6e} FD {fd_nu:.6e} rel {e_nu:.2e}")
+    print(f"{scheme} {n}^2 B={batch} {steps} steps: loss {float(loss.detach()):.6e}; d/dnu adjoint {gnu:.6e} FD {fd_nu:.6e} rel {e_nu:.2e}")
     assert e_nu <= TOL
     gen = torch.Generator().manual_seed(23)
     for i in range(3):
@@ -102,21 +103,31 @@ def test_adjoint_vs_oracle_autograd_u0(dev):
     from torchfsm_amd.workloads import fourier_modes_field
 
     n, batch, steps, dt = 64, 2, 30, 0.01
-    u0 = fourier_modes_field(n, 2, 2, batch, seed=31).to(dev)
-    target = fourier_modes_field(n, 2, 2, batch, seed=32).to(dev)
+    u0 = (0.1 * fourier_modes_field(n, 2, 2, batch, seed=31)).to(dev)
+    target = (0.1 * fourier_modes_field(n, 2, 2, batch, seed=32)).to(dev)
     u = u0.clone().requires_grad_(True)
     op = pde.Burgers(0.01)
     op.set_integrator(ETDRKIntegrator.ETDRK2)
     op.adjoint_checkpoint_every = 7
     res = op.integrate(u, mesh=tfa.MeshGrid([(0, 1, n)] * 2, device=dev), dt=dt, step=steps)
     ((res - target) ** 2).mean().backward()
-    nu = torch.tensor(0.01, dtype=torch.float64, device=dev, requires_grad=True)
     u64 = u0.double().requires_grad_(True)
-    out = refpath.integrate(refpath.burgers_terms(nu), [(0, 1, n)] * 2, u64, dt, steps, scheme="ETDRK2")
+    out = refpath.integrate(refpath.burgers_terms(0.01), [(0, 1, n)] * 2, u64, dt, steps, scheme="ETDRK2")
     ((out - target.double()) ** 2).mean().backward()
     e = _rel(u.grad.cpu().numpy(), u64.grad.cpu().numpy())
-    print(f"ETDRK2 u0 grad vs oracle autograd f64: {e:.2e}; oracle nu grad {float(nu.grad)}")
+    print(f"ETDRK2 u0 grad vs oracle autograd f64: {e:.2e}")
     assert e <= TOL
+    # with a tensor nu, the reference's ETDRK2 nu gradient is NaN (torch.where); the adjoint's is finite
+    nu = torch.tensor(0.01, dtype=torch.float64, device=dev, requires_grad=True)
+    out = refpath.integrate(refpath.burgers_terms(nu), [(0, 1, n)] * 2, u0.double(), dt, steps, scheme="ETDRK2")
+    ((out - target.double()) ** 2).mean().backward()
+    nu32 = torch.tensor(0.01, device=dev, requires_grad=True)
+    op = pde.Burgers(nu32)
+    op.set_integrator(ETDRKIntegrator.ETDRK2)
+    res = op.integrate(u0.clone(), mesh=tfa.MeshGrid([(0, 1, n)] * 2, device=dev), dt=dt, step=steps)
+    ((res - target) ** 2).mean().backward()
+    print(f"nu grad: oracle autograd {float(nu.grad)}, adjoint {float(nu32.grad):.6e}")
+    assert math.isfinite(float(nu32.grad))
 
 
 def test_adjoint_1d_and_errors(dev):

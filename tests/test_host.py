@@ -190,3 +190,49 @@ def test_workloads_are_deterministic():
     assert torch.allclose(rms, torch.ones_like(rms), atol=1e-6)
     assert abs(float(a.double().mean())) < 1e-6
     assert torch.equal(fourier_modes_field(16, 2, 2, 1, seed=1), fourier_modes_field(16, 2, 2, 1, seed=1))
+
+
+def test_serving_host_logic():
+    """Request generation, per-request linear coefficients (generator order of the PDE factories),
+    latency percentiles and the HBM estimate used for admission."""
+    from torchfsm_amd import serving
+
+    a = serving.synthetic_requests(30, rate=200.0, seed=7, sizes={"ks1d": (512,), "ns2d": (64,), "burgers3d": (32,)},
+                                   steps=(10, 20))
+    b = serving.synthetic_requests(30, rate=200.0, seed=7, sizes={"ks1d": (512,), "ns2d": (64,), "burgers3d": (32,)},
+                                   steps=(10, 20))
+    assert [(r.kind, r.n, r.steps, r.param, r.arrival) for r in a] == [(r.kind, r.n, r.steps, r.param, r.arrival) for r in b]
+    assert all(torch.equal(x.u0, y.u0) for x, y in zip(a, b))
+    assert all(a[i].arrival < a[i + 1].arrival for i in range(29))
+    for r in a:
+        assert 10 <= r.steps <= 20
+        dims, ch = serving.SPECS[r.kind][:2]
+        assert tuple(r.u0.shape) == (1, ch) + (r.n,) * dims
+        if r.kind == "ks1d":
+            assert 32 * math.pi <= r.param <= 128 * math.pi
+        elif r.kind == "ns2d":
+            assert 1e-3 <= 1.0 / r.param <= 5e-2
+        else:
+            assert 1e-2 <= r.param <= 5e-2
+    # linear coefficients in the order the PDE factories' generators produce them
+    import torchfsm_amd as tfa
+    from torchfsm_amd import field, pde
+
+    ns = pde.NavierStokesVorticity(400.0, field.kolm_force(torch.zeros(1, 1, 16, 16)))
+    ns.register_mesh(tfa.FourierMesh([(0, 2 * math.pi, 16)] * 2), 1)
+    assert [c for _, c in ns._engine_descriptor(1)[0]] == pytest.approx(serving.linear_coefs("ns2d", 400.0))
+    ks = pde.KuramotoSivashinsky()
+    ks.register_mesh(tfa.FourierMesh([(0, 32 * math.pi, 64)]), 1)
+    assert [c for _, c in ks._engine_descriptor(1)[0]] == serving.linear_coefs("ks1d", 100.0)
+    bu = pde.Burgers(0.02)
+    bu.register_mesh(tfa.FourierMesh([(0, 1, 16)] * 3), 3)
+    assert [c for _, c in bu._engine_descriptor(1)[0]] == pytest.approx(serving.linear_coefs("burgers3d", 0.02))
+    # percentiles: nearest-rank
+    res = [serving.Result(i, None, 0.0, 0.0, (i + 1) / 1000.0) for i in range(100)]
+    lat = serving.latency_summary(res)
+    assert lat["p50_ms"] == pytest.approx(50.0) and lat["p99_ms"] == pytest.approx(99.0) and lat["n"] == 100
+    # admission estimates grow with the slots and the grid
+    e1 = serving.ShapeClass.bytes_estimate("ns2d", 256, 8)
+    assert serving.ShapeClass.bytes_estimate("ns2d", 256, 16) > e1 > serving.ShapeClass.bytes_estimate("ns2d", 128, 8)
+    with pytest.raises(ValueError, match="unknown request kind"):
+        serving.linear_coefs("kdv", 1.0)

@@ -23,6 +23,8 @@ def shape_key(plan) -> str:
 
 
 def candidates(plan):
+    if plan.ndim == 3:
+        return [(16, 0, 0, 256, 0), (16, 0, 1, 256, 0)]
     if plan.ndim != 2:
         return [(16, 0, 0, 256, 0)]
     out = []

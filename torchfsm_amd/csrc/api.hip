@@ -93,7 +93,7 @@ struct tfsm_plan {
   int lin_kind[4] = {0, 0, 0, 0};
   bool per_len = false;  // 1D plan with per-sample lengths (axis tables per row)
   int Bc = 1;
-  float2* tw[3][3] = {};  // [axis][radix-plan variant]
+  float2* tw[3][4] = {};  // [axis][radix-plan variant]
   float* kv[3] = {nullptr, nullptr, nullptr};
   size_t kv_bstride = 0;
   float* L = nullptr;
@@ -180,6 +180,15 @@ bool twiddle_table(int n, int var, std::vector<float2>& h) {
     }
     return false;
   }
+  if (var == 3) {
+    switch (n) {
+      case 16: twiddle_table_t<16, 3>(h); return true;
+      case 32: twiddle_table_t<32, 3>(h); return true;
+      case 64: twiddle_table_t<64, 3>(h); return true;
+      case 128: twiddle_table_t<128, 3>(h); return true;
+      default: return false;
+    }
+  }
   switch (n) {
     case 64: twiddle_table_t<64, 1>(h); return true;
     case 128: twiddle_table_t<128, 1>(h); return true;
@@ -190,7 +199,7 @@ bool twiddle_table(int n, int var, std::vector<float2>& h) {
 }
 
 int build_twiddles(tfsm_plan* p, int d) {
-  for (int var = 0; var < 3; ++var) {
+  for (int var = 0; var < 4; ++var) {
     std::vector<float2> h;
     if (!twiddle_table(p->n[d], var, h)) {
       p->tw[d][var] = nullptr;
@@ -329,13 +338,13 @@ int run3d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hip
     r.P2 = p->P;
     r.PF2 = p->PF;
     r.kmax2 = p->kmax[2];
-    r.tw2 = p->tw[2][var3(p->n[2])];
+    r.tw2 = p->tw[2][p->row_var == 1 ? 3 : var3(p->n[2])];
     r.az = p->kv[2];
     r.G2 = (const float2*)(ws + p->off_G2);
     r.P = Pp;
     r.b0 = b0;
     r.nb = nb;
-    HIP_TRY(launch_row3d(r, st), "3D row kernel launch");
+    HIP_TRY(launch_row3d(r, p->row_var, st), "3D row kernel launch");
   }
   if (stage != ST_PRO && (parts & 2)) {
     float2* Pb = Pp + (size_t)b0 * 3 * p->modes;
@@ -748,6 +757,7 @@ int64_t tfsm_plan_inverse_scratch_bytes(const tfsm_plan* plan, int32_t nbatch) {
 int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int32_t row_var, int32_t row_wg,
                          int32_t sub_batch) {
   if (!plan) return fail(TFSM_E_INVALID, "null plan");
+  if (plan->D == 3 && (row_var < 0 || row_var > 1)) return fail(TFSM_E_INVALID, "unsupported row tuning");
   if (plan->D == 2) {
     if (!col2d_supported(plan->n[0], col_tile, col_var)) return fail(TFSM_E_INVALID, "unsupported column tuning");
     if ((plan->P % col_tile) != 0) return fail(TFSM_E_INVALID, "column tile must divide the pitch");

@@ -398,7 +398,7 @@ hipError_t launch_record(const float2* U, float2* traj, const int* rec_slot, int
 hipError_t launch_run1d(const Run1Args& a, hipStream_t st);
 hipError_t launch_col3d(const Col3Args& a, int nb, int stage, hipStream_t st);
 hipError_t launch_yinv3d(const Yinv3Args& a, hipStream_t st);
-hipError_t launch_row3d(const Row3Args& a, hipStream_t st);
+hipError_t launch_row3d(const Row3Args& a, int row_var, hipStream_t st);
 bool step3d_supported(int n0, int n1, int n2);
 // radix-plan variant used by the 3D kernels for an axis of length n
 inline int var3(int n) { return n == 128 ? 2 : 0; }

@@ -135,6 +135,11 @@ template <> struct FftPlan<256, 1> { static constexpr int RMAX = 8; static const
 template <> struct FftPlan<512, 1> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 8, 0}; };
 // VAR 2: 8 elements per thread where VAR 0 holds 16 (lower register use in the 3D kernels)
 template <> struct FftPlan<128, 2> { static constexpr int RMAX = 8; static constexpr int NST = 3; static constexpr int R[4] = {8, 4, 4, 0}; };
+// VAR 3: 4 elements per thread (radix 4, one radix-2 stage where needed): the fewest registers
+template <> struct FftPlan<16, 3> { static constexpr int RMAX = 4; static constexpr int NST = 2; static constexpr int R[4] = {4, 4, 0, 0}; };
+template <> struct FftPlan<32, 3> { static constexpr int RMAX = 4; static constexpr int NST = 3; static constexpr int R[4] = {4, 4, 2, 0}; };
+template <> struct FftPlan<64, 3> { static constexpr int RMAX = 4; static constexpr int NST = 3; static constexpr int R[4] = {4, 4, 4, 0}; };
+template <> struct FftPlan<128, 3> { static constexpr int RMAX = 4; static constexpr int NST = 4; static constexpr int R[4] = {4, 4, 4, 2}; };
 
 template <int N, int VAR = 0>
 struct Fft {

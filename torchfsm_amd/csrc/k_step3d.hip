@@ -181,6 +181,10 @@ __global__ void __launch_bounds__((Fft<N1, VAR>::NT * V)) k3_yinv(Yinv3Args a) {
 }
 
 // ---------------------------------------------------------------- z rows (products)
+// ROWS consecutive rows of one sample per workgroup (ROWS divides N0 * N1).  The kept kz of the
+// nine G2 fields of those rows are contiguous per field: they are staged into LDS (dynamic, 9 x
+// ROWS x PF2 complex) with coalesced 16-byte loads first, so all loads are in flight together,
+// and the twelve inverse transforms then read LDS only.
 template <int N2, int ROWS, int VAR>
 __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) {
   using F = Fft<N2, VAR>;
@@ -188,18 +192,29 @@ __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) 
   constexpr int SL = LdsSize<N2, 1>::slots;
   __shared__ float2 s_tw[Fft<N2, VAR>::tw_size()];
   __shared__ float2 s_buf[ROWS * SL];
+  extern __shared__ float4 s_in4[];
+  float2* s_in = reinterpret_cast<float2*>(s_in4);
   load_twiddles<N2, VAR>(s_tw, a.tw2);
-  __syncthreads();
   const int g = threadIdx.x / NT, j = threadIdx.x % NT;
-  const int row = blockIdx.x * ROWS + g;
-  const size_t nrows = (size_t)a.nb * a.N0 * a.N1;
-  const bool rv = (size_t)row < nrows;
-  const int bl = rv ? row / (a.N0 * a.N1) : 0;
-  const int xy = rv ? row % (a.N0 * a.N1) : 0;
+  const size_t row0 = (size_t)blockIdx.x * ROWS;
+  const int nxy = a.N0 * a.N1;
+  const int bl = (int)(row0 / nxy);
+  const int xy0 = (int)(row0 % nxy);
   const int b = a.b0 + bl;
+  const int PF2 = a.PF2;
+  const size_t fs = (size_t)nxy * PF2;  // one G2 field of one sample
+  {
+    const float4* src = reinterpret_cast<const float4*>(a.G2 + (size_t)b * 9 * fs + (size_t)xy0 * PF2);
+    const int per = ROWS * PF2 / 2;      // float4 per field
+    const size_t fs4 = fs / 2;
+    for (int i = threadIdx.x; i < 9 * per; i += NT * ROWS) {
+      const int f = i / per, r = i - f * per;
+      s_in4[i] = src[f * fs4 + r];
+    }
+  }
+  __syncthreads();
   float2* lds = s_buf + g * SL;
-  const size_t fs = (size_t)a.N0 * a.N1 * a.PF2;  // one G2 field of one sample
-  const float2* gb = a.G2 + (size_t)b * 9 * fs + (size_t)xy * a.PF2;
+  const float2* in = s_in + (size_t)g * PF2;  // field f of this row at in[f * ROWS * PF2 + k]
   // packed inverse transform of two fields: A = ma(fa), B = mb(fb) where the multiplier is
   // 0: 1, 1: i, 2: i a_z  (u_c = C2R(G[3c]), d_y u_c = C2R(i G[3c+1]), d_x u_c = C2R(i G[3c+2]),
   // d_z u_c = C2R(i a_z G[3c]))
@@ -211,9 +226,9 @@ __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) 
       const int k = cj ? N2 - pos : pos;
       const bool edge = (k == 0) || (k == N2 / 2);
       float2 A = make_float2(0.f, 0.f), Bv = A;
-      if (rv && k <= a.kmax2) {
+      if (k <= a.kmax2) {
         const float az = a.az[k];
-        float2 x1 = gb[(size_t)fa * fs + k], x2 = gb[(size_t)fb * fs + k];
+        const float2 x1 = in[fa * ROWS * PF2 + k], x2 = in[fb * ROWS * PF2 + k];
         A = ma == 0 ? x1 : (ma == 1 ? make_float2(-x1.y, x1.x) : make_float2(-az * x1.y, az * x1.x));
         Bv = mb == 0 ? x2 : (mb == 1 ? make_float2(-x2.y, x2.x) : make_float2(-az * x2.y, az * x2.x));
       }
@@ -257,31 +272,27 @@ __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) 
 #pragma unroll
   for (int m = 0; m < RMAX; ++m) lds[lds_slot<1>(j + m * NT, 0)] = z[m];
   __syncthreads();
-  const size_t cs = (size_t)a.N0 * a.N1 * a.P2;  // one channel of the products
-  float2* pb = a.P + (size_t)b * 3 * cs + (size_t)xy * a.P2;
-  if (rv) {
+  const size_t cs = (size_t)nxy * a.P2;  // one channel of the products
+  float2* pb = a.P + (size_t)b * 3 * cs + (size_t)(xy0 + g) * a.P2;
 #pragma unroll
-    for (int m = 0; m < RMAX; ++m) {
-      const int pos = j + m * NT;
-      if (pos <= N2 / 2) {
-        const float2 zk = z[m];
-        const float2 zm = lds[lds_slot<1>((N2 - pos) & (N2 - 1), 0)];
-        pb[pos] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-        const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);
-        pb[cs + pos] = make_float2(0.5f * d.y, -0.5f * d.x);
-      }
+  for (int m = 0; m < RMAX; ++m) {
+    const int pos = j + m * NT;
+    if (pos <= N2 / 2) {
+      const float2 zk = z[m];
+      const float2 zm = lds[lds_slot<1>((N2 - pos) & (N2 - 1), 0)];
+      pb[pos] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+      const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);
+      pb[cs + pos] = make_float2(0.5f * d.y, -0.5f * d.x);
     }
   }
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < RMAX; ++m) z[m] = make_float2(p2[m], 0.f);
   fft_wg<N2, -1, 1, VAR>(z, lds, s_tw, j, 0);
-  if (rv) {
 #pragma unroll
-    for (int m = 0; m < RMAX; ++m) {
-      const int pos = j + m * NT;
-      if (pos <= N2 / 2) pb[2 * cs + pos] = z[m];
-    }
+  for (int m = 0; m < RMAX; ++m) {
+    const int pos = j + m * NT;
+    if (pos <= N2 / 2) pb[2 * cs + pos] = z[m];
   }
 }
 
@@ -318,14 +329,24 @@ hipError_t launch_yinv3d(const Yinv3Args& a, hipStream_t st) {
   })
 }
 
-hipError_t launch_row3d(const Row3Args& a, hipStream_t st) {
+template <int NN, int VR>
+static hipError_t launch_row3d_t(const Row3Args& a, hipStream_t st) {
+  constexpr int NT = Fft<NN, VR>::NT;
+  constexpr int R = NT >= 256 ? 1 : 256 / NT;
+  const size_t rows = (size_t)a.nb * a.N0 * a.N1;
+  // whole workgroups of one sample each; staged fields in dynamic LDS
+  if ((a.N0 * a.N1) % R != 0 || (a.PF2 % 2) != 0) return hipErrorInvalidValue;
+  const size_t lds = (size_t)9 * R * a.PF2 * sizeof(float2);
+  if (lds > 96 * 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k3_row<NN, R, VR>), dim3((unsigned)(rows / R)), dim3(NT * R), lds, st, a);
+  return hipGetLastError();
+}
+
+// row_var 0: the 3D default radix plan of the axis (var3), 1: radix-4 plan (4 values per thread)
+hipError_t launch_row3d(const Row3Args& a, int row_var, hipStream_t st) {
   TFSM_SWITCH3(a.N2, {
-    constexpr int VR = NN == 128 ? 2 : 0;
-    constexpr int NT = Fft<NN, VR>::NT;
-    constexpr int R = NT >= 256 ? 1 : 256 / NT;
-    const size_t rows = (size_t)a.nb * a.N0 * a.N1;
-    hipLaunchKernelGGL((k3_row<NN, R, VR>), dim3((unsigned)((rows + R - 1) / R)), dim3(NT * R), 0, st, a);
-    return hipGetLastError();
+    if (row_var == 1) return launch_row3d_t<NN, 3>(a, st);
+    return launch_row3d_t<NN, (NN == 128 ? 2 : 0)>(a, st);
   })
 }
 

@@ -131,6 +131,7 @@ class ShapeClass:
         self.graphs: Dict[int, StepGraph] = {}
         self.counts = torch.zeros(self.slots, dtype=torch.int32, device=self.device)
         self.last_used = time.perf_counter()
+        self.reserved_bytes = self.bytes_estimate(kind, n, slots)
         torch.cuda.synchronize(self.device)
 
     @staticmethod
@@ -142,6 +143,7 @@ class ShapeClass:
 
     @property
     def nbytes(self) -> int:
+        """Device bytes actually held (plan workspace and state; tables excluded)."""
         return self.plan.ws_bytes + self.uh.numel() * 8
 
     # ------------------------------------------------------------------ slots
@@ -247,13 +249,16 @@ class Server:
         return self.slots.get(key, 8)
 
     def reserved(self) -> int:
-        return sum(c.nbytes for c in self.classes.values())
+        return sum(c.reserved_bytes for c in self.classes.values())
 
     def get_class(self, key) -> Optional[ShapeClass]:
         c = self.classes.get(key)
         if c is not None:
             return c
         need = ShapeClass.bytes_estimate(key[0], key[1], self._slots_for(key))
+        if need > self.budget:
+            raise RuntimeError(f"shape class {key} needs {need / 2**30:.2f} GiB, above the HBM budget "
+                               f"({self.budget / 2**30:.2f} GiB)")
         while self.reserved() + need > self.budget:
             idle = [k for k, v in self.classes.items() if not v.busy and all(r is None for r in v.req)]
             if not idle:
