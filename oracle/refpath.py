@@ -326,6 +326,38 @@ def apply(terms: List[Term], mesh_info, u, dealias_rate=2 / 3, return_in_fourier
     return out if return_in_fourier else mesh.ifft(out).real
 
 
+# ----------------------------------------------------------------------------- initial-condition generators
+def normalise_field(u, zero_centered=True, unit_variance=False, unit_magnitude=True):
+    """Normalisation tail of diffused_noise / truncated_fourier_series (field.py:55-62, 117-124)."""
+    dims = [i for i in range(1, u.ndim)]
+    if zero_centered:
+        u = u - u.mean(dim=dims, keepdim=True)
+    if unit_variance:
+        u = u / u.std(dim=dims, keepdim=True)
+    if unit_magnitude:
+        u = u / torch.max(u.abs().view(u.size(0), -1), dim=1).values.view([u.shape[0]] + [1] * (u.ndim - 1))
+    return u
+
+
+def diffused_noise_from(u0, mesh_info, diffusion_coef=1.0, **norm):
+    """diffused_noise (field.py:8-63) given its Gaussian draw u0: one ETDRK0 step (dt = 1) of
+    diffusion_coef * Laplacian(), then normalisation."""
+    out = integrate([(diffusion_coef, "lap", None)], mesh_info, u0, 1, 1)
+    return normalise_field(out, **norm)
+
+
+def truncated_fourier_series_from(magnitude, angle, mesh_info, freq_threshold=5, **norm):
+    """truncated_fourier_series (field.py:65-125) given its uniform draws (already scaled to the
+    amplitude and angle ranges)."""
+    mesh = Mesh(mesh_info, device=magnitude.device, dtype=magnitude.dtype)
+    noise = magnitude * torch.exp(1j * angle)
+    mask = torch.ones_like(mesh.lap().real)
+    for d in range(mesh.n_dim):  # FourierMesh.abs_low_pass_filter (mesh.py:463-479)
+        mask = mask * torch.where(mesh.bf(d).abs() > freq_threshold, 0, 1)
+    noise = noise * mask.to(dtype=mesh.dtype)
+    return normalise_field(mesh.ifft(noise).real, **norm)
+
+
 # ----------------------------------------------------------------------------- canonical operators
 def burgers_terms(nu) -> List[Term]:
     """pde.Burgers (pde.py:13-25): nu*Laplacian() - Convection()."""

@@ -225,3 +225,33 @@ def test_convergence_order(scheme, order):
         errs.append(np.abs(numref.integrate([(0, 1, n)], terms, u0, T / m, m, scheme) - ref).max())
     rates = [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
     assert abs(rates[-1] - order) < 0.5, rates
+
+
+def test_ic_generators_oracle_closed_forms():
+    """diffused_noise: before normalisation the spectrum is the draw's spectrum times
+    exp(coef * lap) (one exact heat step, dt = 1).  truncated_fourier_series: the spectrum of the
+    output is the Hermitian part of the masked coefficients, zero for |f| > threshold."""
+    g = torch.Generator().manual_seed(3)
+    mi = [(0, 4.0, 32), (0, 2.0, 16)]
+    u0 = torch.randn(2, 1, 32, 16, generator=g, dtype=torch.float64)
+    raw = R.diffused_noise_from(u0, mi, 0.05, zero_centered=False, unit_variance=False, unit_magnitude=False)
+    m = R.Mesh(mi, dtype=torch.float64)
+    want = m.ifft(m.fft(u0) * torch.exp(0.05 * m.lap())).real
+    assert torch.allclose(raw, want, rtol=0, atol=1e-12)
+    out = R.diffused_noise_from(u0, mi, 0.05)
+    assert torch.allclose(out.abs().flatten(1).amax(1), torch.ones(2, dtype=torch.float64))
+    assert out.mean(dim=(1, 2, 3)).abs().max() < 1e-12
+    mag = torch.rand(2, 1, 32, 16, generator=g, dtype=torch.float64) * 2 - 1
+    ang = torch.rand(2, 1, 32, 16, generator=g, dtype=torch.float64) * 2 * math.pi
+    t = R.truncated_fourier_series_from(mag, ang, mi, 3, zero_centered=False, unit_variance=False,
+                                        unit_magnitude=False)
+    X = mag * torch.exp(1j * ang)
+    fx = torch.fft.fftfreq(32, 4.0 / 32, dtype=torch.float64).abs().reshape(1, 1, -1, 1)
+    fy = torch.fft.fftfreq(16, 2.0 / 16, dtype=torch.float64).abs().reshape(1, 1, 1, -1)
+    X = X * ((fx <= 3) & (fy <= 3))
+    Xm = torch.roll(torch.flip(X, dims=(-2, -1)), shifts=(1, 1), dims=(-2, -1))  # X(-k)
+    herm = (X + torch.conj(Xm)) / 2
+    assert torch.allclose(torch.fft.fftn(t, dim=(-2, -1)), herm, atol=1e-12)
+    assert torch.fft.fftn(t, dim=(-2, -1))[..., ((fx > 3) | (fy > 3)).squeeze(0).squeeze(0)].abs().max() < 1e-12
+    t2 = R.truncated_fourier_series_from(mag, ang, mi, 3, unit_magnitude=False, unit_variance=True)
+    assert torch.allclose(t2.flatten(1).std(1), torch.ones(2, dtype=torch.float64))

@@ -132,6 +132,9 @@ class ShapeClass:
         self.counts = torch.zeros(self.slots, dtype=torch.int32, device=self.device)
         self.last_used = time.perf_counter()
         self.reserved_bytes = self.bytes_estimate(kind, n, slots)
+        # per-class counters: chunks, steps, slot-steps of active requests, device time (ms)
+        self.stats = {"chunks": 0, "steps": 0, "slot_steps": 0, "gpu_ms": 0.0}
+        self.ev0 = None
         torch.cuda.synchronize(self.device)
 
     @staticmethod
@@ -187,6 +190,8 @@ class ShapeClass:
         if not act:
             return 0
         with torch.cuda.stream(self.stream):
+            self.ev0 = torch.cuda.Event(enable_timing=True)
+            self.ev0.record(self.stream)
             if self.dims == 1:
                 k = min(max(self.left[b] for b in act), 1 << MAX_GRAPH_LOG2)
                 counts = [min(self.left[b], k) if b in act else 0 for b in range(self.slots)]
@@ -194,6 +199,7 @@ class ShapeClass:
                 self.plan.step_varying(self.uh, self.counts, k)
                 for b in act:
                     self.left[b] -= counts[b]
+                self.stats["slot_steps"] += sum(counts)
             else:
                 k = min(min(self.left[b] for b in act), (1 << (MAX_GRAPH_LOG2 + 1)) - 1)
                 rest = k
@@ -203,8 +209,11 @@ class ShapeClass:
                         rest -= 1 << e
                 for b in act:
                     self.left[b] -= k
-            self.event = torch.cuda.Event()
+                self.stats["slot_steps"] += k * len(act)
+            self.event = torch.cuda.Event(enable_timing=True)
             self.event.record(self.stream)
+        self.stats["chunks"] += 1
+        self.stats["steps"] += k
         self.busy = True
         self.last_used = time.perf_counter()
         return k
@@ -298,6 +307,7 @@ class Server:
                     if not c.event.query():
                         continue
                     c.busy = False
+                    c.stats["gpu_ms"] += c.ev0.elapsed_time(c.event)
                     results.extend(c.harvest(t0))
                 q = waiting.get(key)
                 if q:
