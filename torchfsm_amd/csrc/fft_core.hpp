@@ -174,8 +174,23 @@ struct LdsSize {
   static constexpr int slots = (V == 1) ? (N + N / 16) : N * (V + 1);
 };
 
+// Synchronisation between the LDS phases of a transform.  WS = false: the transform spans
+// several wavefronts (workgroup barrier).  WS = true: every transform lives inside one wavefront
+// (rows with N / RMAX <= 64 threads, 64 a multiple of it): LDS operations of one wavefront execute
+// in order, so a compiler fence at wavefront scope is all that is needed.
+template <bool WS>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (WS) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One Stockham stage (radix R, Ns = product of earlier radices).  See header comment.
-template <int N, int S, int DIR, int V, int VAR = 0>
+template <int N, int S, int DIR, int V, int VAR = 0, bool WS = false>
 __device__ __forceinline__ void fft_stage(float2 (&v)[(Fft<N, VAR>::RMAX)], float2* lds, const float2* tw, int j,
                                           int vi) {
   using F = Fft<N, VAR>;
@@ -190,7 +205,7 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[(Fft<N, VAR>::RMAX)], floa
   if constexpr (!FIRST) {
 #pragma unroll
     for (int m = 0; m < RMAX; ++m) v[m] = lds[lds_slot<V>(j + m * NT, vi)];
-    __syncthreads();
+    lds_sync<WS>();
   }
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -216,20 +231,27 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[(Fft<N, VAR>::RMAX)], floa
       for (int r = 0; r < R; ++r) lds[lds_slot<V>(idxd + r * NS, vi)] = x[r];
     }
   }
-  if constexpr (!LAST) __syncthreads();
+  if constexpr (!LAST) lds_sync<WS>();
 }
 
 // Full transform: registers (positions j + m*NT) -> registers (same positions, natural order).
 // All threads of the workgroup must call it (it contains barriers).
-template <int N, int DIR, int V, int VAR = 0>
+template <int N, int DIR, int V, int VAR = 0, bool WS = false>
 __device__ __forceinline__ void fft_wg(float2 (&v)[(Fft<N, VAR>::RMAX)], float2* lds, const float2* tw, int j,
                                        int vi) {
   using F = Fft<N, VAR>;
-  fft_stage<N, 0, DIR, V, VAR>(v, lds, tw, j, vi);
-  if constexpr (F::NST > 1) fft_stage<N, 1, DIR, V, VAR>(v, lds, tw, j, vi);
-  if constexpr (F::NST > 2) fft_stage<N, 2, DIR, V, VAR>(v, lds, tw, j, vi);
-  if constexpr (F::NST > 3) fft_stage<N, 3, DIR, V, VAR>(v, lds, tw, j, vi);
+  static_assert(!WS || (V == 1 && F::NT <= 64 && 64 % F::NT == 0), "wave-local transforms are rows of one wavefront");
+  fft_stage<N, 0, DIR, V, VAR, WS>(v, lds, tw, j, vi);
+  if constexpr (F::NST > 1) fft_stage<N, 1, DIR, V, VAR, WS>(v, lds, tw, j, vi);
+  if constexpr (F::NST > 2) fft_stage<N, 2, DIR, V, VAR, WS>(v, lds, tw, j, vi);
+  if constexpr (F::NST > 3) fft_stage<N, 3, DIR, V, VAR, WS>(v, lds, tw, j, vi);
 }
+
+// row transforms (V = 1): wave-local synchronisation whenever a row fits one wavefront
+template <int N, int VAR>
+struct RowSync {
+  static constexpr bool WS = Fft<N, VAR>::NT <= 64 && 64 % Fft<N, VAR>::NT == 0;
+};
 
 // Cooperative copy of the twiddle table into LDS (call before the first fft_wg, then barrier).
 template <int N, int VAR = 0>

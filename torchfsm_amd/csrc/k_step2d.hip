@@ -193,12 +193,12 @@ __global__ void __launch_bounds__((Fft<N1, VAR>::NT * ROWS)) k2_row(Row2Args a) 
       Bv = make_float2(-ay * f3.y, ay * f3.x);
       z2[m] = pack_pair(A, Bv, cj, edge);
     }
-    fft_wg<N1, +1, 1, VAR>(z1, lds, s_tw, j, 0);
-    fft_wg<N1, +1, 1, VAR>(z2, lds, s_tw, j, 0);
+    fft_wg<N1, +1, 1, VAR, RowSync<N1, VAR>::WS>(z1, lds, s_tw, j, 0);
+    fft_wg<N1, +1, 1, VAR, RowSync<N1, VAR>::WS>(z2, lds, s_tw, j, 0);
     float2 v[RMAX];
 #pragma unroll
     for (int m = 0; m < RMAX; ++m) v[m] = make_float2(z1[m].x * z2[m].x + z1[m].y * z2[m].y, 0.f);
-    fft_wg<N1, -1, 1, VAR>(v, lds, s_tw, j, 0);
+    fft_wg<N1, -1, 1, VAR, RowSync<N1, VAR>::WS>(v, lds, s_tw, j, 0);
     if (rv) {
       float2* p = a.P + ((size_t)b * a.N0 + x) * a.P1;
 #pragma unroll
@@ -229,9 +229,9 @@ __global__ void __launch_bounds__((Fft<N1, VAR>::NT * ROWS)) k2_row(Row2Args a) 
       z2[m] = pack_pair(make_float2(-f1.y, f1.x), make_float2(-ay * f0.y, ay * f0.x), cj, edge);  // d0u0, d1u0
       z3[m] = pack_pair(make_float2(-f3.y, f3.x), make_float2(-ay * f2.y, ay * f2.x), cj, edge);  // d0u1, d1u1
     }
-    fft_wg<N1, +1, 1, VAR>(z1, lds, s_tw, j, 0);
-    fft_wg<N1, +1, 1, VAR>(z2, lds, s_tw, j, 0);
-    fft_wg<N1, +1, 1, VAR>(z3, lds, s_tw, j, 0);
+    fft_wg<N1, +1, 1, VAR, RowSync<N1, VAR>::WS>(z1, lds, s_tw, j, 0);
+    fft_wg<N1, +1, 1, VAR, RowSync<N1, VAR>::WS>(z2, lds, s_tw, j, 0);
+    fft_wg<N1, +1, 1, VAR, RowSync<N1, VAR>::WS>(z3, lds, s_tw, j, 0);
     float2 v[RMAX];
 #pragma unroll
     for (int m = 0; m < RMAX; ++m) {
@@ -239,11 +239,11 @@ __global__ void __launch_bounds__((Fft<N1, VAR>::NT * ROWS)) k2_row(Row2Args a) 
       const float p1 = z1[m].x * z3[m].x + z1[m].y * z3[m].y;
       v[m] = make_float2(p0, p1);
     }
-    fft_wg<N1, -1, 1, VAR>(v, lds, s_tw, j, 0);
+    fft_wg<N1, -1, 1, VAR, RowSync<N1, VAR>::WS>(v, lds, s_tw, j, 0);
     // unpack two real spectra: P0 = (Z[k] + conj Z[N-k]) / 2, P1 = (Z[k] - conj Z[N-k]) / (2i)
 #pragma unroll
     for (int m = 0; m < RMAX; ++m) lds[lds_slot<1>(j + m * NT, 0)] = v[m];
-    __syncthreads();
+    lds_sync<RowSync<N1, VAR>::WS>();
     if (rv) {
       float2* p0 = a.P + (((size_t)b * 2 + 0) * a.N0 + x) * a.P1;
       float2* p1 = a.P + (((size_t)b * 2 + 1) * a.N0 + x) * a.P1;

@@ -234,7 +234,7 @@ __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) 
       }
       z[m] = pack_pair3(A, Bv, cj, edge);
     }
-    fft_wg<N2, +1, 1, VAR>(z, lds, s_tw, j, 0);
+    fft_wg<N2, +1, 1, VAR, RowSync<N2, VAR>::WS>(z, lds, s_tw, j, 0);
   };
   float u0[RMAX], u1[RMAX], u2[RMAX], p0[RMAX], p1[RMAX], p2[RMAX];
   float2 z[RMAX];
@@ -268,10 +268,10 @@ __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) 
   // forward: (p0 + i p1) packed, then p2
 #pragma unroll
   for (int m = 0; m < RMAX; ++m) z[m] = make_float2(p0[m], p1[m]);
-  fft_wg<N2, -1, 1, VAR>(z, lds, s_tw, j, 0);
+  fft_wg<N2, -1, 1, VAR, RowSync<N2, VAR>::WS>(z, lds, s_tw, j, 0);
 #pragma unroll
   for (int m = 0; m < RMAX; ++m) lds[lds_slot<1>(j + m * NT, 0)] = z[m];
-  __syncthreads();
+  lds_sync<RowSync<N2, VAR>::WS>();
   const size_t cs = (size_t)nxy * a.P2;  // one channel of the products
   float2* pb = a.P + (size_t)b * 3 * cs + (size_t)(xy0 + g) * a.P2;
 #pragma unroll
@@ -285,10 +285,10 @@ __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) 
       pb[cs + pos] = make_float2(0.5f * d.y, -0.5f * d.x);
     }
   }
-  __syncthreads();
+  lds_sync<RowSync<N2, VAR>::WS>();
 #pragma unroll
   for (int m = 0; m < RMAX; ++m) z[m] = make_float2(p2[m], 0.f);
-  fft_wg<N2, -1, 1, VAR>(z, lds, s_tw, j, 0);
+  fft_wg<N2, -1, 1, VAR, RowSync<N2, VAR>::WS>(z, lds, s_tw, j, 0);
 #pragma unroll
   for (int m = 0; m < RMAX; ++m) {
     const int pos = j + m * NT;

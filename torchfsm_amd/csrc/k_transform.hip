@@ -23,7 +23,7 @@ __global__ void __launch_bounds__(Fft<N>::NT* ROWS)
   float2 v[RMAX];
 #pragma unroll
   for (int m = 0; m < RMAX; ++m) v[m] = make_float2(rv ? in[(size_t)row * in_ld + j + m * NT] : 0.f, 0.f);
-  fft_wg<N, -1, 1>(v, s_buf + g * SL, s_tw, j, 0);
+  fft_wg<N, -1, 1, 0, RowSync<N, 0>::WS>(v, s_buf + g * SL, s_tw, j, 0);
   if (rv) {
 #pragma unroll
     for (int m = 0; m < RMAX; ++m) {
@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(Fft<N>::NT* ROWS)
     if (k == 0 || k == N / 2) x.y = 0.f;
     v[m] = cj ? make_float2(x.x, -x.y) : x;
   }
-  fft_wg<N, +1, 1>(v, s_buf + g * SL, s_tw, j, 0);
+  fft_wg<N, +1, 1, 0, RowSync<N, 0>::WS>(v, s_buf + g * SL, s_tw, j, 0);
   if (rv) {
 #pragma unroll
     for (int m = 0; m < RMAX; ++m) out[(size_t)row * out_ld + j + m * NT] = v[m].x * scale;
@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(Fft<N>::NT* ROWS)
   float2 v[RMAX];
 #pragma unroll
   for (int m = 0; m < RMAX; ++m) v[m] = rv ? in[(size_t)row * in_ld + j + m * NT] : make_float2(0.f, 0.f);
-  fft_wg<N, DIR, 1>(v, s_buf + g * SL, s_tw, j, 0);
+  fft_wg<N, DIR, 1, 0, RowSync<N, 0>::WS>(v, s_buf + g * SL, s_tw, j, 0);
   if (rv) {
 #pragma unroll
     for (int m = 0; m < RMAX; ++m) out[(size_t)row * out_ld + j + m * NT] = make_float2(v[m].x * scale, v[m].y * scale);
