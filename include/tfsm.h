@@ -32,6 +32,8 @@ extern "C" {
 #define TFSM_TERM_NONE 0
 #define TFSM_TERM_CONV 1      /* Convection: sum_i u_i d_i u_c, C == D   (_convection.py:18-46) */
 #define TFSM_TERM_VC 2        /* VorticityConvection, D == 2, C == 1     (_navier_stokes.py:27-46) */
+#define TFSM_TERM_NSP 3       /* NSPressureConvection without force: -(I - k k^T/|k|^2) applied to
+                                 Convection, C == D in {2, 3}       (_navier_stokes.py:221-262) */
 
 /* linear symbols (LinearCoef cores) */
 #define TFSM_LIN_LAP 1        /* Laplacian       (_laplacian.py:12-15)  */

@@ -21,7 +21,7 @@ import torch
 
 Term = Tuple[object, str, Optional[torch.Tensor]]
 LINEAR_KINDS = ("lap", "biharm", "unit")
-NONLINEAR_KINDS = ("conv", "vc", "src")
+NONLINEAR_KINDS = ("conv", "vc", "nsp", "src")
 
 
 class Mesh:
@@ -135,6 +135,15 @@ def vorticity_convection(u_fft, mesh: Mesh, u=None):
     return mesh.fft(ux * wx + uy * wy)
 
 
+def ns_pressure_convection(u_fft, mesh: Mesh, u=None):
+    """_NSPressureConvectionCore without external force (_navier_stokes.py:221-262):
+    p = invlap * sum_i(nabla_i * conv_i);  N = nabla * p - conv."""
+    conv = convection(u_fft, mesh, u)
+    nab = mesh.nabla_vector(1)
+    p = mesh.inv_lap() * torch.sum(nab * conv, dim=1, keepdim=True)
+    return nab * p - conv
+
+
 def source_spectrum(src: torch.Tensor):
     """_ExplicitSourceCore.__init__ (_base.py:1002-1005): fftn over all dims after (B, C)."""
     return torch.fft.fftn(src, dim=[i + 2 for i in range(src.dim() - 2)])
@@ -142,7 +151,7 @@ def source_spectrum(src: torch.Tensor):
 
 def check_terms(mesh: Mesh, terms: List[Term], n_channel: int):
     for _, kind, _ in terms:
-        if kind == "conv" and mesh.n_dim != n_channel:  # _convection.py:57-64
+        if kind in ("conv", "nsp") and mesh.n_dim != n_channel:  # _convection.py:57-64
             raise ValueError("convection operator only works for vector field with the same dimension as mesh")
         if kind == "vc" and (mesh.n_dim != 2 or n_channel != 1):  # _navier_stokes.py:56-59
             raise ValueError("Only vorticity in 2Dmesh is supported")
@@ -162,7 +171,7 @@ def build_closure(mesh: Mesh, terms: List[Term], dealias_rate=2 / 3) -> Optional
     for c, k, p in terms:
         if k == "src":
             nl.append((c, k, source_spectrum(p).to(mesh.device)))
-        elif k in ("conv", "vc"):
+        elif k in ("conv", "vc", "nsp"):
             nl.append((c, k, None))
     if not nl:
         return None
@@ -173,11 +182,11 @@ def build_closure(mesh: Mesh, terms: List[Term], dealias_rate=2 / 3) -> Optional
         d_u = None
         u = None
         for c, k, p in nl:
-            if k in ("conv", "vc"):
+            if k in ("conv", "vc", "nsp"):
                 if d_fft is None:
                     d_fft = u_fft * mesh.mask(dealias_rate)
                     d_u = mesh.ifft(d_fft).real
-                f = convection if k == "conv" else vorticity_convection
+                f = {"conv": convection, "vc": vorticity_convection, "nsp": ns_pressure_convection}[k]
                 result += c * f(d_fft, mesh, d_u)
             else:  # explicit source: no dealiasing, u computed but unused (_base.py:394-401)
                 if u is None:
@@ -362,6 +371,11 @@ def truncated_fourier_series_from(magnitude, angle, mesh_info, freq_threshold=5,
 def burgers_terms(nu) -> List[Term]:
     """pde.Burgers (pde.py:13-25): nu*Laplacian() - Convection()."""
     return [(nu, "lap", None), (-1, "conv", None)]
+
+
+def ns_velocity_terms(Re) -> List[Term]:
+    """pde.NavierStokes without force (pde.py:85-100): NSPressureConvection() + 1/Re Laplacian()."""
+    return [(1, "nsp", None), (1 / Re, "lap", None)]
 
 
 def ks_terms() -> List[Term]:

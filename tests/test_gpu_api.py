@@ -217,3 +217,37 @@ def test_linear_etdrk0_3d(dev):
     e = _rel(out, ref)
     print(f"ETDRK0 3D: rel-L2 {e:.2e}")
     assert e <= TOL
+
+
+@pytest.mark.parametrize("n,dims", [(64, 2), (128, 2), (32, 3), (64, 3)])
+def test_navier_stokes_velocity_form(dev, n, dims):
+    """pde.NavierStokes(Re) = NSPressureConvection() + Laplacian()/Re (pde.py:85-100) in 2D and 3D,
+    every launch shape of the plan, against the oracle; and one direct evaluation of the term."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import autotune, pde
+    from torchfsm_amd.workloads import fourier_modes_field
+
+    mi = [(0, 2 * math.pi, n)] * dims
+    u = fourier_modes_field(n, dims, dims, 2, seed=80 + n, length=2 * math.pi)
+    op = pde.NavierStokes(100.0)
+    out = op.integrate(u.to(dev), mesh=tfa.MeshGrid(mi, device=dev), dt=2e-3, step=10)
+    ref = refpath.integrate(refpath.ns_velocity_terms(100.0), mi, u, 2e-3, 10, scheme="SETDRK4")
+    e = _rel(out, ref)
+    print(f"NavierStokes {dims}D {n}: rel-L2 {e:.2e}")
+    assert e <= TOL
+    plan = op._prepare(2, dev, 2e-3)
+    for cand in autotune.candidates(plan):
+        plan.set_tuning(*cand)
+        uh = plan.forward(u.to(dev))
+        plan.step(uh, 10)
+        e1 = _rel(plan.inverse(uh), ref)
+        assert e1 <= TOL, (cand, e1)
+    core = tfa.NSPressureConvection()
+    got = core(u=u.to(dev), mesh=tfa.FourierMesh(mi, device=dev), return_in_fourier=True)
+    want = refpath.apply([(1, "nsp", None)], mi, u, return_in_fourier=True)
+    e2 = _rel(got, want)
+    print(f"  direct NSPressureConvection rel-L2 {e2:.2e}")
+    assert e2 <= TOL
+    with pytest.raises(ValueError, match="external force"):
+        pde.NavierStokes(100.0, force=tfa.ImplicitSource()).register_mesh(tfa.FourierMesh(mi, device=dev), dims)

@@ -255,3 +255,22 @@ def test_ic_generators_oracle_closed_forms():
     assert torch.fft.fftn(t, dim=(-2, -1))[..., ((fx > 3) | (fy > 3)).squeeze(0).squeeze(0)].abs().max() < 1e-12
     t2 = R.truncated_fourier_series_from(mag, ang, mi, 3, unit_magnitude=False, unit_variance=True)
     assert torch.allclose(t2.flatten(1).std(1), torch.ones(2, dtype=torch.float64))
+
+
+def test_ns_pressure_convection_is_the_leray_projection():
+    """NSPressureConvection (_navier_stokes.py:221-262) equals -(I - k k^T / |k|^2) conv for k != 0
+    and -conv at k = 0, so its output is divergence free: sum_i k_i N_i = 0."""
+    g = torch.Generator().manual_seed(8)
+    for mi in ([(0, 1, 16)] * 2, [(0, 1, 8), (0, 2, 8), (0, 1, 8)]):
+        D = len(mi)
+        m = R.Mesh(mi, dtype=torch.float64)
+        u = torch.randn((2, D) + tuple(x[2] for x in mi), generator=g, dtype=torch.float64)
+        uf = m.fft(u)
+        N = R.ns_pressure_convection(uf, m)
+        conv = R.convection(uf, m)
+        f = m.bf_vector().to(torch.float64)
+        f2 = (f ** 2).sum(1, keepdim=True)
+        proj = conv - f * (f * conv).sum(1, keepdim=True) / torch.where(f2 == 0, 1.0, f2)
+        assert torch.allclose(N, -proj, atol=1e-10 * float(conv.abs().max()))
+        div = (m.nabla_vector(1) * N).sum(1)
+        assert float(div.abs().max()) < 1e-9 * float(conv.abs().max())

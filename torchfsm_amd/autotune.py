@@ -24,7 +24,7 @@ def shape_key(plan) -> str:
 
 def candidates(plan):
     if plan.ndim == 3:
-        return [(16, 0, 0, 256, 0), (16, 0, 1, 256, 0)]
+        return [(16, cv, rv, 256, 0) for cv in (0, 1) for rv in (0, 1)]
     if plan.ndim != 2:
         return [(16, 0, 0, 256, 0)]
     out = []

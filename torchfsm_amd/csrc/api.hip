@@ -305,10 +305,13 @@ Col3Args col3_args(tfsm_plan* p, float2* U, char* ws, int b0, int nb) {
   c.kmax0 = p->kmax[0];
   c.kmax1 = p->kmax[1];
   c.kmax2 = p->kmax[2];
+  c.term = p->term;
   c.coef_t = p->coef_t;
   c.scale = (float)(1.0 / ((double)p->n[0] * p->n[1] * p->n[2]));
-  c.tw0 = p->tw[0][var3(p->n[0])];
+  c.tw0 = p->tw[0][p->col_var == 1 ? 3 : var3(p->n[0])];
   c.ax = p->kv[0];
+  c.ay = p->kv[1];
+  c.az = p->kv[2];
   c.P = (const float2*)(ws + p->off_P);
   c.U = U;
   c.A1 = (float2*)(ws + p->off_A1);
@@ -352,7 +355,7 @@ int run3d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hip
                             p->tw[1][0], st),
             "3D y-forward launch");
   }
-  if (parts & 4) HIP_TRY(launch_col3d(col3_args(p, U, ws, b0, nb), nb, stage, st), "3D column kernel launch");
+  if (parts & 4) HIP_TRY(launch_col3d(col3_args(p, U, ws, b0, nb), nb, stage, p->col_var, st), "3D column kernel launch");
   if (!(parts & 8)) return TFSM_OK;
   Yinv3Args y;
   y.N0 = N0;
@@ -361,13 +364,13 @@ int run3d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hip
   y.K1 = p->K1;
   y.kmax1 = p->kmax[1];
   y.kmax2 = p->kmax[2];
-  y.tw1 = p->tw[1][var3(N1)];
+  y.tw1 = p->tw[1][p->col_var == 1 ? 3 : var3(N1)];
   y.ay = p->kv[1];
   y.G1 = (const float2*)(ws + p->off_G);
   y.G2 = (float2*)(ws + p->off_G2);
   y.b0 = b0;
   y.nb = nb;
-  HIP_TRY(launch_yinv3d(y, st), "3D y-inverse launch");
+  HIP_TRY(launch_yinv3d(y, p->col_var, st), "3D y-inverse launch");
   return TFSM_OK;
 }
 
@@ -590,12 +593,15 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
       if (geom->n[d] < 16 || geom->n[d] > 128) return fail(TFSM_E_INVALID, "3D meshes must have 16..128 points per axis");
   }
   const int term = terms->nonlinear;
-  if (term != TERM_NONE && term != TERM_CONV && term != TERM_VC) return fail(TFSM_E_INVALID, "unknown nonlinear term");
-  if (term == TERM_CONV && geom->channels != D)
+  if (term != TERM_NONE && term != TERM_CONV && term != TERM_VC && term != TERM_NSP)
+    return fail(TFSM_E_INVALID, "unknown nonlinear term");
+  if ((term == TERM_CONV || term == TERM_NSP) && geom->channels != D)
     return fail(TFSM_E_INVALID, "convection operator only works for vector field with the same dimension as mesh");
+  if (term == TERM_NSP && D == 1)
+    return fail(TFSM_E_INVALID, "term not supported by engine: NSPressureConvection on a 1D mesh");
   if (term == TERM_VC && (D != 2 || geom->channels != 1))
     return fail(TFSM_E_INVALID, "Only vorticity in 2Dmesh is supported");
-  if (term == TERM_CONV && D == 3 && !step3d_supported(geom->n[0], geom->n[1], geom->n[2]))
+  if ((term == TERM_CONV || term == TERM_NSP) && D == 3 && !step3d_supported(geom->n[0], geom->n[1], geom->n[2]))
     return fail(TFSM_E_INVALID, "3D convection needs 16, 32, 64 or 128 points per axis");
   if (terms->n_linear < 0 || terms->n_linear > 4) return fail(TFSM_E_INVALID, "at most 4 linear terms");
   if (terms->n_linear > 0 && !terms->linear_coef) return fail(TFSM_E_INVALID, "linear coefficients missing");
@@ -757,7 +763,8 @@ int64_t tfsm_plan_inverse_scratch_bytes(const tfsm_plan* plan, int32_t nbatch) {
 int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int32_t row_var, int32_t row_wg,
                          int32_t sub_batch) {
   if (!plan) return fail(TFSM_E_INVALID, "null plan");
-  if (plan->D == 3 && (row_var < 0 || row_var > 1)) return fail(TFSM_E_INVALID, "unsupported row tuning");
+  if (plan->D == 3 && (row_var < 0 || row_var > 1 || col_var < 0 || col_var > 1))
+    return fail(TFSM_E_INVALID, "unsupported 3D tuning");
   if (plan->D == 2) {
     if (!col2d_supported(plan->n[0], col_tile, col_var)) return fail(TFSM_E_INVALID, "unsupported column tuning");
     if ((plan->P % col_tile) != 0) return fail(TFSM_E_INVALID, "column tile must divide the pitch");

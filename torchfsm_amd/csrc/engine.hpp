@@ -15,7 +15,7 @@
 
 namespace tfsm {
 
-enum TermKind : int { TERM_NONE = 0, TERM_CONV = 1, TERM_VC = 2 };
+enum TermKind : int { TERM_NONE = 0, TERM_CONV = 1, TERM_VC = 2, TERM_NSP = 3 };
 
 enum SchemeId : int { ETDRK0 = 0, ETDRK1 = 1, ETDRK2 = 2, SETDRK1 = 3, SETDRK2 = 4, SETDRK3 = 5, SETDRK4 = 6 };
 
@@ -239,6 +239,23 @@ __device__ __forceinline__ float2 stage_combine_v(int stage, float2 nval, float2
   }
 }
 
+// NSPressureConvection of one mode from the convection spectra conv_c (_navier_stokes.py:244-262),
+// in the reference's op order: t_i = (i a_i) conv_i, p = invlap * sum_i t_i,
+// N_c = (i a_c) p - conv_c, with invlap = 1 / lap (1 at lap = 0), lap = sum_i -(a_i a_i).
+template <int C>
+__device__ __forceinline__ void nsp_project(float2 (&cv)[C], const float (&av)[C]) {
+  float lap = -mulr(av[0], av[0]);
+#pragma unroll
+  for (int i = 1; i < C; ++i) lap = addr(lap, -mulr(av[i], av[i]));
+  const float il = (lap == 0.f) ? 1.0f : __fdiv_rn(1.0f, lap);
+  float2 sum = make_float2(-mulr(av[0], cv[0].y), mulr(av[0], cv[0].x));
+#pragma unroll
+  for (int i = 1; i < C; ++i) sum = radd(sum, make_float2(-mulr(av[i], cv[i].y), mulr(av[i], cv[i].x)));
+  const float2 p = make_float2(mulr(il, sum.x), mulr(il, sum.y));
+#pragma unroll
+  for (int c = 0; c < C; ++c) cv[c] = rsub(make_float2(-mulr(av[c], p.y), mulr(av[c], p.x)), cv[c]);
+}
+
 // Nonlinear value of one mode: N = coef_t * F + S   (S = sum of coef_s * source spectra)
 __device__ __forceinline__ float2 nl_value(float coef_t, float2 f, const float2* __restrict__ src, size_t si) {
   float2 n = rscale(coef_t, f);
@@ -283,9 +300,12 @@ struct Row2Args {
 struct Col3Args {
   int B, N0, N1, N2, H2, P2, PF2, K1;
   int kmax0, kmax1, kmax2;
+  int term;                   // TERM_CONV or TERM_NSP
   float coef_t, scale;
   const float2* tw0;
   const float* ax;
+  const float* ay;
+  const float* az;
   const float2* P;            // y-forward products [B][3][N0][N1][P2]
   float2* U;
   float2* A1;
@@ -396,8 +416,8 @@ hipError_t launch_linear_steps(float2* U, const float* E, size_t coef_bstride, s
 hipError_t launch_record(const float2* U, float2* traj, const int* rec_slot, int step, int B, size_t per_sample,
                          int T, hipStream_t st);
 hipError_t launch_run1d(const Run1Args& a, hipStream_t st);
-hipError_t launch_col3d(const Col3Args& a, int nb, int stage, hipStream_t st);
-hipError_t launch_yinv3d(const Yinv3Args& a, hipStream_t st);
+hipError_t launch_col3d(const Col3Args& a, int nb, int stage, int col_var, hipStream_t st);
+hipError_t launch_yinv3d(const Yinv3Args& a, int col_var, hipStream_t st);
 hipError_t launch_row3d(const Row3Args& a, int row_var, hipStream_t st);
 bool step3d_supported(int n0, int n1, int n2);
 // radix-plan variant used by the 3D kernels for an axis of length n

@@ -37,6 +37,71 @@ template <>
 struct Term2<TERM_CONV> {
   static constexpr int NC = 2, NF = 4;
 };
+template <>
+struct Term2<TERM_NSP> {  // velocity-form Navier-Stokes: Convection fields, projected spectrum
+  static constexpr int NC = 2, NF = 4;
+};
+
+// Velocity-form Navier-Stokes (NSPressureConvection): both channels' convection spectra are
+// transformed first, projected mode by mode (nsp_project), then combined channel by channel.
+template <int N0, int V, int VAR>
+__device__ __forceinline__ void col_nsp_combine(const Col2Args& a, int stage, float2 (&s)[2][Fft<N0, VAR>::RMAX],
+                                                float2* s_buf, const float2* s_tw, int b, int ky, bool colv, int j,
+                                                int vi) {
+  using F = Fft<N0, VAR>;
+  constexpr int RMAX = F::RMAX, NT = F::NT;
+  const size_t plane = (size_t)N0 * a.P1;
+  const float2 z = make_float2(0.f, 0.f);
+  if (stage == ST_PRO) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) s[c][m] = colv ? a.U[((size_t)b * 2 + c) * plane + (size_t)(j + m * NT) * a.P1 + ky] : z;
+    return;
+  }
+  float2 v[2][RMAX];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) v[c][m] = colv ? a.P[((size_t)b * 2 + c) * plane + (size_t)(j + m * NT) * a.P1 + ky] : z;
+    fft_wg<N0, -1, V, VAR>(v[c], s_buf, s_tw, j, vi);
+  }
+  const float ayv = colv ? a.ay[ky] : 0.f;
+#pragma unroll
+  for (int m = 0; m < RMAX; ++m) {
+    float2 cv[2] = {v[0][m], v[1][m]};
+    const float av[2] = {a.ax[j + m * NT], ayv};
+    nsp_project<2>(cv, av);
+    v[0][m] = cv[0];
+    v[1][m] = cv[1];
+  }
+  const float4* cpk = a.cpk + (size_t)b * a.cpk_bstride;
+  const int chalf = stage_coef_half(stage);
+  const bool rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
+  const int tgt = stage_target(stage);
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const size_t base = ((size_t)b * 2 + c) * plane;
+    const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride : nullptr;
+    float2* dst = tgt == 0 ? a.U : (tgt == 1 ? a.A1 : a.A2);
+    float2 wv[RMAX];
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) {
+      const size_t ci = (size_t)(j + m * NT) * a.P1 + ky;
+      const float2 uu = colv ? a.U[base + ci] : z;
+      const float2 q1 = (colv && rd1) ? a.A1[base + ci] : z;
+      const float2 q2 = (colv && rd2) ? a.A2[base + ci] : z;
+      const float4 cc = colv ? cpk[ci * 2 + chalf] : make_float4(0.f, 0.f, 0.f, 0.f);
+      float2 nv = rscale(a.coef_t, v[c][m]);
+      if (src && colv) nv = radd(nv, src[ci]);
+      s[c][m] = colv ? stage_combine_v(stage, nv, uu, q1, q2, cc, &wv[m]) : z;
+    }
+    if (colv) {
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) dst[base + (size_t)(j + m * NT) * a.P1 + ky] = wv[m];
+    }
+  }
+}
 
 template <int N0, int V, int TERM, int VAR>
 __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int stage) {
@@ -71,6 +136,9 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
   const bool rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
   const int tgt = stage_target(stage);
   float2 s[NC][RMAX];
+  if constexpr (TERM == TERM_NSP) {
+    col_nsp_combine<N0, V, VAR>(a, stage, s, s_buf, s_tw, b, ky, colv, j, vi);
+  } else {
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const size_t base = ((size_t)b * NC + c) * plane;
@@ -106,6 +174,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
         for (int m = 0; m < RMAX; ++m) dst[base + (size_t)(j + m * NT) * a.P1 + ky] = wv[m];
       }
     }
+  }
   }
   if (tile * V > a.kmax1) return;  // uniform per workgroup: no kept column in this tile
   const float ay = colv ? a.ay[ky] : 0.f;
@@ -313,12 +382,15 @@ static hipError_t launch_row_n(const Row2Args& a, int var, int wgt, hipStream_t 
 hipError_t launch_col2d(int term, const Col2Args& a, int nb, int stage, int v, int var, hipStream_t st) {
   if (term == TERM_VC) return launch_col_n<TERM_VC>(a, nb, stage, v, var, st);
   if (term == TERM_CONV) return launch_col_n<TERM_CONV>(a, nb, stage, v, var, st);
+  if (term == TERM_NSP) return launch_col_n<TERM_NSP>(a, nb, stage, v, var, st);
   return hipErrorInvalidValue;
 }
 
+// the velocity-form Navier-Stokes term has the products of Convection (its projection is applied
+// in the column kernel), so it shares Convection's row kernels
 hipError_t launch_row2d(int term, const Row2Args& a, int var, int wgt, hipStream_t st) {
   if (term == TERM_VC) return launch_row_n<TERM_VC>(a, var, wgt, st);
-  if (term == TERM_CONV) return launch_row_n<TERM_CONV>(a, var, wgt, st);
+  if (term == TERM_CONV || term == TERM_NSP) return launch_row_n<TERM_CONV>(a, var, wgt, st);
   return hipErrorInvalidValue;
 }
 

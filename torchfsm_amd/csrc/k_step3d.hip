@@ -1,4 +1,6 @@
-// 3D fused ETD evaluation kernels for Convection with C = D = 3 (3D vector Burgers).
+// 3D fused ETD evaluation kernels for Convection with C = D = 3 (3D vector Burgers) and for the
+// velocity-form Navier-Stokes term (NSPressureConvection: the same fields and products, the
+// projection applied to the convection spectrum in the x-column kernel).
 //
 // One nonlinear evaluation = four passes over the half spectrum [B][3][N0][N1][P2]:
 //   k3_row   z rows: 12 C2R (packed in pairs) of the y-inverse fields -> the three products
@@ -33,7 +35,7 @@ __device__ __forceinline__ int ky_compress(int ky, int N1, int kmax1) {
 }
 
 // ---------------------------------------------------------------- x columns (fused combine)
-template <int N0, int V, int VAR>
+template <int N0, int V, int VAR, int TERM>
 __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int stage) {
   using F = Fft<N0, VAR>;
   constexpr int RMAX = F::RMAX, NT = F::NT;
@@ -68,6 +70,48 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
       const size_t base = ((size_t)b * 3 + c) * plane + coff;
 #pragma unroll
       for (int m = 0; m < RMAX; ++m) s[c][m] = colv ? a.U[base + (size_t)(j + m * NT) * inner] : make_float2(0.f, 0.f);
+    }
+  } else if constexpr (TERM == TERM_NSP) {
+    // velocity-form Navier-Stokes: the three convection spectra first, projected mode by mode
+    float2 v3[3][RMAX];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t base = ((size_t)b * 3 + c) * plane + coff;
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) v3[c][m] = colv ? a.P[base + (size_t)(j + m * NT) * inner] : make_float2(0.f, 0.f);
+      fft_wg<N0, -1, V, VAR>(v3[c], s_buf, s_tw, j, vi);
+    }
+    const float ayv = a.ay[ky], azv = colv ? a.az[kz] : 0.f;
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) {
+      float2 cv[3] = {v3[0][m], v3[1][m], v3[2][m]};
+      const float av[3] = {a.ax[j + m * NT], ayv, azv};
+      nsp_project<3>(cv, av);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v3[c][m] = cv[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t base = ((size_t)b * 3 + c) * plane + coff;
+      const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride + coff : nullptr;
+      float2* dst = tgt == 0 ? a.U : (tgt == 1 ? a.A1 : a.A2);
+      float2 wv[RMAX];
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) {
+        const size_t xo = (size_t)(j + m * NT) * inner;
+        const float2 z = make_float2(0.f, 0.f);
+        const float2 uu = colv ? a.U[base + xo] : z;
+        const float2 q1 = (colv && rd1) ? a.A1[base + xo] : z;
+        const float2 q2 = (colv && rd2) ? a.A2[base + xo] : z;
+        const float4 cc = colv ? cpk[(coff + xo) * 2 + chalf] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float2 nv = rscale(a.coef_t, v3[c][m]);
+        if (src && colv) nv = radd(nv, src[xo]);
+        s[c][m] = colv ? stage_combine_v(stage, nv, uu, q1, q2, cc, &wv[m]) : z;
+      }
+      if (colv) {
+#pragma unroll
+        for (int m = 0; m < RMAX; ++m) dst[base + (size_t)(j + m * NT) * inner] = wv[m];
+      }
     }
   } else {
 #pragma unroll
@@ -306,26 +350,41 @@ __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) 
     default: return hipErrorInvalidValue;              \
   }
 
-hipError_t launch_col3d(const Col3Args& a, int nb, int stage, hipStream_t st) {
+template <int NN, int VR>
+static hipError_t launch_col3d_t(const Col3Args& a, int nb, int stage, hipStream_t st) {
+  constexpr int V = 16;
+  const int ntiles = a.N1 * (a.P2 / V);
+  const int slots = (ntiles + 7) / 8;
+  Col3Args b = a;
+  b.nb = nb;
+  if (a.term == TERM_NSP)
+    hipLaunchKernelGGL((k3_col<NN, V, VR, TERM_NSP>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b, stage);
+  else
+    hipLaunchKernelGGL((k3_col<NN, V, VR, TERM_CONV>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b,
+                       stage);
+  return hipGetLastError();
+}
+
+template <int NN, int VR>
+static hipError_t launch_yinv3d_t(const Yinv3Args& a, hipStream_t st) {
+  constexpr int V = 16;
+  const int nzt = a.PF2 / V;
+  hipLaunchKernelGGL((k3_yinv<NN, V, VR>), dim3(nzt * a.N0 * a.nb), dim3(Fft<NN, VR>::NT * V), 0, st, a);
+  return hipGetLastError();
+}
+
+// col_var 0: the 3D default radix plan of the axis (var3), 1: radix-4 plan (4 values per thread)
+hipError_t launch_col3d(const Col3Args& a, int nb, int stage, int col_var, hipStream_t st) {
   TFSM_SWITCH3(a.N0, {
-    constexpr int V = 16;
-    constexpr int VR = NN == 128 ? 2 : 0;
-    const int ntiles = a.N1 * (a.P2 / V);
-    const int slots = (ntiles + 7) / 8;
-    Col3Args b = a;
-    b.nb = nb;
-    hipLaunchKernelGGL((k3_col<NN, V, VR>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b, stage);
-    return hipGetLastError();
+    if (col_var == 1) return launch_col3d_t<NN, 3>(a, nb, stage, st);
+    return launch_col3d_t<NN, (NN == 128 ? 2 : 0)>(a, nb, stage, st);
   })
 }
 
-hipError_t launch_yinv3d(const Yinv3Args& a, hipStream_t st) {
+hipError_t launch_yinv3d(const Yinv3Args& a, int col_var, hipStream_t st) {
   TFSM_SWITCH3(a.N1, {
-    constexpr int V = 16;
-    constexpr int VR = NN == 128 ? 2 : 0;
-    const int nzt = a.PF2 / V;
-    hipLaunchKernelGGL((k3_yinv<NN, V, VR>), dim3(nzt * a.N0 * a.nb), dim3(Fft<NN, VR>::NT * V), 0, st, a);
-    return hipGetLastError();
+    if (col_var == 1) return launch_yinv3d_t<NN, 3>(a, st);
+    return launch_yinv3d_t<NN, (NN == 128 ? 2 : 0)>(a, st);
   })
 }
 

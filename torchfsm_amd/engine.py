@@ -57,7 +57,7 @@ class SpectralPlan:
     """
 
     KIND = {"lap": _lib.LIN_LAP, "biharm": _lib.LIN_BIHARM, "unit": _lib.LIN_UNIT}
-    TERM = {None: _lib.TERM_NONE, "conv": _lib.TERM_CONV, "vc": _lib.TERM_VC}
+    TERM = {None: _lib.TERM_NONE, "conv": _lib.TERM_CONV, "vc": _lib.TERM_VC, "nsp": _lib.TERM_NSP}
 
     def __init__(self, mesh_info, batch, channels, nonlinear=None, nonlinear_coef=0.0, linear=(), device="cuda",
                  dealias_rate=2 / 3, lengths_per_sample: Optional[Sequence[float]] = None):

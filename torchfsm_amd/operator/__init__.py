@@ -11,7 +11,7 @@ from ._base import (
     OperatorLike,
     check_value_with_mesh,
 )
-from ._cores import Biharmonic, Convection, ImplicitSource, Laplacian, VorticityConvection
+from ._cores import Biharmonic, Convection, ImplicitSource, Laplacian, NSPressureConvection, VorticityConvection
 
 __all__ = [
     "Operator",
@@ -27,5 +27,6 @@ __all__ = [
     "Biharmonic",
     "Convection",
     "VorticityConvection",
+    "NSPressureConvection",
     "check_value_with_mesh",
 ]

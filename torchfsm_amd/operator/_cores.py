@@ -136,3 +136,37 @@ class VorticityConvection(NonlinearOperator):
 
     def __init__(self) -> None:
         super().__init__(_VorticityConvectionGenerator())
+
+
+class _NSPressureConvectionCore(NonlinearFunc):
+    """Velocity-form Navier-Stokes nonlinear term without external force
+    (_navier_stokes.py:221-262): p = lap^-1 (nabla . conv), N = nabla p - conv, conv = (u . grad) u."""
+
+    _engine_kind = "nsp"
+
+    def __init__(self) -> None:
+        super().__init__()
+
+    def __call__(self, u_fft, f_mesh: FourierMesh, u=None):
+        return _core_direct("nsp", u_fft, f_mesh)
+
+
+class _NSPressureConvectionGenerator(CoreGenerator):
+    def __init__(self, external_force=None) -> None:
+        self.external_force = external_force
+
+    def __call__(self, f_mesh: FourierMesh, n_channel: int) -> NonlinearFunc:
+        if self.external_force is not None:
+            raise ValueError("term not supported by engine: NSPressureConvection with an external force "
+                             "(add the force to the operator instead)")
+        if f_mesh.n_dim != n_channel or f_mesh.n_dim == 1:
+            raise ValueError("term not supported by engine: NSPressureConvection needs a 2D or 3D velocity field "
+                             "with one channel per dimension")
+        return _NSPressureConvectionCore()
+
+
+class NSPressureConvection(NonlinearOperator):
+    """-grad(lap^-1 div((u . grad) u - f)) - (u . grad) u + f with f = None (_navier_stokes.py:265-280)."""
+
+    def __init__(self, external_force=None) -> None:
+        super().__init__(_NSPressureConvectionGenerator(external_force))

@@ -2,7 +2,7 @@
 
 from typing import Optional
 
-from .operator import Biharmonic, Convection, Laplacian, Operator, VorticityConvection
+from .operator import Biharmonic, Convection, Laplacian, NSPressureConvection, Operator, VorticityConvection
 
 
 def Burgers(nu) -> Operator:
@@ -23,3 +23,9 @@ def NavierStokesVorticity(Re: float, force: Optional[Operator] = None) -> Operat
     if force is not None:
         ns_vorticity += force
     return ns_vorticity
+
+
+def NavierStokes(Re: float, force: Optional[Operator] = None) -> Operator:
+    """u_t = -grad p - (u . grad) u + lap u / Re (+ f), velocity form in 2D/3D (pde.py:85-100).
+    The engine covers force = None; add a forcing term to the returned operator instead."""
+    return NSPressureConvection(force) + 1 / Re * Laplacian()
