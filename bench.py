@@ -315,6 +315,7 @@ def main():
     ap.add_argument("--row-var", type=int, default=0)
     ap.add_argument("--row-wg", type=int, default=256)
     ap.add_argument("--sub-batch", type=int, default=0)
+    ap.add_argument("--streams", type=int, default=1, help="1, or 2 to overlap alternate sub-batches")
     ap.add_argument("--serve-requests", type=int, default=240, help="requests of the serving leg (0 = skip it)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate of the serving leg (1/s)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -344,12 +345,13 @@ def main():
     plan, uh = w.plan, w.uh
     if args.tile:
         plan.set_tuning(args.tile, args.col_var, args.row_var, args.row_wg, args.sub_batch)
+        plan.set_streams(args.streams)
     else:
         from torchfsm_amd.autotune import autotune
 
         autotune(plan, uh, steps=6, log=log)
-    log(f"[bench] launch shape (col tile, col var, row var, row wg, sub-batch) = {plan.tuning}; "
-        f"warmup {args.warmup} steps")
+    log(f"[bench] launch shape (col tile, col var, row var, row wg, sub-batch) = {plan.tuning}, "
+        f"streams {plan.streams}; warmup {args.warmup} steps")
     plan.step(uh, args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -364,7 +366,7 @@ def main():
     cfg = w.config()
     cfg["launch_shape"] = {"col_tile": plan.tuning[0], "col_radix_variant": plan.tuning[1],
                            "row_radix_variant": plan.tuning[2], "row_workgroup": plan.tuning[3],
-                           "sub_batch": plan.tuning[4]}
+                           "sub_batch": plan.tuning[4], "streams": plan.streams}
     cfg["parallelism"] = "single GPU, batch-parallel"
     result = {
         "metric": METRIC,

@@ -111,6 +111,12 @@ int64_t tfsm_plan_inverse_scratch_bytes(const tfsm_plan* plan, int32_t nbatch);
 int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int32_t row_var, int32_t row_wg,
                          int32_t sub_batch);
 
+/* Run alternate sub-batches of tfsm_step (see tfsm_plan_set_tuning) on the caller's stream and
+ * on an auxiliary stream of the plan (nstreams = 2), forked from and joined back into the
+ * caller's stream with events, so that their kernels overlap; 1 (default) keeps one stream.
+ * Results are identical.  (No reference counterpart: launch configuration.) */
+int tfsm_plan_set_streams(tfsm_plan* plan, int32_t nstreams);
+
 /* ETD coefficients on the device: SETDRK1-4 as float64 contour means of n_points points on a
  * circle of radius `radius`; ETDRK0-2 with the reference's complex64 closed forms.
  * Replaces OperatorLike._build_integrator (_base.py:441-526) with ETDRK0/1/2.__init__

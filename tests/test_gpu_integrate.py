@@ -283,8 +283,9 @@ def test_every_launch_shape_is_exact(dev):
 
 
 def test_sub_batch_blocking_is_bit_identical(dev):
-    """Temporal blocking (sub-batches stepped through all steps in turn) changes the launch order
-    only: states and recorded frames are bit-identical to whole-batch stepping."""
+    """Temporal blocking (sub-batches stepped through all steps in turn, on one stream or
+    alternating over two) changes the launch order only: states and recorded frames are
+    bit-identical to whole-batch stepping."""
     import torchfsm_amd as tfa
     from torchfsm_amd import field, pde
     from torchfsm_amd.workloads import ns_initial_vorticity
@@ -296,8 +297,9 @@ def test_sub_batch_blocking_is_bit_identical(dev):
     plan = op._prepare(16, dev, 1e-3)
     w0 = ns_initial_vorticity(64, 16, seed=3).to(dev)
     outs = []
-    for sb in (0, 8, 24):
+    for sb, ns in ((0, 1), (8, 1), (24, 1), (8, 2), (4, 2)):
         plan.set_tuning(16, 0, 0, 256, sb)
+        plan.set_streams(ns)
         uh = plan.forward(w0)
         rec = torch.tensor([0, -1, 1, -1, 2, -1, 3], dtype=torch.int32, device=dev)
         traj = torch.zeros((16, 4) + tuple(plan.spec_shape[1:]), dtype=torch.complex64, device=dev)
@@ -306,3 +308,16 @@ def test_sub_batch_blocking_is_bit_identical(dev):
     for uh, traj in outs[1:]:
         assert torch.equal(uh, outs[0][0])
         assert torch.equal(traj, outs[0][1])
+    # two streams under HIP-graph capture (fork/join by events): replays equal eager stepping
+    plan.set_tuning(16, 0, 0, 256, 4)
+    plan.set_streams(2)
+    ug = plan.forward(w0)
+    ue = ug.clone()
+    g = plan.graph(ug, 3)
+    for _ in range(3):
+        g.launch()
+        plan.step(ue, 3)
+        torch.cuda.synchronize()
+        assert torch.equal(ug, ue)
+    g.close()
+    plan.set_streams(1)

@@ -110,6 +110,9 @@ struct tfsm_plan {
   int row_var = 0;   // radix-plan variant of the row transforms
   int row_wg = 256;  // row-kernel workgroup size
   int sub_batch = 0; // samples stepped together through all steps (0 = whole batch)
+  int nstreams = 1;  // 2: alternate sub-batches run on the caller's stream and an auxiliary one
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int PF = 0, NF = 0;
   int K1 = 0;  // 3D: kept ky values (2 kmax1 + 1)
   size_t off_A1 = 0, off_A2 = 0, off_P = 0, off_G = 0, off_G2 = 0, ws_bytes = 0;
@@ -429,10 +432,20 @@ int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* tra
     // Temporal blocking: a sub-batch of samples goes through all nsteps before the next one, so
     // its state, auxiliary stages, products and fields stay resident in the last-level cache
     // across the 2 * evals launches of every step.  Samples are independent: same results.
+    // With two streams, alternate sub-batches run concurrently (fork from and join back into the
+    // caller's stream with events, which also holds under graph capture): their kernels overlap
+    // and fill each other's latency.  Sub-batches use disjoint parts of the workspace.
     const int nev = scheme_evals(p->scheme);
     const int sb = (p->sub_batch > 0 && p->sub_batch < p->B) ? p->sub_batch : p->B;
     const size_t per = (size_t)p->C * p->modes;
-    for (int b0 = 0; b0 < p->B; b0 += sb) {
+    const bool two = p->nstreams == 2 && sb < p->B && p->aux;
+    if (two) {
+      HIP_TRY(hipEventRecord(p->ev_fork, st), "fork event");
+      HIP_TRY(hipStreamWaitEvent(p->aux, p->ev_fork, 0), "fork wait");
+    }
+    const hipStream_t caller = st;
+    for (int b0 = 0, k = 0; b0 < p->B; b0 += sb, ++k) {
+      st = (two && (k & 1)) ? p->aux : caller;
       const int nb = (p->B - b0) < sb ? (p->B - b0) : sb;
       float2* Ub = U + (size_t)b0 * per;
       float2* Tb = traj ? (float2*)traj + (size_t)b0 * T * per : nullptr;
@@ -446,6 +459,10 @@ int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* tra
         }
       }
       if (rec_slot) HIP_TRY(launch_record(Ub, Tb, rec_slot, nsteps, nb, per, T, st), "record launch");
+    }
+    if (two) {
+      HIP_TRY(hipEventRecord(p->ev_join, p->aux), "join event");
+      HIP_TRY(hipStreamWaitEvent(caller, p->ev_join, 0), "join wait");
     }
     return TFSM_OK;
   }
@@ -743,6 +760,12 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
 
 int tfsm_plan_destroy(tfsm_plan* plan) {
   if (!plan) return TFSM_OK;
+  if (plan->aux) {
+    (void)hipStreamSynchronize(plan->aux);
+    (void)hipStreamDestroy(plan->aux);
+  }
+  if (plan->ev_fork) (void)hipEventDestroy(plan->ev_fork);
+  if (plan->ev_join) (void)hipEventDestroy(plan->ev_join);
   for (void* q : plan->allocs) (void)hipFree(q);
   delete plan;
   return TFSM_OK;
@@ -775,6 +798,19 @@ int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int
   plan->row_var = row_var;
   plan->row_wg = row_wg;
   plan->sub_batch = sub_batch < 0 ? 0 : sub_batch;
+  return TFSM_OK;
+}
+
+int tfsm_plan_set_streams(tfsm_plan* plan, int32_t nstreams) {
+  if (!plan) return fail(TFSM_E_INVALID, "null plan");
+  if (nstreams != 1 && nstreams != 2) return fail(TFSM_E_INVALID, "nstreams must be 1 or 2");
+  if (nstreams == 2 && !plan->aux) {
+    // default priority, non-blocking with respect to the null stream
+    HIP_TRY(hipStreamCreateWithFlags(&plan->aux, hipStreamNonBlocking), "auxiliary stream");
+    HIP_TRY(hipEventCreateWithFlags(&plan->ev_fork, hipEventDisableTiming), "fork event");
+    HIP_TRY(hipEventCreateWithFlags(&plan->ev_join, hipEventDisableTiming), "join event");
+  }
+  plan->nstreams = nstreams;
   return TFSM_OK;
 }
 

@@ -123,6 +123,7 @@ class SpectralPlan:
         self.scheme = None
         self.dt = None
         self.tuning = (16, 0, 0, 256, 0)
+        self.streams = 1
 
     # ------------------------------------------------------------------ lifecycle
     def close(self):
@@ -151,6 +152,11 @@ class SpectralPlan:
         check(lib.tfsm_plan_set_tuning(self._plan, int(col_tile), int(col_var), int(row_var), int(row_wg),
                                        int(sub_batch)), "tuning")
         self.tuning = (int(col_tile), int(col_var), int(row_var), int(row_wg), int(sub_batch))
+
+    def set_streams(self, nstreams: int):
+        """1: one stream; 2: alternate sub-batches overlap on an auxiliary stream (tfsm_plan_set_streams)."""
+        check(lib.tfsm_plan_set_streams(self._plan, int(nstreams)), "streams")
+        self.streams = int(nstreams)
 
     def set_scheme(self, scheme: str, dt: float, n_points: int = 16, radius: float = 1.0):
         with torch.cuda.device(self.device):
