@@ -310,7 +310,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--torch-steps", type=int, default=20)
     ap.add_argument("--no-baselines", action="store_true", help="engine and kernel legs only")
-    ap.add_argument("--tile", type=int, default=0, help="column tile (8/16); 0 = autotune the launch shape")
+    ap.add_argument("--tile", type=int, default=0, help="column tile (8/16); 0 = the plan's default launch shape")
+    ap.add_argument("--autotune", action="store_true", help="search the launch shape first (torchfsm_amd.autotune)")
     ap.add_argument("--col-var", type=int, default=0)
     ap.add_argument("--row-var", type=int, default=0)
     ap.add_argument("--row-wg", type=int, default=256)
@@ -346,7 +347,7 @@ def main():
     if args.tile:
         plan.set_tuning(args.tile, args.col_var, args.row_var, args.row_wg, args.sub_batch)
         plan.set_streams(args.streams)
-    else:
+    elif args.autotune:
         from torchfsm_amd.autotune import autotune
 
         autotune(plan, uh, steps=6, log=log)

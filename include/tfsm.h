@@ -111,6 +111,12 @@ int64_t tfsm_plan_inverse_scratch_bytes(const tfsm_plan* plan, int32_t nbatch);
 int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int32_t row_var, int32_t row_wg,
                          int32_t sub_batch);
 
+/* Current launch shape: out[6] = column tile, column variant, row variant, row workgroup,
+ * sub-batch, streams.  A new plan starts from a default derived from its shape (2D: sub-batch
+ * sized to keep >= 512 column workgroups per launch; 3D: radix-4 z rows from 64 points).
+ * (No reference counterpart: launch configuration.) */
+int tfsm_plan_get_tuning(const tfsm_plan* plan, int32_t* out);
+
 /* Run alternate sub-batches of tfsm_step (see tfsm_plan_set_tuning) on the caller's stream and
  * on an auxiliary stream of the plan (nstreams = 2), forked from and joined back into the
  * caller's stream with events, so that their kernels overlap; 1 (default) keeps one stream.

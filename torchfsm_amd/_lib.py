@@ -60,6 +60,7 @@ SIGNATURES = {
     "tfsm_plan_inverse_scratch_bytes": (I64, [VP, I32]),
     "tfsm_plan_set_tuning": (ctypes.c_int, [VP, I32, I32, I32, I32, I32]),
     "tfsm_plan_set_streams": (ctypes.c_int, [VP, I32]),
+    "tfsm_plan_get_tuning": (ctypes.c_int, [VP, VP]),
     "tfsm_plan_set_scheme": (ctypes.c_int, [VP, I32, ctypes.c_double, I32, ctypes.c_double, VP]),
     "tfsm_plan_coefficients": (ctypes.c_int, [VP, I32, VP, VP]),
     "tfsm_plan_set_sample": (ctypes.c_int, [VP, I32, VP, ctypes.c_double, VP]),

@@ -750,6 +750,20 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
       p->ws_bytes = p->off_A2 + roundup(sb, 256);
     }
   }
+  // default launch shape (the autotuner may refine it): 2D columns of 512 points run the
+  // half-thread radix plan; temporal blocking uses the smallest sub-batch (multiple of 8) that
+  // still gives >= 512 column workgroups (two per CU) while its working set stays under 256 MB
+  if (!rc && D == 2) {
+    p->col_var = (p->n[0] == 512) ? 1 : 0;
+    const size_t per_sample = 4 * (size_t)p->C * p->modes * sizeof(float2) +
+                              (size_t)p->NF * p->n[0] * p->PF * sizeof(float2);
+    const int tiles = (p->H + p->V - 1) / p->V;
+    int sb = 8;
+    while (sb < p->B && (size_t)tiles * sb < 512) sb += 8;
+    if (sb >= p->B || (size_t)sb * per_sample > ((size_t)256 << 20)) sb = 0;
+    p->sub_batch = sb;
+  }
+  if (!rc && D == 3) p->row_var = p->n[2] >= 64 ? 1 : 0;  // radix-4 z rows (fewer registers)
   if (rc) {
     tfsm_plan_destroy(p);
     return rc;
@@ -798,6 +812,17 @@ int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int
   plan->row_var = row_var;
   plan->row_wg = row_wg;
   plan->sub_batch = sub_batch < 0 ? 0 : sub_batch;
+  return TFSM_OK;
+}
+
+int tfsm_plan_get_tuning(const tfsm_plan* plan, int32_t* out) {
+  if (!plan || !out) return fail(TFSM_E_INVALID, "null argument");
+  out[0] = plan->V;
+  out[1] = plan->col_var;
+  out[2] = plan->row_var;
+  out[3] = plan->row_wg;
+  out[4] = plan->sub_batch;
+  out[5] = plan->nstreams;
   return TFSM_OK;
 }
 

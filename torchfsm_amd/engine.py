@@ -122,8 +122,10 @@ class SpectralPlan:
         self._ws = None
         self.scheme = None
         self.dt = None
-        self.tuning = (16, 0, 0, 256, 0)
-        self.streams = 1
+        cur = (ctypes.c_int32 * 6)()
+        check(lib.tfsm_plan_get_tuning(self._plan, ctypes.cast(cur, ctypes.c_void_p)), "tuning")
+        self.tuning = tuple(int(x) for x in cur[:5])
+        self.streams = int(cur[5])
 
     # ------------------------------------------------------------------ lifecycle
     def close(self):
