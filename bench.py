@@ -81,8 +81,9 @@ class NS2D:
         A = nb * self.N * H * 8
         rho = (self.N // 3 + 1) / H
         per = self.B // nb
-        row = ("k2_row<512,4,VC,*>", 0, ST_R4_E0, (4 * rho + 1) * A, 4 * per)
-        cols = [(f"k2_col<512,tile,VC,*> e{i}", 1, st, (r + 4 * rho) * A, per)
+        row = ("k2_row (512-point rows: 2 packed C2R, product u.grad w, R2C)", 0, ST_R4_E0, (4 * rho + 1) * A, 4 * per)
+        cols = [(f"k2_col (512-point columns: fwd FFT, SETDRK4 combine, 4 inverse FFTs) e{i}", 1, st,
+                 (r + 4 * rho) * A, per)
                 for i, (st, r) in enumerate([(ST_R4_E0, 3), (ST_R4_E1, 3), (ST_R4_E2, 5), (ST_R4_E3, 5)])]
         return [row] + cols
 
@@ -124,7 +125,7 @@ class Burgers3D:
     name = "burgers3d_128cubed_c3_b8_etdrk2"
     N, B, C, NU, DT, SCHEME = 128, 8, 3, 0.01, 5e-3, "ETDRK2"
     default_steps = 200
-    traffic_file = "k3_col_burgers128_b8_traffic.txt"
+    traffic_file = "k3_burgers128_b8_traffic.txt"
     data = "synthetic: per (b, c) 8 seeded cosines, integer |k| in [1, 4], a ~ U(-1,1), phase ~ U(0, 2pi)"
 
     def __init__(self, dev):
@@ -161,11 +162,11 @@ class Burgers3D:
         ryz = (2 * k + 1) / N * rz
         per = self.B // nb
         return [
-            ("k3_row<128,16,2> (z rows: 12 C2R, products, 3 R2C)", 4, ST_R2_E0, (9 * rz + 3) * A1, 2 * per),
-            ("k_cols_c2c<128,*,-1> (y forward of the products)", 5, ST_R2_E0, 6 * A1, 2 * per),
-            ("k3_col<128,16,2> e0 (x fwd + combine + 6 x inverse)", 6, ST_R2_E0, (9 + 6 * ryz) * A1, per),
-            ("k3_col<128,16,2> e1", 6, ST_R2_E1, (12 + 6 * ryz) * A1, per),
-            ("k3_yinv<128,16,2> (6 -> 9 fields, y inverse)", 7, ST_R2_E0, (6 * ryz + 9 * rz) * A1, 2 * per),
+            ("k3_row (z rows: 12 C2R, products, 3 R2C)", 4, ST_R2_E0, (9 * rz + 3) * A1, 2 * per),
+            ("k_cols_c2c (y forward of the products)", 5, ST_R2_E0, 6 * A1, 2 * per),
+            ("k3_col (x fwd + combine + 6 x inverse) e0", 6, ST_R2_E0, (9 + 6 * ryz) * A1, per),
+            ("k3_col (x fwd + combine + 6 x inverse) e1", 6, ST_R2_E1, (12 + 6 * ryz) * A1, per),
+            ("k3_yinv (6 -> 9 fields, y inverse)", 7, ST_R2_E0, (6 * ryz + 9 * rz) * A1, 2 * per),
         ]
 
     def prime(self):
@@ -397,7 +398,7 @@ def main():
     # dominant kernel family = the largest share of the step's kernel time
     fam = {}
     for name, pid, stage, nbytes, per_step in ks:
-        f = name.split(" e")[0] if " e" in name else name
+        f = name.rsplit(" e", 1)[0] if name[-2:-1] == "e" and name[-1].isdigit() else name
         t, b, n = fam.get(f, (0.0, 0.0, 0))
         fam[f] = (t + kt[name] * per_step, b + nbytes * per_step, n + per_step)
     dom = max(fam, key=lambda f: fam[f][0])
