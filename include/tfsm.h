@@ -188,6 +188,14 @@ int tfsm_step_varying(tfsm_plan* plan, void* uh, const int32_t* nsteps_per_sampl
  * (_base.py:408-439). */
 int tfsm_apply(tfsm_plan* plan, const void* uh, void* out, void* ws, void* stream);
 
+/* Runge-Kutta stage combination on half spectra of the plan's state size:
+ * out = x + dt * ((coefs[0] ks[0] + coefs[1] ks[1]) + ...), nk <= 8 (host arrays of device
+ * pointers and coefficients), in the order of the reference's stage sums.  With tfsm_apply as the
+ * right-hand side L u + N(u) this runs the explicit Runge-Kutta integrators.  Replaces the stage
+ * and update sums of _RKBase._rk_step (_rk.py:43-58). */
+int tfsm_rk_combine(tfsm_plan* plan, void* out, const void* x, const void* const* ks, const float* coefs, int32_t nk,
+                    float dt, void* stream);
+
 /* out = uh * where(L == 0, 1, 1 / L) (half spectra).  Replaces LinearOperator.solve
  * (_base.py:217-262). */
 int tfsm_solve(tfsm_plan* plan, const void* uh, void* out, void* stream);

@@ -275,6 +275,56 @@ class Scheme:
         return self.E * u + self.c4 * n0 + self.c5 * 2 * (na + nb) + self.c6 * nc
 
 
+# Butcher tableaux of the reference's explicit Runge-Kutta family (_rk.py:82-255): each row of
+# CA is [c_i, a_i1, ...], B the update weights.
+RK_TABLEAUX = {
+    "Euler": ([[1.0]], [1.0]),
+    "Midpoint": ([[1 / 2, 1 / 2]], [0, 1]),
+    "Heun12": ([[1, 1]], [1 / 2, 1 / 2]),
+    "Ralston12": ([[2 / 3, 2 / 3]], [1 / 4, 3 / 4]),
+    "BogackiShampine23": ([[1 / 2, 1 / 2], [3 / 4, 0, 3 / 4], [1, 2 / 9, 1 / 3, 4 / 9]], [2 / 9, 1 / 3, 4 / 9, 0]),
+    "RK4": ([[1 / 2, 1 / 2], [1 / 2, 0, 1 / 2], [1, 0, 0, 1]], [1 / 6, 1 / 3, 1 / 3, 1 / 6]),
+    "RK4_38Rule": ([[1 / 3, 1 / 3], [2 / 3, -1 / 3, 1], [1, -1, 1, 1]], [1 / 8, 3 / 8, 3 / 8, 1 / 8]),
+    "Dorpi45": ([[1 / 5, 1 / 5], [3 / 10, 3 / 40, 9 / 40], [4 / 5, 44 / 45, -56 / 15, 32 / 9],
+                 [8 / 9, 19372 / 6561, -25360 / 2187, 64448 / 6561, -212 / 729],
+                 [1, 9017 / 3168, -355 / 33, 46732 / 5247, 49 / 176, -5103 / 18656],
+                 [1, 35 / 384, 0, 500 / 1113, 125 / 192, -2187 / 6784, 11 / 84]],
+                [35 / 384, 0, 500 / 1113, 125 / 192, -2187 / 6784, 11 / 84]),
+    "Fehlberg45": ([[1 / 4, 1 / 4], [3 / 8, 3 / 32, 9 / 32], [12 / 13, 1932 / 2197, -7200 / 2197, 7296 / 2197],
+                    [1, 439 / 216, -8, 3680 / 513, -845 / 4104], [1 / 2, -8 / 27, 2, -3544 / 2565, 1859 / 4104, -11 / 40]],
+                   [16 / 135, 0, 6656 / 12825, 28561 / 56430, -9 / 50, 2 / 55]),
+    "CashKarp45": ([[1 / 5, 1 / 5], [3 / 10, 3 / 40, 9 / 40], [3 / 5, 3 / 10, -9 / 10, 6 / 5],
+                    [1, -11 / 54, 5 / 2, -70 / 27, 35 / 27],
+                    [7 / 8, 1631 / 55296, 175 / 512, 575 / 13824, 44275 / 110592, 253 / 4096]],
+                   [37 / 378, 0, 250 / 621, 125 / 594, 0, 512 / 1771]),
+}
+
+
+def rk_step(f: Callable, x, dt, name: str):
+    """One explicit Runge-Kutta step of _rk.py:43-58 (fixed step, every stage evaluated)."""
+    ca, b = RK_TABLEAUX[name]
+    ks = [f(x)]
+    for row in ca:
+        ks.append(f(x + dt * sum([a_i * k for a_i, k in zip(row[1:], ks)])))
+    return x + dt * sum([b_i * k for b_i, k in zip(b, ks)])
+
+
+class RKScheme:
+    """Explicit Runge-Kutta stepping of the operator f(u) = L u + N(u) (_base.py:408-439)."""
+
+    def __init__(self, name: str, dt: float, L, closure):
+        self.name, self.dt = name, dt
+        if closure is None:
+            self.f = lambda u: L * u
+        elif L is None:
+            self.f = closure
+        else:
+            self.f = lambda u: L * u + closure(u)
+
+    def step(self, u):
+        return rk_step(self.f, u, self.dt, self.name)
+
+
 def choose_scheme(terms: List[Term], scheme: str) -> str:
     """_build_integrator 'auto' (_base.py:451-455): ETDRK0 if linear else SETDRK4."""
     if scheme != "auto":
@@ -296,9 +346,12 @@ def integrate(terms: List[Term], mesh_info, u0, dt, step, scheme="auto", record=
     L = build_linear(mesh, terms, n_channel)
     closure = build_closure(mesh, terms, dealias_rate)
     name = choose_scheme(terms, scheme)
-    if name != "ETDRK0" and L is None:  # _base.py:473-478
-        L = torch.tensor([0.0], dtype=mesh.dtype, device=mesh.device)
-    sch = Scheme(name, dt, L, closure, n_points, radius)
+    if name in RK_TABLEAUX:  # _base.py:492-500
+        sch = RKScheme(name, dt, L, closure)
+    else:
+        if name != "ETDRK0" and L is None:  # _base.py:473-478
+            L = torch.tensor([0.0], dtype=mesh.dtype, device=mesh.device)
+        sch = Scheme(name, dt, L, closure, n_points, radius)
     u_fft = mesh.fft(u0) if u0_fft is None else u0_fft
     frames = []
     i = -1

@@ -1,6 +1,6 @@
 """Public API paths of OperatorLike on the engine against the oracle on the same inputs:
 trajectory recorders (interval, start, include_initial_state, CPU), return_in_fourier, u_0_fft
-input, every ETD scheme in 1D/2D/3D, per-sample coefficients, LinearOperator.solve,
+input, every ETD scheme and every explicit Runge-Kutta tableau in 1D/2D/3D, per-sample coefficients, LinearOperator.solve,
 Operator.__call__ in 3D, progressive integration, HIP-graph replay and per-sample domain lengths.
 Tolerance: rel-L2 <= 1e-5 in float32 (_base.py:217-262, 676-790; traj_recorder.py:7-148)."""
 
@@ -83,6 +83,59 @@ def test_return_in_fourier_and_u0_fft(dev):
     e2 = _rel(out2, ref2)
     print(f"return_in_fourier {e1:.2e}; u_0_fft input {e2:.2e}")
     assert e1 <= TOL and e2 <= TOL
+
+
+RK_NAMES = ["Euler", "Midpoint", "Heun12", "Ralston12", "BogackiShampine23", "RK4", "RK4_38Rule", "Dorpi45",
+            "Fehlberg45", "CashKarp45"]
+
+
+@pytest.mark.parametrize("name", RK_NAMES)
+def test_every_rk_integrator_1d_2d_3d(dev, name):
+    """Explicit Runge-Kutta stepping (_rk.py:43-58) on the engine's L u + N(u), every tableau."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import pde
+    from torchfsm_amd.integrator import RKIntegrator
+    from torchfsm_amd.workloads import fourier_modes_field
+
+    for dims, n, steps in ((1, 256, 20), (2, 64, 10), (3, 32, 5)):
+        u = 0.5 * fourier_modes_field(n, dims, dims, 2, seed=50 + dims)
+        mi = [(0, 1, n)] * dims
+        op = pde.Burgers(0.01)
+        op.set_integrator(RKIntegrator[name])
+        dt = 2e-4 if dims == 1 else 5e-4
+        out = op.integrate(u.to(dev), mesh=tfa.MeshGrid(mi, device=dev), dt=dt, step=steps)
+        ref = refpath.integrate(refpath.burgers_terms(0.01), mi, u, dt, steps, scheme=name)
+        e = _rel(out, ref)
+        print(f"{name} {dims}D {n}: rel-L2 {e:.2e}")
+        assert e <= TOL
+
+
+def test_rk_forced_ns_recorder_and_options(dev):
+    """RK4 on forced 2D Navier-Stokes (vorticity form, drag and Kolmogorov forcing) with an
+    AutoRecorder, return_in_fourier, and the option checks of set_integrator."""
+    from oracle import refpath
+    from torchfsm_amd.integrator import RKIntegrator
+    from torchfsm_amd.traj_recorder import AutoRecorder, IntervalController
+    from torchfsm_amd.workloads import ns_initial_vorticity
+
+    op, mesh, y = _ns(dev)
+    op.set_integrator(RKIntegrator.RK4)
+    w0 = ns_initial_vorticity(64, 2, seed=9)
+    terms = refpath.ns_terms(1000, y.cpu())
+    mi = [(0, 2 * math.pi, 64)] * 2
+    rec = AutoRecorder(IntervalController(3))
+    traj = op.integrate(w0.to(dev), mesh=mesh, dt=1e-3, step=10, trajectory_recorder=rec)
+    ref = refpath.integrate(terms, mi, w0, 1e-3, 10, scheme="RK4", record=lambda s: s % 3 == 0)
+    assert tuple(traj.shape) == tuple(ref.shape)
+    e1 = _rel(traj, ref)
+    out = op.integrate(w0.to(dev), mesh=mesh, dt=1e-3, step=10, return_in_fourier=True)
+    e2 = _rel(out, refpath.integrate(terms, mi, w0, 1e-3, 10, scheme="RK4", return_in_fourier=True))
+    print(f"RK4 NS recorder {e1:.2e}; fourier {e2:.2e}")
+    assert e1 <= TOL and e2 <= TOL
+    op.set_integrator(RKIntegrator.Dorpi45, adaptive=True)
+    with pytest.raises(ValueError, match="adaptive"):
+        op.integrate(w0.to(dev), mesh=mesh, dt=1e-3, step=2)
 
 
 @pytest.mark.parametrize("scheme", ["ETDRK1", "ETDRK2", "SETDRK1", "SETDRK2", "SETDRK3", "SETDRK4"])

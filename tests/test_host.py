@@ -236,3 +236,13 @@ def test_serving_host_logic():
     assert serving.ShapeClass.bytes_estimate("ns2d", 256, 16) > e1 > serving.ShapeClass.bytes_estimate("ns2d", 128, 8)
     with pytest.raises(ValueError, match="unknown request kind"):
         serving.linear_coefs("kdv", 1.0)
+
+
+def test_rk_tableaux_match_the_oracle():
+    """The product's Butcher tableaux are the oracle's restatement of _rk.py:82-255, entry by entry."""
+    from oracle import refpath
+    from torchfsm_amd.integrator import RKIntegrator, rk_tableau
+
+    assert sorted(m.name for m in RKIntegrator) == sorted(refpath.RK_TABLEAUX)
+    for m in RKIntegrator:
+        assert rk_tableau(m) == refpath.RK_TABLEAUX[m.name], m

@@ -274,3 +274,61 @@ def test_ns_pressure_convection_is_the_leray_projection():
         assert torch.allclose(N, -proj, atol=1e-10 * float(conv.abs().max()))
         div = (m.nabla_vector(1) * N).sum(1)
         assert float(div.abs().max()) < 1e-9 * float(conv.abs().max())
+
+
+# ---------------------------------------------------------------- explicit Runge-Kutta family
+# RK4_38Rule is second order as the reference writes it: its last row is a_41 = -1, a_42 = 1
+# (_rk.py:165-173), where the classical 3/8 rule has a_41 = 1, a_42 = -1; parity keeps it.
+RK_ORDERS = {"Euler": 1, "Midpoint": 2, "Heun12": 2, "Ralston12": 2, "BogackiShampine23": 3, "RK4": 4,
+             "RK4_38Rule": 2, "Dorpi45": 5, "Fehlberg45": 5, "CashKarp45": 5}
+
+
+def _pendulum(name, m, T=2.0):
+    u = torch.tensor([1.0, 0.5], dtype=torch.float64)
+    for _ in range(m):
+        u = R.rk_step(lambda v: torch.stack([v[1], -torch.sin(v[0])]), u, T / m, name)
+    return u
+
+
+@pytest.mark.parametrize("name", sorted(RK_ORDERS))
+def test_rk_tableau_consistency_and_order(name):
+    """Each tableau row satisfies c_i = sum_j a_ij, the weights sum to one, and the step converges at
+    the method's order on the pendulum x' = y, y' = -sin x in float64 (reference solution: RK4 at
+    4000 steps), so a wrong entry fails."""
+    ca, b = R.RK_TABLEAUX[name]
+    for row in ca[:len(b) - 1]:  # rows past len(b) - 1 feed stages the update weights never use
+        assert abs(row[0] - sum(row[1:])) < 1e-12, row
+    assert abs(sum(b) - 1) < 1e-12
+    ref = _pendulum("RK4", 4000)
+    errs = [float((_pendulum(name, m) - ref).abs().max()) for m in (8, 16, 32)]
+    rate = math.log2(errs[1] / errs[2])
+    assert abs(rate - RK_ORDERS[name]) < 0.25, (errs, rate)
+
+
+def test_rk_linear_is_the_stability_polynomial():
+    """RK4 on nu*lap (f = L u): each step multiplies mode k by sum_{j<=4} (dt L)^j / j!
+    (dt inside the stability interval for every mode of the mesh)."""
+    n, nu, dt, steps = 32, 0.01, 0.005, 7
+    x = np.arange(n) / n
+    u0 = np.cos(2 * np.pi * 3 * x)[None, None]
+    z = -nu * (2 * np.pi * 3) ** 2 * dt
+    amp = sum(z ** j / math.factorial(j) for j in range(5)) ** steps
+    o = R.integrate([(nu, "lap", None)], [(0, 1, n)], torch.from_numpy(u0), dt, steps, scheme="RK4").numpy()
+    assert np.abs(o - amp * u0).max() < 1e-12
+
+
+def test_rk_agrees_with_etd_on_burgers():
+    """f = L u + N(u) carries the nonlinear term with the reference's sign: RK4 and SETDRK4 on 1D
+    viscous Burgers agree to the schemes' truncation error at a small step (float64)."""
+    n = 64
+    x = torch.arange(n, dtype=torch.float64) / n
+    u0 = (torch.sin(2 * math.pi * x) + 0.5).reshape(1, 1, n)
+    terms = R.burgers_terms(0.02)
+    d = []
+    for dt, m in ((1e-3, 200), (5e-4, 400)):
+        a = R.integrate(terms, [(0, 1, n)], u0, dt, m, scheme="RK4")
+        b = R.integrate(terms, [(0, 1, n)], u0, dt, m, scheme="SETDRK4")
+        d.append(rel(a.numpy(), b.numpy()))
+    assert d[0] < 1e-8 and math.log2(d[0] / d[1]) > 3.5, d
+    e = R.integrate(terms, [(0, 1, n)], u0, 1e-3, 200, scheme="Euler")
+    assert 1e-4 < rel(e.numpy(), b.numpy()) < 1e-2

@@ -74,6 +74,7 @@ SIGNATURES = {
     "tfsm_step_varying": (ctypes.c_int, [VP, VP, VP, I32, VP, VP]),
     "tfsm_apply": (ctypes.c_int, [VP, VP, VP, VP, VP]),
     "tfsm_solve": (ctypes.c_int, [VP, VP, VP, VP]),
+    "tfsm_rk_combine": (ctypes.c_int, [VP, VP, VP, VP, VP, I32, ctypes.c_float, VP]),
     "tfsm_launch_pass": (ctypes.c_int, [VP, VP, VP, I32, I32, VP]),
     "tfsm_plan_adjoint_workspace_bytes": (I64, [VP]),
     "tfsm_adjoint_prepare": (ctypes.c_int, [VP, I32, VP]),

@@ -1120,6 +1120,25 @@ int tfsm_apply(tfsm_plan* p, const void* uh, void* out, void* ws, void* stream) 
   return TFSM_OK;
 }
 
+int tfsm_rk_combine(tfsm_plan* p, void* out, const void* x, const void* const* ks, const float* coefs, int32_t nk,
+                    float dt, void* stream) {
+  if (!p || !out || !x || (nk > 0 && (!ks || !coefs))) return fail(TFSM_E_INVALID, "null argument");
+  if (nk < 0 || nk > RK_MAX_TERMS) return fail(TFSM_E_INVALID, "at most 8 stage terms");
+  RkArgs a;
+  a.x = (const float2*)x;
+  for (int i = 0; i < RK_MAX_TERMS; ++i) {
+    a.k[i] = i < nk ? (const float2*)ks[i] : nullptr;
+    a.c[i] = i < nk ? coefs[i] : 0.f;
+    if (i < nk && !ks[i]) return fail(TFSM_E_INVALID, "null stage pointer");
+  }
+  a.nk = nk;
+  a.dt = dt;
+  a.out = (float2*)out;
+  a.total = p->state_elems();
+  HIP_TRY(launch_rk_combine(a, (hipStream_t)stream), "rk combine launch");
+  return TFSM_OK;
+}
+
 int tfsm_solve(tfsm_plan* p, const void* uh, void* out, void* stream) {
   if (!p || !uh || !out) return fail(TFSM_E_INVALID, "null argument");
   if (p->nlin == 0) return fail(TFSM_E_INVALID, "solve needs a linear operator");

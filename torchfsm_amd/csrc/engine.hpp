@@ -464,6 +464,18 @@ hipError_t launch_adj_reduce(const float* gpart, size_t per, int B, double* out,
 hipError_t launch_adj_dcoefs(const float* L, size_t modes, int Bc, double dt, float* dcoef, hipStream_t st);
 hipError_t launch_adj_sym(const AdjArgs& a, int kind, float* S, hipStream_t st);
 
+constexpr int RK_MAX_TERMS = 8;
+struct RkArgs {
+  const float2* x;
+  const float2* k[RK_MAX_TERMS];
+  float c[RK_MAX_TERMS];
+  int nk;
+  float dt;
+  float2* out;
+  size_t total;
+};
+hipError_t launch_rk_combine(const RkArgs& a, hipStream_t st);
+
 hipError_t launch_apply_linear(const float2* U, const float2* Nv, const float* L, size_t l_bstride, size_t modes, int C,
                                size_t total, int have_lin, int have_nl, float2* out, hipStream_t st);
 

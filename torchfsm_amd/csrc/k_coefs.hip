@@ -283,6 +283,24 @@ hipError_t launch_accum_source(float2* S, const float2* X, float coef, size_t to
   return hipGetLastError();
 }
 
+// Runge-Kutta stage combination out = x + dt * ((c0 k0 + c1 k1) + ...), in the order of the
+// reference's python sum (_rk.py:43-58: sum([a_i * k ...]) starts from 0, then dt * ..., then x + ...)
+__global__ void k_rk_combine(RkArgs a) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < a.total; e += (size_t)gridDim.x * blockDim.x) {
+    float2 acc = make_float2(0.f, 0.f);
+    for (int i = 0; i < a.nk; ++i) {
+      const float2 t = rscale(a.c[i], a.k[i][e]);
+      acc = i == 0 ? t : radd(acc, t);
+    }
+    a.out[e] = radd(a.x[e], rscale(a.dt, acc));
+  }
+}
+
+hipError_t launch_rk_combine(const RkArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_rk_combine, dim3(grid_for(a.total)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_apply_linear(const float2* U, const float2* Nv, const float* L, size_t l_bstride, size_t modes, int C,
                                size_t total, int have_lin, int have_nl, float2* out, hipStream_t st) {
   hipLaunchKernelGGL(k_apply_linear, dim3(grid_for(total)), dim3(256), 0, st, U, Nv, L, l_bstride, modes, C, total,

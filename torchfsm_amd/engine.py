@@ -275,6 +275,19 @@ class SpectralPlan:
             check(lib.tfsm_apply(self._plan, _ptr(uh), _ptr(out), _ptr(ws), _stream(self.device)), "apply")
         return out
 
+    def rk_combine(self, x: torch.Tensor, ks, coefs, dt: float, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """x + dt * sum_i coefs[i] ks[i] (half spectra), the reference's stage-sum order."""
+        if len(ks) != len(coefs):
+            raise ValueError("one coefficient per stage")
+        out = torch.empty_like(x) if out is None else out
+        ptrs = (ctypes.c_void_p * max(1, len(ks)))(*[k.data_ptr() for k in ks])
+        cs = (ctypes.c_float * max(1, len(coefs)))(*[float(c) for c in coefs])
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_rk_combine(self._plan, _ptr(out), _ptr(x), ctypes.cast(ptrs, ctypes.c_void_p),
+                                      ctypes.cast(cs, ctypes.c_void_p), len(ks), float(dt), _stream(self.device)),
+                  "rk_combine")
+        return out
+
     def solve(self, uh: torch.Tensor) -> torch.Tensor:
         """uh / L with 1/0 := 1 (half spectra)."""
         out = torch.empty_like(uh)

@@ -348,6 +348,10 @@ class OperatorLike(_MutableMixIn):
         step = int(step)
         if trajectory_recorder is not None and step <= 0:
             raise UnboundLocalError("cannot access local variable 'i' where it is not associated with a value")
+        if isinstance(self._integrator, RKIntegrator):
+            if torch.is_grad_enabled() and self._grad_requested(u_0, u_0_fft):
+                raise ValueError("term not supported by engine: gradients through Runge-Kutta integrators")
+            return self._integrate_rk(u_0, u_0_fft, dt, step, progressive, trajectory_recorder, return_in_fourier)
         if torch.is_grad_enabled() and self._grad_requested(u_0, u_0_fft):
             return self._integrate_with_adjoint(u_0, u_0_fft, dt, step, trajectory_recorder, return_in_fourier)
         plan = self._prepare(x.shape[0], x.device, dt)
@@ -385,6 +389,48 @@ class OperatorLike(_MutableMixIn):
                 "Original error message: {}".format(str(e)),
                 "Please try to use a smaller mesh or a low-order integrator.",
             ]))
+        if trajectory_recorder is not None:
+            trajectory_recorder.return_in_fourier = return_in_fourier
+            return trajectory_recorder.trajectory
+        if return_in_fourier:
+            return plan.half_to_full(uh)
+        return plan.inverse(uh)
+
+    def _integrate_rk(self, u_0, u_0_fft, dt, step, progressive, trajectory_recorder, return_in_fourier):
+        """Explicit Runge-Kutta stepping (_rk.py:43-58 with the operator L u + N(u) of
+        _base.py:408-439): each stage is one tfsm_apply on the engine, the stage and update sums
+        are tfsm_rk_combine."""
+        from ..integrator import rk_tableau
+
+        cfg = dict(self._integrator_config)
+        name = self._integrator.name
+        if cfg and name in ("Euler", "Midpoint", "RK4", "RK4_38Rule"):  # constructors without options
+            raise TypeError(f"{name}.__init__() got an unexpected keyword argument '{next(iter(cfg))}'")
+        if cfg.pop("adaptive", False):
+            raise ValueError("term not supported by engine: adaptive Runge-Kutta steps")
+        cfg.pop("atol", None)
+        cfg.pop("rtol", None)
+        if cfg:
+            raise TypeError(f"{name}.__init__() got an unexpected keyword argument '{next(iter(cfg))}'")
+        ca, b = rk_tableau(self._integrator)
+        x = u_0 if u_0 is not None else u_0_fft
+        plan = self._engine(x.shape[0], x.device)
+        self._state_dict["integrator"] = plan
+        uh = self._to_half(plan, u_0, u_0_fft)
+        sched = [bool(trajectory_recorder.control_func(i)) for i in range(step + 1)] if trajectory_recorder else None
+        it = range(step)
+        if progressive:
+            it = tqdm(it, desc="Integrating")
+        for i in it:
+            if sched and sched[i]:
+                trajectory_recorder.record(i, plan.half_to_full(uh))
+            ks = [plan.apply(uh)]
+            for s in range(1, len(b)):
+                a = ca[s - 1][1:]
+                ks.append(plan.apply(plan.rk_combine(uh, ks[:len(a)], a, dt)))
+            uh = plan.rk_combine(uh, ks, b, dt)
+        if sched and sched[step]:
+            trajectory_recorder.record(step, plan.half_to_full(uh))
         if trajectory_recorder is not None:
             trajectory_recorder.return_in_fourier = return_in_fourier
             return trajectory_recorder.trajectory
