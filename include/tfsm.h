@@ -56,7 +56,7 @@ typedef struct tfsm_graph tfsm_graph;
  * state registered by OperatorLike.register_mesh (_base.py:581-624). */
 typedef struct tfsm_geometry {
   int32_t ndim;                      /* 1, 2 or 3 */
-  int32_t n[3];                      /* points per axis, powers of two */
+  int32_t n[3];                      /* points per axis: powers of two or a listed mixed-radix size */
   double length[3];                  /* end - start per axis */
   int32_t batch;                     /* B */
   int32_t channels;                  /* C */

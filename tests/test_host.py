@@ -27,10 +27,13 @@ def test_library_exports_every_declared_symbol():
 
 @pytest.mark.parametrize("ndim,n,batch,ch,term,err", [
     (4, (16, 16, 16), 1, 1, 0, "1D, 2D and 3D"),
-    (3, (256, 16, 16), 1, 3, 1, "16..128 points"),
+    (3, (256, 16, 16), 1, 3, 1, "powers of two 16..128"),
     (3, (16, 16, 16), 1, 2, 1, "convection operator only works"),
-    (2, (100, 100, 1), 1, 1, 0, "powers of two"),
-    (1, (16, 1, 1), 1, 1, 0, "32..4096"),
+    (2, (90, 90, 1), 1, 1, 0, "2D mesh sizes per axis"),
+    (1, (102, 1, 1), 1, 1, 0, "1D mesh sizes"),
+    (3, (96, 96, 48), 1, 3, 1, "cubic 48, 96 or 100"),
+    (3, (100, 100, 101), 1, 1, 0, "3D mesh sizes per axis"),
+    (1, (16, 1, 1), 1, 1, 0, "powers of two 32..4096"),
     (2, (64, 64, 1), 1, 1, 1, "convection operator only works"),
     (1, (64, 1, 1), 1, 1, 2, "Only vorticity in 2Dmesh"),
     (2, (64, 64, 1), 0, 1, 0, "batch and channels"),

@@ -173,6 +173,16 @@ bool twiddle_table(int n, int var, std::vector<float2>& h) {
       case 1024: twiddle_table_t<1024, 0>(h); return true;
       case 2048: twiddle_table_t<2048, 0>(h); return true;
       case 4096: twiddle_table_t<4096, 0>(h); return true;
+      case 48: twiddle_table_t<48, 0>(h); return true;
+      case 96: twiddle_table_t<96, 0>(h); return true;
+      case 100: twiddle_table_t<100, 0>(h); return true;
+      case 112: twiddle_table_t<112, 0>(h); return true;
+      case 192: twiddle_table_t<192, 0>(h); return true;
+      case 200: twiddle_table_t<200, 0>(h); return true;
+      case 384: twiddle_table_t<384, 0>(h); return true;
+      case 400: twiddle_table_t<400, 0>(h); return true;
+      case 768: twiddle_table_t<768, 0>(h); return true;
+      case 800: twiddle_table_t<800, 0>(h); return true;
       default: return false;
     }
   }
@@ -596,18 +606,16 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   if (D < 1 || D > 3) return fail(TFSM_E_INVALID, "the engine supports 1D, 2D and 3D meshes");
   if (geom->batch < 1 || geom->channels < 1) return fail(TFSM_E_INVALID, "batch and channels must be >= 1");
   for (int d = 0; d < D; ++d) {
-    if (!is_pow2(geom->n[d])) return fail(TFSM_E_INVALID, "mesh sizes must be powers of two");
     if (!(geom->length[d] > 0)) return fail(TFSM_E_INVALID, "mesh lengths must be positive");
-  }
-  if (D == 1 && (geom->n[0] < 32 || geom->n[0] > 4096))
-    return fail(TFSM_E_INVALID, "1D meshes must have 32..4096 points");
-  if (D == 2) {
-    for (int d = 0; d < 2; ++d)
-      if (geom->n[d] < 16 || geom->n[d] > 512) return fail(TFSM_E_INVALID, "2D meshes must have 16..512 points per axis");
-  }
-  if (D == 3) {
-    for (int d = 0; d < 3; ++d)
-      if (geom->n[d] < 16 || geom->n[d] > 128) return fail(TFSM_E_INVALID, "3D meshes must have 16..128 points per axis");
+    const int n = geom->n[d];
+    const bool ok = D == 1   ? ((is_pow2(n) && n >= 32 && n <= 4096) || is_mixed_size(n))
+                    : D == 2 ? ((is_pow2(n) && n >= 16 && n <= 512) || is_mixed_size_2d(n))
+                             : ((is_pow2(n) && n >= 16 && n <= 128) || is_mixed_size_3d(n));
+    if (!ok)
+      return fail(TFSM_E_INVALID, D == 1 ? "1D mesh sizes: powers of two 32..4096, or 48, 96, 100, 112, 192, 200, 384, "
+                                           "400, 768, 800"
+                                  : D == 2 ? "2D mesh sizes per axis: powers of two 16..512, or 48, 96, 100, 112, 192, 200"
+                                           : "3D mesh sizes per axis: powers of two 16..128, or 48, 96, 100");
   }
   const int term = terms->nonlinear;
   if (term != TERM_NONE && term != TERM_CONV && term != TERM_VC && term != TERM_NSP)
@@ -619,7 +627,7 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   if (term == TERM_VC && (D != 2 || geom->channels != 1))
     return fail(TFSM_E_INVALID, "Only vorticity in 2Dmesh is supported");
   if ((term == TERM_CONV || term == TERM_NSP) && D == 3 && !step3d_supported(geom->n[0], geom->n[1], geom->n[2]))
-    return fail(TFSM_E_INVALID, "3D convection needs 16, 32, 64 or 128 points per axis");
+    return fail(TFSM_E_INVALID, "3D convection needs 16, 32, 64 or 128 points per axis, or a cubic 48, 96 or 100 mesh");
   if (terms->n_linear < 0 || terms->n_linear > 4) return fail(TFSM_E_INVALID, "at most 4 linear terms");
   if (terms->n_linear > 0 && !terms->linear_coef) return fail(TFSM_E_INVALID, "linear coefficients missing");
   const int Bc = terms->coef_batch < 1 ? 1 : terms->coef_batch;
@@ -763,7 +771,7 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     if (sb >= p->B || (size_t)sb * per_sample > ((size_t)256 << 20)) sb = 0;
     p->sub_batch = sb;
   }
-  if (!rc && D == 3) p->row_var = p->n[2] >= 64 ? 1 : 0;  // radix-4 z rows (fewer registers)
+  if (!rc && D == 3) p->row_var = (p->n[2] >= 64 && is_pow2(p->n[2])) ? 1 : 0;  // radix-4 z rows (fewer registers)
   if (rc) {
     tfsm_plan_destroy(p);
     return rc;
@@ -802,6 +810,9 @@ int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int
   if (!plan) return fail(TFSM_E_INVALID, "null plan");
   if (plan->D == 3 && (row_var < 0 || row_var > 1 || col_var < 0 || col_var > 1))
     return fail(TFSM_E_INVALID, "unsupported 3D tuning");
+  if (plan->D == 3 && (col_var == 1 || row_var == 1) &&
+      (!is_pow2(plan->n[0]) || !is_pow2(plan->n[1]) || !is_pow2(plan->n[2])))
+    return fail(TFSM_E_INVALID, "the radix-4 3D plans need power-of-two axes");
   if (plan->D == 2) {
     if (!col2d_supported(plan->n[0], col_tile, col_var)) return fail(TFSM_E_INVALID, "unsupported column tuning");
     if ((plan->P % col_tile) != 0) return fail(TFSM_E_INVALID, "column tile must divide the pitch");

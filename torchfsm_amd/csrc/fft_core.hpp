@@ -12,6 +12,7 @@
 //     whole 128-byte rows of a column tile (the HBM layout of the tile, no transpose needed);
 //   * G independent groups (contiguous rows), each with its own LDS region.
 //
+// Lengths may mix radices 2, 4, 8, 16 with 3, 5, 7 (Stockham needs no particular order of them).
 // Twiddles come from per-stage tables (Fft<N>::tw_offset, computed in double on the host, rounded
 // once), staged in LDS at kernel start.
 #pragma once
@@ -104,13 +105,81 @@ __device__ __forceinline__ void dft16(float2* x) {
   }
 }
 
+// Odd prime radix R in {3, 5, 7}, direct from the definition folded over the conjugate pairs
+// (n, R-n):  X_k = x_0 + sum_{n=1}^{(R-1)/2} [ (x_n + x_{R-n}) cos(2 pi n k/R)
+//                                            + DIR i (x_n - x_{R-n}) sin(2 pi n k/R) ],
+// X_{R-k} the same with the sine part negated.
+template <int R>
+struct OddRoots;
+template <>
+struct OddRoots<3> {
+  static constexpr float C[3] = {1.0f, -0.5f, -0.5f};
+  static constexpr float S[3] = {0.0f, 0.86602540378443864676f, -0.86602540378443864676f};
+};
+template <>
+struct OddRoots<5> {
+  static constexpr float C[5] = {1.0f, 0.30901699437494742410f, -0.80901699437494742410f, -0.80901699437494742410f,
+                                 0.30901699437494742410f};
+  static constexpr float S[5] = {0.0f, 0.95105651629515357212f, 0.58778525229247312917f, -0.58778525229247312917f,
+                                 -0.95105651629515357212f};
+};
+template <>
+struct OddRoots<7> {
+  static constexpr float C[7] = {1.0f,
+                                 0.62348980185873353053f,
+                                 -0.22252093395631440429f,
+                                 -0.90096886790241912624f,
+                                 -0.90096886790241912624f,
+                                 -0.22252093395631440429f,
+                                 0.62348980185873353053f};
+  static constexpr float S[7] = {0.0f,
+                                 0.78183148246802980871f,
+                                 0.97492791218182360702f,
+                                 0.43388373911755812048f,
+                                 -0.43388373911755812048f,
+                                 -0.97492791218182360702f,
+                                 -0.78183148246802980871f};
+};
+
+template <int R, int DIR>
+__device__ __forceinline__ void dft_odd(float2* x) {
+  constexpr int H = (R - 1) / 2;
+  float2 t[H + 1], d[H + 1];
+#pragma unroll
+  for (int n = 1; n <= H; ++n) {
+    t[n] = cadd(x[n], x[R - n]);
+    d[n] = csub(x[n], x[R - n]);
+  }
+  float2 out[R];
+  float2 s0 = x[0];
+#pragma unroll
+  for (int n = 1; n <= H; ++n) s0 = cadd(s0, t[n]);
+  out[0] = s0;
+#pragma unroll
+  for (int k = 1; k <= H; ++k) {
+    float2 a = x[0], b = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int n = 1; n <= H; ++n) {
+      const float c = OddRoots<R>::C[(n * k) % R], sn = OddRoots<R>::S[(n * k) % R];
+      a = make_float2(a.x + c * t[n].x, a.y + c * t[n].y);
+      b = make_float2(b.x + sn * d[n].x, b.y + sn * d[n].y);
+    }
+    const float2 ib = mul_i<DIR>(b);
+    out[k] = cadd(a, ib);
+    out[R - k] = csub(a, ib);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) x[r] = out[r];
+}
+
 template <int R, int DIR>
 __device__ __forceinline__ void dft(float2* x) {
   if constexpr (R == 2) dft2<DIR>(x);
   else if constexpr (R == 4) dft4<DIR>(x);
   else if constexpr (R == 8) dft8<DIR>(x);
   else if constexpr (R == 16) dft16<DIR>(x);
-  else static_assert(R == 2 || R == 4 || R == 8 || R == 16, "unsupported radix");
+  else if constexpr (R == 3 || R == 5 || R == 7) dft_odd<R, DIR>(x);
+  else static_assert(R == 2, "unsupported radix");
 }
 
 // ---------------------------------------------------------------- radix plans
@@ -118,28 +187,40 @@ __device__ __forceinline__ void dft(float2* x) {
 template <int N, int VAR = 0>
 struct FftPlan;
 // VAR 0: default plans
-template <> struct FftPlan<16, 0> { static constexpr int RMAX = 4; static constexpr int NST = 2; static constexpr int R[4] = {4, 4, 0, 0}; };
-template <> struct FftPlan<32, 0> { static constexpr int RMAX = 8; static constexpr int NST = 2; static constexpr int R[4] = {8, 4, 0, 0}; };
-template <> struct FftPlan<64, 0> { static constexpr int RMAX = 8; static constexpr int NST = 2; static constexpr int R[4] = {8, 8, 0, 0}; };
-template <> struct FftPlan<128, 0> { static constexpr int RMAX = 16; static constexpr int NST = 2; static constexpr int R[4] = {16, 8, 0, 0}; };
-template <> struct FftPlan<256, 0> { static constexpr int RMAX = 16; static constexpr int NST = 2; static constexpr int R[4] = {16, 16, 0, 0}; };
-template <> struct FftPlan<512, 0> { static constexpr int RMAX = 8; static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 8, 0}; };
-template <> struct FftPlan<1024, 0> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 8, 8, 0}; };
-template <> struct FftPlan<2048, 0> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 16, 8, 0}; };
-template <> struct FftPlan<4096, 0> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {16, 16, 16, 0}; };
+template <> struct FftPlan<16, 0> { static constexpr int RMAX = 4; static constexpr int NST = 2; static constexpr int R[5] = {4, 4, 0, 0, 0}; };
+template <> struct FftPlan<32, 0> { static constexpr int RMAX = 8; static constexpr int NST = 2; static constexpr int R[5] = {8, 4, 0, 0, 0}; };
+template <> struct FftPlan<64, 0> { static constexpr int RMAX = 8; static constexpr int NST = 2; static constexpr int R[5] = {8, 8, 0, 0, 0}; };
+template <> struct FftPlan<128, 0> { static constexpr int RMAX = 16; static constexpr int NST = 2; static constexpr int R[5] = {16, 8, 0, 0, 0}; };
+template <> struct FftPlan<256, 0> { static constexpr int RMAX = 16; static constexpr int NST = 2; static constexpr int R[5] = {16, 16, 0, 0, 0}; };
+template <> struct FftPlan<512, 0> { static constexpr int RMAX = 8; static constexpr int NST = 3; static constexpr int R[5] = {8, 8, 8, 0, 0}; };
+template <> struct FftPlan<1024, 0> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[5] = {16, 8, 8, 0, 0}; };
+template <> struct FftPlan<2048, 0> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[5] = {16, 16, 8, 0, 0}; };
+template <> struct FftPlan<4096, 0> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[5] = {16, 16, 16, 0, 0}; };
+// Mixed-radix lengths (even N with factors 3, 5, 7): RMAX is a common multiple of the radices,
+// so every stage's R divides the values a thread holds.
+template <> struct FftPlan<48, 0> { static constexpr int RMAX = 12; static constexpr int NST = 3; static constexpr int R[5] = {4, 4, 3, 0, 0}; };
+template <> struct FftPlan<96, 0> { static constexpr int RMAX = 12; static constexpr int NST = 4; static constexpr int R[5] = {4, 4, 2, 3, 0}; };
+template <> struct FftPlan<100, 0> { static constexpr int RMAX = 20; static constexpr int NST = 3; static constexpr int R[5] = {4, 5, 5, 0, 0}; };
+template <> struct FftPlan<112, 0> { static constexpr int RMAX = 28; static constexpr int NST = 3; static constexpr int R[5] = {4, 4, 7, 0, 0}; };
+template <> struct FftPlan<192, 0> { static constexpr int RMAX = 12; static constexpr int NST = 4; static constexpr int R[5] = {4, 4, 4, 3, 0}; };
+template <> struct FftPlan<200, 0> { static constexpr int RMAX = 20; static constexpr int NST = 4; static constexpr int R[5] = {4, 2, 5, 5, 0}; };
+template <> struct FftPlan<384, 0> { static constexpr int RMAX = 12; static constexpr int NST = 5; static constexpr int R[5] = {4, 4, 4, 2, 3}; };
+template <> struct FftPlan<400, 0> { static constexpr int RMAX = 20; static constexpr int NST = 4; static constexpr int R[5] = {4, 4, 5, 5, 0}; };
+template <> struct FftPlan<768, 0> { static constexpr int RMAX = 12; static constexpr int NST = 5; static constexpr int R[5] = {4, 4, 4, 4, 3}; };
+template <> struct FftPlan<800, 0> { static constexpr int RMAX = 20; static constexpr int NST = 5; static constexpr int R[5] = {4, 4, 2, 5, 5}; };
 // VAR 1: twice the elements per thread (half the threads per transform), same radix sequence
 // where possible; lets a column tile of the same width run with half the workgroup size.
-template <> struct FftPlan<64, 1> { static constexpr int RMAX = 16; static constexpr int NST = 2; static constexpr int R[4] = {8, 8, 0, 0}; };
-template <> struct FftPlan<128, 1> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {8, 4, 4, 0}; };
-template <> struct FftPlan<256, 1> { static constexpr int RMAX = 8; static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 4, 0}; };
-template <> struct FftPlan<512, 1> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[4] = {8, 8, 8, 0}; };
+template <> struct FftPlan<64, 1> { static constexpr int RMAX = 16; static constexpr int NST = 2; static constexpr int R[5] = {8, 8, 0, 0, 0}; };
+template <> struct FftPlan<128, 1> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[5] = {8, 4, 4, 0, 0}; };
+template <> struct FftPlan<256, 1> { static constexpr int RMAX = 8; static constexpr int NST = 3; static constexpr int R[5] = {8, 8, 4, 0, 0}; };
+template <> struct FftPlan<512, 1> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[5] = {8, 8, 8, 0, 0}; };
 // VAR 2: 8 elements per thread where VAR 0 holds 16 (lower register use in the 3D kernels)
-template <> struct FftPlan<128, 2> { static constexpr int RMAX = 8; static constexpr int NST = 3; static constexpr int R[4] = {8, 4, 4, 0}; };
+template <> struct FftPlan<128, 2> { static constexpr int RMAX = 8; static constexpr int NST = 3; static constexpr int R[5] = {8, 4, 4, 0, 0}; };
 // VAR 3: 4 elements per thread (radix 4, one radix-2 stage where needed): the fewest registers
-template <> struct FftPlan<16, 3> { static constexpr int RMAX = 4; static constexpr int NST = 2; static constexpr int R[4] = {4, 4, 0, 0}; };
-template <> struct FftPlan<32, 3> { static constexpr int RMAX = 4; static constexpr int NST = 3; static constexpr int R[4] = {4, 4, 2, 0}; };
-template <> struct FftPlan<64, 3> { static constexpr int RMAX = 4; static constexpr int NST = 3; static constexpr int R[4] = {4, 4, 4, 0}; };
-template <> struct FftPlan<128, 3> { static constexpr int RMAX = 4; static constexpr int NST = 4; static constexpr int R[4] = {4, 4, 4, 2}; };
+template <> struct FftPlan<16, 3> { static constexpr int RMAX = 4; static constexpr int NST = 2; static constexpr int R[5] = {4, 4, 0, 0, 0}; };
+template <> struct FftPlan<32, 3> { static constexpr int RMAX = 4; static constexpr int NST = 3; static constexpr int R[5] = {4, 4, 2, 0, 0}; };
+template <> struct FftPlan<64, 3> { static constexpr int RMAX = 4; static constexpr int NST = 3; static constexpr int R[5] = {4, 4, 4, 0, 0}; };
+template <> struct FftPlan<128, 3> { static constexpr int RMAX = 4; static constexpr int NST = 4; static constexpr int R[5] = {4, 4, 4, 2, 0}; };
 
 template <int N, int VAR = 0>
 struct Fft {
@@ -161,6 +242,15 @@ struct Fft {
     return off;
   }
   static constexpr int tw_size() { return tw_offset(NST) > 0 ? tw_offset(NST) : 1; }
+  static constexpr bool plan_ok() {
+    int p = 1;
+    for (int i = 0; i < NST; ++i) {
+      if (RMAX % radix(i) != 0) return false;
+      p *= radix(i);
+    }
+    return p == N && NST <= 5;
+  }
+  static_assert(plan_ok(), "radix plan must multiply to N and each radix must divide RMAX");
 };
 
 // LDS slot of position p of interleaved transform vi (V transforms interleaved).
@@ -245,6 +335,28 @@ __device__ __forceinline__ void fft_wg(float2 (&v)[(Fft<N, VAR>::RMAX)], float2*
   if constexpr (F::NST > 1) fft_stage<N, 1, DIR, V, VAR, WS>(v, lds, tw, j, vi);
   if constexpr (F::NST > 2) fft_stage<N, 2, DIR, V, VAR, WS>(v, lds, tw, j, vi);
   if constexpr (F::NST > 3) fft_stage<N, 3, DIR, V, VAR, WS>(v, lds, tw, j, vi);
+  if constexpr (F::NST > 4) fft_stage<N, 4, DIR, V, VAR, WS>(v, lds, tw, j, vi);
+}
+
+// Supported lengths.  Powers of two, and the mixed-radix lengths above (each also needs its
+// kernels instantiated, which the size lists below drive).
+#define TFSM_MIXED_SIZES_1D(X) X(48) X(96) X(100) X(112) X(192) X(200) X(384) X(400) X(768) X(800)
+#define TFSM_MIXED_SIZES_2D(X) X(48) X(96) X(100) X(112) X(192) X(200)
+#define TFSM_MIXED_SIZES_3D(X) X(48) X(96) X(100)
+constexpr bool is_mixed_size(int n) {
+#define TFSM_MS_EQ(N) n == N ||
+  return TFSM_MIXED_SIZES_1D(TFSM_MS_EQ) false;
+#undef TFSM_MS_EQ
+}
+constexpr bool is_mixed_size_2d(int n) {
+#define TFSM_MS_EQ(N) n == N ||
+  return TFSM_MIXED_SIZES_2D(TFSM_MS_EQ) false;
+#undef TFSM_MS_EQ
+}
+constexpr bool is_mixed_size_3d(int n) {
+#define TFSM_MS_EQ(N) n == N ||
+  return TFSM_MIXED_SIZES_3D(TFSM_MS_EQ) false;
+#undef TFSM_MS_EQ
 }
 
 // row transforms (V = 1): wave-local synchronisation whenever a row fits one wavefront

@@ -323,7 +323,7 @@ __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) 
     const int pos = j + m * NT;
     if (pos <= N2 / 2) {
       const float2 zk = z[m];
-      const float2 zm = lds[lds_slot<1>((N2 - pos) & (N2 - 1), 0)];
+      const float2 zm = lds[lds_slot<1>(pos == 0 ? 0 : N2 - pos, 0)];
       pb[pos] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
       const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);
       pb[cs + pos] = make_float2(0.5f * d.y, -0.5f * d.x);
@@ -347,6 +347,9 @@ __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) 
     case 32: { constexpr int NN = 32; __VA_ARGS__ }    \
     case 64: { constexpr int NN = 64; __VA_ARGS__ }    \
     case 128: { constexpr int NN = 128; __VA_ARGS__ }  \
+    case 48: { constexpr int NN = 48; __VA_ARGS__ }  \
+    case 96: { constexpr int NN = 96; __VA_ARGS__ }  \
+    case 100: { constexpr int NN = 100; __VA_ARGS__ }  \
     default: return hipErrorInvalidValue;              \
   }
 
@@ -376,22 +379,37 @@ static hipError_t launch_yinv3d_t(const Yinv3Args& a, hipStream_t st) {
 // col_var 0: the 3D default radix plan of the axis (var3), 1: radix-4 plan (4 values per thread)
 hipError_t launch_col3d(const Col3Args& a, int nb, int stage, int col_var, hipStream_t st) {
   TFSM_SWITCH3(a.N0, {
-    if (col_var == 1) return launch_col3d_t<NN, 3>(a, nb, stage, st);
+    if constexpr (!is_mixed_size(NN)) {
+      if (col_var == 1) return launch_col3d_t<NN, 3>(a, nb, stage, st);
+    }
     return launch_col3d_t<NN, (NN == 128 ? 2 : 0)>(a, nb, stage, st);
   })
 }
 
 hipError_t launch_yinv3d(const Yinv3Args& a, int col_var, hipStream_t st) {
   TFSM_SWITCH3(a.N1, {
-    if (col_var == 1) return launch_yinv3d_t<NN, 3>(a, st);
+    if constexpr (!is_mixed_size(NN)) {
+      if (col_var == 1) return launch_yinv3d_t<NN, 3>(a, st);
+    }
     return launch_yinv3d_t<NN, (NN == 128 ? 2 : 0)>(a, st);
   })
+}
+
+// rows per workgroup of the 3D row kernel: about 256 threads, a divisor of NN^2 (the row count of
+// one sample of a cubic mesh) whose staged fields fit 96 KiB of LDS at the largest field pitch
+// (kmax <= NN/3)
+template <int NN, int NT>
+constexpr int rows3d() {
+  int r = NT >= 256 ? 1 : 256 / NT;
+  const int pf = ((NN / 3 + 1) + 15) / 16 * 16;
+  while (r > 1 && ((NN * NN) % r != 0 || 9 * r * pf * 8 > 96 * 1024)) --r;
+  return r;
 }
 
 template <int NN, int VR>
 static hipError_t launch_row3d_t(const Row3Args& a, hipStream_t st) {
   constexpr int NT = Fft<NN, VR>::NT;
-  constexpr int R = NT >= 256 ? 1 : 256 / NT;
+  constexpr int R = rows3d<NN, NT>();
   const size_t rows = (size_t)a.nb * a.N0 * a.N1;
   // whole workgroups of one sample each; staged fields in dynamic LDS
   if ((a.N0 * a.N1) % R != 0 || (a.PF2 % 2) != 0) return hipErrorInvalidValue;
@@ -404,14 +422,18 @@ static hipError_t launch_row3d_t(const Row3Args& a, hipStream_t st) {
 // row_var 0: the 3D default radix plan of the axis (var3), 1: radix-4 plan (4 values per thread)
 hipError_t launch_row3d(const Row3Args& a, int row_var, hipStream_t st) {
   TFSM_SWITCH3(a.N2, {
-    if (row_var == 1) return launch_row3d_t<NN, 3>(a, st);
+    if constexpr (!is_mixed_size(NN)) {
+      if (row_var == 1) return launch_row3d_t<NN, 3>(a, st);
+    }
     return launch_row3d_t<NN, (NN == 128 ? 2 : 0)>(a, st);
   })
 }
 
+// powers of two 16..128 per axis, or a cubic mesh of one of the mixed-radix 3D sizes
 bool step3d_supported(int n0, int n1, int n2) {
   auto ok = [](int n) { return n == 16 || n == 32 || n == 64 || n == 128; };
-  return ok(n0) && ok(n1) && ok(n2);
+  if (ok(n0) && ok(n1) && ok(n2)) return true;
+  return n0 == n1 && n1 == n2 && is_mixed_size_3d(n0);
 }
 
 }  // namespace tfsm

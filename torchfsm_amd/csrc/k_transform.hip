@@ -341,6 +341,16 @@ static constexpr int rows_per_wg() {
     case 1024: { constexpr int NN = 1024; BODY; } \
     case 2048: { constexpr int NN = 2048; BODY; } \
     case 4096: { constexpr int NN = 4096; BODY; } \
+    case 48: { constexpr int NN = 48; BODY; } \
+    case 96: { constexpr int NN = 96; BODY; } \
+    case 100: { constexpr int NN = 100; BODY; } \
+    case 112: { constexpr int NN = 112; BODY; } \
+    case 192: { constexpr int NN = 192; BODY; } \
+    case 200: { constexpr int NN = 200; BODY; } \
+    case 384: { constexpr int NN = 384; BODY; } \
+    case 400: { constexpr int NN = 400; BODY; } \
+    case 768: { constexpr int NN = 768; BODY; } \
+    case 800: { constexpr int NN = 800; BODY; } \
     default: return hipErrorInvalidValue; \
   }
 
@@ -452,6 +462,16 @@ hipError_t launch_run1d(const Run1Args& a, hipStream_t st) {
     case 1024: return launch_run1_t<1024>(a, st);
     case 2048: return launch_run1_t<2048>(a, st);
     case 4096: return launch_run1_t<4096>(a, st);
+    case 48: return launch_run1_t<48>(a, st);
+    case 96: return launch_run1_t<96>(a, st);
+    case 100: return launch_run1_t<100>(a, st);
+    case 112: return launch_run1_t<112>(a, st);
+    case 192: return launch_run1_t<192>(a, st);
+    case 200: return launch_run1_t<200>(a, st);
+    case 384: return launch_run1_t<384>(a, st);
+    case 400: return launch_run1_t<400>(a, st);
+    case 768: return launch_run1_t<768>(a, st);
+    case 800: return launch_run1_t<800>(a, st);
     default: return hipErrorInvalidValue;
   }
 }
