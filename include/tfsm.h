@@ -34,6 +34,9 @@ extern "C" {
 #define TFSM_TERM_VC 2        /* VorticityConvection, D == 2, C == 1     (_navier_stokes.py:27-46) */
 #define TFSM_TERM_NSP 3       /* NSPressureConvection without force: -(I - k k^T/|k|^2) applied to
                                  Convection, C == D in {2, 3}       (_navier_stokes.py:221-262) */
+#define TFSM_TERM_KS 4        /* KSConvection(remove_mean=True): 1/2 |grad phi|^2 minus its mean over
+                                 batch and space, C == 1, D in {1, 2, 3} (_ks_convection.py:8-38) */
+#define TFSM_TERM_KS_KEEP_MEAN 5  /* KSConvection(remove_mean=False)                             */
 
 /* linear symbols (LinearCoef cores) */
 #define TFSM_LIN_LAP 1        /* Laplacian       (_laplacian.py:12-15)  */

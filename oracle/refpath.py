@@ -11,6 +11,8 @@ An operator is described by a list of terms in generator order, each ``(coef, ki
   kind "unit"    ones_like(bf_x) (ImplicitSource() drag)        (_source.py:9-17)
   kind "conv"    Convection, dealiased input                    (_convection.py:18-46)
   kind "vc"      VorticityConvection, dealiased input           (_navier_stokes.py:27-46)
+  kind "nsp"     NSPressureConvection, dealiased input          (_navier_stokes.py:221-262)
+  kind "ks"      KSConvection (remove_mean), dealiased input    (_ks_convection.py:18-38)
   kind "src"     ExplicitSource; payload = spatial source field (_base.py:994-1015)
 """
 
@@ -21,7 +23,7 @@ import torch
 
 Term = Tuple[object, str, Optional[torch.Tensor]]
 LINEAR_KINDS = ("lap", "biharm", "unit")
-NONLINEAR_KINDS = ("conv", "vc", "nsp", "src")
+NONLINEAR_KINDS = ("conv", "vc", "nsp", "ks", "ks_keep_mean", "src")
 
 
 class Mesh:
@@ -144,6 +146,16 @@ def ns_pressure_convection(u_fft, mesh: Mesh, u=None):
     return nab * p - conv
 
 
+def ks_convection(u_fft, mesh: Mesh, u=None, remove_mean=True):
+    """_KSConvectionCore (_ks_convection.py:18-38): 1/2 sum_i (d_i phi)^2, minus its mean over the
+    whole tensor (batch included) when remove_mean."""
+    grad_u = mesh.ifft(mesh.nabla_vector(1) * u_fft).real
+    re = 0.5 * torch.sum(grad_u**2, dim=1, keepdim=True)
+    if remove_mean:
+        re = re - re.mean()
+    return mesh.fft(re)
+
+
 def source_spectrum(src: torch.Tensor):
     """_ExplicitSourceCore.__init__ (_base.py:1002-1005): fftn over all dims after (B, C)."""
     return torch.fft.fftn(src, dim=[i + 2 for i in range(src.dim() - 2)])
@@ -171,7 +183,7 @@ def build_closure(mesh: Mesh, terms: List[Term], dealias_rate=2 / 3) -> Optional
     for c, k, p in terms:
         if k == "src":
             nl.append((c, k, source_spectrum(p).to(mesh.device)))
-        elif k in ("conv", "vc", "nsp"):
+        elif k in ("conv", "vc", "nsp", "ks", "ks_keep_mean"):
             nl.append((c, k, None))
     if not nl:
         return None
@@ -182,12 +194,15 @@ def build_closure(mesh: Mesh, terms: List[Term], dealias_rate=2 / 3) -> Optional
         d_u = None
         u = None
         for c, k, p in nl:
-            if k in ("conv", "vc", "nsp"):
+            if k in ("conv", "vc", "nsp", "ks", "ks_keep_mean"):
                 if d_fft is None:
                     d_fft = u_fft * mesh.mask(dealias_rate)
                     d_u = mesh.ifft(d_fft).real
-                f = {"conv": convection, "vc": vorticity_convection, "nsp": ns_pressure_convection}[k]
-                result += c * f(d_fft, mesh, d_u)
+                if k in ("ks", "ks_keep_mean"):
+                    result += c * ks_convection(d_fft, mesh, d_u, remove_mean=(k == "ks"))
+                else:
+                    f = {"conv": convection, "vc": vorticity_convection, "nsp": ns_pressure_convection}[k]
+                    result += c * f(d_fft, mesh, d_u)
             else:  # explicit source: no dealiasing, u computed but unused (_base.py:394-401)
                 if u is None:
                     u = mesh.ifft(u_fft).real
@@ -434,6 +449,11 @@ def ns_velocity_terms(Re) -> List[Term]:
 def ks_terms() -> List[Term]:
     """pde.KuramotoSivashinsky (pde.py:27-39): -Laplacian() - Biharmonic() - Convection()."""
     return [(-1, "lap", None), (-1, "biharm", None), (-1, "conv", None)]
+
+
+def ks_highdim_terms(remove_mean=True) -> List[Term]:
+    """pde.KuramotoSivashinskyHighDim (pde.py:41-49): -Laplacian() - Biharmonic() - KSConvection()."""
+    return [(-1, "lap", None), (-1, "biharm", None), (-1, "ks" if remove_mean else "ks_keep_mean", None)]
 
 
 def ns_terms(Re, y=None, drag=-0.1, k=4.0, length_scale=1.0) -> List[Term]:

@@ -332,3 +332,40 @@ def test_rk_agrees_with_etd_on_burgers():
     assert d[0] < 1e-8 and math.log2(d[0] / d[1]) > 3.5, d
     e = R.integrate(terms, [(0, 1, n)], u0, 1e-3, 200, scheme="Euler")
     assert 1e-4 < rel(e.numpy(), b.numpy()) < 1e-2
+
+
+# ---------------------------------------------------------------- KSConvection
+def test_ks_convection_closed_form_and_batch_mean():
+    """1/2 |grad phi|^2 for phi_b = s_b (sin x + cos 2y) on [0, 2 pi)^2: 1/2 s^2 (cos^2 x + 4 sin^2 2y)
+    = s^2 (5/4 + 1/4 cos 2x - cos 4y); remove_mean subtracts the mean over the whole batch
+    (5/4 mean_b s_b^2), so the samples' DC differ from zero in opposite directions."""
+    n = 32
+    mi = [(0, 2 * math.pi, n)] * 2
+    mesh = R.Mesh(mi, dtype=torch.float64)
+    x = torch.arange(n, dtype=torch.float64) * 2 * math.pi / n
+    X, Y = torch.meshgrid(x, x, indexing="ij")
+    s = torch.tensor([1.0, 2.0], dtype=torch.float64).reshape(2, 1, 1, 1)
+    phi = s * (torch.sin(X) + torch.cos(2 * Y))
+    base = 5 / 4 + 0.25 * torch.cos(2 * X) - torch.cos(4 * Y)
+    keep = mesh.ifft(R.ks_convection(mesh.fft(phi), mesh, remove_mean=False)).real
+    assert torch.allclose(keep, s**2 * base, atol=1e-12)
+    rem = mesh.ifft(R.ks_convection(mesh.fft(phi), mesh, remove_mean=True)).real
+    assert torch.allclose(rem, s**2 * base - 5 / 4 * 2.5, atol=1e-12)
+    assert abs(float(rem[0].mean()) + 1.875) < 1e-12 and abs(float(rem[1].mean()) - 1.875) < 1e-12
+
+
+def test_ks_highdim_1d_is_the_derivative_of_ks():
+    """d/dx of phi_t = -phi_xx - phi_xxxx - 1/2 phi_x^2 is the KS equation of v = phi_x with
+    Convection; the dealias mask and the ETD combines act per mode, so in float64 the two
+    integrations agree mode by mode: v^ = i a phi^ at every step (the mean only moves k = 0)."""
+    n, L = 64, 22.0
+    mi = [(0, L, n)]
+    mesh = R.Mesh(mi, dtype=torch.float64)
+    x = torch.arange(n, dtype=torch.float64) * L / n
+    phi0 = (torch.cos(2 * math.pi * x / L) + 0.3 * torch.sin(4 * math.pi * x / L)).reshape(1, 1, n)
+    v0 = mesh.ifft(mesh.nabla_vector(1) * mesh.fft(phi0)).real
+    for remove_mean in (True, False):
+        ph = R.integrate(R.ks_highdim_terms(remove_mean), mi, phi0, 0.05, 40, scheme="SETDRK4", return_in_fourier=True)
+        v = R.integrate(R.ks_terms(), mi, v0, 0.05, 40, scheme="SETDRK4", return_in_fourier=True)
+        dv = mesh.nabla_vector(1) * ph
+        assert rel(dv.numpy(), v.numpy()) < 1e-10

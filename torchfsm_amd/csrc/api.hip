@@ -86,6 +86,7 @@ struct tfsm_plan {
   int H = 0, P = 0;
   size_t modes = 0;  // complex elements of one [N...] block in half-spectrum layout
   int term = TERM_NONE;
+  bool ks_mean = false;  // KSConvection(remove_mean=True): the batch mean of the product is removed
   float coef_t = 0.f;
   double rate = 2.0 / 3.0;
   int kmax[3] = {0, 0, 0};
@@ -116,6 +117,8 @@ struct tfsm_plan {
   int PF = 0, NF = 0;
   int K1 = 0;  // 3D: kept ky values (2 kmax1 + 1)
   size_t off_A1 = 0, off_A2 = 0, off_P = 0, off_G = 0, off_G2 = 0, ws_bytes = 0;
+  size_t off_DC = 0;              // KSConvection: batch mean of the product DC (one complex value)
+  size_t off_S = 0, off_N = 0;    // 1D KSConvection: stage spectrum and raw product spectrum
   std::vector<void*> allocs;
   // adjoint (tfsm_adjoint_prepare)
   float* dcoef = nullptr;  // [Bc][3][modes]
@@ -278,6 +281,7 @@ Col2Args col_args(tfsm_plan* p, float2* U, char* ws) {
   c.src_bstride = p->src_bstride();
   c.src_cstride = p->src_cstride();
   c.G = (float2*)(ws + p->off_G);
+  c.dc = p->ks_mean ? (const float2*)(ws + p->off_DC) : nullptr;
   c.b0 = 0;
   c.nb = p->B;
   return c;
@@ -300,6 +304,11 @@ int run2d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hip
   if (stage != ST_PRO) {
     int rc = launch_row(p, ws, b0, nb, st);
     if (rc) return rc;
+    // KSConvection's mean couples the batch (the plan never splits it into sub-batches)
+    if (p->ks_mean)
+      HIP_TRY(launch_ks_dc((const float2*)(ws + p->off_P), p->B, p->modes, p->n[0], (size_t)p->P,
+                           (float2*)(ws + p->off_DC), st),
+              "batch-mean launch");
   }
   return launch_col(p, U, ws, stage, b0, nb, st);
 }
@@ -319,6 +328,7 @@ Col3Args col3_args(tfsm_plan* p, float2* U, char* ws, int b0, int nb) {
   c.kmax1 = p->kmax[1];
   c.kmax2 = p->kmax[2];
   c.term = p->term;
+  c.nc = p->C;
   c.coef_t = p->coef_t;
   c.scale = (float)(1.0 / ((double)p->n[0] * p->n[1] * p->n[2]));
   c.tw0 = p->tw[0][p->col_var == 1 ? 3 : var3(p->n[0])];
@@ -335,6 +345,7 @@ Col3Args col3_args(tfsm_plan* p, float2* U, char* ws, int b0, int nb) {
   c.src_bstride = p->src_bstride();
   c.src_cstride = p->src_cstride();
   c.G1 = (float2*)(ws + p->off_G);
+  c.dc = p->ks_mean ? (const float2*)(ws + p->off_DC) : nullptr;
   c.b0 = b0;
   c.nb = nb;
   return c;
@@ -348,6 +359,7 @@ int run3d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hip
   float2* Pp = (float2*)(ws + p->off_P);
   if (stage != ST_PRO && (parts & 1)) {
     Row3Args r;
+    r.term = p->term;
     r.N0 = N0;
     r.N1 = N1;
     r.N2 = p->n[2];
@@ -363,10 +375,13 @@ int run3d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hip
     HIP_TRY(launch_row3d(r, p->row_var, st), "3D row kernel launch");
   }
   if (stage != ST_PRO && (parts & 2)) {
-    float2* Pb = Pp + (size_t)b0 * 3 * p->modes;
-    HIP_TRY(launch_cols_c2c(N1, -1, Pb, Pb, (size_t)nb * 3 * N0, (size_t)N1 * p->P, (size_t)p->P, p->H, 1.0f,
+    float2* Pb = Pp + (size_t)b0 * p->C * p->modes;
+    HIP_TRY(launch_cols_c2c(N1, -1, Pb, Pb, (size_t)nb * p->C * N0, (size_t)N1 * p->P, (size_t)p->P, p->H, 1.0f,
                             p->tw[1][0], st),
             "3D y-forward launch");
+    if (p->ks_mean)
+      HIP_TRY(launch_ks_dc(Pp, p->B, p->modes, N0, (size_t)N1 * p->P, (float2*)(ws + p->off_DC), st),
+              "batch-mean launch");
   }
   if (parts & 4) HIP_TRY(launch_col3d(col3_args(p, U, ws, b0, nb), nb, stage, p->col_var, st), "3D column kernel launch");
   if (!(parts & 8)) return TFSM_OK;
@@ -377,6 +392,7 @@ int run3d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hip
   y.K1 = p->K1;
   y.kmax1 = p->kmax[1];
   y.kmax2 = p->kmax[2];
+  y.nc = p->C;
   y.tw1 = p->tw[1][p->col_var == 1 ? 3 : var3(N1)];
   y.ay = p->kv[1];
   y.G1 = (const float2*)(ws + p->off_G);
@@ -429,6 +445,25 @@ int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* tra
     return TFSM_OK;
   }
   if (!ws) return fail(TFSM_E_INVALID, "workspace pointer is null");
+  if (p->D == 1 && p->term == TERM_KS) {
+    // one launch per stage (the batch mean couples the samples): raw product -> mean -> combine
+    Run1Args a = run1_args(p, U, (char*)ws);
+    float2* S = (float2*)((char*)ws + p->off_S);
+    float2* Nr = (float2*)((char*)ws + p->off_N);
+    float2* dc = p->ks_mean ? (float2*)((char*)ws + p->off_DC) : nullptr;
+    const int nev = scheme_evals(p->scheme);
+    const size_t per = (size_t)p->C * p->modes;
+    for (int i = 0; i < nsteps; ++i) {
+      if (rec_slot) HIP_TRY(launch_record(U, (float2*)traj, rec_slot, i, p->B, per, T, st), "record launch");
+      for (int e = 0; e < nev; ++e) {
+        HIP_TRY(launch_k1_eval_ks(a, e == 0 ? U : S, Nr, st), "1D product launch");
+        if (dc) HIP_TRY(launch_ks_dc(Nr, a.B, (size_t)p->P, 1, 0, dc, st), "batch-mean launch");
+        HIP_TRY(launch_k1_stage(a, scheme_stage(p->scheme, e), Nr, dc, S, st), "1D stage launch");
+      }
+    }
+    if (rec_slot) HIP_TRY(launch_record(U, (float2*)traj, rec_slot, nsteps, p->B, per, T, st), "record launch");
+    return TFSM_OK;
+  }
   if (p->D == 1) {
     Run1Args a = run1_args(p, U, (char*)ws);
     a.nsteps = nsteps;
@@ -617,9 +652,13 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
                                   : D == 2 ? "2D mesh sizes per axis: powers of two 16..512, or 48, 96, 100, 112, 192, 200"
                                            : "3D mesh sizes per axis: powers of two 16..128, or 48, 96, 100");
   }
-  const int term = terms->nonlinear;
-  if (term != TERM_NONE && term != TERM_CONV && term != TERM_VC && term != TERM_NSP)
+  const bool ks_keep = terms->nonlinear == TFSM_TERM_KS_KEEP_MEAN;
+  const int term = ks_keep ? TERM_KS : terms->nonlinear;
+  if (term != TERM_NONE && term != TERM_CONV && term != TERM_VC && term != TERM_NSP && term != TERM_KS)
     return fail(TFSM_E_INVALID, "unknown nonlinear term");
+  if (term == TERM_KS && geom->channels != 1) return fail(TFSM_E_INVALID, "KSConvection only supports scalar field");
+  if (term == TERM_KS && D == 3 && !step3d_supported(geom->n[0], geom->n[1], geom->n[2]))
+    return fail(TFSM_E_INVALID, "3D KSConvection needs 16, 32, 64 or 128 points per axis, or a cubic 48, 96 or 100 mesh");
   if ((term == TERM_CONV || term == TERM_NSP) && geom->channels != D)
     return fail(TFSM_E_INVALID, "convection operator only works for vector field with the same dimension as mesh");
   if (term == TERM_NSP && D == 1)
@@ -648,6 +687,7 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   p->modes = (size_t)p->P;
   for (int d = 0; d < D - 1; ++d) p->modes *= p->n[d];
   p->term = term;
+  p->ks_mean = term == TERM_KS && !ks_keep;
   p->coef_t = terms->nonlinear_coef;
   p->rate = terms->dealias_rate > 0 ? terms->dealias_rate : 2.0 / 3.0;
   p->nlin = terms->n_linear;
@@ -749,14 +789,21 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
       p->off_G2 = p->off_G + roundup((size_t)p->B * 6 * p->n[0] * p->K1 * p->PF * sizeof(float2), 256);
       p->ws_bytes = p->off_G2 + roundup((size_t)p->B * 9 * p->n[0] * p->n[1] * p->PF * sizeof(float2), 256);
     } else if (D == 2) {
-      p->NF = 4;
+      p->NF = term == TERM_KS ? 2 : 4;
       p->PF = (int)roundup((size_t)p->kmax[1] + 1, 16);
       p->off_P = p->off_A2 + roundup(sb, 256);
       p->off_G = p->off_P + roundup(sb, 256);
       p->ws_bytes = p->off_G + roundup((size_t)p->B * p->NF * p->n[0] * p->PF * sizeof(float2), 256);
     } else {
       p->ws_bytes = p->off_A2 + roundup(sb, 256);
+      if (term == TERM_KS) {
+        p->off_S = p->ws_bytes;
+        p->off_N = p->off_S + roundup(sb, 256);
+        p->ws_bytes = p->off_N + roundup(sb, 256);
+      }
     }
+    p->off_DC = p->ws_bytes;
+    p->ws_bytes += 256;
   }
   // default launch shape (the autotuner may refine it): 2D columns of 512 points run the
   // half-thread radix plan; temporal blocking uses the smallest sub-batch (multiple of 8) that
@@ -768,7 +815,7 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     const int tiles = (p->H + p->V - 1) / p->V;
     int sb = 8;
     while (sb < p->B && (size_t)tiles * sb < 512) sb += 8;
-    if (sb >= p->B || (size_t)sb * per_sample > ((size_t)256 << 20)) sb = 0;
+    if (sb >= p->B || (size_t)sb * per_sample > ((size_t)256 << 20) || p->ks_mean) sb = 0;
     p->sub_batch = sb;
   }
   if (!rc && D == 3) p->row_var = (p->n[2] >= 64 && is_pow2(p->n[2])) ? 1 : 0;  // radix-4 z rows (fewer registers)
@@ -818,6 +865,8 @@ int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int
     if ((plan->P % col_tile) != 0) return fail(TFSM_E_INVALID, "column tile must divide the pitch");
     if (!row2d_supported(plan->n[1], row_var, row_wg)) return fail(TFSM_E_INVALID, "unsupported row tuning");
   }
+  if (sub_batch > 0 && sub_batch < plan->B && plan->ks_mean)
+    return fail(TFSM_E_INVALID, "KSConvection couples the batch through its mean: no sub-batches");
   plan->V = col_tile;
   plan->col_var = col_var;
   plan->row_var = row_var;
@@ -840,6 +889,8 @@ int tfsm_plan_get_tuning(const tfsm_plan* plan, int32_t* out) {
 int tfsm_plan_set_streams(tfsm_plan* plan, int32_t nstreams) {
   if (!plan) return fail(TFSM_E_INVALID, "null plan");
   if (nstreams != 1 && nstreams != 2) return fail(TFSM_E_INVALID, "nstreams must be 1 or 2");
+  if (nstreams == 2 && plan->ks_mean)
+    return fail(TFSM_E_INVALID, "KSConvection couples the batch through its mean: one stream only");
   if (nstreams == 2 && !plan->aux) {
     // default priority, non-blocking with respect to the null stream
     HIP_TRY(hipStreamCreateWithFlags(&plan->aux, hipStreamNonBlocking), "auxiliary stream");
@@ -1098,6 +1149,7 @@ int tfsm_step_varying(tfsm_plan* p, void* uh, const int32_t* nsteps_per_sample, 
                       void* stream) {
   if (!p || !uh || !nsteps_per_sample || !ws) return fail(TFSM_E_INVALID, "null argument");
   if (p->D != 1 || p->scheme == ETDRK0) return fail(TFSM_E_INVALID, "per-sample step counts need a nonlinear 1D plan");
+  if (p->term == TERM_KS) return fail(TFSM_E_INVALID, "per-sample step counts are not available for KSConvection");
   Run1Args a = run1_args(p, (float2*)uh, (char*)ws);
   a.nsteps = max_steps;
   a.nsteps_per = nsteps_per_sample;
@@ -1113,7 +1165,14 @@ int tfsm_apply(tfsm_plan* p, const void* uh, void* out, void* ws, void* stream) 
   float2* U = (float2*)uh;
   if (have_nl) {
     if (!ws) return fail(TFSM_E_INVALID, "workspace pointer is null");
-    if (p->D == 1) {
+    if (p->D == 1 && p->term == TERM_KS) {
+      Run1Args a = run1_args(p, U, (char*)ws);
+      float2* Nr = (float2*)((char*)ws + p->off_N);
+      float2* dc = p->ks_mean ? (float2*)((char*)ws + p->off_DC) : nullptr;
+      HIP_TRY(launch_k1_eval_ks(a, U, Nr, st), "1D product launch");
+      if (dc) HIP_TRY(launch_ks_dc(Nr, a.B, (size_t)p->P, 1, 0, dc, st), "batch-mean launch");
+      HIP_TRY(launch_k1_stage(a, ST_EVAL, Nr, dc, (float2*)((char*)ws + p->off_S), st), "1D stage launch");
+    } else if (p->D == 1) {
       Run1Args a = run1_args(p, U, (char*)ws);
       a.eval_only = 1;
       a.nsteps = 1;
@@ -1184,6 +1243,7 @@ int tfsm_launch_pass(tfsm_plan* p, void* uh, void* ws, int32_t pass_id, int32_t 
   }
   if (p->D == 1 && pass_id == 2) {
     if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
+    if (p->term == TERM_KS) return fail(TFSM_E_INVALID, "1D KSConvection steps stage by stage (tfsm_step)");
     Run1Args a = run1_args(p, (float2*)uh, (char*)ws);
     a.nsteps = 1;
     HIP_TRY(launch_run1d(a, st), "1D stepper launch");

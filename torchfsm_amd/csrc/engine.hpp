@@ -15,7 +15,7 @@
 
 namespace tfsm {
 
-enum TermKind : int { TERM_NONE = 0, TERM_CONV = 1, TERM_VC = 2, TERM_NSP = 3 };
+enum TermKind : int { TERM_NONE = 0, TERM_CONV = 1, TERM_VC = 2, TERM_NSP = 3, TERM_KS = 4 };
 
 enum SchemeId : int { ETDRK0 = 0, ETDRK1 = 1, ETDRK2 = 2, SETDRK1 = 3, SETDRK2 = 4, SETDRK3 = 5, SETDRK4 = 6 };
 
@@ -284,6 +284,7 @@ struct Col2Args {
   const float2* src;          // [Bs][Cs][N0][P1] or null
   size_t src_bstride, src_cstride;
   float2* G;                  // fields [B][NF][N0][PF]
+  const float2* dc;           // KSConvection: batch mean of the product DC (subtracted at k = 0), or null
   int b0;                     // first sample handled by this launch
   int nb;                     // samples handled by this launch (set by the launcher)
 };
@@ -300,7 +301,8 @@ struct Row2Args {
 struct Col3Args {
   int B, N0, N1, N2, H2, P2, PF2, K1;
   int kmax0, kmax1, kmax2;
-  int term;                   // TERM_CONV or TERM_NSP
+  int term;                   // TERM_CONV, TERM_NSP or TERM_KS
+  int nc;                     // channels: 3 (Convection, NSP) or 1 (KSConvection)
   float coef_t, scale;
   const float2* tw0;
   const float* ax;
@@ -314,25 +316,28 @@ struct Col3Args {
   size_t cpk_bstride;
   const float2* src;
   size_t src_bstride, src_cstride;
-  float2* G1;                 // [B][6][N0][K1][PF2]
+  float2* G1;                 // [B][2 nc][N0][K1][PF2]
+  const float2* dc;           // KSConvection: batch mean of the product DC, or null
   int b0, nb;
 };
 
 struct Yinv3Args {
   int N0, N1, PF2, K1, kmax1, kmax2;
+  int nc;                     // channels (fields 2 nc -> 3 nc)
   const float2* tw1;
   const float* ay;
   const float2* G1;
-  float2* G2;                 // [B][9][N0][N1][PF2]
+  float2* G2;                 // [B][3 nc][N0][N1][PF2]
   int b0, nb;
 };
 
 struct Row3Args {
+  int term;                   // TERM_CONV / TERM_NSP (Convection products) or TERM_KS
   int N0, N1, N2, P2, PF2, kmax2;
   const float2* tw2;
   const float* az;
   const float2* G2;
-  float2* P;                  // products [B][3][N0][N1][P2]
+  float2* P;                  // products [B][nc][N0][N1][P2]
   int b0, nb;
 };
 
@@ -416,6 +421,14 @@ hipError_t launch_linear_steps(float2* U, const float* E, size_t coef_bstride, s
 hipError_t launch_record(const float2* U, float2* traj, const int* rec_slot, int step, int B, size_t per_sample,
                          int T, hipStream_t st);
 hipError_t launch_run1d(const Run1Args& a, hipStream_t st);
+// KSConvection in 1D, one launch per stage: the raw product spectrum of stage spectrum S, then the
+// stage combine (with the batch-mean correction of the DC mode) into S, U, A1, A2
+hipError_t launch_k1_eval_ks(const Run1Args& a, const float2* S, float2* Nraw, hipStream_t st);
+hipError_t launch_k1_stage(const Run1Args& a, int stage, const float2* Nraw, const float2* dc, float2* S,
+                           hipStream_t st);
+// out = (1/B) sum_b sum_x P[b * bstride + x * xstride]: the DC of the product spectra averaged over
+// the batch (KSConvection's remove_mean subtracts the mean over batch and space)
+hipError_t launch_ks_dc(const float2* P, int B, size_t bstride, int nx, size_t xstride, float2* out, hipStream_t st);
 hipError_t launch_col3d(const Col3Args& a, int nb, int stage, int col_var, hipStream_t st);
 hipError_t launch_yinv3d(const Yinv3Args& a, int col_var, hipStream_t st);
 hipError_t launch_row3d(const Row3Args& a, int row_var, hipStream_t st);

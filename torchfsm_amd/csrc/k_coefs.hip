@@ -296,6 +296,30 @@ __global__ void k_rk_combine(RkArgs a) {
   }
 }
 
+// batch mean of the product DC (one workgroup, fixed summation order: deterministic)
+__global__ void __launch_bounds__(256) k_ks_dc(const float2* __restrict__ P, int B, size_t bstride, int nx,
+                                               size_t xstride, float2* __restrict__ out) {
+  __shared__ float2 red[256];
+  float2 acc = make_float2(0.f, 0.f);
+  const int total = B * nx;
+  for (int i = threadIdx.x; i < total; i += 256) {
+    const int b = i / nx, x = i % nx;
+    acc = radd(acc, P[(size_t)b * bstride + (size_t)x * xstride]);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = radd(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = make_float2(__fdiv_rn(red[0].x, (float)B), __fdiv_rn(red[0].y, (float)B));
+}
+
+hipError_t launch_ks_dc(const float2* P, int B, size_t bstride, int nx, size_t xstride, float2* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_ks_dc, dim3(1), dim3(256), 0, st, P, B, bstride, nx, xstride, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_rk_combine(const RkArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(k_rk_combine, dim3(grid_for(a.total)), dim3(256), 0, st, a);
   return hipGetLastError();

@@ -11,7 +11,8 @@
 //   here)  ->  physical product  ->  R2C  ->  products P (all H1 modes; the output is not masked)
 //
 // Reference path replaced: the nonlinear closure _base.py:375-403 with _VorticityConvectionCore
-// (_navier_stokes.py:27-46) or _ConvectionCore (_convection.py:18-46), and the stage combines of
+// (_navier_stokes.py:27-46), _ConvectionCore (_convection.py:18-46), _NSPressureConvectionCore
+// (_navier_stokes.py:221-262) or _KSConvectionCore (_ks_convection.py:8-38), and the stage combines of
 // _setdrk_step.py / _etdrk.py.
 #pragma once
 #include "engine.hpp"
@@ -42,6 +43,10 @@ struct Term2<TERM_CONV> {
 template <>
 struct Term2<TERM_NSP> {  // velocity-form Navier-Stokes: Convection fields, projected spectrum
   static constexpr int NC = 2, NF = 4;
+};
+template <>
+struct Term2<TERM_KS> {  // KSConvection: fields phi^, a_x phi^; one product 1/2 |grad phi|^2
+  static constexpr int NC = 1, NF = 2;
 };
 
 // Velocity-form Navier-Stokes (NSPressureConvection): both channels' convection spectra are
@@ -154,6 +159,10 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
       for (int m = 0; m < RMAX; ++m)
         v[m] = colv ? a.P[base + (size_t)(j + m * NT) * a.P1 + ky] : make_float2(0.f, 0.f);
       fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
+      if constexpr (TERM == TERM_KS) {
+        // remove_mean: fft(p - mean(p)) differs from fft(p) only at k = 0, by the batch mean of the DC
+        if (a.dc && ky == 0 && j == 0) v[0] = rsub(v[0], a.dc[0]);
+      }
       // combine: every store of this tile is deferred until all loads have been issued, so the
       // loads need not wait behind stores to arrays the compiler cannot prove disjoint
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride : nullptr;
@@ -278,6 +287,39 @@ __global__ void __launch_bounds__((Fft<N1, VAR>::NT * ROWS)) k2_row(Row2Args a) 
         if (pos <= N1 / 2) p[pos] = v[m];
       }
     }
+  } else if constexpr (TERM == TERM_KS) {
+    // KSConvection, C = 1: fields F0 = phi, F1 = kx phi (x-inverse done);
+    // 1/2 (d_x phi^2 + d_y phi^2), d_x phi = C2R(i F1), d_y phi = C2R(i a_y F0)  (_ks_convection.py:27-38)
+    float2 z1[RMAX];
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) {
+      const int pos = j + m * NT;
+      const bool cj = pos > N1 / 2;
+      const int k = cj ? N1 - pos : pos;
+      const bool edge = (k == 0) || (k == N1 / 2);
+      float2 f0 = make_float2(0.f, 0.f), f1 = f0;
+      float ay = 0.f;
+      if (rv && k <= a.kmax1) {
+        f0 = gb[k];
+        f1 = gb[fs + k];
+        ay = a.ay[k];
+      }
+      z1[m] = pack_pair(make_float2(-f1.y, f1.x), make_float2(-ay * f0.y, ay * f0.x), cj, edge);
+    }
+    fft_wg<N1, +1, 1, VAR, RowSync<N1, VAR>::WS>(z1, lds, s_tw, j, 0);
+    float2 v[RMAX];
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m)
+      v[m] = make_float2(mulr(0.5f, addr(mulr(z1[m].x, z1[m].x), mulr(z1[m].y, z1[m].y))), 0.f);
+    fft_wg<N1, -1, 1, VAR, RowSync<N1, VAR>::WS>(v, lds, s_tw, j, 0);
+    if (rv) {
+      float2* p = a.P + ((size_t)b * a.N0 + x) * a.P1;
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) {
+        const int pos = j + m * NT;
+        if (pos <= N1 / 2) p[pos] = v[m];
+      }
+    }
   } else {
     // Convection, C = D = 2: fields F0 = u0, F1 = kx u0, F2 = u1, F3 = kx u1 (x-inverse done)
     float2 z1[RMAX], z2[RMAX], z3[RMAX];
@@ -384,5 +426,7 @@ hipError_t launch_col2d_mixed(int term, const Col2Args& a, int nb, int stage, in
 hipError_t launch_row2d_vc(const Row2Args& a, int var, int wgt, hipStream_t st);
 hipError_t launch_row2d_conv(const Row2Args& a, int var, int wgt, hipStream_t st);
 hipError_t launch_row2d_mixed(int term, const Row2Args& a, int var, int wgt, hipStream_t st);
+hipError_t launch_col2d_ks(const Col2Args& a, int nb, int stage, int v, int var, hipStream_t st);
+hipError_t launch_row2d_ks(const Row2Args& a, int var, int wgt, hipStream_t st);
 
 }  // namespace tfsm

@@ -1,6 +1,7 @@
-// 3D fused ETD evaluation kernels for Convection with C = D = 3 (3D vector Burgers) and for the
+// 3D fused ETD evaluation kernels for Convection with C = D = 3 (3D vector Burgers), for the
 // velocity-form Navier-Stokes term (NSPressureConvection: the same fields and products, the
-// projection applied to the convection spectrum in the x-column kernel).
+// projection applied to the convection spectrum in the x-column kernel), and for KSConvection
+// (C = 1: fields phi^, a_x phi^ -> phi, a_y phi, a_x phi; one product 1/2 |grad phi|^2).
 //
 // One nonlinear evaluation = four passes over the half spectrum [B][3][N0][N1][P2]:
 //   k3_row   z rows: 12 C2R (packed in pairs) of the y-inverse fields -> the three products
@@ -63,11 +64,12 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
   const int chalf = stage_coef_half(stage);
   const bool rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
   const int tgt = stage_target(stage);
-  float2 s[3][RMAX];
+  constexpr int NC3 = TERM == TERM_KS ? 1 : 3;  // channels
+  float2 s[NC3][RMAX];
   if (stage == ST_PRO) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const size_t base = ((size_t)b * 3 + c) * plane + coff;
+    for (int c = 0; c < NC3; ++c) {
+      const size_t base = ((size_t)b * NC3 + c) * plane + coff;
 #pragma unroll
       for (int m = 0; m < RMAX; ++m) s[c][m] = colv ? a.U[base + (size_t)(j + m * NT) * inner] : make_float2(0.f, 0.f);
     }
@@ -115,12 +117,16 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
     }
   } else {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const size_t base = ((size_t)b * 3 + c) * plane + coff;
+    for (int c = 0; c < NC3; ++c) {
+      const size_t base = ((size_t)b * NC3 + c) * plane + coff;
       float2 v[RMAX];
 #pragma unroll
       for (int m = 0; m < RMAX; ++m) v[m] = colv ? a.P[base + (size_t)(j + m * NT) * inner] : make_float2(0.f, 0.f);
       fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
+      if constexpr (TERM == TERM_KS) {
+        // remove_mean: fft(p - mean(p)) differs from fft(p) only at k = 0, by the batch mean of the DC
+        if (a.dc && ky == 0 && kz == 0 && j == 0) v[0] = rsub(v[0], a.dc[0]);
+      }
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride + coff : nullptr;
       float2* dst = tgt == 0 ? a.U : (tgt == 1 ? a.A1 : a.A2);
       float2 wv[RMAX];
@@ -145,7 +151,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
   const int kyc = ky_compress(ky, a.N1, a.kmax1);
   if (kyc < 0 || (tile % nzt) * V > a.kmax2) return;  // uniform: no kept column in this tile
 #pragma unroll
-  for (int f = 0; f < 6; ++f) {
+  for (int f = 0; f < 2 * NC3; ++f) {
     float2 v[RMAX];
 #pragma unroll
     for (int m = 0; m < RMAX; ++m) {
@@ -167,7 +173,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
     }
     fft_wg<N0, +1, V, VAR>(v, s_buf, s_tw, j, vi);
     if (kz < a.PF2) {
-      float2* g = a.G1 + ((((size_t)b * 6 + f) * N0) * a.K1 + kyc) * a.PF2 + kz;
+      float2* g = a.G1 + ((((size_t)b * 2 * NC3 + f) * N0) * a.K1 + kyc) * a.PF2 + kz;
 #pragma unroll
       for (int m = 0; m < RMAX; ++m) g[(size_t)(j + m * NT) * a.K1 * a.PF2] = v[m];
     }
@@ -175,7 +181,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
 }
 
 // ---------------------------------------------------------------- y columns (inverse, 6 -> 9 fields)
-template <int N1, int V, int VAR>
+template <int N1, int V, int VAR, int NCH>
 __global__ void __launch_bounds__((Fft<N1, VAR>::NT * V)) k3_yinv(Yinv3Args a) {
   using F = Fft<N1, VAR>;
   constexpr int RMAX = F::RMAX, NT = F::NT;
@@ -190,9 +196,9 @@ __global__ void __launch_bounds__((Fft<N1, VAR>::NT * V)) k3_yinv(Yinv3Args a) {
   const int b = a.b0 + blockIdx.x / (nzt * a.N0);
   const bool kv = kz <= a.kmax2;
 #pragma unroll 1
-  for (int c = 0; c < 3; ++c) {
-    const float2* g0 = a.G1 + ((((size_t)b * 6 + 2 * c) * a.N0 + x) * a.K1) * a.PF2 + kz;
-    const float2* g1 = a.G1 + ((((size_t)b * 6 + 2 * c + 1) * a.N0 + x) * a.K1) * a.PF2 + kz;
+  for (int c = 0; c < NCH; ++c) {
+    const float2* g0 = a.G1 + ((((size_t)b * 2 * NCH + 2 * c) * a.N0 + x) * a.K1) * a.PF2 + kz;
+    const float2* g1 = a.G1 + ((((size_t)b * 2 * NCH + 2 * c + 1) * a.N0 + x) * a.K1) * a.PF2 + kz;
     float2 f0[RMAX], f1[RMAX];
 #pragma unroll
     for (int m = 0; m < RMAX; ++m) {
@@ -216,7 +222,7 @@ __global__ void __launch_bounds__((Fft<N1, VAR>::NT * V)) k3_yinv(Yinv3Args a) {
       }
       fft_wg<N1, +1, V, VAR>(v, s_buf, s_tw, j, vi);
       if (kz < a.PF2) {
-        float2* g = a.G2 + ((((size_t)b * 9 + 3 * c + k) * a.N0 + x) * N1) * a.PF2 + kz;
+        float2* g = a.G2 + ((((size_t)b * 3 * NCH + 3 * c + k) * a.N0 + x) * N1) * a.PF2 + kz;
 #pragma unroll
         for (int m = 0; m < RMAX; ++m) g[(size_t)(j + m * NT) * a.PF2] = v[m];
       }
@@ -229,10 +235,11 @@ __global__ void __launch_bounds__((Fft<N1, VAR>::NT * V)) k3_yinv(Yinv3Args a) {
 // nine G2 fields of those rows are contiguous per field: they are staged into LDS (dynamic, 9 x
 // ROWS x PF2 complex) with coalesced 16-byte loads first, so all loads are in flight together,
 // and the twelve inverse transforms then read LDS only.
-template <int N2, int ROWS, int VAR>
+template <int N2, int ROWS, int VAR, int TERM>
 __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) {
   using F = Fft<N2, VAR>;
   constexpr int RMAX = F::RMAX, NT = F::NT;
+  constexpr int NFS = TERM == TERM_KS ? 3 : 9;  // staged G2 fields
   constexpr int SL = LdsSize<N2, 1>::slots;
   __shared__ float2 s_tw[Fft<N2, VAR>::tw_size()];
   __shared__ float2 s_buf[ROWS * SL];
@@ -248,10 +255,10 @@ __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) 
   const int PF2 = a.PF2;
   const size_t fs = (size_t)nxy * PF2;  // one G2 field of one sample
   {
-    const float4* src = reinterpret_cast<const float4*>(a.G2 + (size_t)b * 9 * fs + (size_t)xy0 * PF2);
+    const float4* src = reinterpret_cast<const float4*>(a.G2 + (size_t)b * NFS * fs + (size_t)xy0 * PF2);
     const int per = ROWS * PF2 / 2;      // float4 per field
     const size_t fs4 = fs / 2;
-    for (int i = threadIdx.x; i < 9 * per; i += NT * ROWS) {
+    for (int i = threadIdx.x; i < NFS * per; i += NT * ROWS) {
       const int f = i / per, r = i - f * per;
       s_in4[i] = src[f * fs4 + r];
     }
@@ -280,6 +287,25 @@ __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) 
     }
     fft_wg<N2, +1, 1, VAR, RowSync<N2, VAR>::WS>(z, lds, s_tw, j, 0);
   };
+  if constexpr (TERM == TERM_KS) {
+    // KSConvection, C = 1: G2 fields 0: phi, 1: a_y phi, 2: a_x phi;
+    // p = 1/2 ((d_x phi^2 + d_y phi^2) + d_z phi^2)   (_ks_convection.py:27-38)
+    float2 z[RMAX], w[RMAX];
+    inv_pair(z, 2, 1, 1, 1);  // (d_x phi, d_y phi)
+    inv_pair(w, 0, 2, 0, 0);  // (d_z phi, phi)
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m)
+      z[m] = make_float2(
+          mulr(0.5f, addr(addr(mulr(z[m].x, z[m].x), mulr(z[m].y, z[m].y)), mulr(w[m].x, w[m].x))), 0.f);
+    fft_wg<N2, -1, 1, VAR, RowSync<N2, VAR>::WS>(z, lds, s_tw, j, 0);
+    float2* pb = a.P + (size_t)b * nxy * a.P2 + (size_t)(xy0 + g) * a.P2;
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) {
+      const int pos = j + m * NT;
+      if (pos <= N2 / 2) pb[pos] = z[m];
+    }
+    return;
+  }
   float u0[RMAX], u1[RMAX], u2[RMAX], p0[RMAX], p1[RMAX], p2[RMAX];
   float2 z[RMAX];
   inv_pair(z, 0, 0, 3, 0);  // (u0, u1)
@@ -362,6 +388,8 @@ static hipError_t launch_col3d_t(const Col3Args& a, int nb, int stage, hipStream
   b.nb = nb;
   if (a.term == TERM_NSP)
     hipLaunchKernelGGL((k3_col<NN, V, VR, TERM_NSP>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b, stage);
+  else if (a.term == TERM_KS)
+    hipLaunchKernelGGL((k3_col<NN, V, VR, TERM_KS>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b, stage);
   else
     hipLaunchKernelGGL((k3_col<NN, V, VR, TERM_CONV>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b,
                        stage);
@@ -372,7 +400,10 @@ template <int NN, int VR>
 static hipError_t launch_yinv3d_t(const Yinv3Args& a, hipStream_t st) {
   constexpr int V = 16;
   const int nzt = a.PF2 / V;
-  hipLaunchKernelGGL((k3_yinv<NN, V, VR>), dim3(nzt * a.N0 * a.nb), dim3(Fft<NN, VR>::NT * V), 0, st, a);
+  if (a.nc == 1)
+    hipLaunchKernelGGL((k3_yinv<NN, V, VR, 1>), dim3(nzt * a.N0 * a.nb), dim3(Fft<NN, VR>::NT * V), 0, st, a);
+  else
+    hipLaunchKernelGGL((k3_yinv<NN, V, VR, 3>), dim3(nzt * a.N0 * a.nb), dim3(Fft<NN, VR>::NT * V), 0, st, a);
   return hipGetLastError();
 }
 
@@ -413,9 +444,13 @@ static hipError_t launch_row3d_t(const Row3Args& a, hipStream_t st) {
   const size_t rows = (size_t)a.nb * a.N0 * a.N1;
   // whole workgroups of one sample each; staged fields in dynamic LDS
   if ((a.N0 * a.N1) % R != 0 || (a.PF2 % 2) != 0) return hipErrorInvalidValue;
-  const size_t lds = (size_t)9 * R * a.PF2 * sizeof(float2);
+  const bool ks = a.term == TERM_KS;
+  const size_t lds = (size_t)(ks ? 3 : 9) * R * a.PF2 * sizeof(float2);
   if (lds > 96 * 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k3_row<NN, R, VR>), dim3((unsigned)(rows / R)), dim3(NT * R), lds, st, a);
+  if (ks)
+    hipLaunchKernelGGL((k3_row<NN, R, VR, TERM_KS>), dim3((unsigned)(rows / R)), dim3(NT * R), lds, st, a);
+  else
+    hipLaunchKernelGGL((k3_row<NN, R, VR, TERM_CONV>), dim3((unsigned)(rows / R)), dim3(NT * R), lds, st, a);
   return hipGetLastError();
 }
 

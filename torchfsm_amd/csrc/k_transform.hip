@@ -324,6 +324,70 @@ __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
   }
 }
 
+// ---------------------------------------------------------------- 1D KSConvection, one stage per launch
+// KSConvection's remove_mean subtracts the mean over the whole batch, so the stages of different
+// samples are coupled through the DC mode: each stage is evaluated for every sample (raw product
+// spectrum), the batch mean of the DC is reduced, then the stage combine runs.
+// Nraw = fft(1/2 (d_x phi)^2) of the dealiased stage spectrum S (_ks_convection.py:27-38).
+template <int N, int V>
+__global__ void __launch_bounds__(Fft<N>::NT* V) k1_eval_ks(Run1Args a, const float2* S, float2* Nraw) {
+  using F = Fft<N>;
+  constexpr int RMAX = F::RMAX, NT = F::NT;
+  __shared__ float2 s_tw[Fft<N>::tw_size()];
+  __shared__ float2 s_buf[LdsSize<N, V>::slots];
+  load_twiddles<N>(s_tw, a.tw);
+  __syncthreads();
+  const int vi = threadIdx.x % V, j = threadIdx.x / V;
+  const int smp = blockIdx.x * V + vi;
+  const bool valid = smp < a.B;
+  const float2* Sb = S + (size_t)(valid ? smp : 0) * a.P;
+  const float* ax = a.ax + (size_t)(valid ? smp : 0) * a.ax_bstride;
+  float2 v[RMAX];
+#pragma unroll
+  for (int m = 0; m < RMAX; ++m) {
+    const int pos = j + m * NT;
+    const bool cj = pos > N / 2;
+    const int k = cj ? N - pos : pos;
+    float2 g = make_float2(0.f, 0.f);
+    if (valid && k <= a.kmax) {
+      const float2 val = make_float2(Sb[k].x * a.scale, Sb[k].y * a.scale);
+      const float axk = ax[k];
+      g = make_float2(-axk * val.y, axk * val.x);  // i a_k phi^
+    }
+    if (k == 0 || k == N / 2) g.y = 0.f;
+    v[m] = cj ? make_float2(g.x, -g.y) : g;
+  }
+  fft_wg<N, +1, V>(v, s_buf, s_tw, j, vi);
+#pragma unroll
+  for (int m = 0; m < RMAX; ++m) v[m] = make_float2(mulr(0.5f, mulr(v[m].x, v[m].x)), 0.f);
+  fft_wg<N, -1, V>(v, s_buf, s_tw, j, vi);
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) {
+      const int pos = j + m * NT;
+      if (pos <= N / 2) Nraw[(size_t)smp * a.P + pos] = v[m];
+    }
+  }
+}
+
+// stage combine of every (sample, k <= N/2): N = coef_t (Nraw - [k = 0] dc) + S_src, then the
+// stage's update of U / A1 / A2; S receives the next stage spectrum
+__global__ void k1_stage(Run1Args a, int stage, const float2* __restrict__ Nraw, const float2* __restrict__ dc,
+                         float2* __restrict__ S) {
+  const size_t total = (size_t)a.B * a.H;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t smp = e / a.H;
+    const int pos = (int)(e % a.H);
+    const size_t so = smp * a.P;
+    float2 v = Nraw[so + pos];
+    if (pos == 0 && dc) v = rsub(v, dc[0]);
+    const float2* src = a.src ? a.src + smp * a.src_bstride : nullptr;
+    const float2 nv = nl_value(a.coef_t, v, src, pos);
+    const float* cf = a.coef + smp * a.coef_bstride;
+    S[so + pos] = stage_combine(stage, nv, a.U + so, a.A1 + so, a.A2 + so, pos, cf, a.P, pos, true);
+  }
+}
+
 // ---------------------------------------------------------------- host launchers
 template <int N>
 static constexpr int rows_per_wg() {
@@ -449,6 +513,27 @@ static hipError_t launch_run1_t(const Run1Args& a, hipStream_t st) {
   constexpr int NT = Fft<N>::NT;
   constexpr int V = NT >= 64 ? 1 : 64 / NT;
   hipLaunchKernelGGL((k1_run<N, V>), dim3((a.B + V - 1) / V), dim3(NT * V), 0, st, a);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_eval_ks_t(const Run1Args& a, const float2* S, float2* Nraw, hipStream_t st) {
+  constexpr int NT = Fft<N>::NT;
+  constexpr int V = NT >= 64 ? 1 : 64 / NT;
+  hipLaunchKernelGGL((k1_eval_ks<N, V>), dim3((a.B + V - 1) / V), dim3(NT * V), 0, st, a, S, Nraw);
+  return hipGetLastError();
+}
+
+hipError_t launch_k1_eval_ks(const Run1Args& a, const float2* S, float2* Nraw, hipStream_t st) {
+  TFSM_SWITCH_N(a.N, { return launch_eval_ks_t<NN>(a, S, Nraw, st); })
+}
+
+hipError_t launch_k1_stage(const Run1Args& a, int stage, const float2* Nraw, const float2* dc, float2* S,
+                           hipStream_t st) {
+  const size_t total = (size_t)a.B * a.H;
+  unsigned blocks = (unsigned)((total + 255) / 256);
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k1_stage, dim3(blocks), dim3(256), 0, st, a, stage, Nraw, dc, S);
   return hipGetLastError();
 }
 

@@ -48,7 +48,8 @@ class SpectralPlan:
     Args:
         mesh_info: sequence of (start, end, n) per axis.
         batch, channels: B and C of the state.
-        nonlinear: None, "conv" or "vc" (the dealiased product term) and its coefficient.
+        nonlinear: None, "conv", "vc", "nsp", "ks" or "ks_keep_mean" (the dealiased product term)
+            and its coefficient.
         linear: list of (kind, coef) with kind in {"lap", "biharm", "unit"}; coef a float or a
             per-sample sequence of length `batch`.
         device: CUDA device.
@@ -57,7 +58,8 @@ class SpectralPlan:
     """
 
     KIND = {"lap": _lib.LIN_LAP, "biharm": _lib.LIN_BIHARM, "unit": _lib.LIN_UNIT}
-    TERM = {None: _lib.TERM_NONE, "conv": _lib.TERM_CONV, "vc": _lib.TERM_VC, "nsp": _lib.TERM_NSP}
+    TERM = {None: _lib.TERM_NONE, "conv": _lib.TERM_CONV, "vc": _lib.TERM_VC, "nsp": _lib.TERM_NSP,
+            "ks": _lib.TERM_KS, "ks_keep_mean": _lib.TERM_KS_KEEP_MEAN}
 
     def __init__(self, mesh_info, batch, channels, nonlinear=None, nonlinear_coef=0.0, linear=(), device="cuda",
                  dealias_rate=2 / 3, lengths_per_sample: Optional[Sequence[float]] = None):

@@ -11,7 +11,15 @@ from ._base import (
     OperatorLike,
     check_value_with_mesh,
 )
-from ._cores import Biharmonic, Convection, ImplicitSource, Laplacian, NSPressureConvection, VorticityConvection
+from ._cores import (
+    Biharmonic,
+    Convection,
+    ImplicitSource,
+    KSConvection,
+    Laplacian,
+    NSPressureConvection,
+    VorticityConvection,
+)
 
 __all__ = [
     "Operator",
@@ -28,5 +36,6 @@ __all__ = [
     "Convection",
     "VorticityConvection",
     "NSPressureConvection",
+    "KSConvection",
     "check_value_with_mesh",
 ]

@@ -226,7 +226,7 @@ class OperatorLike(_MutableMixIn):
                 if isinstance(c, list):
                     raise ValueError("term not supported by engine: per-sample coefficient on an explicit source")
                 sources.append((c, core._source_spatial))
-            elif kind in ("conv", "vc", "nsp"):
+            elif kind in ("conv", "vc", "nsp", "ks", "ks_keep_mean"):
                 if prod is not None:
                     raise ValueError("term not supported by engine: more than one product term")
                 if not core._dealiasing_swtich:

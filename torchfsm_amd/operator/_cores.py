@@ -1,5 +1,5 @@
 """Operator cores the engine compiles (generic/_laplacian.py, _biharmonic.py, _convection.py,
-_source.py, dedicated/_navier_stokes.py:18-71).
+_source.py, dedicated/_navier_stokes.py:18-71, 221-280, dedicated/_ks_convection.py).
 
 Each core carries an ``_engine_kind`` that register_mesh maps onto the engine's term
 descriptor.  LinearCoef cores still return the reference's symbol tensors when called directly.
@@ -170,3 +170,34 @@ class NSPressureConvection(NonlinearOperator):
 
     def __init__(self, external_force=None) -> None:
         super().__init__(_NSPressureConvectionGenerator(external_force))
+
+
+class _KSConvectionCore(NonlinearFunc):
+    """1/2 |grad phi|^2 of a scalar field, minus its mean when remove_mean (_ks_convection.py:8-38).
+    The reference's mean runs over the whole tensor, batch included, so with remove_mean the
+    samples of a batch share it."""
+
+    def __init__(self, remove_mean: bool) -> None:
+        super().__init__()
+        self.remove_mean = remove_mean
+        self._engine_kind = "ks" if remove_mean else "ks_keep_mean"
+
+    def __call__(self, u_fft, f_mesh: FourierMesh, u=None):
+        return _core_direct(self._engine_kind, u_fft, f_mesh)
+
+
+class _KSConvectionGenerator(CoreGenerator):
+    def __init__(self, remove_mean: bool) -> None:
+        self.remove_mean = remove_mean
+
+    def __call__(self, f_mesh: FourierMesh, n_channel: int) -> NonlinearFunc:
+        if n_channel != 1:
+            raise NotImplementedError("KSConvection only supports scalar field")
+        return _KSConvectionCore(self.remove_mean)
+
+
+class KSConvection(NonlinearOperator):
+    """Kuramoto-Sivashinsky convection 1/2 |grad phi|^2 (_ks_convection.py:55-67)."""
+
+    def __init__(self, remove_mean: bool = True) -> None:
+        super().__init__(_KSConvectionGenerator(remove_mean))

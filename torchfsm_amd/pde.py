@@ -2,7 +2,15 @@
 
 from typing import Optional
 
-from .operator import Biharmonic, Convection, Laplacian, NSPressureConvection, Operator, VorticityConvection
+from .operator import (
+    Biharmonic,
+    Convection,
+    KSConvection,
+    Laplacian,
+    NSPressureConvection,
+    Operator,
+    VorticityConvection,
+)
 
 
 def Burgers(nu) -> Operator:
@@ -15,6 +23,11 @@ def KuramotoSivashinsky() -> Operator:
     ks_eqn = -Laplacian() - Biharmonic() - Convection()
     ks_eqn.register_additional_check(lambda dim_value, dim_mesh: dim_value == 1 and dim_mesh == 1)
     return ks_eqn
+
+
+def KuramotoSivashinskyHighDim() -> Operator:
+    """phi_t = -lap phi - lap^2 phi - 1/2 |grad phi|^2 in 1D, 2D or 3D (pde.py:41-49)."""
+    return -Laplacian() - Biharmonic() - KSConvection()
 
 
 def NavierStokesVorticity(Re: float, force: Optional[Operator] = None) -> Operator:
