@@ -42,6 +42,8 @@ extern "C" {
 #define TFSM_LIN_LAP 1        /* Laplacian       (_laplacian.py:12-15)  */
 #define TFSM_LIN_BIHARM 2     /* Biharmonic      (_biharmonic.py:13-16) */
 #define TFSM_LIN_UNIT 3       /* ImplicitSource() unit symbol (_source.py:9-17) */
+#define TFSM_LIN_DERIV 4      /* SpatialDerivative(dim, order): (i a_dim)^order (_spatial_derivative.py:7-20,
+                                 mesh.py:399-404); an odd order makes L imaginary (complex-L plan) */
 
 /* time integration schemes (ETDRKIntegrator / SETDRKIntegrator members) */
 #define TFSM_ETDRK0 0
@@ -77,6 +79,8 @@ typedef struct tfsm_terms {
   int32_t linear_kind[4];            /* TFSM_LIN_* */
   const float* linear_coef;          /* host [n_linear][coef_batch] */
   int32_t coef_batch;                /* 1, or batch for per-sample coefficients (batched nu) */
+  int32_t linear_dim[4];             /* TFSM_LIN_DERIV: axis index */
+  int32_t linear_order[4];           /* TFSM_LIN_DERIV: derivative order (1..8) */
 } tfsm_terms;
 
 /* Message of the last failure on this thread ("" if none). */

@@ -9,6 +9,7 @@ An operator is described by a list of terms in generator order, each ``(coef, ki
   kind "lap"     Laplacian symbol, concatenated over channels   (_laplacian.py:12-15)
   kind "biharm"  lap*lap, concatenated over channels            (_biharmonic.py:13-16)
   kind "unit"    ones_like(bf_x) (ImplicitSource() drag)        (_source.py:9-17)
+  kind "deriv"   SpatialDerivative; payload = (dim, order)      (_spatial_derivative.py:7-20)
   kind "conv"    Convection, dealiased input                    (_convection.py:18-46)
   kind "vc"      VorticityConvection, dealiased input           (_navier_stokes.py:27-46)
   kind "nsp"     NSPressureConvection, dealiased input          (_navier_stokes.py:221-262)
@@ -22,7 +23,7 @@ from typing import Callable, List, Optional, Sequence, Tuple
 import torch
 
 Term = Tuple[object, str, Optional[torch.Tensor]]
-LINEAR_KINDS = ("lap", "biharm", "unit")
+LINEAR_KINDS = ("lap", "biharm", "unit", "deriv")
 NONLINEAR_KINDS = ("conv", "vc", "nsp", "ks", "ks_keep_mean", "src")
 
 
@@ -109,7 +110,9 @@ class Mesh:
 
 
 # ----------------------------------------------------------------------------- terms
-def linear_symbol(mesh: Mesh, kind: str, n_channel: int):
+def linear_symbol(mesh: Mesh, kind: str, n_channel: int, payload=None):
+    if kind == "deriv":  # _spatial_derivative.py:18-20: payload = (dim_index, order)
+        return mesh.grad(payload[0], payload[1])
     if kind == "lap":  # _laplacian.py:12-15
         return torch.cat([mesh.lap()] * n_channel, dim=1)
     if kind == "biharm":  # _biharmonic.py:13-16
@@ -171,10 +174,10 @@ def check_terms(mesh: Mesh, terms: List[Term], n_channel: int):
 
 def build_linear(mesh: Mesh, terms: List[Term], n_channel: int):
     """_build_linear_coefs (_base.py:339-357): sum(coef * symbol) in generator order."""
-    lin = [(c, k) for c, k, _ in terms if k in LINEAR_KINDS]
+    lin = [(c, k, p) for c, k, p in terms if k in LINEAR_KINDS]
     if not lin:
         return None
-    return sum([c * linear_symbol(mesh, k, n_channel) for c, k in lin])
+    return sum([c * linear_symbol(mesh, k, n_channel, p) for c, k, p in lin])
 
 
 def build_closure(mesh: Mesh, terms: List[Term], dealias_rate=2 / 3) -> Optional[Callable]:
@@ -449,6 +452,11 @@ def ns_velocity_terms(Re) -> List[Term]:
 def ks_terms() -> List[Term]:
     """pde.KuramotoSivashinsky (pde.py:27-39): -Laplacian() - Biharmonic() - Convection()."""
     return [(-1, "lap", None), (-1, "biharm", None), (-1, "conv", None)]
+
+
+def kdv_terms(dispersion_coef=1.0, convection_coef=6.0) -> List[Term]:
+    """pde.KortewegDeVries (pde.py:51-64): -c1 SpatialDerivative(0, 3) + c2 Convection()."""
+    return [(-dispersion_coef, "deriv", (0, 3)), (convection_coef, "conv", None)]
 
 
 def ks_highdim_terms(remove_mean=True) -> List[Term]:

@@ -369,3 +369,44 @@ def test_ks_highdim_1d_is_the_derivative_of_ks():
         v = R.integrate(R.ks_terms(), mi, v0, 0.05, 40, scheme="SETDRK4", return_in_fourier=True)
         dv = mesh.nabla_vector(1) * ph
         assert rel(dv.numpy(), v.numpy()) < 1e-10
+
+
+# ---------------------------------------------------------------- complex L (SpatialDerivative)
+def test_airy_equation_exact():
+    """u_t = -c u_xxx (SpatialDerivative(0, 3), imaginary symbol): ETDRK0 moves cos(k x) to
+    cos(k x + c k^3 t) exactly."""
+    n, L, c, dt, steps = 64, 2 * math.pi, 0.3, 0.05, 9
+    x = torch.arange(n, dtype=torch.float64) * L / n
+    k = 3.0
+    u0 = torch.cos(k * x).reshape(1, 1, n)
+    o = R.integrate([(-c, "deriv", (0, 3))], [(0, L, n)], u0, dt, steps, scheme="ETDRK0")
+    ex = torch.cos(k * x + c * k**3 * dt * steps).reshape(1, 1, n)
+    assert (o - ex).abs().max() < 1e-12
+
+
+def test_kdv_soliton_converges():
+    """The reference's default KdV u_t = -u_xxx + 6 u u_x carries u = -(c/2) sech^2(sqrt(c)/2 (x - c t
+    - x0)); ETDRK2 (complex c1, c2) converges to it at second order in float64."""
+    n, L, cs, T = 256, 40.0, 1.0, 1.0
+    x = torch.arange(n, dtype=torch.float64) * L / n
+
+    def sol(t):
+        return (-(cs / 2) / torch.cosh(math.sqrt(cs) / 2 * (x - cs * t - 15.0)) ** 2).reshape(1, 1, n)
+
+    errs = []
+    for m in (40, 80):
+        o = R.integrate(R.kdv_terms(), [(0, L, n)], sol(0.0), T / m, m, scheme="ETDRK2")
+        errs.append(float((o - sol(T)).abs().max()))
+    assert errs[1] < 1e-4 and 3.5 < errs[0] / errs[1] < 4.6, errs
+
+
+def test_setdrk_complex_l_takes_the_real_part():
+    """With L = i w the reference's SETDRK coefficient is dt Re(phi_1(i w dt)) = dt sin(w dt)/(w dt)
+    (the contour mean is taken .real), while E = exp(i w dt) stays complex."""
+    dt = 0.1
+    w = torch.tensor([0.5, 10.0, 50.0, 300.0], dtype=torch.float64)
+    L = 1j * w
+    sch = R.Scheme("SETDRK1", dt, L, lambda u: u * 0)
+    th = w * dt
+    assert torch.allclose(sch.c1, dt * torch.sin(th) / th, rtol=1e-11, atol=0)
+    assert torch.allclose(sch.E, torch.exp(1j * th), rtol=1e-14, atol=0)

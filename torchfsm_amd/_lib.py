@@ -17,7 +17,7 @@ TFSM_E_DEVICE = -3
 TFSM_E_OOM = -4
 
 TERM_NONE, TERM_CONV, TERM_VC, TERM_NSP, TERM_KS, TERM_KS_KEEP_MEAN = 0, 1, 2, 3, 4, 5
-LIN_LAP, LIN_BIHARM, LIN_UNIT = 1, 2, 3
+LIN_LAP, LIN_BIHARM, LIN_UNIT, LIN_DERIV = 1, 2, 3, 4
 ETDRK0, ETDRK1, ETDRK2, SETDRK1, SETDRK2, SETDRK3, SETDRK4 = range(7)
 
 
@@ -41,6 +41,8 @@ class Terms(ctypes.Structure):
         ("linear_kind", ctypes.c_int32 * 4),
         ("linear_coef", ctypes.POINTER(ctypes.c_float)),
         ("coef_batch", ctypes.c_int32),
+        ("linear_dim", ctypes.c_int32 * 4),
+        ("linear_order", ctypes.c_int32 * 4),
     ]
 
 

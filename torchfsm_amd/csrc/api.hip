@@ -92,6 +92,10 @@ struct tfsm_plan {
   int kmax[3] = {0, 0, 0};
   int nlin = 0;
   int lin_kind[4] = {0, 0, 0, 0};
+  int lin_dim[4] = {0, 0, 0, 0}, lin_order[4] = {0, 0, 0, 0};
+  bool cplx = false;        // complex L (odd-order SpatialDerivative): complex symbol and tables
+  float2* Lc = nullptr;     // [Bc][modes]
+  float2* coefc = nullptr;  // [Bc][NCOEF][modes]
   bool per_len = false;  // 1D plan with per-sample lengths (axis tables per row)
   int Bc = 1;
   float2* tw[3][4] = {};  // [axis][radix-plan variant]
@@ -425,6 +429,7 @@ Run1Args run1_args(tfsm_plan* p, float2* U, char* ws) {
   a.A2 = (float2*)(ws + p->off_A2);
   a.coef = p->coef;
   a.coef_bstride = p->coef_bstride();
+  a.coefc = p->cplx ? p->coefc : nullptr;
   a.src = p->src;
   a.src_bstride = p->src_bstride();
   a.nsteps = 0;
@@ -439,9 +444,14 @@ Run1Args run1_args(tfsm_plan* p, float2* U, char* ws) {
 int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* traj, int T, void* ws, hipStream_t st) {
   float2* U = (float2*)uh;
   if (p->scheme == ETDRK0) {
-    HIP_TRY(launch_linear_steps(U, p->coef + CS_E * p->modes, p->coef_bstride(), p->modes, p->C, p->state_elems(),
-                                nsteps, rec_slot, (float2*)traj, T, st),
-            "linear step launch");
+    if (p->cplx)
+      HIP_TRY(launch_linear_steps_c(U, p->coefc + CS_E * p->modes, p->coef_bstride(), p->modes, p->C,
+                                    p->state_elems(), nsteps, rec_slot, (float2*)traj, T, st),
+              "linear step launch");
+    else
+      HIP_TRY(launch_linear_steps(U, p->coef + CS_E * p->modes, p->coef_bstride(), p->modes, p->C, p->state_elems(),
+                                  nsteps, rec_slot, (float2*)traj, T, st),
+              "linear step launch");
     return TFSM_OK;
   }
   if (!ws) return fail(TFSM_E_INVALID, "workspace pointer is null");
@@ -668,6 +678,21 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   if ((term == TERM_CONV || term == TERM_NSP) && D == 3 && !step3d_supported(geom->n[0], geom->n[1], geom->n[2]))
     return fail(TFSM_E_INVALID, "3D convection needs 16, 32, 64 or 128 points per axis, or a cubic 48, 96 or 100 mesh");
   if (terms->n_linear < 0 || terms->n_linear > 4) return fail(TFSM_E_INVALID, "at most 4 linear terms");
+  bool cplx = false;
+  for (int t = 0; t < terms->n_linear; ++t) {
+    const int kind = terms->linear_kind[t];
+    if (kind < TFSM_LIN_LAP || kind > TFSM_LIN_DERIV) return fail(TFSM_E_INVALID, "unknown linear term");
+    if (kind != TFSM_LIN_DERIV) continue;
+    if (terms->linear_dim[t] < 0 || terms->linear_dim[t] >= D)
+      return fail(TFSM_E_INVALID, "SpatialDerivative axis out of range");
+    if (terms->linear_order[t] < 1 || terms->linear_order[t] > 8)
+      return fail(TFSM_E_INVALID, "SpatialDerivative order must be 1..8");
+    if (geom->channels != 1) return fail(TFSM_E_INVALID, "The SpatialDerivative operator only supports scalar field.");
+    if (terms->linear_order[t] & 1) cplx = true;
+  }
+  if (cplx && term != TERM_NONE && D > 1)
+    return fail(TFSM_E_INVALID,
+                "term not supported by engine: an odd-order SpatialDerivative (complex L) with a product term in 2D/3D");
   if (terms->n_linear > 0 && !terms->linear_coef) return fail(TFSM_E_INVALID, "linear coefficients missing");
   const int Bc = terms->coef_batch < 1 ? 1 : terms->coef_batch;
   if (Bc != 1 && Bc != geom->batch) return fail(TFSM_E_INVALID, "coef_batch must be 1 or the batch size");
@@ -691,7 +716,12 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   p->coef_t = terms->nonlinear_coef;
   p->rate = terms->dealias_rate > 0 ? terms->dealias_rate : 2.0 / 3.0;
   p->nlin = terms->n_linear;
-  for (int t = 0; t < p->nlin; ++t) p->lin_kind[t] = terms->linear_kind[t];
+  for (int t = 0; t < p->nlin; ++t) {
+    p->lin_kind[t] = terms->linear_kind[t];
+    p->lin_dim[t] = terms->linear_dim[t];
+    p->lin_order[t] = terms->linear_order[t];
+  }
+  p->cplx = cplx;
   p->per_len = D == 1 && geom->length_per_sample;
   p->Bc = Bc;
   int rc = TFSM_OK;
@@ -737,6 +767,8 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   if (!rc) rc = dev_alloc(p, &p->L, (size_t)Lb * p->modes, "linear symbol");
   if (!rc) rc = dev_alloc(p, &p->coef, (size_t)Lb * NCOEF * p->modes, "coefficient table");
   if (!rc) rc = dev_alloc(p, &p->cpk, (size_t)Lb * 2 * p->modes, "packed coefficient table");
+  if (!rc && cplx) rc = dev_alloc(p, &p->Lc, (size_t)Lb * p->modes, "complex linear symbol");
+  if (!rc && cplx) rc = dev_alloc(p, &p->coefc, (size_t)Lb * NCOEF * p->modes, "complex coefficient table");
   if (!rc) {
     hipError_t e = hipMemsetAsync(p->coef, 0, (size_t)Lb * NCOEF * p->modes * sizeof(float), st);
     if (e != hipSuccess) rc = hip_fail(e, "coefficient table clear");
@@ -753,9 +785,14 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     la.modes = p->modes;
     la.kv_bstride = p->kv_bstride;
     la.nterm = p->nlin;
-    for (int t = 0; t < 4; ++t) la.kind[t] = t < p->nlin ? terms->linear_kind[t] : 0;
+    for (int t = 0; t < 4; ++t) {
+      la.kind[t] = t < p->nlin ? terms->linear_kind[t] : 0;
+      la.dim[t] = p->lin_dim[t];
+      la.order[t] = p->lin_order[t];
+    }
     la.Bc = Lb;
     la.L = p->L;
+    la.Lc = p->Lc;
     float* dcoef = nullptr;
     std::vector<float> hc((size_t)(p->nlin > 0 ? p->nlin : 1) * Lb, 0.f);
     for (int t = 0; t < p->nlin; ++t)
@@ -921,8 +958,10 @@ int tfsm_plan_set_scheme(tfsm_plan* plan, int32_t scheme, double dt, int32_t n_p
   a.L = plan->L;
   a.coef = plan->coef;
   a.cpk = plan->cpk;
+  a.Lc = plan->Lc;
+  a.coefc = plan->coefc;
   HIP_TRY(launch_coefs(a, (hipStream_t)stream), "coefficient launch");
-  HIP_TRY(launch_pack_coefs(a, (hipStream_t)stream), "coefficient pack launch");
+  if (!plan->cplx) HIP_TRY(launch_pack_coefs(a, (hipStream_t)stream), "coefficient pack launch");
   plan->scheme = scheme;
   plan->dt = dt;
   plan->n_points = n_points;
@@ -959,11 +998,14 @@ int tfsm_plan_set_sample(tfsm_plan* p, int32_t b, const float* linear_coef, doub
       la.nterm = p->nlin;
       for (int t = 0; t < 4; ++t) {
         la.kind[t] = t < p->nlin ? p->lin_kind[t] : 0;
+        la.dim[t] = p->lin_dim[t];
+        la.order[t] = p->lin_order[t];
         la.cval[t] = t < p->nlin ? linear_coef[t] : 0.f;
       }
       la.coef = nullptr;
       la.Bc = 1;
       la.L = p->L + (size_t)r * p->modes;
+      la.Lc = p->cplx ? p->Lc + (size_t)r * p->modes : nullptr;
       HIP_TRY(launch_linear_symbol(la, st), "linear symbol launch");
     }
     if (p->scheme >= 0) {
@@ -979,8 +1021,10 @@ int tfsm_plan_set_sample(tfsm_plan* p, int32_t b, const float* linear_coef, doub
       a.L = p->L + (size_t)r * p->modes;
       a.coef = p->coef + (size_t)r * NCOEF * p->modes;
       a.cpk = p->cpk + (size_t)r * 2 * p->modes;
+      a.Lc = p->cplx ? p->Lc + (size_t)r * p->modes : nullptr;
+      a.coefc = p->cplx ? p->coefc + (size_t)r * NCOEF * p->modes : nullptr;
       HIP_TRY(launch_coefs(a, st), "coefficient launch");
-      HIP_TRY(launch_pack_coefs(a, st), "coefficient pack launch");
+      if (!p->cplx) HIP_TRY(launch_pack_coefs(a, st), "coefficient pack launch");
     }
   }
   return TFSM_OK;
@@ -1184,9 +1228,14 @@ int tfsm_apply(tfsm_plan* p, const void* uh, void* out, void* ws, void* stream) 
     }
   }
   const float2* Nv = have_nl ? (const float2*)((char*)ws + p->off_A1) : nullptr;
-  HIP_TRY(launch_apply_linear(U, Nv, p->L, p->Bc > 1 ? p->modes : 0, p->modes, p->C, p->state_elems(), have_lin,
-                              have_nl, (float2*)out, st),
-          "apply launch");
+  if (p->cplx)
+    HIP_TRY(launch_apply_linear_c(U, Nv, p->Lc, p->Bc > 1 ? p->modes : 0, p->modes, p->C, p->state_elems(), have_lin,
+                                  have_nl, (float2*)out, st),
+            "apply launch");
+  else
+    HIP_TRY(launch_apply_linear(U, Nv, p->L, p->Bc > 1 ? p->modes : 0, p->modes, p->C, p->state_elems(), have_lin,
+                                have_nl, (float2*)out, st),
+            "apply launch");
   return TFSM_OK;
 }
 
@@ -1212,9 +1261,14 @@ int tfsm_rk_combine(tfsm_plan* p, void* out, const void* x, const void* const* k
 int tfsm_solve(tfsm_plan* p, const void* uh, void* out, void* stream) {
   if (!p || !uh || !out) return fail(TFSM_E_INVALID, "null argument");
   if (p->nlin == 0) return fail(TFSM_E_INVALID, "solve needs a linear operator");
-  HIP_TRY(launch_apply_linear((const float2*)uh, nullptr, p->L, p->Bc > 1 ? p->modes : 0, p->modes, p->C,
-                              p->state_elems(), 2, 0, (float2*)out, (hipStream_t)stream),
-          "solve launch");
+  if (p->cplx)
+    HIP_TRY(launch_apply_linear_c((const float2*)uh, nullptr, p->Lc, p->Bc > 1 ? p->modes : 0, p->modes, p->C,
+                                  p->state_elems(), 2, 0, (float2*)out, (hipStream_t)stream),
+            "solve launch");
+  else
+    HIP_TRY(launch_apply_linear((const float2*)uh, nullptr, p->L, p->Bc > 1 ? p->modes : 0, p->modes, p->C,
+                                p->state_elems(), 2, 0, (float2*)out, (hipStream_t)stream),
+            "solve launch");
   return TFSM_OK;
 }
 
@@ -1264,6 +1318,7 @@ int tfsm_adjoint_prepare(tfsm_plan* p, int32_t grad_term, void* stream) {
   if (p->term != TERM_CONV || p->D > 2)
     return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint covers Convection in 1D and 2D");
   if (p->src) return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint does not cover explicit sources");
+  if (p->cplx) return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint needs a real linear symbol");
   if (p->per_len) return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint needs shared lengths");
   if (grad_term < -1 || grad_term >= p->nlin) return fail(TFSM_E_INVALID, "grad_term out of range");
   hipStream_t st = (hipStream_t)stream;

@@ -78,6 +78,22 @@ __device__ __forceinline__ float subr(float a, float b) {
 __device__ __forceinline__ float2 rscale(float c, float2 z) { return make_float2(mulr(c, z.x), mulr(c, z.y)); }
 __device__ __forceinline__ float2 radd(float2 a, float2 b) { return make_float2(addr(a.x, b.x), addr(a.y, b.y)); }
 __device__ __forceinline__ float2 rsub(float2 a, float2 b) { return make_float2(subr(a.x, b.x), subr(a.y, b.y)); }
+// complex64 product as torch computes it: (ac - bd) + (ad + bc) i, each product rounded
+__device__ __forceinline__ float2 cmulr(float2 c, float2 z) {
+  return make_float2(subr(mulr(c.x, z.x), mulr(c.y, z.y)), addr(mulr(c.x, z.y), mulr(c.y, z.x)));
+}
+// complex64 quotient a / b with Smith's scaling (c10::complex operator/)
+__device__ __forceinline__ float2 cdivr(float2 a, float2 b) {
+  const float ac = fabsf(b.x), ad = fabsf(b.y);
+  if (ac >= ad) {
+    const float rat = __fdiv_rn(b.y, b.x);
+    const float scl = __fdiv_rn(1.0f, addr(b.x, mulr(b.y, rat)));
+    return make_float2(mulr(addr(a.x, mulr(a.y, rat)), scl), mulr(subr(a.y, mulr(a.x, rat)), scl));
+  }
+  const float rat = __fdiv_rn(b.x, b.y);
+  const float scl = __fdiv_rn(1.0f, addr(b.y, mulr(b.x, rat)));
+  return make_float2(mulr(addr(mulr(a.x, rat), a.y), scl), mulr(subr(mulr(a.y, rat), a.x), scl));
+}
 
 // Per-mode stage combine.  Arrays are indexed by the same element index; coefficient pointer
 // `cf` points at slot 0 of this sample's table, `cstride` = modes per table.  Returns s.
@@ -153,6 +169,73 @@ __device__ __forceinline__ float2 stage_combine(int stage, float2 nval, float2* 
       if (write) A1[ei] = nval;
       return nval;
     }
+    default:
+      return U[ei];
+  }
+}
+
+// Complex-coefficient stage combine (complex-L plans: E, E2 complex; ETDRK c1, c2 complex; SETDRK
+// c's real, stored with a zero imaginary part).  Same stage formulas and op order as
+// stage_combine, with every coefficient product a complex64 product.
+__device__ __forceinline__ float2 stage_combine_c(int stage, float2 nval, float2* __restrict__ U,
+                                                  float2* __restrict__ A1, float2* __restrict__ A2, size_t ei,
+                                                  const float2* __restrict__ cf, size_t cstride, size_t ci) {
+  auto C = [&](int slot) { return cf[(size_t)slot * cstride + ci]; };
+  switch (stage) {
+    case ST_R1_E0: {
+      const float2 un = radd(cmulr(C(CS_E), U[ei]), cmulr(C(CS_C1), nval));
+      U[ei] = un;
+      return un;
+    }
+    case ST_R2_E0:
+      A1[ei] = nval;
+      return radd(cmulr(C(CS_E), U[ei]), cmulr(C(CS_C1), nval));
+    case ST_R2_E1: {
+      const float2 n0 = A1[ei];
+      const float2 a = radd(cmulr(C(CS_E), U[ei]), cmulr(C(CS_C1), n0));
+      const float2 un = radd(a, cmulr(C(CS_C2), rsub(nval, n0)));
+      U[ei] = un;
+      return un;
+    }
+    case ST_R3_E0:
+      A1[ei] = nval;
+      return radd(cmulr(C(CS_E2), U[ei]), cmulr(C(CS_C1), nval));
+    case ST_R3_E1: {
+      const float2 n0 = A1[ei];
+      A2[ei] = nval;
+      const float2 two_na = make_float2(2.0f * nval.x, 2.0f * nval.y);
+      return radd(cmulr(C(CS_E), U[ei]), cmulr(C(CS_C2), rsub(two_na, n0)));
+    }
+    case ST_R3_E2: {
+      const float2 un = radd(radd(radd(cmulr(C(CS_E), U[ei]), cmulr(C(CS_C3), A1[ei])), cmulr(C(CS_C4), A2[ei])),
+                             cmulr(C(CS_C5), nval));
+      U[ei] = un;
+      return un;
+    }
+    case ST_R4_E0:
+      A1[ei] = nval;
+      return radd(cmulr(C(CS_E2), U[ei]), cmulr(C(CS_C1), nval));
+    case ST_R4_E1:
+      A2[ei] = nval;
+      return radd(cmulr(C(CS_E2), U[ei]), cmulr(C(CS_C1), nval));
+    case ST_R4_E2: {
+      const float2 e2 = C(CS_E2), c1 = C(CS_C1), n0 = A1[ei], na = A2[ei];
+      const float2 a = radd(cmulr(e2, U[ei]), cmulr(c1, n0));
+      const float2 two_nb = make_float2(2.0f * nval.x, 2.0f * nval.y);
+      A2[ei] = radd(na, nval);
+      return radd(cmulr(e2, a), cmulr(c1, rsub(two_nb, n0)));
+    }
+    case ST_R4_E3: {
+      const float2 c5 = C(CS_C5);
+      const float2 c5x2 = make_float2(2.0f * c5.x, 2.0f * c5.y);
+      const float2 un = radd(radd(radd(cmulr(C(CS_E), U[ei]), cmulr(C(CS_C4), A1[ei])), cmulr(c5x2, A2[ei])),
+                             cmulr(C(CS_C6), nval));
+      U[ei] = un;
+      return un;
+    }
+    case ST_EVAL:
+      A1[ei] = nval;
+      return nval;
     default:
       return U[ei];
   }
@@ -284,7 +367,7 @@ struct Col2Args {
   const float2* src;          // [Bs][Cs][N0][P1] or null
   size_t src_bstride, src_cstride;
   float2* G;                  // fields [B][NF][N0][PF]
-  const float2* dc;           // KSConvection: batch mean of the product DC (subtracted at k = 0), or null
+  const float2* dc = nullptr;  // KSConvection: batch mean of the product DC (subtracted at k = 0), or null
   int b0;                     // first sample handled by this launch
   int nb;                     // samples handled by this launch (set by the launcher)
 };
@@ -317,7 +400,7 @@ struct Col3Args {
   const float2* src;
   size_t src_bstride, src_cstride;
   float2* G1;                 // [B][2 nc][N0][K1][PF2]
-  const float2* dc;           // KSConvection: batch mean of the product DC, or null
+  const float2* dc = nullptr;  // KSConvection: batch mean of the product DC, or null
   int b0, nb;
 };
 
@@ -353,6 +436,7 @@ struct Run1Args {
   float2* A2;
   const float* coef;
   size_t coef_bstride;
+  const float2* coefc = nullptr;  // complex-L plans: complex coefficient tables (same strides), else null
   const float2* src;
   size_t src_bstride;
   int nsteps;
@@ -364,7 +448,7 @@ struct Run1Args {
 };
 
 
-enum LinKind : int { LIN_LAP = 1, LIN_BIHARM = 2, LIN_UNIT = 3 };
+enum LinKind : int { LIN_LAP = 1, LIN_BIHARM = 2, LIN_UNIT = 3, LIN_DERIV = 4 };
 
 struct LinArgs {
   int D;
@@ -375,10 +459,12 @@ struct LinArgs {
   size_t kv_bstride;       // per-sample axis tables (1D serving); 0 = shared
   int nterm;
   int kind[4];
+  int dim[4] = {0, 0, 0, 0}, order[4] = {0, 0, 0, 0};  // LIN_DERIV parameters
   const float* coef;       // [nterm][Bc]; null: use cval (one row, coefficients by value)
   float cval[4];
   int Bc;
-  float* L;                // [Bc][modes]
+  float* L;                // [Bc][modes] real part
+  float2* Lc = nullptr;    // [Bc][modes] complex L (complex-L plans), or null
 };
 
 struct CoefArgs {
@@ -392,6 +478,8 @@ struct CoefArgs {
   int Bc;
   const float* L;          // [Bc][modes]
   float* coef;             // [Bc][NCOEF][modes]
+  const float2* Lc = nullptr;  // complex-L plans: [Bc][modes], and the complex tables below
+  float2* coefc = nullptr;     // [Bc][NCOEF][modes]
 };
 
 struct ConvArgs {
@@ -418,6 +506,8 @@ hipError_t launch_full_to_half(const float2* full, float2* half, const ConvArgs&
 hipError_t launch_half_to_full(const float2* half, float2* full, const ConvArgs& a, hipStream_t st);
 hipError_t launch_linear_steps(float2* U, const float* E, size_t coef_bstride, size_t modes, int C, size_t total,
                                int nsteps, const int* rec_slot, float2* traj, int T, hipStream_t st);
+hipError_t launch_linear_steps_c(float2* U, const float2* E, size_t coef_bstride, size_t modes, int C, size_t total,
+                                 int nsteps, const int* rec_slot, float2* traj, int T, hipStream_t st);
 hipError_t launch_record(const float2* U, float2* traj, const int* rec_slot, int step, int B, size_t per_sample,
                          int T, hipStream_t st);
 hipError_t launch_run1d(const Run1Args& a, hipStream_t st);
@@ -491,5 +581,7 @@ hipError_t launch_rk_combine(const RkArgs& a, hipStream_t st);
 
 hipError_t launch_apply_linear(const float2* U, const float2* Nv, const float* L, size_t l_bstride, size_t modes, int C,
                                size_t total, int have_lin, int have_nl, float2* out, hipStream_t st);
+hipError_t launch_apply_linear_c(const float2* U, const float2* Nv, const float2* L, size_t l_bstride, size_t modes,
+                                 int C, size_t total, int have_lin, int have_nl, float2* out, hipStream_t st);
 
 }  // namespace tfsm

@@ -1,6 +1,6 @@
 // Setup kernels: linear symbol L per mode (float32, the reference's op order), ETD coefficient
-// tables (SETDRK: float64 contour means; ETDRK: the reference's complex64 closed forms), and the
-// accumulation of explicit source spectra.
+// tables (SETDRK: float64 contour means; ETDRK: the reference's complex64 closed forms; complex L:
+// complex tables), and the accumulation of explicit source spectra.
 #include "engine.hpp"
 
 namespace tfsm {
@@ -9,7 +9,23 @@ namespace tfsm {
 
 // L = sum_t coef_t * sym_t in generator order (_base.py:339-357), each product and sum rounded in
 // float32: lap = (-(a0 a0)) + (-(a1 a1)) [+ (-(a2 a2))] (mesh.py:406-426), biharm = lap * lap,
-// unit = 1 (_source.py:9-17).  Padding modes (k >= H) get 0.
+// unit = 1 (_source.py:9-17), SpatialDerivative (i a_d)^n (mesh.py:399-404: n = 1 the value,
+// n = 2 and 3 complex products, higher n from double).  The reference's symbols are complex64;
+// only odd derivative orders give them a nonzero imaginary part, so L is written as its real part
+// (and, for complex-L plans, as complex).  Padding modes (k >= H) get 0.
+__device__ __forceinline__ float2 deriv_symbol(float a, int n) {
+  if (n == 1) return make_float2(0.f, a);
+  if (n == 2) return make_float2(-mulr(a, a), 0.f);
+  if (n == 3) return make_float2(0.f, -mulr(mulr(a, a), a));
+  const double p = pow((double)a, (double)n);
+  switch (n & 3) {
+    case 0: return make_float2((float)p, 0.f);
+    case 1: return make_float2(0.f, (float)p);
+    case 2: return make_float2((float)-p, 0.f);
+    default: return make_float2(0.f, (float)-p);
+  }
+}
+
 __global__ void k_linear_symbol(LinArgs a) {
   const size_t total = a.modes * a.Bc;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
@@ -24,7 +40,7 @@ __global__ void k_linear_symbol(LinArgs a) {
       r /= a.n[1];
     }
     if (a.D >= 2) idx[0] = (int)r;
-    float out = 0.f;
+    float2 out = make_float2(0.f, 0.f);
     if (k < a.H) {
       float lap = 0.f;
       for (int d = 0; d < a.D; ++d) {
@@ -34,14 +50,17 @@ __global__ void k_linear_symbol(LinArgs a) {
       }
       for (int t = 0; t < a.nterm; ++t) {
         const float c = a.coef ? a.coef[t * a.Bc + b] : a.cval[t];
-        float sym = 1.f;
-        if (a.kind[t] == LIN_LAP) sym = lap;
-        else if (a.kind[t] == LIN_BIHARM) sym = mulr(lap, lap);
-        const float v = mulr(c, sym);
-        out = (t == 0) ? v : addr(out, v);
+        float2 sym = make_float2(1.f, 0.f);
+        if (a.kind[t] == LIN_LAP) sym.x = lap;
+        else if (a.kind[t] == LIN_BIHARM) sym.x = mulr(lap, lap);
+        else if (a.kind[t] == LIN_DERIV)
+          sym = deriv_symbol(a.kv[a.dim[t]][(size_t)b * a.kv_bstride + idx[a.dim[t]]], a.order[t]);
+        const float2 v = make_float2(mulr(c, sym.x), mulr(c, sym.y));
+        out = (t == 0) ? v : radd(out, v);
       }
     }
-    a.L[e] = out;
+    a.L[e] = out.x;
+    if (a.Lc) a.Lc[e] = out;
   }
 }
 
@@ -182,6 +201,104 @@ __global__ void k_etdrk_coefs(CoefArgs a) {
   }
 }
 
+// Complex-L plans (odd-order SpatialDerivative, e.g. KdV): complex E = exp(fl32(dt L)) and
+// E2 = exp(fl32(dt/2 L)) (double, rounded once); SETDRK c's are the real parts of the contour
+// means, as the reference takes .real of them (_uncached.py:72, 103-104, 137-156, 193-211);
+// ETDRK c1 = where(L == 0, dt, (E - 1) / L), c2 = where(L == 0, dt/2, ((E - 1) - L dt) / ((L L) dt))
+// in complex64 ops (_etdrk.py:43-45, 66-70).
+__device__ __forceinline__ float2 cexp_rounded(float2 z) {
+  const double e = exp((double)z.x);
+  double s, c;
+  sincos((double)z.y, &s, &c);
+  return make_float2((float)(e * c), (float)(e * s));
+}
+
+__global__ void k_coefs_c(CoefArgs a) {
+  const size_t total = a.modes * a.Bc;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t mode = e % a.modes;
+    const size_t b = e / a.modes;
+    const float2 l = a.Lc[e];
+    const float2 z = make_float2(mulr(l.x, a.dt32), mulr(l.y, a.dt32));
+    float2* out = a.coefc + b * (size_t)NCOEF * a.modes + mode;
+    const float2 E = cexp_rounded(z);
+    out[CS_E * a.modes] = E;
+    if (a.scheme == ETDRK0) continue;
+    const bool l0 = l.x == 0.f && l.y == 0.f;
+    if (a.scheme == ETDRK1 || a.scheme == ETDRK2) {
+      const float2 em1 = make_float2(subr(E.x, 1.0f), E.y);
+      out[CS_C1 * a.modes] = l0 ? make_float2(a.dt32, 0.f) : cdivr(em1, l);
+      if (a.scheme == ETDRK2) {
+        const float2 num = rsub(em1, z);
+        const float2 ll = cmulr(l, l);
+        const float2 den = make_float2(mulr(ll.x, a.dt32), mulr(ll.y, a.dt32));
+        out[CS_C2 * a.modes] = l0 ? make_float2((float)(a.dt / 2), 0.f) : cdivr(num, den);
+      }
+      continue;
+    }
+    const float2 z2 = make_float2(mulr(l.x, a.hdt32), mulr(l.y, a.hdt32));
+    if (a.scheme == SETDRK3 || a.scheme == SETDRK4) out[CS_E2 * a.modes] = cexp_rounded(z2);
+    cd s1 = {0, 0}, s2 = {0, 0}, s3 = {0, 0}, s4 = {0, 0}, s5 = {0, 0};
+    const cd one = {1.0, 0.0};
+    for (int jj = 1; jj <= a.M; ++jj) {
+      const double th = 2.0 * M_PI * (jj - 0.5) / a.M;
+      const cd lr = {a.radius * cos(th) + (double)z.x, a.radius * sin(th) + (double)z.y};
+      const cd el = cd_exp(lr);
+      const cd lr2 = cd_mul(lr, lr);
+      const cd lr3 = cd_mul(lr2, lr);
+      const cd eh = cd_exp(cd_scale(lr, 0.5));
+      cd t;
+      switch (a.scheme) {
+        case SETDRK1:
+          s1 = cd_add(s1, cd_div(cd_sub(el, one), lr));
+          break;
+        case SETDRK2:
+          s1 = cd_add(s1, cd_div(cd_sub(el, one), lr));
+          s2 = cd_add(s2, cd_div(cd_sub(cd_sub(el, one), lr), lr2));
+          break;
+        case SETDRK3:
+          s1 = cd_add(s1, cd_div(cd_sub(eh, one), lr));
+          s2 = cd_add(s2, cd_div(cd_sub(el, one), lr));
+          t = cd_add(cd_sub({4.0, 0.0}, cd_scale(lr, 3.0)), lr2);
+          s3 = cd_add(s3, cd_div(cd_add(cd_sub({-4.0, 0.0}, lr), cd_mul(el, t)), lr3));
+          t = cd_add({-2.0, 0.0}, lr);
+          s4 = cd_add(s4, cd_div(cd_scale(cd_add(cd_add({2.0, 0.0}, lr), cd_mul(el, t)), 4.0), lr3));
+          t = cd_sub({4.0, 0.0}, lr);
+          s5 = cd_add(s5, cd_div(cd_add(cd_sub(cd_sub({-4.0, 0.0}, cd_scale(lr, 3.0)), lr2), cd_mul(el, t)), lr3));
+          break;
+        default:  // SETDRK4
+          s1 = cd_add(s1, cd_div(cd_sub(eh, one), lr));
+          t = cd_add(cd_sub({4.0, 0.0}, cd_scale(lr, 3.0)), lr2);
+          s2 = cd_add(s2, cd_div(cd_add(cd_sub({-4.0, 0.0}, lr), cd_mul(el, t)), lr3));
+          t = cd_add({-2.0, 0.0}, lr);
+          s3 = cd_add(s3, cd_div(cd_add(cd_add({2.0, 0.0}, lr), cd_mul(el, t)), lr3));
+          t = cd_sub({4.0, 0.0}, lr);
+          s4 = cd_add(s4, cd_div(cd_add(cd_sub(cd_sub({-4.0, 0.0}, cd_scale(lr, 3.0)), lr2), cd_mul(el, t)), lr3));
+          break;
+      }
+    }
+    const double w = a.dt / a.M;
+    auto re = [&](cd v) { return make_float2((float)(w * v.x), 0.f); };
+    switch (a.scheme) {
+      case SETDRK1: out[CS_C1 * a.modes] = re(s1); break;
+      case SETDRK2: out[CS_C1 * a.modes] = re(s1); out[CS_C2 * a.modes] = re(s2); break;
+      case SETDRK3:
+        out[CS_C1 * a.modes] = re(s1);
+        out[CS_C2 * a.modes] = re(s2);
+        out[CS_C3 * a.modes] = re(s3);
+        out[CS_C4 * a.modes] = re(s4);
+        out[CS_C5 * a.modes] = re(s5);
+        break;
+      default:
+        out[CS_C1 * a.modes] = re(s1);
+        out[CS_C4 * a.modes] = re(s2);
+        out[CS_C5 * a.modes] = re(s3);
+        out[CS_C6 * a.modes] = re(s4);
+        break;
+    }
+  }
+}
+
 // Pack the slot tables into the per-mode layout read by the fused column kernel (see engine.hpp).
 __global__ void k_pack_coefs(CoefArgs a) {
   const size_t total = a.modes * a.Bc;
@@ -239,6 +356,26 @@ __global__ void k_apply_linear(const float2* __restrict__ U, const float2* __res
   }
 }
 
+// complex-L plans: out = L * U + N, or (solve) where(L == 0, 1, 1 / L) * U with complex64 ops
+__global__ void k_apply_linear_c(const float2* __restrict__ U, const float2* __restrict__ Nv,
+                                 const float2* __restrict__ L, size_t l_bstride, size_t modes, int C, size_t total,
+                                 int have_lin, int have_nl, float2* __restrict__ out) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t mode = e % modes;
+    const size_t b = e / modes / C;
+    const float2 l = L[b * l_bstride + mode];
+    if (have_lin == 2) {
+      const float2 inv = (l.x == 0.f && l.y == 0.f) ? make_float2(1.f, 0.f) : cdivr(make_float2(1.f, 0.f), l);
+      out[e] = cmulr(inv, U[e]);
+      continue;
+    }
+    float2 r = make_float2(0.f, 0.f);
+    if (have_lin) r = cmulr(l, U[e]);
+    if (have_nl) r = have_lin ? radd(r, Nv[e]) : Nv[e];
+    out[e] = r;
+  }
+}
+
 static unsigned grid_for(size_t total) {
   size_t b = (total + 255) / 256;
   if (b > 65535u * 4) b = 65535u * 4;
@@ -265,8 +402,17 @@ hipError_t launch_axis_row(float* kv, int n, float s, hipStream_t st) {
   return hipGetLastError();
 }
 
+hipError_t launch_apply_linear_c(const float2* U, const float2* Nv, const float2* L, size_t l_bstride, size_t modes,
+                                 int C, size_t total, int have_lin, int have_nl, float2* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_apply_linear_c, dim3(grid_for(total)), dim3(256), 0, st, U, Nv, L, l_bstride, modes, C, total,
+                     have_lin, have_nl, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_coefs(const CoefArgs& a, hipStream_t st) {
-  if (a.scheme >= SETDRK1)
+  if (a.Lc)
+    hipLaunchKernelGGL(k_coefs_c, dim3(grid_for(a.modes * a.Bc)), dim3(256), 0, st, a);
+  else if (a.scheme >= SETDRK1)
     hipLaunchKernelGGL(k_setdrk_coefs, dim3(grid_for(a.modes * a.Bc)), dim3(256), 0, st, a);
   else
     hipLaunchKernelGGL(k_etdrk_coefs, dim3(grid_for(a.modes * a.Bc)), dim3(256), 0, st, a);

@@ -217,6 +217,27 @@ __global__ void k_linear_steps(float2* __restrict__ U, const float* __restrict__
 }
 
 
+// complex-L plans: U <- E U with complex E (complex64 product)
+__global__ void k_linear_steps_c(float2* __restrict__ U, const float2* __restrict__ E, size_t coef_bstride,
+                                 size_t modes, int C, size_t total, int nsteps, const int* __restrict__ rec_slot,
+                                 float2* __restrict__ traj, int T) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t mode = e % modes;
+    const size_t bc = e / modes;
+    const size_t b = bc / C, c = bc % C;
+    const float2 ev = E[b * coef_bstride + mode];
+    float2 u = U[e];
+    for (int s = 0; s <= nsteps; ++s) {
+      if (rec_slot) {
+        const int sl = rec_slot[s];
+        if (sl >= 0) traj[((b * T + sl) * C + c) * modes + mode] = u;
+      }
+      if (s < nsteps) u = cmulr(ev, u);
+    }
+    U[e] = u;
+  }
+}
+
 // copy the state U [B][per_sample] to traj[b][rec_slot[step]] when that slot is >= 0
 // (_TrajRecorder.record before step i, traj_recorder.py:46-55); the schedule is read on the
 // device so a captured step sequence does not depend on it.
@@ -235,7 +256,8 @@ __global__ void k_record(const float2* __restrict__ U, float2* __restrict__ traj
 // V samples per workgroup (interleaved in LDS); every sample's state, stage spectrum and
 // auxiliary arrays stay owned by fixed threads for the whole run: thread j owns modes k = j + m*NT.
 // Convection (C = D = 1): N = coef_t * fft(u * u_x) [+ S], u from the dealiased stage spectrum.
-template <int N, int V>
+// CPLX: complex-L plan (complex coefficient tables, stage_combine_c).
+template <int N, int V, bool CPLX>
 __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
   using F = Fft<N>;
   constexpr int RMAX = F::RMAX, NT = F::NT, H = N / 2 + 1;
@@ -260,6 +282,7 @@ __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
   float2* A1 = a.A1 + so;
   float2* A2 = a.A2 + so;
   const float* cf = a.coef + (size_t)(valid ? smp : 0) * a.coef_bstride;
+  const float2* cfc = CPLX ? a.coefc + (size_t)(valid ? smp : 0) * a.coef_bstride : nullptr;
   const float* ax = a.ax + (size_t)(valid ? smp : 0) * a.ax_bstride;
   const float2* src = a.src ? a.src + (size_t)(valid ? smp : 0) * a.src_bstride : nullptr;
   float2* stg = s_stage + vi * H;
@@ -309,7 +332,8 @@ __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
           float2 sn = make_float2(0.f, 0.f);
           if (active || (a.eval_only && valid)) {
             float2 nv = nl_value(a.coef_t, v[m], src, pos);
-            sn = stage_combine(stage, nv, U, A1, A2, pos, cf, a.P, pos, true);
+            if constexpr (CPLX) sn = stage_combine_c(stage, nv, U, A1, A2, pos, cfc, a.P, pos);
+            else sn = stage_combine(stage, nv, U, A1, A2, pos, cf, a.P, pos, true);
           }
           stg[pos] = sn;
         }
@@ -383,8 +407,12 @@ __global__ void k1_stage(Run1Args a, int stage, const float2* __restrict__ Nraw,
     if (pos == 0 && dc) v = rsub(v, dc[0]);
     const float2* src = a.src ? a.src + smp * a.src_bstride : nullptr;
     const float2 nv = nl_value(a.coef_t, v, src, pos);
-    const float* cf = a.coef + smp * a.coef_bstride;
-    S[so + pos] = stage_combine(stage, nv, a.U + so, a.A1 + so, a.A2 + so, pos, cf, a.P, pos, true);
+    if (a.coefc)
+      S[so + pos] = stage_combine_c(stage, nv, a.U + so, a.A1 + so, a.A2 + so, pos, a.coefc + smp * a.coef_bstride,
+                                    a.P, pos);
+    else
+      S[so + pos] = stage_combine(stage, nv, a.U + so, a.A1 + so, a.A2 + so, pos, a.coef + smp * a.coef_bstride, a.P,
+                                  pos, true);
   }
 }
 
@@ -499,6 +527,15 @@ hipError_t launch_linear_steps(float2* U, const float* E, size_t coef_bstride, s
   return hipGetLastError();
 }
 
+hipError_t launch_linear_steps_c(float2* U, const float2* E, size_t coef_bstride, size_t modes, int C, size_t total,
+                                 int nsteps, const int* rec_slot, float2* traj, int T, hipStream_t st) {
+  unsigned blocks = (unsigned)((total + 255) / 256);
+  if (blocks > 65535u * 4) blocks = 65535u * 4;
+  hipLaunchKernelGGL(k_linear_steps_c, dim3(blocks), dim3(256), 0, st, U, E, coef_bstride, modes, C, total, nsteps,
+                     rec_slot, traj, T);
+  return hipGetLastError();
+}
+
 hipError_t launch_record(const float2* U, float2* traj, const int* rec_slot, int step, int B, size_t per_sample,
                          int T, hipStream_t st) {
   size_t total = (size_t)B * per_sample;
@@ -512,7 +549,10 @@ template <int N>
 static hipError_t launch_run1_t(const Run1Args& a, hipStream_t st) {
   constexpr int NT = Fft<N>::NT;
   constexpr int V = NT >= 64 ? 1 : 64 / NT;
-  hipLaunchKernelGGL((k1_run<N, V>), dim3((a.B + V - 1) / V), dim3(NT * V), 0, st, a);
+  if (a.coefc)
+    hipLaunchKernelGGL((k1_run<N, V, true>), dim3((a.B + V - 1) / V), dim3(NT * V), 0, st, a);
+  else
+    hipLaunchKernelGGL((k1_run<N, V, false>), dim3((a.B + V - 1) / V), dim3(NT * V), 0, st, a);
   return hipGetLastError();
 }
 

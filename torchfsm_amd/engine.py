@@ -50,8 +50,8 @@ class SpectralPlan:
         batch, channels: B and C of the state.
         nonlinear: None, "conv", "vc", "nsp", "ks" or "ks_keep_mean" (the dealiased product term)
             and its coefficient.
-        linear: list of (kind, coef) with kind in {"lap", "biharm", "unit"}; coef a float or a
-            per-sample sequence of length `batch`.
+        linear: list of (kind, coef) with kind in {"lap", "biharm", "unit"} or ("deriv", dim, order)
+            (SpatialDerivative); coef a float or a per-sample sequence of length `batch`.
         device: CUDA device.
         dealias_rate: 2/3 by default.
         lengths_per_sample: 1D only, per-sample domain lengths (serving of mixed 1D requests).
@@ -97,9 +97,14 @@ class SpectralPlan:
         cb = self.batch if per_sample else 1
         flat = []
         for t, (kind, c) in enumerate(lin):
-            if kind not in self.KIND:
+            if isinstance(kind, tuple) and kind[0] == "deriv":
+                terms.linear_kind[t] = _lib.LIN_DERIV
+                terms.linear_dim[t] = int(kind[1])
+                terms.linear_order[t] = int(kind[2])
+            elif kind in self.KIND:
+                terms.linear_kind[t] = self.KIND[kind]
+            else:
                 raise ValueError(f"term not supported by engine: linear kind {kind}")
-            terms.linear_kind[t] = self.KIND[kind]
             if isinstance(c, (int, float)):
                 flat += [float(c)] * cb
             else:

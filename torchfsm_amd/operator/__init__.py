@@ -18,6 +18,7 @@ from ._cores import (
     KSConvection,
     Laplacian,
     NSPressureConvection,
+    SpatialDerivative,
     VorticityConvection,
 )
 
@@ -37,5 +38,6 @@ __all__ = [
     "VorticityConvection",
     "NSPressureConvection",
     "KSConvection",
+    "SpatialDerivative",
     "check_value_with_mesh",
 ]

@@ -215,7 +215,9 @@ class OperatorLike(_MutableMixIn):
         lin = []
         for coef, core in self._linear_cores:
             kind = getattr(core, "_engine_kind", None)
-            if kind not in ("lap", "biharm", "unit"):
+            if kind == "deriv":
+                kind = ("deriv", core.dim_index, core.order)
+            elif kind not in ("lap", "biharm", "unit"):
                 raise ValueError(f"term not supported by engine: linear core {type(core).__name__}")
             lin.append((kind, _coef_value(coef, batch)))
         prod, prod_coef, sources = None, 0.0, []

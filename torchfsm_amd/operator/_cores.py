@@ -43,6 +43,38 @@ class Biharmonic(LinearOperator):
         super().__init__(_BiharmonicCore())
 
 
+class _SpatialDerivativeCore(LinearCoef):
+    """(i a_dim)^order (_spatial_derivative.py:7-20); odd orders make L imaginary."""
+
+    _engine_kind = "deriv"
+
+    def __init__(self, dim_index: int, order: int) -> None:
+        super().__init__()
+        self.dim_index = dim_index
+        self.order = order
+
+    def __call__(self, f_mesh: FourierMesh, n_channel: int):
+        return f_mesh.grad(self.dim_index, self.order)
+
+
+class _SpatialDerivativeGenerator(CoreGenerator):
+    def __init__(self, dim_index: int, order: int) -> None:
+        self.dim_index = dim_index
+        self.order = order
+
+    def __call__(self, f_mesh: FourierMesh, n_channel: int) -> LinearCoef:
+        if n_channel != 1:
+            raise ValueError("The SpatialDerivative operator only supports scalar field.")
+        return _SpatialDerivativeCore(self.dim_index, self.order)
+
+
+class SpatialDerivative(LinearOperator):
+    """d^order / d x_dim^order of a scalar field (_spatial_derivative.py:40-55)."""
+
+    def __init__(self, dim_index: int, order: int) -> None:
+        super().__init__(_SpatialDerivativeGenerator(dim_index, order))
+
+
 class _ImplicitUnitSourceCore(LinearCoef):
     _engine_kind = "unit"
 

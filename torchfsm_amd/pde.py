@@ -9,6 +9,7 @@ from .operator import (
     Laplacian,
     NSPressureConvection,
     Operator,
+    SpatialDerivative,
     VorticityConvection,
 )
 
@@ -28,6 +29,11 @@ def KuramotoSivashinsky() -> Operator:
 def KuramotoSivashinskyHighDim() -> Operator:
     """phi_t = -lap phi - lap^2 phi - 1/2 |grad phi|^2 in 1D, 2D or 3D (pde.py:41-49)."""
     return -Laplacian() - Biharmonic() - KSConvection()
+
+
+def KortewegDeVries(dispersion_coef=1, convection_coef: float = 6.0) -> Operator:
+    """phi_t = -c1 phi_xxx + c2 phi phi_x (pde.py:51-64): an imaginary linear symbol."""
+    return -dispersion_coef * SpatialDerivative(0, 3) + convection_coef * Convection()
 
 
 def NavierStokesVorticity(Re: float, force: Optional[Operator] = None) -> Operator:
