@@ -200,6 +200,28 @@ int tfsm_apply(tfsm_plan* plan, const void* uh, void* out, void* ws, void* strea
  * pointers and coefficients), in the order of the reference's stage sums.  With tfsm_apply as the
  * right-hand side L u + N(u) this runs the explicit Runge-Kutta integrators.  Replaces the stage
  * and update sums of _RKBase._rk_step (_rk.py:43-58). */
+/* Diagnostic spectral maps (single applications of operators that change the channel count), on
+ * half spectra of the plan's mesh and batch; out = coef * map(in):
+ *   TFSM_MAP_DEALIAS   C -> C   u^ * low_pass_filter (mesh.py:443-461), C = the plan's channels
+ *   TFSM_MAP_DIV       D -> 1   sum_i (i a_i) u_i                       (_div.py:17-28)
+ *   TFSM_MAP_GRAD      1 -> D   (i a_i) p                               (_grad.py:12-15)
+ *   TFSM_MAP_CURL      2 -> 1 (2D), 3 -> 3 (3D)                         (_curl.py:18-56)
+ *   TFSM_MAP_V2V       1 -> 2   -invlap [i a_1, -i a_0] w, 2D           (_navier_stokes.py:75-91)
+ *   TFSM_MAP_PRESSURE  D -> 1   -(sum_i (i a_i) conv_i) invlap          (_navier_stokes.py:140-144, 202-206)
+ *   TFSM_MAP_CONSDIV   D*D -> D sum_i (i a_i) X[i][c]                  (_conservative_convection.py:23-27) */
+#define TFSM_MAP_DEALIAS 0
+#define TFSM_MAP_DIV 1
+#define TFSM_MAP_GRAD 2
+#define TFSM_MAP_CURL 3
+#define TFSM_MAP_V2V 4
+#define TFSM_MAP_PRESSURE 5
+#define TFSM_MAP_CONSDIV 6
+int tfsm_spectral_map(tfsm_plan* plan, int32_t op, float coef, const void* in, void* out, void* stream);
+
+/* Physical-space products out[b][i][j] = u[b][i] * u[b][j] of a [batch][C][N...] float field of the
+ * plan's mesh and batch (ConservativeConvection's u u^T, _conservative_convection.py:25). */
+int tfsm_pointwise_products(tfsm_plan* plan, const float* u, float* out, void* stream);
+
 int tfsm_rk_combine(tfsm_plan* plan, void* out, const void* x, const void* const* ks, const float* coefs, int32_t nk,
                     float dt, void* stream);
 

@@ -406,6 +406,61 @@ def apply(terms: List[Term], mesh_info, u, dealias_rate=2 / 3, return_in_fourier
     return out if return_in_fourier else mesh.ifft(out).real
 
 
+# ----------------------------------------------------------------------------- diagnostic operators
+def div(u_fft, mesh: Mesh):
+    """_DivCore (_div.py:17-28)."""
+    return torch.sum(mesh.nabla_vector(1) * u_fft, dim=1, keepdim=True)
+
+
+def grad(u_fft, mesh: Mesh):
+    """Grad: linear coefficient nabla_vector(1) times u^ (_grad.py:12-15)."""
+    return mesh.nabla_vector(1) * u_fft
+
+
+def curl(u_fft, mesh: Mesh):
+    """_Curl2DCore / _Curl3DCore (_curl.py:18-56)."""
+    if mesh.n_dim == 2:
+        return mesh.grad(0, 1) * u_fft[:, 1:2] - mesh.grad(1, 1) * u_fft[:, 0:1]
+    return torch.cat([mesh.grad(1, 1) * u_fft[:, 2:3] - mesh.grad(2, 1) * u_fft[:, 1:2],
+                      mesh.grad(2, 1) * u_fft[:, 0:1] - mesh.grad(0, 1) * u_fft[:, 2:3],
+                      mesh.grad(0, 1) * u_fft[:, 1:2] - mesh.grad(1, 1) * u_fft[:, 0:1]], dim=1)
+
+
+def vorticity2velocity_coef(mesh: Mesh):
+    """_Vorticity2VelocityCore (_navier_stokes.py:75-91)."""
+    n0, n1 = mesh.mesh_info[0][-1], mesh.mesh_info[1][-1]
+    return -1 * mesh.inv_lap() * torch.cat([mesh.grad(1, 1).repeat([1, 1, n0, 1]),
+                                           -mesh.grad(0, 1).repeat([1, 1, 1, n1])], dim=1)
+
+
+def pressure_of_convection(conv, mesh: Mesh):
+    """p = -1 * div(conv) * invlap (_navier_stokes.py:140-144, 202-206)."""
+    p = torch.sum(mesh.nabla_vector(1) * conv, dim=1, keepdim=True)
+    return -1 * p * mesh.inv_lap()
+
+
+def vorticity2pressure(w_fft, mesh: Mesh, dealias_rate=2 / 3):
+    """Vorticity2Pressure()(u_fft=w^) (_navier_stokes.py:119-170): closure mask, velocity,
+    Convection core, pressure."""
+    wd = w_fft * mesh.mask(dealias_rate)
+    vel = wd * vorticity2velocity_coef(mesh)
+    return pressure_of_convection(convection(vel, mesh, mesh.ifft(vel).real), mesh)
+
+
+def velocity2pressure(u_fft, mesh: Mesh, dealias_rate=2 / 3):
+    """Velocity2Pressure()(u_fft=u^) (_navier_stokes.py:174-217)."""
+    ud = u_fft * mesh.mask(dealias_rate)
+    return pressure_of_convection(convection(ud, mesh, mesh.ifft(ud).real), mesh)
+
+
+def conservative_convection(u_fft, mesh: Mesh, dealias_rate=2 / 3):
+    """ConservativeConvection()(u_fft=u^) (_conservative_convection.py:17-27), closure mask first."""
+    ud = u_fft * mesh.mask(dealias_rate)
+    u = mesh.ifft(ud).real
+    uu_fft = mesh.fft(u.unsqueeze(2) * u.unsqueeze(1))
+    return (mesh.nabla_vector(1).unsqueeze(2) * uu_fft).sum(1)
+
+
 # ----------------------------------------------------------------------------- initial-condition generators
 def normalise_field(u, zero_centered=True, unit_variance=False, unit_magnitude=True):
     """Normalisation tail of diffused_noise / truncated_fourier_series (field.py:55-62, 117-124)."""

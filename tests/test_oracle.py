@@ -410,3 +410,38 @@ def test_setdrk_complex_l_takes_the_real_part():
     th = w * dt
     assert torch.allclose(sch.c1, dt * torch.sin(th) / th, rtol=1e-11, atol=0)
     assert torch.allclose(sch.E, torch.exp(1j * th), rtol=1e-14, atol=0)
+
+
+# ---------------------------------------------------------------- diagnostic operators
+def test_diagnostic_operators_closed_forms():
+    """Taylor-Green on [0, 2 pi)^2: psi = sin x sin y, w = -lap psi = 2 sin x sin y, velocity
+    u = (d_y psi, -d_x psi) = (sin x cos y, -cos x sin y); div u = 0; curl u = w; grad psi = (cos x
+    sin y, sin x cos y); (u . grad) u = div(u u^T) = (sin 2x, sin 2y) / 2; pressure
+    -lap^-1 div((u . grad) u) = (cos 2x + cos 2y) / 4."""
+    n = 32
+    mi = [(0, 2 * math.pi, n)] * 2
+    m = R.Mesh(mi, dtype=torch.float64)
+    x = torch.arange(n, dtype=torch.float64) * 2 * math.pi / n
+    X, Y = torch.meshgrid(x, x, indexing="ij")
+    psi = (torch.sin(X) * torch.sin(Y))[None, None]
+    w = 2 * psi
+    vel = torch.stack([torch.sin(X) * torch.cos(Y), -torch.cos(X) * torch.sin(Y)])[None]
+    vf = m.fft(vel)
+    sp = lambda f: m.ifft(f).real  # noqa: E731
+    assert sp(R.div(vf, m)).abs().max() < 1e-12
+    assert (sp(R.curl(vf, m)) - w).abs().max() < 1e-12
+    g = torch.stack([torch.cos(X) * torch.sin(Y), torch.sin(X) * torch.cos(Y)])[None]
+    assert (sp(R.grad(m.fft(psi), m)) - g).abs().max() < 1e-12
+    assert (sp(m.fft(w) * R.vorticity2velocity_coef(m)) - vel).abs().max() < 1e-12
+    adv = torch.stack([torch.sin(2 * X), torch.sin(2 * Y)])[None] / 2
+    assert (sp(R.conservative_convection(vf, m)) - adv).abs().max() < 1e-12
+    p = ((torch.cos(2 * X) + torch.cos(2 * Y)) / 4)[None, None]
+    assert (sp(R.velocity2pressure(vf, m)) - p).abs().max() < 1e-12
+    assert (sp(R.vorticity2pressure(m.fft(w), m)) - p).abs().max() < 1e-12
+    # 3D curl of (sin z, sin x, sin y) = (cos y, cos z, cos x)
+    m3 = R.Mesh([(0, 2 * math.pi, 16)] * 3, dtype=torch.float64)
+    z = torch.arange(16, dtype=torch.float64) * 2 * math.pi / 16
+    X3, Y3, Z3 = torch.meshgrid(z, z, z, indexing="ij")
+    v3 = torch.stack([torch.sin(Z3), torch.sin(X3), torch.sin(Y3)])[None]
+    c3 = torch.stack([torch.cos(Y3), torch.cos(Z3), torch.cos(X3)])[None]
+    assert (m3.ifft(R.curl(m3.fft(v3), m3)).real - c3).abs().max() < 1e-12

@@ -18,6 +18,7 @@ TFSM_E_OOM = -4
 
 TERM_NONE, TERM_CONV, TERM_VC, TERM_NSP, TERM_KS, TERM_KS_KEEP_MEAN = 0, 1, 2, 3, 4, 5
 LIN_LAP, LIN_BIHARM, LIN_UNIT, LIN_DERIV = 1, 2, 3, 4
+MAP_DEALIAS, MAP_DIV, MAP_GRAD, MAP_CURL, MAP_V2V, MAP_PRESSURE, MAP_CONSDIV = 0, 1, 2, 3, 4, 5, 6
 ETDRK0, ETDRK1, ETDRK2, SETDRK1, SETDRK2, SETDRK3, SETDRK4 = range(7)
 
 
@@ -77,6 +78,8 @@ SIGNATURES = {
     "tfsm_apply": (ctypes.c_int, [VP, VP, VP, VP, VP]),
     "tfsm_solve": (ctypes.c_int, [VP, VP, VP, VP]),
     "tfsm_rk_combine": (ctypes.c_int, [VP, VP, VP, VP, VP, I32, ctypes.c_float, VP]),
+    "tfsm_spectral_map": (ctypes.c_int, [VP, I32, ctypes.c_float, VP, VP, VP]),
+    "tfsm_pointwise_products": (ctypes.c_int, [VP, VP, VP, VP]),
     "tfsm_launch_pass": (ctypes.c_int, [VP, VP, VP, I32, I32, VP]),
     "tfsm_plan_adjoint_workspace_bytes": (I64, [VP]),
     "tfsm_adjoint_prepare": (ctypes.c_int, [VP, I32, VP]),

@@ -1239,6 +1239,52 @@ int tfsm_apply(tfsm_plan* p, const void* uh, void* out, void* ws, void* stream) 
   return TFSM_OK;
 }
 
+int tfsm_spectral_map(tfsm_plan* p, int32_t op, float coef, const void* in, void* out, void* stream) {
+  if (!p || !in || !out) return fail(TFSM_E_INVALID, "null argument");
+  MapArgs a;
+  a.op = op;
+  a.D = p->D;
+  a.B = p->B;
+  switch (op) {
+    case MAP_DEALIAS: a.cin = p->C; a.cout = p->C; break;
+    case MAP_DIV: a.cin = p->D; a.cout = 1; break;
+    case MAP_GRAD: a.cin = 1; a.cout = p->D; break;
+    case MAP_CURL:
+      if (p->D < 2) return fail(TFSM_E_INVALID, "curl operator only works for 2D or 3D vector field");
+      a.cin = p->D; a.cout = p->D == 2 ? 1 : 3;
+      break;
+    case MAP_V2V:
+      if (p->D != 2) return fail(TFSM_E_INVALID, "Only vorticity in 2Dmesh is supported");
+      a.cin = 1; a.cout = 2;
+      break;
+    case MAP_PRESSURE: a.cin = p->D; a.cout = 1; break;
+    case MAP_CONSDIV: a.cin = p->D * p->D; a.cout = p->D; break;
+    default: return fail(TFSM_E_INVALID, "unknown spectral map");
+  }
+  for (int d = 0; d < 3; ++d) {
+    a.n[d] = p->n[d];
+    a.kv[d] = p->kv[d];
+    a.kmax[d] = p->kmax[d];
+  }
+  if (p->kv_bstride) return fail(TFSM_E_INVALID, "spectral maps need shared lengths");
+  a.H = p->H;
+  a.P = p->P;
+  a.modes = p->modes;
+  a.coef = coef;
+  a.in = (const float2*)in;
+  a.out = (float2*)out;
+  HIP_TRY(launch_spectral_map(a, (hipStream_t)stream), "spectral map launch");
+  return TFSM_OK;
+}
+
+int tfsm_pointwise_products(tfsm_plan* p, const float* u, float* out, void* stream) {
+  if (!p || !u || !out) return fail(TFSM_E_INVALID, "null argument");
+  size_t npts = 1;
+  for (int d = 0; d < p->D; ++d) npts *= p->n[d];
+  HIP_TRY(launch_pointwise_products(u, out, p->B, p->C, npts, (hipStream_t)stream), "products launch");
+  return TFSM_OK;
+}
+
 int tfsm_rk_combine(tfsm_plan* p, void* out, const void* x, const void* const* ks, const float* coefs, int32_t nk,
                     float dt, void* stream) {
   if (!p || !out || !x || (nk > 0 && (!ks || !coefs))) return fail(TFSM_E_INVALID, "null argument");

@@ -567,6 +567,23 @@ hipError_t launch_adj_reduce(const float* gpart, size_t per, int B, double* out,
 hipError_t launch_adj_dcoefs(const float* L, size_t modes, int Bc, double dt, float* dcoef, hipStream_t st);
 hipError_t launch_adj_sym(const AdjArgs& a, int kind, float* S, hipStream_t st);
 
+// ---------------------------------------------------------------- diagnostic maps (k_maps.hip)
+enum MapOp : int { MAP_DEALIAS = 0, MAP_DIV = 1, MAP_GRAD = 2, MAP_CURL = 3, MAP_V2V = 4, MAP_PRESSURE = 5,
+                   MAP_CONSDIV = 6 };
+struct MapArgs {
+  int op, D, B, cin, cout;
+  int n[3];
+  int H, P;
+  size_t modes;
+  const float* kv[3];
+  int kmax[3];
+  float coef;
+  const float2* in;  // [B][cin][modes]
+  float2* out;       // [B][cout][modes]
+};
+hipError_t launch_spectral_map(const MapArgs& a, hipStream_t st);
+hipError_t launch_pointwise_products(const float* u, float* out, int B, int C, size_t npts, hipStream_t st);
+
 constexpr int RK_MAX_TERMS = 8;
 struct RkArgs {
   const float2* x;

@@ -295,6 +295,24 @@ class SpectralPlan:
                   "rk_combine")
         return out
 
+    def spectral_map(self, op: int, x: torch.Tensor, cout: int, coef: float = 1.0) -> torch.Tensor:
+        """coef * map(x) for a diagnostic operator (tfsm_spectral_map); x and the result are half
+        spectra of this plan's mesh and batch with the map's channel counts."""
+        out = torch.empty((self.batch, cout) + tuple(self.spec_shape[2:]), dtype=torch.complex64, device=self.device)
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_spectral_map(self._plan, int(op), float(coef), _ptr(x.contiguous()), _ptr(out),
+                                        _stream(self.device)), "spectral map")
+        return out
+
+    def pointwise_products(self, u: torch.Tensor) -> torch.Tensor:
+        """u_i u_j of a physical field [B][C][N...] (tfsm_pointwise_products) -> [B][C*C][N...]."""
+        u = u.contiguous()
+        out = torch.empty((self.batch, self.channels * self.channels) + tuple(u.shape[2:]), dtype=torch.float32,
+                          device=self.device)
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_pointwise_products(self._plan, _ptr(u), _ptr(out), _stream(self.device)), "products")
+        return out
+
     def solve(self, uh: torch.Tensor) -> torch.Tensor:
         """uh / L with 1/0 := 1 (half spectra)."""
         out = torch.empty_like(uh)

@@ -11,6 +11,15 @@ from ._base import (
     OperatorLike,
     check_value_with_mesh,
 )
+from ._diag import (
+    ConservativeConvection,
+    Curl,
+    Div,
+    Grad,
+    Velocity2Pressure,
+    Vorticity2Pressure,
+    Vorticity2Velocity,
+)
 from ._cores import (
     Biharmonic,
     Convection,
@@ -39,5 +48,12 @@ __all__ = [
     "NSPressureConvection",
     "KSConvection",
     "SpatialDerivative",
+    "Div",
+    "Grad",
+    "Curl",
+    "Vorticity2Velocity",
+    "Vorticity2Pressure",
+    "Velocity2Pressure",
+    "ConservativeConvection",
     "check_value_with_mesh",
 ]

@@ -502,6 +502,15 @@ class OperatorLike(_MutableMixIn):
             raise ValueError("Both linear coefficient and nonlinear function are None. Cannot build operator.")
         x = u if u is not None else u_fft
         require_gpu_tensor(x, "u" if u is not None else "u_fft")
+        cores = list(self._linear_cores) + list(self._nonlinear_funcs)
+        if any(getattr(core, "_engine_map", None) for _, core in cores):
+            if len(cores) != 1:
+                raise ValueError("term not supported by engine: a channel-changing operator (Div, Grad, Curl, "
+                                 "Vorticity2Velocity, pressure, ConservativeConvection) combined with other terms")
+            from ._diag import map_call
+
+            coef, core = cores[0]
+            return map_call(core, coef, u, u_fft, self._state_dict["f_mesh"], return_in_fourier)
         plan = self._engine(x.shape[0], x.device)
         uh = self._to_half(plan, u, u_fft)
         out = plan.apply(uh)
