@@ -242,6 +242,7 @@ struct Fft {
     return off;
   }
   static constexpr int tw_size() { return tw_offset(NST) > 0 ? tw_offset(NST) : 1; }
+  static constexpr int PD = FftPlan<N, VAR>::R[0];  // LDS padding period of single transforms
   static constexpr bool plan_ok() {
     int p = 1;
     for (int i = 0; i < NST; ++i) {
@@ -253,15 +254,19 @@ struct Fft {
   static_assert(plan_ok(), "radix plan must multiply to N and each radix must divide RMAX");
 };
 
-// LDS slot of position p of interleaved transform vi (V transforms interleaved).
-template <int V>
+// LDS slot of position p of interleaved transform vi (V transforms interleaved).  A single
+// transform (V = 1) is padded by one slot per PD positions, PD = the plan's first radix: the
+// first stage writes at stride PD and the second at stride 1 within groups of PD, and with this
+// padding both fill all banks without conflicts (PD is a power of two).
+template <int V, int PD = 16>
 __device__ __forceinline__ int lds_slot(int p, int vi) {
-  if constexpr (V == 1) return p + (p >> 4);
+  if constexpr (V == 1) return p + (int)((unsigned)p / PD);
   else return p * (V + 1) + vi;
 }
+// slots for any plan (PD >= 4)
 template <int N, int V>
 struct LdsSize {
-  static constexpr int slots = (V == 1) ? (N + N / 16) : N * (V + 1);
+  static constexpr int slots = (V == 1) ? (N + N / 4) : N * (V + 1);
 };
 
 // Synchronisation between the LDS phases of a transform.  WS = false: the transform spans
@@ -294,7 +299,7 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[(Fft<N, VAR>::RMAX)], floa
   constexpr int TWO = F::tw_offset(S);
   if constexpr (!FIRST) {
 #pragma unroll
-    for (int m = 0; m < RMAX; ++m) v[m] = lds[lds_slot<V>(j + m * NT, vi)];
+    for (int m = 0; m < RMAX; ++m) v[m] = lds[lds_slot<V, F::PD>(j + m * NT, vi)];
     lds_sync<WS>();
   }
 #pragma unroll
@@ -318,7 +323,7 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[(Fft<N, VAR>::RMAX)], floa
     } else {
       const int idxd = (jb / NS) * NS * R + (jb % NS);
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[lds_slot<V>(idxd + r * NS, vi)] = x[r];
+      for (int r = 0; r < R; ++r) lds[lds_slot<V, F::PD>(idxd + r * NS, vi)] = x[r];
     }
   }
   if constexpr (!LAST) lds_sync<WS>();
