@@ -304,3 +304,56 @@ def test_navier_stokes_velocity_form(dev, n, dims):
     assert e2 <= TOL
     with pytest.raises(ValueError, match="external force"):
         pde.NavierStokes(100.0, force=tfa.ImplicitSource()).register_mesh(tfa.FourierMesh(mi, device=dev), dims)
+
+
+def test_cpu_recorder_streams_frames_while_stepping(dev):
+    """CPURecorder: frames go to pinned host memory in chunks on a copy stream while the next chunk
+    steps; the result equals AutoRecorder's bit for bit (NS 2D, several ring chunks, physical and
+    Fourier output, a last frame before the final step, and a schedule that records nothing)."""
+    import time
+
+    from torchfsm_amd.traj_recorder import AutoRecorder, CPURecorder, IntervalController
+    from torchfsm_amd.workloads import ns_initial_vorticity
+
+    op, mesh, y = _ns(dev)
+    w0 = ns_initial_vorticity(64, 3, seed=11).to(dev)
+    frame_bytes = 3 * 64 * 48 * 8  # B * spectrum (pitch 48) * complex64
+    for ring_frames, ctrl, fourier in ((2, IntervalController(3), False), (5, IntervalController(2, 1), True),
+                                       (1000, IntervalController(1), False)):
+        op._stream_ring_bytes = ring_frames * frame_bytes
+        crec = CPURecorder(ctrl)
+        ct = op.integrate(w0, mesh=mesh, dt=1e-3, step=20, trajectory_recorder=crec, return_in_fourier=fourier)
+        arec = AutoRecorder(ctrl)
+        at = op.integrate(w0, mesh=mesh, dt=1e-3, step=20, trajectory_recorder=arec, return_in_fourier=fourier)
+        print(f"ring {ring_frames} frames, fourier={fourier}: T={ct.shape[1]}, pinned={ct.is_pinned()}")
+        assert ct.device.type == "cpu" and ct.is_pinned() and torch.equal(ct, at.cpu())
+    op._stream_ring_bytes = 2 * frame_bytes
+    none = op.integrate(w0, mesh=mesh, dt=1e-3, step=5, trajectory_recorder=CPURecorder(lambda s: False))
+    assert none is None
+    # overlap: 400 steps of 256^2 x 16 recorded every step (1.7 GB of frames) with the default ring
+    from torchfsm_amd import field, pde
+    import torchfsm_amd as tfa
+
+    mesh2 = tfa.MeshGrid([(0, 2 * math.pi, 256)] * 2, device=dev)
+    op2 = pde.NavierStokesVorticity(1000, field.kolm_force(mesh2.bc_mesh_grid()[1]))
+    w2 = ns_initial_vorticity(256, 16, seed=12).to(dev)
+    op2.integrate(w2, mesh=mesh2, dt=1e-3, step=2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op2.integrate(w2, mesh=mesh2, dt=1e-3, step=400)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tr = op2.integrate(w2, mesh=mesh2, dt=1e-3, step=400, trajectory_recorder=CPURecorder())
+    t_stream = time.perf_counter() - t0
+    # the same frames kept on the device, transformed, then copied to pinned host memory in one go
+    t0 = time.perf_counter()
+    arec = AutoRecorder()
+    op2.integrate(w2, mesh=mesh2, dt=1e-3, step=400, trajectory_recorder=arec)
+    dev_traj = arec.trajectory
+    host = torch.empty(dev_traj.shape, dtype=dev_traj.dtype, pin_memory=True)
+    host.copy_(dev_traj)
+    t_serial = time.perf_counter() - t0
+    print(f"400 steps 256^2 B=16: {t_plain * 1e3:.1f} ms without frames; {tr.numel() * 4 / 1e9:.2f} GB of frames to "
+          f"host: {t_stream * 1e3:.1f} ms streamed during stepping, {t_serial * 1e3:.1f} ms recorded then copied")
+    assert tr.shape == (16, 401, 1, 256, 256) and torch.equal(tr, host)

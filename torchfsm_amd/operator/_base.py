@@ -19,7 +19,7 @@ from tqdm.auto import tqdm
 from ..engine import SpectralPlan, require_gpu_tensor
 from ..integrator import ETDRKIntegrator, RKIntegrator, SETDRKIntegrator
 from ..mesh import FourierMesh, MeshGrid
-from ..traj_recorder import AutoRecorder, _TrajRecorder
+from ..traj_recorder import AutoRecorder, CPURecorder, _TrajRecorder
 from ..utils import default
 
 
@@ -361,6 +361,9 @@ class OperatorLike(_MutableMixIn):
             uh = self._to_half(plan, u_0, u_0_fft)
             if trajectory_recorder is None:
                 self._run_steps(plan, uh, step, progressive)
+            elif isinstance(trajectory_recorder, CPURecorder):
+                sched = [bool(trajectory_recorder.control_func(i)) for i in range(step + 1)]
+                self._integrate_streamed(plan, uh, step, sched, trajectory_recorder, return_in_fourier)
             elif isinstance(trajectory_recorder, AutoRecorder):
                 sched = [bool(trajectory_recorder.control_func(i)) for i in range(step + 1)]
                 slots, t = [], 0
@@ -397,6 +400,70 @@ class OperatorLike(_MutableMixIn):
         if return_in_fourier:
             return plan.half_to_full(uh)
         return plan.inverse(uh)
+
+    # device bytes of one half of the frame ring used when frames stream to host memory
+    _stream_ring_bytes = 256 << 20
+
+    def _integrate_streamed(self, plan, uh, step, sched, recorder, return_in_fourier):
+        """CPURecorder (traj_recorder.py:127-148) with the frames streamed to pinned host memory while
+        the stepping continues: the steps are cut into chunks of F recorded frames; each chunk records
+        into one half of a device ring, is transformed on the GPU (inverse, or the full spectrum with
+        return_in_fourier) and copied to host on a second stream, overlapping the next chunk's steps.
+        The ring half is reused only after its copy has finished (event wait on the compute stream)."""
+        frame_steps = [i for i, s in enumerate(sched) if s]
+        T = len(frame_steps)
+        if T == 0:
+            self._run_steps(plan, uh, step, False)
+            recorder._take_host_frames(None, return_in_fourier)
+            return
+        B, dev = plan.batch, plan.device
+        half_shape = tuple(plan.spec_shape[1:])
+        fe = 1
+        for d in half_shape:
+            fe *= int(d)
+        F = max(1, min(T, self._stream_ring_bytes // max(1, B * fe * 8)))
+        ring = [torch.empty(B * F * fe, dtype=torch.complex64, device=dev) for _ in range(2)]
+        # every chunk's record schedule, uploaded once: chunk k covers frames [t0, t1), steps [s0, s_end]
+        chunks, slots = [], []
+        s0, t0 = 0, 0
+        while t0 < T:
+            t1 = min(T, t0 + F)
+            s_end = frame_steps[t1 - 1]
+            sl = [-1] * (s_end - s0 + 1)
+            for t in range(t0, t1):
+                sl[frame_steps[t] - s0] = t - t0
+            chunks.append((s0, s_end, t0, t1, len(slots)))
+            slots += sl
+            s0, t0 = s_end, t1
+        rec_all = torch.tensor(slots, dtype=torch.int32).to(dev)
+        compute = torch.cuda.current_stream(dev)
+        copy_stream = torch.cuda.Stream(device=dev)
+        out_shape = (plan.channels,) + tuple(plan.n)
+        host = torch.empty((B, T) + out_shape, dtype=torch.complex64 if return_in_fourier else torch.float32,
+                           pin_memory=True)
+        busy = [None, None]
+        for k, (s0, s_end, t0, t1, off) in enumerate(chunks):
+            h, nf = k % 2, t1 - t0
+            if busy[h] is not None:
+                compute.wait_event(busy[h][0])
+            frames = ring[h][: B * nf * fe].view((B, nf) + half_shape)
+            plan.step(uh, s_end - s0, rec_all[off:off + s_end - s0 + 1], frames)
+            flat = frames.reshape((B * nf,) + half_shape)
+            out = plan.half_to_full(flat) if return_in_fourier else plan.inverse(flat)
+            out = out.view((B, nf) + tuple(out.shape[1:]))
+            ready = torch.cuda.Event()
+            ready.record(compute)
+            with torch.cuda.stream(copy_stream):
+                copy_stream.wait_event(ready)
+                for b in range(B):  # host[b, t0:t1] is contiguous
+                    host[b, t0:t1].copy_(out[b], non_blocking=True)
+                done = torch.cuda.Event()
+                done.record(copy_stream)
+            busy[h] = (done, out)  # keeps `out` alive until its copy has finished
+        if step > frame_steps[-1]:
+            self._run_steps(plan, uh, step - frame_steps[-1], False)
+        copy_stream.synchronize()
+        recorder._take_host_frames(host, return_in_fourier)
 
     def _integrate_rk(self, u_0, u_0_fft, dt, step, progressive, trajectory_recorder, return_in_fourier):
         """Explicit Runge-Kutta stepping (_rk.py:43-58 with the operator L u + N(u) of

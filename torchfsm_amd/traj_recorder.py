@@ -96,8 +96,27 @@ class AutoRecorder(_TrajRecorder):
 
 
 class CPURecorder(AutoRecorder):
-    """Keeps frames in host memory (traj_recorder.py:127-148): the engine's device frames are moved
-    to pinned host memory; ``trajectory`` returns a CPU tensor (transformed on the GPU in chunks)."""
+    """Keeps frames in host memory (traj_recorder.py:127-148).  With the engine the frames are
+    transformed on the GPU and streamed to pinned host memory while the stepping continues
+    (OperatorLike._integrate_streamed); ``trajectory`` returns that CPU tensor."""
+
+    def __init__(self, control_func: Optional[Callable[[int], bool]] = None, include_initial_state: bool = True):
+        super().__init__(control_func, include_initial_state)
+        self._host = None  # (frames (B, T, ...) on the host, already in the requested space)
+
+    def _take_host_frames(self, frames: Optional[torch.Tensor], in_fourier: bool):
+        if self._trajectory or self._engine_frames is not None:
+            raise RuntimeError("The recorder already holds frames from another integration.")
+        self._host = (frames, in_fourier)
+
+    @property
+    def trajectory(self):
+        if self._host is not None:
+            frames, in_fourier = self._host
+            if frames is None or in_fourier == self.return_in_fourier:
+                return frames
+            raise RuntimeError("return_in_fourier differs from the integration that recorded the frames")
+        return super().trajectory
 
     def _take_engine_frames(self, plan, frames: torch.Tensor):
         host = torch.empty(frames.shape, dtype=frames.dtype, pin_memory=True)
