@@ -239,6 +239,8 @@ def serving_leg(dev, n_requests, rate, seed=0, plain_sample=12):
     from torchfsm_amd.mesh import MeshGrid
 
     sample = reqs[:plain_sample]
+    if not sample:
+        return out
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     wsum = 0
@@ -320,6 +322,8 @@ def main():
     ap.add_argument("--streams", type=int, default=1, help="1, or 2 to overlap alternate sub-batches")
     ap.add_argument("--serve-requests", type=int, default=240, help="requests of the serving leg (0 = skip it)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate of the serving leg (1/s)")
+    ap.add_argument("--serve-light-rate", type=float, default=25.0,
+                    help="arrival rate of a second, lightly loaded serving run (1/s; 0 = skip it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run 3 plain 256 MiB device copies (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -468,6 +472,14 @@ def main():
         log(f"[bench] serving: {sv['requests']} requests in {sv['makespan_s']:.2f} s, {sv['dof_steps_per_s']:.3e} "
             f"DOF*steps/s, p50 {sv['p50_latency_ms']:.1f} ms, p99 {sv['p99_latency_ms']:.1f} ms; "
             f"{sv['throughput_vs_plain']:.2f}x plain per-request solves")
+        if args.serve_light_rate > 0:
+            n_light = max(1, args.serve_requests // 3)
+            log(f"[bench] serving (light load): {n_light} requests at {args.serve_light_rate}/s")
+            lt = serving_leg(dev, n_light, args.serve_light_rate, seed=1, plain_sample=0)
+            lt.pop("plain_per_request", None)
+            lt.pop("throughput_vs_plain", None)
+            result["serving_light"] = lt
+            log(f"[bench] serving (light load): p50 {lt['p50_latency_ms']:.1f} ms, p99 {lt['p99_latency_ms']:.1f} ms")
     print(json.dumps(result), flush=True)
 
 
