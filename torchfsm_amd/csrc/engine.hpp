@@ -247,6 +247,12 @@ __device__ __forceinline__ float2 stage_combine_c(int stage, float2 nval, float2
 //   ETDRK1/SETDRK1: [E, c1, -, - | -]          ETDRK2/SETDRK2: [E, c1, c2, - | -]
 //   SETDRK3:        [E2, c1, E, c2 | E, c3, c4, c5]
 //   SETDRK4:        [E2, c1, -, - | E, c4, c5, c6]
+// Within a table the halves are interleaved in blocks of 16 modes: [block][half][16] float4, so a
+// column tile's 16 consecutive modes of one stage read 256 contiguous bytes (tables hold a multiple
+// of 16 modes: the pitch is).
+__host__ __device__ __forceinline__ size_t cpk_index(size_t mode, int half) {
+  return 2 * mode - (mode & 15) + ((size_t)half << 4);
+}
 __host__ __device__ inline int stage_coef_half(int stage) {
   return (stage == ST_R3_E2 || stage == ST_R4_E3) ? 1 : 0;
 }

@@ -322,8 +322,8 @@ __global__ void k_pack_coefs(CoefArgs a) {
         break;
       default: break;
     }
-    a.cpk[e * 2] = h0;
-    a.cpk[e * 2 + 1] = h1;
+    a.cpk[cpk_index(e, 0)] = h0;
+    a.cpk[cpk_index(e, 1)] = h1;
   }
 }
 

@@ -84,6 +84,8 @@ __device__ __forceinline__ void col_nsp_combine(const Col2Args& a, int stage, fl
   }
   const float4* cpk = a.cpk + (size_t)b * a.cpk_bstride;
   const int chalf = stage_coef_half(stage);
+  // cpk_index(ci, chalf) = 2 ci + cko: every mode of this lane has ci & 15 == ky & 15
+  const int cko = (chalf << 4) - (ky & 15);
   const bool rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
   const int tgt = stage_target(stage);
 #pragma unroll
@@ -98,7 +100,7 @@ __device__ __forceinline__ void col_nsp_combine(const Col2Args& a, int stage, fl
       const float2 uu = colv ? a.U[base + ci] : z;
       const float2 q1 = (colv && rd1) ? a.A1[base + ci] : z;
       const float2 q2 = (colv && rd2) ? a.A2[base + ci] : z;
-      const float4 cc = colv ? cpk[ci * 2 + chalf] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 cc = colv ? cpk[ci * 2 + cko] : make_float4(0.f, 0.f, 0.f, 0.f);
       float2 nv = rscale(a.coef_t, v[c][m]);
       if (src && colv) nv = radd(nv, src[ci]);
       s[c][m] = colv ? stage_combine_v(stage, nv, uu, q1, q2, cc, &wv[m]) : z;
@@ -140,6 +142,8 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
   const size_t plane = (size_t)N0 * a.P1;
   const float4* cpk = a.cpk + (size_t)b * a.cpk_bstride;
   const int chalf = stage_coef_half(stage);
+  // cpk_index(ci, chalf) = 2 ci + cko: every mode of this lane has ci & 15 == ky & 15
+  const int cko = (chalf << 4) - (ky & 15);
   const bool rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
   const int tgt = stage_target(stage);
   float2 s[NC][RMAX];
@@ -175,7 +179,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
         const float2 uu = colv ? a.U[base + ci] : z;
         const float2 q1 = (colv && rd1) ? a.A1[base + ci] : z;
         const float2 q2 = (colv && rd2) ? a.A2[base + ci] : z;
-        const float4 cc = colv ? cpk[ci * 2 + chalf] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 cc = colv ? cpk[ci * 2 + cko] : make_float4(0.f, 0.f, 0.f, 0.f);
         float2 nv = rscale(a.coef_t, v[m]);
         if (src && colv) nv = radd(nv, src[ci]);
         s[c][m] = colv ? stage_combine_v(stage, nv, uu, q1, q2, cc, &wv[m]) : z;

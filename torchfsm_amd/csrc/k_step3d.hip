@@ -62,6 +62,8 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
   const size_t coff = (size_t)ky * a.P2 + kz;
   const float4* cpk = a.cpk + (size_t)b * a.cpk_bstride;
   const int chalf = stage_coef_half(stage);
+  // cpk_index(coff + xo, chalf) = 2 (coff + xo) + cko: (coff + xo) & 15 == kz & 15
+  const int cko = (chalf << 4) - (kz & 15);
   const bool rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
   const int tgt = stage_target(stage);
   constexpr int NC3 = TERM == TERM_KS ? 1 : 3;  // channels
@@ -105,7 +107,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
         const float2 uu = colv ? a.U[base + xo] : z;
         const float2 q1 = (colv && rd1) ? a.A1[base + xo] : z;
         const float2 q2 = (colv && rd2) ? a.A2[base + xo] : z;
-        const float4 cc = colv ? cpk[(coff + xo) * 2 + chalf] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 cc = colv ? cpk[(coff + xo) * 2 + cko] : make_float4(0.f, 0.f, 0.f, 0.f);
         float2 nv = rscale(a.coef_t, v3[c][m]);
         if (src && colv) nv = radd(nv, src[xo]);
         s[c][m] = colv ? stage_combine_v(stage, nv, uu, q1, q2, cc, &wv[m]) : z;
@@ -137,7 +139,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
         const float2 uu = colv ? a.U[base + xo] : z;
         const float2 q1 = (colv && rd1) ? a.A1[base + xo] : z;
         const float2 q2 = (colv && rd2) ? a.A2[base + xo] : z;
-        const float4 cc = colv ? cpk[(coff + xo) * 2 + chalf] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 cc = colv ? cpk[(coff + xo) * 2 + cko] : make_float4(0.f, 0.f, 0.f, 0.f);
         float2 nv = rscale(a.coef_t, v[m]);
         if (src && colv) nv = radd(nv, src[xo]);
         s[c][m] = colv ? stage_combine_v(stage, nv, uu, q1, q2, cc, &wv[m]) : z;
