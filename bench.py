@@ -328,8 +328,8 @@ def main():
                     help="first run 3 plain 256 MiB device copies (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: every box here has one MI355X and this build has no multi-GPU path "
-              f"(--gpus {args.gpus} requested). The batch shards across GPUs with no communication.",
+        print(f"bench.py measures one GPU: the engine has no multi-GPU path (--gpus {args.gpus} requested). "
+              f"The batch would shard across GPUs with no communication.",
               file=sys.stderr)
         sys.exit(2)
 

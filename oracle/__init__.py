@@ -7,7 +7,7 @@ package, and only as the checker (or as the timed reference baseline).  The prod
 Modules:
   refpath  PyTorch restatement of the reference's schedule (full complex spectrum, the reference's
            op order and dtypes).  Pinned bit-for-bit on CPU against the golden fixtures written
-           from the reference (tests/golden), and run on the GPU box as the same-inputs oracle.
+           from the reference (tests/golden), and run on the GPU as the same-inputs oracle.
   numref   Independent NumPy float64 restatement (half spectrum, integer wavenumbers, phi
            functions from closed forms and Taylor series rather than contour means).  Pins
            refpath in float64 and is itself pinned by closed forms, brute-force DFTs, invariants

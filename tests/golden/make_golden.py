@@ -1,7 +1,7 @@
 """Write golden input/output vectors from the reference torchfsm (run in the build container only).
 
-The reference is pure Python on PyTorch; it is imported here (never shipped, never run on the
-GPU box) to pin the one thing no closed form, invariant or independent FFT can pin: the
+The reference is pure Python on PyTorch; it is imported here (never shipped, never run on a
+GPU) to pin the one thing no closed form, invariant or independent FFT can pin: the
 reference's own float32 results -- its op order, its complex64 ETDRK closed-form coefficients
 (with their float32 cancellation), its float32 contour means and the accumulated rounding of the
 integrate loop.  Every fixture is small (< 1 MB) and stores its inputs next to its outputs.
