@@ -37,6 +37,8 @@ extern "C" {
 #define TFSM_TERM_KS 4        /* KSConvection(remove_mean=True): 1/2 |grad phi|^2 minus its mean over
                                  batch and space, C == 1, D in {1, 2, 3} (_ks_convection.py:8-38) */
 #define TFSM_TERM_KS_KEEP_MEAN 5  /* KSConvection(remove_mean=False)                             */
+#define TFSM_TERM_CONS 6      /* ConservativeConvection: sum_i d_i (u_i u_c), C == D in {1, 2}
+                                 (_conservative_convection.py:17-27)                          */
 
 /* linear symbols (LinearCoef cores) */
 #define TFSM_LIN_LAP 1        /* Laplacian       (_laplacian.py:12-15)  */

@@ -14,6 +14,7 @@ An operator is described by a list of terms in generator order, each ``(coef, ki
   kind "vc"      VorticityConvection, dealiased input           (_navier_stokes.py:27-46)
   kind "nsp"     NSPressureConvection, dealiased input          (_navier_stokes.py:221-262)
   kind "ks"      KSConvection (remove_mean), dealiased input    (_ks_convection.py:18-38)
+  kind "cons"    ConservativeConvection, dealiased input        (_conservative_convection.py:17-27)
   kind "src"     ExplicitSource; payload = spatial source field (_base.py:994-1015)
 """
 
@@ -24,7 +25,7 @@ import torch
 
 Term = Tuple[object, str, Optional[torch.Tensor]]
 LINEAR_KINDS = ("lap", "biharm", "unit", "deriv")
-NONLINEAR_KINDS = ("conv", "vc", "nsp", "ks", "ks_keep_mean", "src")
+NONLINEAR_KINDS = ("conv", "vc", "nsp", "ks", "ks_keep_mean", "cons", "src")
 
 
 class Mesh:
@@ -186,7 +187,7 @@ def build_closure(mesh: Mesh, terms: List[Term], dealias_rate=2 / 3) -> Optional
     for c, k, p in terms:
         if k == "src":
             nl.append((c, k, source_spectrum(p).to(mesh.device)))
-        elif k in ("conv", "vc", "nsp", "ks", "ks_keep_mean"):
+        elif k in ("conv", "vc", "nsp", "ks", "ks_keep_mean", "cons"):
             nl.append((c, k, None))
     if not nl:
         return None
@@ -197,11 +198,14 @@ def build_closure(mesh: Mesh, terms: List[Term], dealias_rate=2 / 3) -> Optional
         d_u = None
         u = None
         for c, k, p in nl:
-            if k in ("conv", "vc", "nsp", "ks", "ks_keep_mean"):
+            if k in ("conv", "vc", "nsp", "ks", "ks_keep_mean", "cons"):
                 if d_fft is None:
                     d_fft = u_fft * mesh.mask(dealias_rate)
                     d_u = mesh.ifft(d_fft).real
-                if k in ("ks", "ks_keep_mean"):
+                if k == "cons":
+                    uu_fft = mesh.fft(d_u.unsqueeze(2) * d_u.unsqueeze(1))
+                    result += c * (mesh.nabla_vector(1).unsqueeze(2) * uu_fft).sum(1)
+                elif k in ("ks", "ks_keep_mean"):
                     result += c * ks_convection(d_fft, mesh, d_u, remove_mean=(k == "ks"))
                 else:
                     f = {"conv": convection, "vc": vorticity_convection, "nsp": ns_pressure_convection}[k]

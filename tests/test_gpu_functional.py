@@ -137,3 +137,51 @@ def test_fourier_io_coefficients_and_errors(dev):
         Grad().integrate(torch.zeros(1, 1, n, n, device=dev), mesh=mesh, dt=0.1, step=1)
     with pytest.raises(ValueError, match="dimension"):
         Div()(torch.zeros(1, 1, n, n, device=dev), mesh=mesh)
+
+
+@pytest.mark.parametrize("dims,n", [(1, 256), (1, 100), (2, 64), (2, 100), (2, 512)])
+def test_conservative_burgers_integrates(dev, dims, n):
+    """nu Laplacian - ConservativeConvection stepped by the engine (SETDRK4 under every launch shape
+    the autotuner tries, and ETDRK2) against the oracle, and its Operator.__call__."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import autotune
+    from torchfsm_amd.integrator import ETDRKIntegrator
+    from torchfsm_amd.operator import ConservativeConvection, Laplacian
+
+    u0 = 0.5 * _field(n, dims, dims, 2, seed=60 + n)
+    mi = [(0, 1, n)] * dims
+    mesh = tfa.MeshGrid(mi, device=dev)
+    terms = [(0.01, "lap", None), (-1, "cons", None)]
+    dt, steps = (2e-4, 20) if dims == 1 else (5e-4, 10)
+    ref = refpath.integrate(terms, mi, u0, dt, steps, scheme="SETDRK4")
+    op = 0.01 * Laplacian() - ConservativeConvection()
+    e = _rel(op.integrate(u0.to(dev), mesh=mesh, dt=dt, step=steps), ref)
+    print(f"conservative Burgers {dims}D {n} SETDRK4: rel-L2 {e:.2e}")
+    assert e <= 1e-5
+    if dims == 2:
+        plan = op._prepare(2, dev, dt)
+        for cand in autotune.candidates(plan):
+            plan.set_tuning(*cand)
+            uh = plan.forward(u0.to(dev))
+            plan.step(uh, steps)
+            e1 = _rel(plan.inverse(uh), ref)
+            print(f"  launch shape {cand}: rel-L2 {e1:.2e}")
+            assert e1 <= 1e-5
+    op.set_integrator(ETDRKIntegrator.ETDRK2)
+    e2 = _rel(op.integrate(u0.to(dev), mesh=mesh, dt=dt, step=steps),
+              refpath.integrate(terms, mi, u0, dt, steps, scheme="ETDRK2"))
+    print(f"conservative Burgers {dims}D {n} ETDRK2: rel-L2 {e2:.2e}")
+    assert e2 <= 1e-5
+    _check(f"conservative Burgers call {dims}D {n}", op(u0.to(dev), mesh=mesh), refpath.apply(terms, mi, u0),
+           refpath.apply(terms, mi, u0.double()))
+
+
+def test_conservative_convection_3d_stepping_is_rejected(dev):
+    import torchfsm_amd as tfa
+    from torchfsm_amd.operator import ConservativeConvection, Laplacian
+
+    mesh = tfa.MeshGrid([(0, 1, 16)] * 3, device=dev)
+    with pytest.raises(ValueError, match="3D"):
+        (0.01 * Laplacian() - ConservativeConvection()).integrate(torch.zeros(1, 3, 16, 16, 16, device=dev), mesh=mesh,
+                                                                   dt=1e-3, step=1)

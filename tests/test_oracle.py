@@ -445,3 +445,20 @@ def test_diagnostic_operators_closed_forms():
     v3 = torch.stack([torch.sin(Z3), torch.sin(X3), torch.sin(Y3)])[None]
     c3 = torch.stack([torch.cos(Y3), torch.cos(Z3), torch.cos(X3)])[None]
     assert (m3.ifft(R.curl(m3.fft(v3), m3)).real - c3).abs().max() < 1e-12
+
+
+def test_conservative_convection_closure_closed_forms():
+    """The closure's ConservativeConvection term (dealiased input, no aliasing for these modes):
+    1D u = sin x + cos(2x)/2 gives d_x(u^2) = 2 u u_x; 2D u = (sin x, sin y) gives
+    (sin 2x + sin x cos y, cos x sin y + sin 2y)."""
+    n = 32
+    x = torch.arange(n, dtype=torch.float64) * 2 * math.pi / n
+    u = (torch.sin(x) + 0.5 * torch.cos(2 * x)).reshape(1, 1, n)
+    ux = (torch.cos(x) - torch.sin(2 * x)).reshape(1, 1, n)
+    o = R.apply([(1, "cons", None)], [(0, 2 * math.pi, n)], u)
+    assert (o - 2 * u * ux).abs().max() < 1e-12
+    X, Y = torch.meshgrid(x, x, indexing="ij")
+    v = torch.stack([torch.sin(X), torch.sin(Y)])[None]
+    ex = torch.stack([torch.sin(2 * X) + torch.sin(X) * torch.cos(Y), torch.cos(X) * torch.sin(Y) + torch.sin(2 * Y)])[None]
+    o2 = R.apply([(1, "cons", None)], [(0, 2 * math.pi, n)] * 2, v)
+    assert (o2 - ex).abs().max() < 1e-12

@@ -15,7 +15,8 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 OBJDIR = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libtfsm.so")
 SOURCES = ["api.hip", "k_coefs.hip", "k_transform.hip", "k_step2d.hip", "k_step2d_vc.hip", "k_step2d_conv.hip",
-           "k_step2d_nsp.hip", "k_step2d_ks.hip", "k_step2d_mixed.hip", "k_step3d.hip", "k_adjoint.hip",
+           "k_step2d_nsp.hip", "k_step2d_ks.hip", "k_step2d_cons.hip",
+           "k_step2d_mixed.hip", "k_step3d.hip", "k_adjoint.hip",
            "k_maps.hip"]
 ARCH = "gfx950"
 # packed FP32 instructions are disabled for all device code

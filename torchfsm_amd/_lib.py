@@ -16,7 +16,7 @@ TFSM_E_STATE = -2
 TFSM_E_DEVICE = -3
 TFSM_E_OOM = -4
 
-TERM_NONE, TERM_CONV, TERM_VC, TERM_NSP, TERM_KS, TERM_KS_KEEP_MEAN = 0, 1, 2, 3, 4, 5
+TERM_NONE, TERM_CONV, TERM_VC, TERM_NSP, TERM_KS, TERM_KS_KEEP_MEAN, TERM_CONS = 0, 1, 2, 3, 4, 5, 6
 LIN_LAP, LIN_BIHARM, LIN_UNIT, LIN_DERIV = 1, 2, 3, 4
 MAP_DEALIAS, MAP_DIV, MAP_GRAD, MAP_CURL, MAP_V2V, MAP_PRESSURE, MAP_CONSDIV = 0, 1, 2, 3, 4, 5, 6
 ETDRK0, ETDRK1, ETDRK2, SETDRK1, SETDRK2, SETDRK3, SETDRK4 = range(7)

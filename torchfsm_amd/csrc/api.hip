@@ -430,6 +430,7 @@ Run1Args run1_args(tfsm_plan* p, float2* U, char* ws) {
   a.coef = p->coef;
   a.coef_bstride = p->coef_bstride();
   a.coefc = p->cplx ? p->coefc : nullptr;
+  a.cons = p->term == TERM_CONS ? 1 : 0;
   a.src = p->src;
   a.src_bstride = p->src_bstride();
   a.nsteps = 0;
@@ -664,8 +665,13 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   }
   const bool ks_keep = terms->nonlinear == TFSM_TERM_KS_KEEP_MEAN;
   const int term = ks_keep ? TERM_KS : terms->nonlinear;
-  if (term != TERM_NONE && term != TERM_CONV && term != TERM_VC && term != TERM_NSP && term != TERM_KS)
+  if (term != TERM_NONE && term != TERM_CONV && term != TERM_VC && term != TERM_NSP && term != TERM_KS &&
+      term != TERM_CONS)
     return fail(TFSM_E_INVALID, "unknown nonlinear term");
+  if (term == TERM_CONS && geom->channels != D)
+    return fail(TFSM_E_INVALID, "convection operator only works for vector field with the same dimension as mesh");
+  if (term == TERM_CONS && D == 3)
+    return fail(TFSM_E_INVALID, "term not supported by engine: stepping ConservativeConvection in 3D");
   if (term == TERM_KS && geom->channels != 1) return fail(TFSM_E_INVALID, "KSConvection only supports scalar field");
   if (term == TERM_KS && D == 3 && !step3d_supported(geom->n[0], geom->n[1], geom->n[2]))
     return fail(TFSM_E_INVALID, "3D KSConvection needs 16, 32, 64 or 128 points per axis, or a cubic 48, 96 or 100 mesh");
@@ -826,10 +832,11 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
       p->off_G2 = p->off_G + roundup((size_t)p->B * 6 * p->n[0] * p->K1 * p->PF * sizeof(float2), 256);
       p->ws_bytes = p->off_G2 + roundup((size_t)p->B * 9 * p->n[0] * p->n[1] * p->PF * sizeof(float2), 256);
     } else if (D == 2) {
-      p->NF = term == TERM_KS ? 2 : 4;
+      p->NF = (term == TERM_KS || term == TERM_CONS) ? 2 : 4;
       p->PF = (int)roundup((size_t)p->kmax[1] + 1, 16);
       p->off_P = p->off_A2 + roundup(sb, 256);
-      p->off_G = p->off_P + roundup(sb, 256);
+      // ConservativeConvection keeps two product planes per channel
+      p->off_G = p->off_P + roundup(term == TERM_CONS ? 2 * sb : sb, 256);
       p->ws_bytes = p->off_G + roundup((size_t)p->B * p->NF * p->n[0] * p->PF * sizeof(float2), 256);
     } else {
       p->ws_bytes = p->off_A2 + roundup(sb, 256);

@@ -15,7 +15,7 @@
 
 namespace tfsm {
 
-enum TermKind : int { TERM_NONE = 0, TERM_CONV = 1, TERM_VC = 2, TERM_NSP = 3, TERM_KS = 4 };
+enum TermKind : int { TERM_NONE = 0, TERM_CONV = 1, TERM_VC = 2, TERM_NSP = 3, TERM_KS = 4, TERM_CONS = 6 };
 
 enum SchemeId : int { ETDRK0 = 0, ETDRK1 = 1, ETDRK2 = 2, SETDRK1 = 3, SETDRK2 = 4, SETDRK3 = 5, SETDRK4 = 6 };
 
@@ -451,6 +451,7 @@ struct Run1Args {
   float2* traj;               // [B][T][P] recorded half spectra
   int T;
   int eval_only;              // 1: one nonlinear evaluation into A1, no stepping
+  int cons = 0;               // 1: ConservativeConvection (N = i a fft(u u)), else Convection
 };
 
 

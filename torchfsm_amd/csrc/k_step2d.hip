@@ -9,6 +9,7 @@ hipError_t launch_col2d(int term, const Col2Args& a, int nb, int stage, int v, i
   if (term == TERM_CONV) return launch_col2d_conv(a, nb, stage, v, var, st);
   if (term == TERM_NSP) return launch_col2d_nsp(a, nb, stage, v, var, st);
   if (term == TERM_KS) return launch_col2d_ks(a, nb, stage, v, var, st);
+  if (term == TERM_CONS) return launch_col2d_cons(a, nb, stage, v, var, st);
   return hipErrorInvalidValue;
 }
 
@@ -19,6 +20,7 @@ hipError_t launch_row2d(int term, const Row2Args& a, int var, int wgt, hipStream
   if (term == TERM_VC) return launch_row2d_vc(a, var, wgt, st);
   if (term == TERM_CONV || term == TERM_NSP) return launch_row2d_conv(a, var, wgt, st);
   if (term == TERM_KS) return launch_row2d_ks(a, var, wgt, st);
+  if (term == TERM_CONS) return launch_row2d_cons(a, var, wgt, st);
   return hipErrorInvalidValue;
 }
 

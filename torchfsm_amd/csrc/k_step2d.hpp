@@ -48,6 +48,10 @@ template <>
 struct Term2<TERM_KS> {  // KSConvection: fields phi^, a_x phi^; one product 1/2 |grad phi|^2
   static constexpr int NC = 1, NF = 2;
 };
+template <>
+struct Term2<TERM_CONS> {  // ConservativeConvection, C = 2: fields u0, u1; product planes per
+  static constexpr int NC = 2, NF = 2;  // channel c: F_y(u0 u_c) and i a_y F_y(u1 u_c)
+};
 
 // Velocity-form Navier-Stokes (NSPressureConvection): both channels' convection spectra are
 // transformed first, projected mode by mode (nsp_project), then combined channel by channel.
@@ -159,10 +163,30 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
         s[c][m] = colv ? a.U[base + (size_t)(j + m * NT) * a.P1 + ky] : make_float2(0.f, 0.f);
     } else {
       float2 v[RMAX];
+      if constexpr (TERM == TERM_CONS) {
+        // N_c = (i a_x) F_x(F_y(u0 u_c)) + F_x(i a_y F_y(u1 u_c))   (_conservative_convection.py:23-27)
+        const size_t pb = ((size_t)b * 4 + 2 * c) * plane;
 #pragma unroll
-      for (int m = 0; m < RMAX; ++m)
-        v[m] = colv ? a.P[base + (size_t)(j + m * NT) * a.P1 + ky] : make_float2(0.f, 0.f);
-      fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
+        for (int m = 0; m < RMAX; ++m) v[m] = colv ? a.P[pb + (size_t)(j + m * NT) * a.P1 + ky] : make_float2(0.f, 0.f);
+        fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
+#pragma unroll
+        for (int m = 0; m < RMAX; ++m) {
+          const float axm = a.ax[j + m * NT];
+          v[m] = make_float2(-mulr(axm, v[m].y), mulr(axm, v[m].x));
+        }
+        float2 w[RMAX];
+#pragma unroll
+        for (int m = 0; m < RMAX; ++m)
+          w[m] = colv ? a.P[pb + plane + (size_t)(j + m * NT) * a.P1 + ky] : make_float2(0.f, 0.f);
+        fft_wg<N0, -1, V, VAR>(w, s_buf, s_tw, j, vi);
+#pragma unroll
+        for (int m = 0; m < RMAX; ++m) v[m] = radd(v[m], w[m]);
+      } else {
+#pragma unroll
+        for (int m = 0; m < RMAX; ++m)
+          v[m] = colv ? a.P[base + (size_t)(j + m * NT) * a.P1 + ky] : make_float2(0.f, 0.f);
+        fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
+      }
       if constexpr (TERM == TERM_KS) {
         // remove_mean: fft(p - mean(p)) differs from fft(p) only at k = 0, by the batch mean of the DC
         if (a.dc && ky == 0 && j == 0) v[0] = rsub(v[0], a.dc[0]);
@@ -216,6 +240,8 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
             float2 psi = make_float2(mulr(-w.x, il), mulr(-w.y, il));
             val = (f == 0) ? psi : make_float2(mulr(ax, psi.x), mulr(ax, psi.y));
           }
+        } else if constexpr (TERM == TERM_CONS) {
+          val = s[f][m];
         } else {
           const float2 u = s[f >> 1][m];
           val = (f & 1) ? make_float2(mulr(ax, u.x), mulr(ax, u.y)) : u;
@@ -289,6 +315,62 @@ __global__ void __launch_bounds__((Fft<N1, VAR>::NT * ROWS)) k2_row(Row2Args a) 
       for (int m = 0; m < RMAX; ++m) {
         const int pos = j + m * NT;
         if (pos <= N1 / 2) p[pos] = v[m];
+      }
+    }
+  } else if constexpr (TERM == TERM_CONS) {
+    // ConservativeConvection, C = D = 2: fields F0 = u0, F1 = u1 (x-inverse done); products
+    // u0 u0, u0 u1, u1 u1; planes [F(u0 u0), i a_y F(u1 u0), F(u0 u1), i a_y F(u1 u1)]
+    float2 z1[RMAX];
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) {
+      const int pos = j + m * NT;
+      const bool cj = pos > N1 / 2;
+      const int k = cj ? N1 - pos : pos;
+      const bool edge = (k == 0) || (k == N1 / 2);
+      float2 f0 = make_float2(0.f, 0.f), f1 = f0;
+      if (rv && k <= a.kmax1) {
+        f0 = gb[k];
+        f1 = gb[fs + k];
+      }
+      z1[m] = pack_pair(f0, f1, cj, edge);  // (u0, u1)
+    }
+    fft_wg<N1, +1, 1, VAR, RowSync<N1, VAR>::WS>(z1, lds, s_tw, j, 0);
+    float2 v[RMAX], w[RMAX];
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) {
+      v[m] = make_float2(z1[m].x * z1[m].x, z1[m].x * z1[m].y);  // u0 u0 + i u0 u1
+      w[m] = make_float2(z1[m].y * z1[m].y, 0.f);                // u1 u1
+    }
+    fft_wg<N1, -1, 1, VAR, RowSync<N1, VAR>::WS>(v, lds, s_tw, j, 0);
+    // unpack two real spectra: F00 = (Z[k] + conj Z[N-k]) / 2, F01 = (Z[k] - conj Z[N-k]) / (2i)
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) lds[lds_slot<1>(j + m * NT, 0)] = v[m];
+    lds_sync<RowSync<N1, VAR>::WS>();
+    float2 f00[RMAX], f01[RMAX];
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) {
+      const int pos = j + m * NT;
+      const float2 zk = v[m];
+      const float2 zm = lds[lds_slot<1>(pos == 0 ? 0 : N1 - pos, 0)];
+      f00[m] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+      const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);
+      f01[m] = make_float2(0.5f * d.y, -0.5f * d.x);
+    }
+    lds_sync<RowSync<N1, VAR>::WS>();
+    fft_wg<N1, -1, 1, VAR, RowSync<N1, VAR>::WS>(w, lds, s_tw, j, 0);
+    if (rv) {
+      const size_t pl = (size_t)a.N0 * a.P1;  // one product plane of one sample
+      float2* p = a.P + (size_t)b * 4 * pl + (size_t)x * a.P1;
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) {
+        const int pos = j + m * NT;
+        if (pos <= N1 / 2) {
+          const float ay = a.ay[pos];
+          p[pos] = f00[m];
+          p[pl + pos] = make_float2(-mulr(ay, f01[m].y), mulr(ay, f01[m].x));
+          p[2 * pl + pos] = f01[m];
+          p[3 * pl + pos] = make_float2(-mulr(ay, w[m].y), mulr(ay, w[m].x));
+        }
       }
     }
   } else if constexpr (TERM == TERM_KS) {
@@ -431,6 +513,8 @@ hipError_t launch_row2d_vc(const Row2Args& a, int var, int wgt, hipStream_t st);
 hipError_t launch_row2d_conv(const Row2Args& a, int var, int wgt, hipStream_t st);
 hipError_t launch_row2d_mixed(int term, const Row2Args& a, int var, int wgt, hipStream_t st);
 hipError_t launch_col2d_ks(const Col2Args& a, int nb, int stage, int v, int var, hipStream_t st);
+hipError_t launch_col2d_cons(const Col2Args& a, int nb, int stage, int v, int var, hipStream_t st);
+hipError_t launch_row2d_cons(const Row2Args& a, int var, int wgt, hipStream_t st);
 hipError_t launch_row2d_ks(const Row2Args& a, int var, int wgt, hipStream_t st);
 
 }  // namespace tfsm

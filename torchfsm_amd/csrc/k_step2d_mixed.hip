@@ -20,6 +20,7 @@ hipError_t launch_col2d_mixed(int term, const Col2Args& a, int nb, int stage, in
   if (term == TERM_CONV) return col_mixed<TERM_CONV>(a, nb, stage, v, var, st);
   if (term == TERM_NSP) return col_mixed<TERM_NSP>(a, nb, stage, v, var, st);
   if (term == TERM_KS) return col_mixed<TERM_KS>(a, nb, stage, v, var, st);
+  if (term == TERM_CONS) return col_mixed<TERM_CONS>(a, nb, stage, v, var, st);
   return hipErrorInvalidValue;
 }
 
@@ -27,6 +28,7 @@ hipError_t launch_row2d_mixed(int term, const Row2Args& a, int var, int wgt, hip
   if (term == TERM_VC) return row_mixed<TERM_VC>(a, var, wgt, st);
   if (term == TERM_CONV || term == TERM_NSP) return row_mixed<TERM_CONV>(a, var, wgt, st);
   if (term == TERM_KS) return row_mixed<TERM_KS>(a, var, wgt, st);
+  if (term == TERM_CONS) return row_mixed<TERM_CONS>(a, var, wgt, st);
   return hipErrorInvalidValue;
 }
 

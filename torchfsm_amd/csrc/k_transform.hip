@@ -255,7 +255,8 @@ __global__ void k_record(const float2* __restrict__ U, float2* __restrict__ traj
 // ---------------------------------------------------------------- 1D persistent stepper
 // V samples per workgroup (interleaved in LDS); every sample's state, stage spectrum and
 // auxiliary arrays stay owned by fixed threads for the whole run: thread j owns modes k = j + m*NT.
-// Convection (C = D = 1): N = coef_t * fft(u * u_x) [+ S], u from the dealiased stage spectrum.
+// Convection (C = D = 1): N = coef_t * fft(u * u_x) [+ S], u from the dealiased stage spectrum;
+// ConservativeConvection: N = coef_t * (i a) fft(u * u) [+ S].
 // CPLX: complex-L plan (complex coefficient tables, stage_combine_c).
 template <int N, int V, bool CPLX>
 __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
@@ -311,7 +312,7 @@ __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
         if (k <= a.kmax) {
           val = stg[k];
           val = make_float2(val.x * a.scale, val.y * a.scale);
-          axk = ax[k];
+          axk = a.cons ? 0.f : ax[k];  // u alone: u_x in the imaginary part would add its round-off to u
         }
         float2 A = val;
         float2 Bv = make_float2(-axk * val.y, axk * val.x);  // i a_k u^
@@ -322,9 +323,18 @@ __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
         v[m] = cj ? make_float2(A.x + Bv.y, Bv.x - A.y) : make_float2(A.x - Bv.y, A.y + Bv.x);
       }
       fft_wg<N, +1, V>(v, s_buf, s_tw, j, vi);
+      // Convection u u_x, or ConservativeConvection's u u (its derivative follows the transform)
 #pragma unroll
-      for (int m = 0; m < RMAX; ++m) v[m] = make_float2(v[m].x * v[m].y, 0.f);
+      for (int m = 0; m < RMAX; ++m) v[m] = make_float2(a.cons ? v[m].x * v[m].x : v[m].x * v[m].y, 0.f);
       fft_wg<N, -1, V>(v, s_buf, s_tw, j, vi);
+      if (a.cons) {
+#pragma unroll
+        for (int m = 0; m < RMAX; ++m) {
+          const int pos = j + m * NT;
+          const float ap = pos <= N / 2 ? ax[pos] : 0.f;
+          v[m] = make_float2(-mulr(ap, v[m].y), mulr(ap, v[m].x));  // (i a) fft(u u)
+        }
+      }
 #pragma unroll
       for (int m = 0; m < RMAX; ++m) {
         const int pos = j + m * NT;

@@ -59,7 +59,7 @@ class SpectralPlan:
 
     KIND = {"lap": _lib.LIN_LAP, "biharm": _lib.LIN_BIHARM, "unit": _lib.LIN_UNIT}
     TERM = {None: _lib.TERM_NONE, "conv": _lib.TERM_CONV, "vc": _lib.TERM_VC, "nsp": _lib.TERM_NSP,
-            "ks": _lib.TERM_KS, "ks_keep_mean": _lib.TERM_KS_KEEP_MEAN}
+            "ks": _lib.TERM_KS, "ks_keep_mean": _lib.TERM_KS_KEEP_MEAN, "cons": _lib.TERM_CONS}
 
     def __init__(self, mesh_info, batch, channels, nonlinear=None, nonlinear_coef=0.0, linear=(), device="cuda",
                  dealias_rate=2 / 3, lengths_per_sample: Optional[Sequence[float]] = None):

@@ -228,7 +228,7 @@ class OperatorLike(_MutableMixIn):
                 if isinstance(c, list):
                     raise ValueError("term not supported by engine: per-sample coefficient on an explicit source")
                 sources.append((c, core._source_spatial))
-            elif kind in ("conv", "vc", "nsp", "ks", "ks_keep_mean"):
+            elif kind in ("conv", "vc", "nsp", "ks", "ks_keep_mean", "cons"):
                 if prod is not None:
                     raise ValueError("term not supported by engine: more than one product term")
                 if not core._dealiasing_swtich:
@@ -570,14 +570,14 @@ class OperatorLike(_MutableMixIn):
         x = u if u is not None else u_fft
         require_gpu_tensor(x, "u" if u is not None else "u_fft")
         cores = list(self._linear_cores) + list(self._nonlinear_funcs)
-        if any(getattr(core, "_engine_map", None) for _, core in cores):
-            if len(cores) != 1:
-                raise ValueError("term not supported by engine: a channel-changing operator (Div, Grad, Curl, "
-                                 "Vorticity2Velocity, pressure, ConservativeConvection) combined with other terms")
+        if len(cores) == 1 and getattr(cores[0][1], "_engine_map", None):
             from ._diag import map_call
 
             coef, core = cores[0]
             return map_call(core, coef, u, u_fft, self._state_dict["f_mesh"], return_in_fourier)
+        if any(getattr(core, "_engine_map", None) and not getattr(core, "_engine_kind", None) for _, core in cores):
+            raise ValueError("term not supported by engine: a channel-changing operator (Div, Grad, Curl, "
+                             "Vorticity2Velocity, pressure) combined with other terms")
         plan = self._engine(x.shape[0], x.device)
         uh = self._to_half(plan, u, u_fft)
         out = plan.apply(uh)

@@ -5,7 +5,8 @@
 Their cores carry an ``_engine_map`` tag; OperatorLike.__call__ routes a single such core to
 map_call, which runs the engine's transforms, the tfsm_spectral_map kernels and, where a product is
 needed, the Convection evaluation or tfsm_pointwise_products.  Integrating them raises ValueError
-("term not supported by engine"), as their output is not a state of the same shape.
+("term not supported by engine"), as their output is not a state of the same shape; the exception
+is ConservativeConvection, which is also an engine product term (1D, 2D).
 """
 
 from typing import Optional
@@ -249,9 +250,12 @@ class Velocity2Pressure(NonlinearOperator):
 
 
 class _ConservativeConvectionCore(NonlinearFunc):
-    """div(u u^T) of a vector field (_conservative_convection.py:9-27), dealiased input."""
+    """div(u u^T) of a vector field (_conservative_convection.py:9-27), dealiased input.  Stepped by
+    the engine in 1D and 2D (a product term like Convection); a single application also runs
+    through the spectral maps, in any dimension."""
 
     _engine_map = "cons"
+    _engine_kind = "cons"
 
     def __init__(self) -> None:
         super().__init__()
@@ -268,8 +272,7 @@ class _ConservativeConvectionGenerator(CoreGenerator):
 
 
 class ConservativeConvection(NonlinearOperator):
-    """Conservative-form convection div(u u^T) (_conservative_convection.py:43-70); evaluated with
-    Operator.__call__, not stepped by the engine."""
+    """Conservative-form convection div(u u^T) (_conservative_convection.py:43-70)."""
 
     def __init__(self) -> None:
         super().__init__(_ConservativeConvectionGenerator())
