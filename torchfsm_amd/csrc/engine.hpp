@@ -328,6 +328,65 @@ __device__ __forceinline__ float2 stage_combine_v(int stage, float2 nval, float2
   }
 }
 
+// Stage combine of one channel's column of RMAX modes per lane into s, storing the stage target
+// (the column kernels, 2D and 3D).  Mode m of lane j is element (j + m NT) * stride of the column
+// whose first element U, A1, A2, src point at; cpk points at the packed coefficient pair of that
+// first element, so mode m's is cpk[2 (j + m NT) stride].  Operands are loaded in batches of BM
+// modes and every load of a batch is issued before its first use, so a batch costs one memory
+// latency instead of one per operand.  BM is 4 for plans of up to 12 values per thread and 1 above,
+// where the transforms already hold most of the 128 VGPRs that keep two 512-thread workgroups per
+// CU.  The loads are unconditional: a lane past the last column (colv false) reads the padded
+// columns of the same rows, which every array and table holds, and its result is discarded.
+// Offsets within one channel are 32-bit (every supported mesh has fewer than 2^28 modes).
+template <int RMAX, int NT>
+__device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1, float2* A2, const float2* src, const float4* cpk, float coef_t,
+                                               unsigned stride, int j, bool colv, const float2 (&v)[RMAX],
+                                               float2 (&s)[RMAX]) {
+  constexpr int BM = RMAX >= 16 ? 1 : 4;
+  static_assert(RMAX % BM == 0, "combine batch must divide the values per thread");
+  const float2 z = make_float2(0.f, 0.f);
+  const bool rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
+  const int tgt = stage_target(stage);
+  float2* dst = tgt == 0 ? U : (tgt == 1 ? A1 : A2);
+  const unsigned o0 = (unsigned)j * stride, ostep = (unsigned)NT * stride;
+#pragma unroll
+  for (int m0 = 0; m0 < RMAX; m0 += BM) {
+    float2 uu[BM], q1[BM], q2[BM], sv[BM];
+    float4 cc[BM];
+#pragma unroll
+    for (int i = 0; i < BM; ++i) {
+      const unsigned o = o0 + (unsigned)(m0 + i) * ostep;
+      uu[i] = U[o];
+      cc[i] = cpk[2 * o];
+      q1[i] = z;
+      q2[i] = z;
+      sv[i] = z;
+    }
+    if (rd1) {
+#pragma unroll
+      for (int i = 0; i < BM; ++i) q1[i] = A1[o0 + (unsigned)(m0 + i) * ostep];
+    }
+    if (rd2) {
+#pragma unroll
+      for (int i = 0; i < BM; ++i) q2[i] = A2[o0 + (unsigned)(m0 + i) * ostep];
+    }
+    if (src) {
+#pragma unroll
+      for (int i = 0; i < BM; ++i) sv[i] = src[o0 + (unsigned)(m0 + i) * ostep];
+    }
+#pragma unroll
+    for (int i = 0; i < BM; ++i) {
+      const int m = m0 + i;
+      float2 nv = rscale(coef_t, v[m]);
+      if (src) nv = radd(nv, sv[i]);
+      float2 w;
+      const float2 r = stage_combine_v(stage, nv, uu[i], q1[i], q2[i], cc[i], &w);
+      s[m] = colv ? r : z;
+      if (colv) dst[o0 + (unsigned)m * ostep] = w;
+    }
+  }
+}
+
 // NSPressureConvection of one mode from the convection spectra conv_c (_navier_stokes.py:244-262),
 // in the reference's op order: t_i = (i a_i) conv_i, p = invlap * sum_i t_i,
 // N_c = (i a_c) p - conv_c, with invlap = 1 / lap (1 at lap = 0), lap = sum_i -(a_i a_i).

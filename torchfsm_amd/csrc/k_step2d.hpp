@@ -53,6 +53,19 @@ struct Term2<TERM_CONS> {  // ConservativeConvection, C = 2: fields u0, u1; prod
   static constexpr int NC = 2, NF = 2;  // channel c: F_y(u0 u_c) and i a_y F_y(u1 u_c)
 };
 
+// Stage combine of one channel's column (see combine_column in engine.hpp): column base ky,
+// element stride P1.
+template <int N0, int VAR>
+__device__ __forceinline__ void combine_column2(const Col2Args& a, int stage, size_t base, const float2* src,
+                                                const float4* cpk, int cko, int ky, bool colv, int j,
+                                                const float2 (&v)[Fft<N0, VAR>::RMAX],
+                                                float2 (&s)[Fft<N0, VAR>::RMAX]) {
+  using F = Fft<N0, VAR>;
+  const size_t cb = base + ky;
+  combine_column<F::RMAX, F::NT>(stage, a.U + cb, a.A1 + cb, a.A2 + cb, src ? src + ky : nullptr,
+                                 cpk + (2 * ky + cko), a.coef_t, (unsigned)a.P1, j, colv, v, s);
+}
+
 // Velocity-form Navier-Stokes (NSPressureConvection): both channels' convection spectra are
 // transformed first, projected mode by mode (nsp_project), then combined channel by channel.
 template <int N0, int V, int VAR>
@@ -67,14 +80,20 @@ __device__ __forceinline__ void col_nsp_combine(const Col2Args& a, int stage, fl
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
-      for (int m = 0; m < RMAX; ++m) s[c][m] = colv ? a.U[((size_t)b * 2 + c) * plane + (size_t)(j + m * NT) * a.P1 + ky] : z;
+      for (int m = 0; m < RMAX; ++m) {
+        const float2 t = a.U[((size_t)b * 2 + c) * plane + (size_t)(j + m * NT) * a.P1 + ky];
+        s[c][m] = colv ? t : z;
+      }
     return;
   }
   float2 v[2][RMAX];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
 #pragma unroll
-    for (int m = 0; m < RMAX; ++m) v[c][m] = colv ? a.P[((size_t)b * 2 + c) * plane + (size_t)(j + m * NT) * a.P1 + ky] : z;
+    for (int m = 0; m < RMAX; ++m) {
+      const float2 t = a.P[((size_t)b * 2 + c) * plane + (size_t)(j + m * NT) * a.P1 + ky];
+      v[c][m] = colv ? t : z;
+    }
     fft_wg<N0, -1, V, VAR>(v[c], s_buf, s_tw, j, vi);
   }
   const float ayv = colv ? a.ay[ky] : 0.f;
@@ -87,32 +106,12 @@ __device__ __forceinline__ void col_nsp_combine(const Col2Args& a, int stage, fl
     v[1][m] = cv[1];
   }
   const float4* cpk = a.cpk + (size_t)b * a.cpk_bstride;
-  const int chalf = stage_coef_half(stage);
-  // cpk_index(ci, chalf) = 2 ci + cko: every mode of this lane has ci & 15 == ky & 15
-  const int cko = (chalf << 4) - (ky & 15);
-  const bool rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
-  const int tgt = stage_target(stage);
+  // cpk_index(ci, half) = 2 ci + cko: every mode of this lane has ci & 15 == ky & 15
+  const int cko = (stage_coef_half(stage) << 4) - (ky & 15);
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    const size_t base = ((size_t)b * 2 + c) * plane;
     const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride : nullptr;
-    float2* dst = tgt == 0 ? a.U : (tgt == 1 ? a.A1 : a.A2);
-    float2 wv[RMAX];
-#pragma unroll
-    for (int m = 0; m < RMAX; ++m) {
-      const size_t ci = (size_t)(j + m * NT) * a.P1 + ky;
-      const float2 uu = colv ? a.U[base + ci] : z;
-      const float2 q1 = (colv && rd1) ? a.A1[base + ci] : z;
-      const float2 q2 = (colv && rd2) ? a.A2[base + ci] : z;
-      const float4 cc = colv ? cpk[ci * 2 + cko] : make_float4(0.f, 0.f, 0.f, 0.f);
-      float2 nv = rscale(a.coef_t, v[c][m]);
-      if (src && colv) nv = radd(nv, src[ci]);
-      s[c][m] = colv ? stage_combine_v(stage, nv, uu, q1, q2, cc, &wv[m]) : z;
-    }
-    if (colv) {
-#pragma unroll
-      for (int m = 0; m < RMAX; ++m) dst[base + (size_t)(j + m * NT) * a.P1 + ky] = wv[m];
-    }
+    combine_column2<N0, VAR>(a, stage, ((size_t)b * 2 + c) * plane, src, cpk, cko, ky, colv, j, v[c], s[c]);
   }
 }
 
@@ -123,7 +122,9 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
   constexpr int NC = Term2<TERM>::NC, NF = Term2<TERM>::NF;
   __shared__ float2 s_tw[Fft<N0, VAR>::tw_size()];
   __shared__ float2 s_buf[LdsSize<N0, V>::slots];
+  __shared__ float s_ax[N0];  // x wavenumbers: read per mode by the field pass without a memory wait
   load_twiddles<N0, VAR>(s_tw, a.tw0);
+  for (int i = threadIdx.x; i < N0; i += NT * V) s_ax[i] = a.ax[i];
   __syncthreads();
   const int vi = threadIdx.x % V, j = threadIdx.x / V;
   // Block -> (tile, sample).  Workgroups are dispatched round-robin over the 8 XCDs (block v on
@@ -145,11 +146,8 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
   const bool colv = ky < a.H1;
   const size_t plane = (size_t)N0 * a.P1;
   const float4* cpk = a.cpk + (size_t)b * a.cpk_bstride;
-  const int chalf = stage_coef_half(stage);
-  // cpk_index(ci, chalf) = 2 ci + cko: every mode of this lane has ci & 15 == ky & 15
-  const int cko = (chalf << 4) - (ky & 15);
-  const bool rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
-  const int tgt = stage_target(stage);
+  // cpk_index(ci, half) = 2 ci + cko: every mode of this lane has ci & 15 == ky & 15
+  const int cko = (stage_coef_half(stage) << 4) - (ky & 15);
   float2 s[NC][RMAX];
   if constexpr (TERM == TERM_NSP) {
     col_nsp_combine<N0, V, VAR>(a, stage, s, s_buf, s_tw, b, ky, colv, j, vi);
@@ -159,15 +157,20 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
     const size_t base = ((size_t)b * NC + c) * plane;
     if (stage == ST_PRO) {
 #pragma unroll
-      for (int m = 0; m < RMAX; ++m)
-        s[c][m] = colv ? a.U[base + (size_t)(j + m * NT) * a.P1 + ky] : make_float2(0.f, 0.f);
+      for (int m = 0; m < RMAX; ++m) {
+        const float2 t = a.U[base + (size_t)(j + m * NT) * a.P1 + ky];
+        s[c][m] = colv ? t : make_float2(0.f, 0.f);
+      }
     } else {
       float2 v[RMAX];
       if constexpr (TERM == TERM_CONS) {
         // N_c = (i a_x) F_x(F_y(u0 u_c)) + F_x(i a_y F_y(u1 u_c))   (_conservative_convection.py:23-27)
         const size_t pb = ((size_t)b * 4 + 2 * c) * plane;
 #pragma unroll
-        for (int m = 0; m < RMAX; ++m) v[m] = colv ? a.P[pb + (size_t)(j + m * NT) * a.P1 + ky] : make_float2(0.f, 0.f);
+        for (int m = 0; m < RMAX; ++m) {
+          const float2 t = a.P[pb + (size_t)(j + m * NT) * a.P1 + ky];
+          v[m] = colv ? t : make_float2(0.f, 0.f);
+        }
         fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
 #pragma unroll
         for (int m = 0; m < RMAX; ++m) {
@@ -176,47 +179,32 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
         }
         float2 w[RMAX];
 #pragma unroll
-        for (int m = 0; m < RMAX; ++m)
-          w[m] = colv ? a.P[pb + plane + (size_t)(j + m * NT) * a.P1 + ky] : make_float2(0.f, 0.f);
+        for (int m = 0; m < RMAX; ++m) {
+          const float2 t = a.P[pb + plane + (size_t)(j + m * NT) * a.P1 + ky];
+          w[m] = colv ? t : make_float2(0.f, 0.f);
+        }
         fft_wg<N0, -1, V, VAR>(w, s_buf, s_tw, j, vi);
 #pragma unroll
         for (int m = 0; m < RMAX; ++m) v[m] = radd(v[m], w[m]);
       } else {
 #pragma unroll
-        for (int m = 0; m < RMAX; ++m)
-          v[m] = colv ? a.P[base + (size_t)(j + m * NT) * a.P1 + ky] : make_float2(0.f, 0.f);
+        for (int m = 0; m < RMAX; ++m) {
+          const float2 t = a.P[base + (size_t)(j + m * NT) * a.P1 + ky];
+          v[m] = colv ? t : make_float2(0.f, 0.f);
+        }
         fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
       }
       if constexpr (TERM == TERM_KS) {
         // remove_mean: fft(p - mean(p)) differs from fft(p) only at k = 0, by the batch mean of the DC
         if (a.dc && ky == 0 && j == 0) v[0] = rsub(v[0], a.dc[0]);
       }
-      // combine: every store of this tile is deferred until all loads have been issued, so the
-      // loads need not wait behind stores to arrays the compiler cannot prove disjoint
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride : nullptr;
-      float2* dst = tgt == 0 ? a.U : (tgt == 1 ? a.A1 : a.A2);
-      float2 wv[RMAX];
-#pragma unroll
-      for (int m = 0; m < RMAX; ++m) {
-        const size_t ci = (size_t)(j + m * NT) * a.P1 + ky;
-        const float2 z = make_float2(0.f, 0.f);
-        const float2 uu = colv ? a.U[base + ci] : z;
-        const float2 q1 = (colv && rd1) ? a.A1[base + ci] : z;
-        const float2 q2 = (colv && rd2) ? a.A2[base + ci] : z;
-        const float4 cc = colv ? cpk[ci * 2 + cko] : make_float4(0.f, 0.f, 0.f, 0.f);
-        float2 nv = rscale(a.coef_t, v[m]);
-        if (src && colv) nv = radd(nv, src[ci]);
-        s[c][m] = colv ? stage_combine_v(stage, nv, uu, q1, q2, cc, &wv[m]) : z;
-      }
-      if (colv) {
-#pragma unroll
-        for (int m = 0; m < RMAX; ++m) dst[base + (size_t)(j + m * NT) * a.P1 + ky] = wv[m];
-      }
+      combine_column2<N0, VAR>(a, stage, base, src, cpk, cko, ky, colv, j, v, s[c]);
     }
   }
   }
   if (tile * V > a.kmax1) return;  // uniform per workgroup: no kept column in this tile
-  const float ay = colv ? a.ay[ky] : 0.f;
+  const float ay = a.ay[ky];  // ky < P1 <= N1: in the table
   (void)ay;
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
@@ -228,7 +216,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
       const bool keep = colv && ky <= a.kmax1 && ks <= a.kmax0;
       float2 val = make_float2(0.f, 0.f);
       if (keep) {
-        const float ax = a.ax[kx];
+        const float ax = s_ax[kx];
         if constexpr (TERM == TERM_VC) {
           const float2 w = s[0][m];
           if (f == 2) val = make_float2(mulr(ax, w.x), mulr(ax, w.y));

@@ -42,7 +42,9 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
   constexpr int RMAX = F::RMAX, NT = F::NT;
   __shared__ float2 s_tw[Fft<N0, VAR>::tw_size()];
   __shared__ float2 s_buf[LdsSize<N0, V>::slots];
+  __shared__ float s_ax[N0];  // x wavenumbers: read per mode by the field pass without a memory wait
   load_twiddles<N0, VAR>(s_tw, a.tw0);
+  for (int i = threadIdx.x; i < N0; i += NT * V) s_ax[i] = a.ax[i];
   __syncthreads();
   const int vi = threadIdx.x % V, j = threadIdx.x / V;
   // tiles (ky, kz-tile) go round-robin to the XCDs (block v on XCD v % 8); the nb samples of one
@@ -61,11 +63,8 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
   const size_t plane = (size_t)N0 * inner;   // modes
   const size_t coff = (size_t)ky * a.P2 + kz;
   const float4* cpk = a.cpk + (size_t)b * a.cpk_bstride;
-  const int chalf = stage_coef_half(stage);
-  // cpk_index(coff + xo, chalf) = 2 (coff + xo) + cko: (coff + xo) & 15 == kz & 15
-  const int cko = (chalf << 4) - (kz & 15);
-  const bool rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
-  const int tgt = stage_target(stage);
+  // cpk_index(coff + xo, half) = 2 (coff + xo) + cko: (coff + xo) & 15 == kz & 15
+  const int cko = (stage_coef_half(stage) << 4) - (kz & 15);
   constexpr int NC3 = TERM == TERM_KS ? 1 : 3;  // channels
   float2 s[NC3][RMAX];
   if (stage == ST_PRO) {
@@ -73,7 +72,10 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
     for (int c = 0; c < NC3; ++c) {
       const size_t base = ((size_t)b * NC3 + c) * plane + coff;
 #pragma unroll
-      for (int m = 0; m < RMAX; ++m) s[c][m] = colv ? a.U[base + (size_t)(j + m * NT) * inner] : make_float2(0.f, 0.f);
+      for (int m = 0; m < RMAX; ++m) {
+        const float2 t = a.U[base + (size_t)(j + m * NT) * inner];
+        s[c][m] = colv ? t : make_float2(0.f, 0.f);
+      }
     }
   } else if constexpr (TERM == TERM_NSP) {
     // velocity-form Navier-Stokes: the three convection spectra first, projected mode by mode
@@ -82,7 +84,10 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
     for (int c = 0; c < 3; ++c) {
       const size_t base = ((size_t)b * 3 + c) * plane + coff;
 #pragma unroll
-      for (int m = 0; m < RMAX; ++m) v3[c][m] = colv ? a.P[base + (size_t)(j + m * NT) * inner] : make_float2(0.f, 0.f);
+      for (int m = 0; m < RMAX; ++m) {
+        const float2 t = a.P[base + (size_t)(j + m * NT) * inner];
+        v3[c][m] = colv ? t : make_float2(0.f, 0.f);
+      }
       fft_wg<N0, -1, V, VAR>(v3[c], s_buf, s_tw, j, vi);
     }
     const float ayv = a.ay[ky], azv = colv ? a.az[kz] : 0.f;
@@ -98,24 +103,8 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
     for (int c = 0; c < 3; ++c) {
       const size_t base = ((size_t)b * 3 + c) * plane + coff;
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride + coff : nullptr;
-      float2* dst = tgt == 0 ? a.U : (tgt == 1 ? a.A1 : a.A2);
-      float2 wv[RMAX];
-#pragma unroll
-      for (int m = 0; m < RMAX; ++m) {
-        const size_t xo = (size_t)(j + m * NT) * inner;
-        const float2 z = make_float2(0.f, 0.f);
-        const float2 uu = colv ? a.U[base + xo] : z;
-        const float2 q1 = (colv && rd1) ? a.A1[base + xo] : z;
-        const float2 q2 = (colv && rd2) ? a.A2[base + xo] : z;
-        const float4 cc = colv ? cpk[(coff + xo) * 2 + cko] : make_float4(0.f, 0.f, 0.f, 0.f);
-        float2 nv = rscale(a.coef_t, v3[c][m]);
-        if (src && colv) nv = radd(nv, src[xo]);
-        s[c][m] = colv ? stage_combine_v(stage, nv, uu, q1, q2, cc, &wv[m]) : z;
-      }
-      if (colv) {
-#pragma unroll
-        for (int m = 0; m < RMAX; ++m) dst[base + (size_t)(j + m * NT) * inner] = wv[m];
-      }
+      combine_column<RMAX, NT>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko), a.coef_t,
+                               (unsigned)inner, j, colv, v3[c], s[c]);
     }
   } else {
 #pragma unroll
@@ -123,31 +112,18 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
       const size_t base = ((size_t)b * NC3 + c) * plane + coff;
       float2 v[RMAX];
 #pragma unroll
-      for (int m = 0; m < RMAX; ++m) v[m] = colv ? a.P[base + (size_t)(j + m * NT) * inner] : make_float2(0.f, 0.f);
+      for (int m = 0; m < RMAX; ++m) {
+        const float2 t = a.P[base + (size_t)(j + m * NT) * inner];
+        v[m] = colv ? t : make_float2(0.f, 0.f);
+      }
       fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
       if constexpr (TERM == TERM_KS) {
         // remove_mean: fft(p - mean(p)) differs from fft(p) only at k = 0, by the batch mean of the DC
         if (a.dc && ky == 0 && kz == 0 && j == 0) v[0] = rsub(v[0], a.dc[0]);
       }
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride + coff : nullptr;
-      float2* dst = tgt == 0 ? a.U : (tgt == 1 ? a.A1 : a.A2);
-      float2 wv[RMAX];
-#pragma unroll
-      for (int m = 0; m < RMAX; ++m) {
-        const size_t xo = (size_t)(j + m * NT) * inner;
-        const float2 z = make_float2(0.f, 0.f);
-        const float2 uu = colv ? a.U[base + xo] : z;
-        const float2 q1 = (colv && rd1) ? a.A1[base + xo] : z;
-        const float2 q2 = (colv && rd2) ? a.A2[base + xo] : z;
-        const float4 cc = colv ? cpk[(coff + xo) * 2 + cko] : make_float4(0.f, 0.f, 0.f, 0.f);
-        float2 nv = rscale(a.coef_t, v[m]);
-        if (src && colv) nv = radd(nv, src[xo]);
-        s[c][m] = colv ? stage_combine_v(stage, nv, uu, q1, q2, cc, &wv[m]) : z;
-      }
-      if (colv) {
-#pragma unroll
-        for (int m = 0; m < RMAX; ++m) dst[base + (size_t)(j + m * NT) * inner] = wv[m];
-      }
+      combine_column<RMAX, NT>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko), a.coef_t,
+                               (unsigned)inner, j, colv, v, s[c]);
     }
   }
   const int kyc = ky_compress(ky, a.N1, a.kmax1);
@@ -164,7 +140,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
       if (keep) {
         const float2 u = s[f >> 1][m];
         if (f & 1) {
-          const float ax = a.ax[kx];
+          const float ax = s_ax[kx];
           val = make_float2(mulr(ax, u.x), mulr(ax, u.y));
         } else {
           val = u;
