@@ -244,6 +244,13 @@ __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) 
   __syncthreads();
   float2* lds = s_buf + g * SL;
   const float2* in = s_in + (size_t)g * PF2;  // field f of this row at in[f * ROWS * PF2 + k]
+  // z wavenumbers of this lane's modes, loaded once (k <= N2/2 is in the table)
+  float azr[RMAX];
+#pragma unroll
+  for (int m = 0; m < RMAX; ++m) {
+    const int pos = j + m * NT;
+    azr[m] = a.az[pos > N2 / 2 ? N2 - pos : pos];
+  }
   // packed inverse transform of two fields: A = ma(fa), B = mb(fb) where the multiplier is
   // 0: 1, 1: i, 2: i a_z  (u_c = C2R(G[3c]), d_y u_c = C2R(i G[3c+1]), d_x u_c = C2R(i G[3c+2]),
   // d_z u_c = C2R(i a_z G[3c]))
@@ -256,7 +263,7 @@ __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) 
       const bool edge = (k == 0) || (k == N2 / 2);
       float2 A = make_float2(0.f, 0.f), Bv = A;
       if (k <= a.kmax2) {
-        const float az = a.az[k];
+        const float az = azr[m];
         const float2 x1 = in[fa * ROWS * PF2 + k], x2 = in[fb * ROWS * PF2 + k];
         A = ma == 0 ? x1 : (ma == 1 ? make_float2(-x1.y, x1.x) : make_float2(-az * x1.y, az * x1.x));
         Bv = mb == 0 ? x2 : (mb == 1 ? make_float2(-x2.y, x2.x) : make_float2(-az * x2.y, az * x2.x));
