@@ -258,7 +258,10 @@ __global__ void __launch_bounds__((Fft<N1, VAR>::NT * ROWS)) k2_row(Row2Args a) 
   load_twiddles<N1, VAR>(s_tw, a.tw1);
   __syncthreads();
   const int g = threadIdx.x / NT, j = threadIdx.x % NT;
-  const int row = blockIdx.x * ROWS + g;
+  int row = blockIdx.x * ROWS + g;
+  // with NT a multiple of 64 a wavefront holds one row: the row (and every row pointer) is uniform,
+  // so the loads and stores take a scalar base and a 32-bit lane offset
+  if constexpr (NT % 64 == 0) row = __builtin_amdgcn_readfirstlane(row);
   const bool rv = row < a.B * a.N0;
   const int b = a.b0 + (rv ? row / a.N0 : 0);
   const int x = rv ? row % a.N0 : 0;
