@@ -333,16 +333,15 @@ __device__ __forceinline__ float2 stage_combine_v(int stage, float2 nval, float2
 // whose first element U, A1, A2, src point at; cpk points at the packed coefficient pair of that
 // first element, so mode m's is cpk[2 (j + m NT) stride].  Operands are loaded in batches of BM
 // modes and every load of a batch is issued before its first use, so a batch costs one memory
-// latency instead of one per operand.  BM is 4 for plans of up to 12 values per thread and 1 above,
-// where the transforms already hold most of the 128 VGPRs that keep two 512-thread workgroups per
-// CU.  The loads are unconditional: a lane past the last column (colv false) reads the padded
+// latency instead of one per operand.  BM defaults to 4 for plans of up to 12 values per thread and
+// to 1 above, where the transforms already hold most of the 128 VGPRs that keep two 512-thread
+// workgroups per CU.  The loads are unconditional: a lane past the last column (colv false) reads the padded
 // columns of the same rows, which every array and table holds, and its result is discarded.
 // Offsets within one channel are 32-bit (every supported mesh has fewer than 2^28 modes).
-template <int RMAX, int NT>
+template <int RMAX, int NT, int BM = (RMAX >= 16 ? 1 : 4)>
 __device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1, float2* A2, const float2* src, const float4* cpk, float coef_t,
                                                unsigned stride, int j, bool colv, const float2 (&v)[RMAX],
                                                float2 (&s)[RMAX]) {
-  constexpr int BM = RMAX >= 16 ? 1 : 4;
   static_assert(RMAX % BM == 0, "combine batch must divide the values per thread");
   const float2 z = make_float2(0.f, 0.f);
   const bool rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
