@@ -357,3 +357,41 @@ def test_cpu_recorder_streams_frames_while_stepping(dev):
     print(f"400 steps 256^2 B=16: {t_plain * 1e3:.1f} ms without frames; {tr.numel() * 4 / 1e9:.2f} GB of frames to "
           f"host: {t_stream * 1e3:.1f} ms streamed during stepping, {t_serial * 1e3:.1f} ms recorded then copied")
     assert tr.shape == (16, 401, 1, 256, 256) and torch.equal(tr, host)
+
+
+def test_plan_rejects_mismatched_tensors(dev):
+    """Every SpectralPlan call checks device, dtype, contiguity and shape before a pointer reaches the
+    C ABI, and the C ABI rejects a batch count below one on its own; nothing is launched."""
+    import ctypes
+
+    from torchfsm_amd._lib import lib
+    from torchfsm_amd.engine import SpectralPlan
+
+    plan = SpectralPlan([(0, 1, 32)] * 2, 2, 2, nonlinear="conv", linear=[("lap", 0.01)], device=dev)
+    plan.set_scheme("SETDRK2", 1e-3)
+    u = torch.zeros(2, 2, 32, 32, device=dev)
+    uh = plan.forward(u)
+    with pytest.raises(ValueError, match="shape"):
+        plan.forward(torch.zeros(2, 2, 32, 16, device=dev))
+    with pytest.raises(RuntimeError, match="GPU only"):
+        plan.forward(torch.zeros(2, 2, 32, 32))
+    with pytest.raises(ValueError, match="samples"):
+        plan.step(uh[:1].contiguous(), 1)
+    with pytest.raises(TypeError, match="complex64"):
+        plan.apply(uh.real.contiguous())
+    with pytest.raises(ValueError, match="contiguous"):
+        plan.solve(uh.transpose(2, 3))
+    with pytest.raises(ValueError, match="nbatch"):
+        plan.inverse(uh, nbatch=3)
+    with pytest.raises(ValueError, match="record slots"):
+        plan.step(uh, 4, torch.zeros(2, dtype=torch.int32, device=dev),
+                  torch.zeros((2, 1) + plan.spec_shape[1:], dtype=torch.complex64, device=dev))
+    with pytest.raises(ValueError, match="channels"):
+        plan.spectral_map(0, uh, 3)
+    s = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    for nb in (0, -1):
+        rc = lib.tfsm_forward(plan.handle, ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(uh.data_ptr()), nb, s)
+        assert rc != 0 and b"nbatch" in lib.tfsm_last_error()
+    # the plan still works after the rejections
+    plan.step(uh, 2)
+    assert torch.isfinite(plan.inverse(uh)).all()

@@ -639,6 +639,16 @@ AdjStageArgs stage_args(tfsm_plan* p, int op) {
 
 }  // namespace
 
+// nbatch blocks of the plan's [C][N...] shape: at least one, and few enough that every launch's row
+// count fits the kernels' 32-bit row index
+static int check_nbatch(const tfsm_plan* p, int32_t nbatch) {
+  if (nbatch < 1) return fail(TFSM_E_INVALID, "nbatch must be >= 1");
+  size_t rows = (size_t)nbatch * p->C;
+  for (int d = 0; d < p->D - 1; ++d) rows *= p->n[d];
+  if (rows * p->n[p->D - 1] > (size_t)INT32_MAX) return fail(TFSM_E_INVALID, "nbatch too large for one call");
+  return TFSM_OK;
+}
+
 extern "C" {
 
 const char* tfsm_last_error(void) { return g_err.c_str(); }
@@ -951,7 +961,7 @@ int tfsm_plan_set_scheme(tfsm_plan* plan, int32_t scheme, double dt, int32_t n_p
   const bool nonlinear = plan->term != TERM_NONE || plan->src != nullptr;
   if (scheme == ETDRK0 && nonlinear) return fail(TFSM_E_INVALID, "The ETDRK0 integrator only supports linear term");
   if (scheme != ETDRK0 && plan->term == TERM_NONE)
-    return fail(TFSM_E_INVALID, "term not supported by engine: a nonlinear scheme needs Convection or VorticityConvection");
+    return fail(TFSM_E_INVALID, "term not supported by engine: a nonlinear scheme needs a product term");
   if (n_points < 1) return fail(TFSM_E_INVALID, "n_integration_points must be >= 1");
   CoefArgs a;
   a.scheme = scheme;
@@ -1051,6 +1061,7 @@ int tfsm_plan_coefficients(tfsm_plan* p, int32_t slot, float* out, void* stream)
 
 int tfsm_forward(tfsm_plan* p, const float* u, void* uh, int32_t nbatch, void* stream) {
   if (!p || !u || !uh) return fail(TFSM_E_INVALID, "null argument");
+  if (int rc = check_nbatch(p, nbatch)) return rc;
   hipStream_t st = (hipStream_t)stream;
   float2* out = (float2*)uh;
   const size_t blocks = (size_t)nbatch * p->C;
@@ -1073,6 +1084,7 @@ int tfsm_forward(tfsm_plan* p, const float* u, void* uh, int32_t nbatch, void* s
 
 int tfsm_inverse(tfsm_plan* p, const void* uh, float* u, int32_t nbatch, void* scratch, void* stream) {
   if (!p || !u || !uh) return fail(TFSM_E_INVALID, "null argument");
+  if (int rc = check_nbatch(p, nbatch)) return rc;
   hipStream_t st = (hipStream_t)stream;
   const size_t blocks = (size_t)nbatch * p->C;
   const int nl = p->n[p->D - 1];
@@ -1106,6 +1118,7 @@ int tfsm_inverse(tfsm_plan* p, const void* uh, float* u, int32_t nbatch, void* s
 int tfsm_c2c(tfsm_plan* p, const void* in, void* out, int32_t nbatch, int32_t dir, void* stream) {
   if (!p || !in || !out) return fail(TFSM_E_INVALID, "null argument");
   if (dir != -1 && dir != 1) return fail(TFSM_E_INVALID, "dir must be -1 or +1");
+  if (int rc = check_nbatch(p, nbatch)) return rc;
   hipStream_t st = (hipStream_t)stream;
   const size_t blocks = (size_t)nbatch * p->C;
   const int nl = p->n[p->D - 1];
@@ -1135,6 +1148,7 @@ int tfsm_c2c(tfsm_plan* p, const void* in, void* out, int32_t nbatch, int32_t di
 
 int tfsm_half_to_full(tfsm_plan* p, const void* uh, void* full, int32_t nbatch, void* stream) {
   if (!p || !uh || !full) return fail(TFSM_E_INVALID, "null argument");
+  if (int rc = check_nbatch(p, nbatch)) return rc;
   ConvArgs a;
   a.D = p->D;
   for (int d = 0; d < 3; ++d) a.n[d] = p->n[d];
@@ -1147,6 +1161,7 @@ int tfsm_half_to_full(tfsm_plan* p, const void* uh, void* full, int32_t nbatch, 
 
 int tfsm_full_to_half(tfsm_plan* p, const void* full, void* uh, int32_t nbatch, void* stream) {
   if (!p || !uh || !full) return fail(TFSM_E_INVALID, "null argument");
+  if (int rc = check_nbatch(p, nbatch)) return rc;
   ConvArgs a;
   a.D = p->D;
   for (int d = 0; d < 3; ++d) a.n[d] = p->n[d];

@@ -48,7 +48,7 @@ class SpectralPlan:
     Args:
         mesh_info: sequence of (start, end, n) per axis.
         batch, channels: B and C of the state.
-        nonlinear: None, "conv", "vc", "nsp", "ks" or "ks_keep_mean" (the dealiased product term)
+        nonlinear: None, "conv", "vc", "nsp", "ks", "ks_keep_mean" or "cons" (the dealiased product term)
             and its coefficient.
         linear: list of (kind, coef) with kind in {"lap", "biharm", "unit"} or ("deriv", dim, order)
             (SpatialDerivative); coef a float or a per-sample sequence of length `batch`.
@@ -203,6 +203,37 @@ class SpectralPlan:
                                            _stream(self.device)), "source")
             torch.cuda.current_stream(self.device).synchronize()
 
+    # ------------------------------------------------------------------ argument checks
+    def _check(self, t: torch.Tensor, trailing, dtype, what: str, nb: Optional[int] = None) -> int:
+        """Every tensor handed to the C ABI is a contiguous tensor of this plan's device and dtype whose
+        shape is (nb, *trailing): the kernels index it from its shape alone, so a mismatch would read or
+        write outside it.  Returns its leading size."""
+        require_gpu_tensor(t, what)
+        if t.device != self.device:
+            raise ValueError(f"{what} is on {t.device}, the plan on {self.device}")
+        if t.dtype != dtype:
+            raise TypeError(f"{what} must be {dtype}, got {t.dtype}")
+        if not t.is_contiguous():
+            raise ValueError(f"{what} must be contiguous")
+        if t.dim() != 1 + len(trailing) or tuple(t.shape[1:]) != tuple(trailing) or t.shape[0] < 1:
+            raise ValueError(f"{what} has shape {tuple(t.shape)}, expected (n, {', '.join(map(str, trailing))})")
+        if nb is not None and t.shape[0] != nb:
+            raise ValueError(f"{what} holds {t.shape[0]} samples, expected {nb}")
+        return int(t.shape[0])
+
+    def _check_state(self, t: torch.Tensor, what: str = "state"):
+        self._check(t, self.spec_shape[1:], torch.complex64, what, self.batch)
+
+    def _map_channels(self, op: int):
+        """(input, output) channel counts of a spectral map, as tfsm_spectral_map forms them."""
+        D, C = self.ndim, self.channels
+        table = {_lib.MAP_DEALIAS: (C, C), _lib.MAP_DIV: (D, 1), _lib.MAP_GRAD: (1, D),
+                 _lib.MAP_CURL: (D, 1 if D == 2 else 3), _lib.MAP_V2V: (1, 2), _lib.MAP_PRESSURE: (D, 1),
+                 _lib.MAP_CONSDIV: (D * D, D)}
+        if op not in table:
+            raise ValueError(f"unknown spectral map {op}")
+        return table[op]
+
     # ------------------------------------------------------------------ buffers
     def empty_spectrum(self, nbatch=None) -> torch.Tensor:
         nb = self.batch if nbatch is None else nbatch
@@ -212,16 +243,21 @@ class SpectralPlan:
     # ------------------------------------------------------------------ transforms
     def forward(self, u: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
         """Real (nb, C, N...) float32 -> half spectrum (nb, C, ..., P)."""
-        nb = u.shape[0]
+        require_gpu_tensor(u, "forward input")
         u = u.to(torch.float32).contiguous()
+        nb = self._check(u, (self.channels,) + self.n, torch.float32, "forward input")
         out = self.empty_spectrum(nb) if out is None else out
+        self._check(out, self.spec_shape[1:], torch.complex64, "forward output", nb)
         with torch.cuda.device(self.device):
             check(lib.tfsm_forward(self._plan, _ptr(u), _ptr(out), int(nb), _stream(self.device)), "forward")
         return out
 
     def inverse(self, uh: torch.Tensor, nbatch: Optional[int] = None) -> torch.Tensor:
         """Half spectrum (nb, C, ..., P) -> real (nb, C, N...) = ifftn(.).real."""
-        nb = uh.shape[0] if nbatch is None else nbatch
+        have = self._check(uh, self.spec_shape[1:], torch.complex64, "inverse input")
+        nb = have if nbatch is None else int(nbatch)
+        if not 1 <= nb <= have:
+            raise ValueError(f"nbatch {nb} outside 1..{have}")
         out = torch.empty((nb, self.channels) + self.n, dtype=torch.float32, device=self.device)
         sb = int(lib.tfsm_plan_inverse_scratch_bytes(self._plan, int(nb)))
         scratch = torch.empty(max(sb, 8), dtype=torch.uint8, device=self.device)
@@ -231,15 +267,19 @@ class SpectralPlan:
         return out
 
     def half_to_full(self, uh: torch.Tensor, nbatch: Optional[int] = None) -> torch.Tensor:
-        nb = uh.shape[0] if nbatch is None else nbatch
+        have = self._check(uh, self.spec_shape[1:], torch.complex64, "half spectrum")
+        nb = have if nbatch is None else int(nbatch)
+        if not 1 <= nb <= have:
+            raise ValueError(f"nbatch {nb} outside 1..{have}")
         out = torch.empty((nb, self.channels) + self.n, dtype=torch.complex64, device=self.device)
         with torch.cuda.device(self.device):
             check(lib.tfsm_half_to_full(self._plan, _ptr(uh), _ptr(out), int(nb), _stream(self.device)), "half_to_full")
         return out
 
     def full_to_half(self, full: torch.Tensor) -> torch.Tensor:
-        nb = full.shape[0]
+        require_gpu_tensor(full, "full spectrum")
         full = full.to(torch.complex64).contiguous()
+        nb = self._check(full, (self.channels,) + self.n, torch.complex64, "full spectrum")
         out = self.empty_spectrum(nb)
         with torch.cuda.device(self.device):
             check(lib.tfsm_full_to_half(self._plan, _ptr(full), _ptr(out), int(nb), _stream(self.device)), "full_to_half")
@@ -247,8 +287,9 @@ class SpectralPlan:
 
     def c2c(self, x: torch.Tensor, direction: int) -> torch.Tensor:
         """Complex (nb, C, N...) -> complex, over the last D axes (direction -1 forward, +1 inverse)."""
-        nb = x.shape[0]
+        require_gpu_tensor(x, "c2c input")
         x = x.to(torch.complex64).contiguous()
+        nb = self._check(x, (self.channels,) + self.n, torch.complex64, "c2c input")
         out = torch.empty_like(x)
         with torch.cuda.device(self.device):
             check(lib.tfsm_c2c(self._plan, _ptr(x), _ptr(out), int(nb), int(direction), _stream(self.device)), "c2c")
@@ -260,7 +301,20 @@ class SpectralPlan:
         """Advance uh in place by nsteps (optionally recording frames into traj [B][T][C][...][P])."""
         if self.scheme is None:
             raise RuntimeError("set_scheme must be called before stepping")
-        T = int(traj.shape[1]) if traj is not None else 0
+        self._check_state(uh)
+        if int(nsteps) < 0:
+            raise ValueError("nsteps must be >= 0")
+        if (rec_slot is None) != (traj is None):
+            raise ValueError("recording needs both the slot schedule and the trajectory buffer")
+        T = 0
+        if traj is not None:
+            if rec_slot.dim() != 1 or rec_slot.numel() < int(nsteps) + 1:
+                raise ValueError(f"record slots: one per step and one after the last ({int(nsteps) + 1})")
+            self._check(rec_slot, (), torch.int32, "record slots")
+            if traj.dim() < 3:
+                raise ValueError("trajectory buffer must be [B][T][C][...][P]")
+            T = int(traj.shape[1])
+            self._check(traj, (T,) + tuple(self.spec_shape[1:]), torch.complex64, "trajectory buffer", self.batch)
         ws = self.workspace()
         with torch.cuda.device(self.device):
             check(lib.tfsm_step(self._plan, _ptr(uh), int(nsteps), _ptr(rec_slot), _ptr(traj), T, _ptr(ws),
@@ -268,6 +322,8 @@ class SpectralPlan:
         return uh
 
     def step_varying(self, uh: torch.Tensor, nsteps_per_sample: torch.Tensor, max_steps: int):
+        self._check_state(uh)
+        self._check(nsteps_per_sample, (), torch.int32, "steps per sample", self.batch)
         ws = self.workspace()
         with torch.cuda.device(self.device):
             check(lib.tfsm_step_varying(self._plan, _ptr(uh), _ptr(nsteps_per_sample), int(max_steps), _ptr(ws),
@@ -276,6 +332,7 @@ class SpectralPlan:
 
     def apply(self, uh: torch.Tensor) -> torch.Tensor:
         """L uh + N(uh) (half spectra)."""
+        self._check_state(uh)
         out = torch.empty_like(uh)
         ws = self.workspace()
         with torch.cuda.device(self.device):
@@ -286,7 +343,11 @@ class SpectralPlan:
         """x + dt * sum_i coefs[i] ks[i] (half spectra), the reference's stage-sum order."""
         if len(ks) != len(coefs):
             raise ValueError("one coefficient per stage")
+        self._check_state(x, "stage base")
+        for k in ks:
+            self._check_state(k, "stage value")
         out = torch.empty_like(x) if out is None else out
+        self._check_state(out, "stage sum")
         ptrs = (ctypes.c_void_p * max(1, len(ks)))(*[k.data_ptr() for k in ks])
         cs = (ctypes.c_float * max(1, len(coefs)))(*[float(c) for c in coefs])
         with torch.cuda.device(self.device):
@@ -298,15 +359,21 @@ class SpectralPlan:
     def spectral_map(self, op: int, x: torch.Tensor, cout: int, coef: float = 1.0) -> torch.Tensor:
         """coef * map(x) for a diagnostic operator (tfsm_spectral_map); x and the result are half
         spectra of this plan's mesh and batch with the map's channel counts."""
+        cin, cmap = self._map_channels(int(op))
+        if cout != cmap:
+            raise ValueError(f"spectral map {op} gives {cmap} channels, not {cout}")
+        x = x.contiguous()
+        self._check(x, (cin,) + tuple(self.spec_shape[2:]), torch.complex64, "spectral map input", self.batch)
         out = torch.empty((self.batch, cout) + tuple(self.spec_shape[2:]), dtype=torch.complex64, device=self.device)
         with torch.cuda.device(self.device):
-            check(lib.tfsm_spectral_map(self._plan, int(op), float(coef), _ptr(x.contiguous()), _ptr(out),
+            check(lib.tfsm_spectral_map(self._plan, int(op), float(coef), _ptr(x), _ptr(out),
                                         _stream(self.device)), "spectral map")
         return out
 
     def pointwise_products(self, u: torch.Tensor) -> torch.Tensor:
         """u_i u_j of a physical field [B][C][N...] (tfsm_pointwise_products) -> [B][C*C][N...]."""
         u = u.contiguous()
+        self._check(u, (self.channels,) + self.n, torch.float32, "product input", self.batch)
         out = torch.empty((self.batch, self.channels * self.channels) + tuple(u.shape[2:]), dtype=torch.float32,
                           device=self.device)
         with torch.cuda.device(self.device):
@@ -315,6 +382,7 @@ class SpectralPlan:
 
     def solve(self, uh: torch.Tensor) -> torch.Tensor:
         """uh / L with 1/0 := 1 (half spectra)."""
+        self._check_state(uh)
         out = torch.empty_like(uh)
         with torch.cuda.device(self.device):
             check(lib.tfsm_solve(self._plan, _ptr(uh), _ptr(out), _stream(self.device)), "solve")
@@ -322,6 +390,7 @@ class SpectralPlan:
 
     def launch_pass(self, uh: torch.Tensor, pass_id: int, stage: int = 0):
         """One row (0) / column (1, stage code) / 1D step (2) pass, for kernel timing."""
+        self._check_state(uh)
         ws = self.workspace()
         with torch.cuda.device(self.device):
             check(lib.tfsm_launch_pass(self._plan, _ptr(uh), _ptr(ws), int(pass_id), int(stage),
@@ -339,6 +408,7 @@ class StepGraph:
         self.uh = uh
         self.nsteps = nsteps
         self._g = ctypes.c_void_p()
+        plan._check_state(uh)
         ws = plan.workspace()
         with torch.cuda.device(plan.device):
             check(lib.tfsm_graph_create(plan.handle, _ptr(uh), int(nsteps), _ptr(ws), _stream(plan.device),
