@@ -1214,6 +1214,8 @@ int tfsm_step(tfsm_plan* p, void* uh, int32_t nsteps, const int32_t* rec_slot, v
 int tfsm_step_varying(tfsm_plan* p, void* uh, const int32_t* nsteps_per_sample, int32_t max_steps, void* ws,
                       void* stream) {
   if (!p || !uh || !nsteps_per_sample || !ws) return fail(TFSM_E_INVALID, "null argument");
+  if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
+  if (max_steps < 0) return fail(TFSM_E_INVALID, "max_steps must be >= 0");
   if (p->D != 1 || p->scheme == ETDRK0) return fail(TFSM_E_INVALID, "per-sample step counts need a nonlinear 1D plan");
   if (p->term == TERM_KS) return fail(TFSM_E_INVALID, "per-sample step counts are not available for KSConvection");
   Run1Args a = run1_args(p, (float2*)uh, (char*)ws);
@@ -1475,6 +1477,7 @@ int tfsm_adjoint_step(tfsm_plan* p, void* u, const void* ubar_next, void* ubar, 
 int tfsm_graph_create(tfsm_plan* p, void* uh, int32_t nsteps, void* ws, void* stream, tfsm_graph** out) {
   if (!p || !uh || !out) return fail(TFSM_E_INVALID, "null argument");
   if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
+  if (nsteps < 1) return fail(TFSM_E_INVALID, "a graph needs nsteps >= 1");
   *out = nullptr;
   hipStream_t cap = nullptr;
   HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "capture stream");
