@@ -84,7 +84,7 @@ class NS2D:
         row = ("k2_row (512-point rows: 2 packed C2R, product u.grad w, R2C)", 0, ST_R4_E0, (4 * rho + 1) * A, 4 * per)
         cols = [(f"k2_col (512-point columns: fwd FFT, SETDRK4 combine, 4 inverse FFTs) e{i}", 1, st,
                  (r + 4 * rho) * A, per)
-                for i, (st, r) in enumerate([(ST_R4_E0, 3), (ST_R4_E1, 3), (ST_R4_E2, 5), (ST_R4_E3, 5)])]
+                for i, (st, r) in enumerate([(ST_R4_E0, 3), (ST_R4_E1, 3), (ST_R4_E2, 5), (ST_R4_E3, 3)])]
         return [row] + cols
 
     def prime(self):

@@ -33,8 +33,8 @@ enum Stage : int {
   ST_R3_E2,     // U' = E U + c3 A1 + c4 A2 + c5 N
   ST_R4_E0,     // A1 = N0;  s = E2 U + c1 N0                        (_setdrk_step.py:55-82)
   ST_R4_E1,     // A2 = Na;  s = E2 U + c1 Na          (coef_2 is coef_1, _uncached.py:194)
-  ST_R4_E2,     // a = E2 U + c1 A1; s = E2 a + c1 (2 N - A1); A2 = A2 + N
-  ST_R4_E3,     // U' = ((E U + c4 A1) + (2 c5) A2) + c6 N
+  ST_R4_E2,     // a = E2 U + c1 A1; s = E2 a + c1 (2 N - A1); A2 = (E U + c4 A1) + (2 c5) (A2 + N)
+  ST_R4_E3,     // U' = A2 + c6 N   (= ((E U + c4 N0) + (2 c5)(Na + Nb)) + c6 Nc, in that order)
   ST_EVAL,      // A1 = N (one nonlinear evaluation, no time step; Operator.__call__)
 };
 
@@ -154,14 +154,14 @@ __device__ __forceinline__ float2 stage_combine(int stage, float2 nval, float2* 
       const float e2 = C(CS_E2), c1 = C(CS_C1);
       float2 a = radd(rscale(e2, u), rscale(c1, n0));
       float2 two_nb = make_float2(2.0f * nval.x, 2.0f * nval.y);
-      if (write) A2[ei] = radd(na, nval);
+      if (write) {
+        const float c5x2 = 2.0f * C(CS_C5);
+        A2[ei] = radd(radd(rscale(C(CS_E), u), rscale(C(CS_C4), n0)), rscale(c5x2, radd(na, nval)));
+      }
       return radd(rscale(e2, a), rscale(c1, rsub(two_nb, n0)));
     }
     case ST_R4_E3: {
-      float2 u = U[ei], n0 = A1[ei], sab = A2[ei];
-      const float c5x2 = 2.0f * C(CS_C5);
-      float2 un = radd(radd(radd(rscale(C(CS_E), u), rscale(C(CS_C4), n0)), rscale(c5x2, sab)),
-                       rscale(C(CS_C6), nval));
+      float2 un = radd(A2[ei], rscale(C(CS_C6), nval));
       if (write) U[ei] = un;
       return un;
     }
@@ -219,17 +219,16 @@ __device__ __forceinline__ float2 stage_combine_c(int stage, float2 nval, float2
       A2[ei] = nval;
       return radd(cmulr(C(CS_E2), U[ei]), cmulr(C(CS_C1), nval));
     case ST_R4_E2: {
-      const float2 e2 = C(CS_E2), c1 = C(CS_C1), n0 = A1[ei], na = A2[ei];
-      const float2 a = radd(cmulr(e2, U[ei]), cmulr(c1, n0));
+      const float2 e2 = C(CS_E2), c1 = C(CS_C1), u = U[ei], n0 = A1[ei], na = A2[ei];
+      const float2 a = radd(cmulr(e2, u), cmulr(c1, n0));
       const float2 two_nb = make_float2(2.0f * nval.x, 2.0f * nval.y);
-      A2[ei] = radd(na, nval);
+      const float2 c5 = C(CS_C5);
+      const float2 c5x2 = make_float2(2.0f * c5.x, 2.0f * c5.y);
+      A2[ei] = radd(radd(cmulr(C(CS_E), u), cmulr(C(CS_C4), n0)), cmulr(c5x2, radd(na, nval)));
       return radd(cmulr(e2, a), cmulr(c1, rsub(two_nb, n0)));
     }
     case ST_R4_E3: {
-      const float2 c5 = C(CS_C5);
-      const float2 c5x2 = make_float2(2.0f * c5.x, 2.0f * c5.y);
-      const float2 un = radd(radd(radd(cmulr(C(CS_E), U[ei]), cmulr(C(CS_C4), A1[ei])), cmulr(c5x2, A2[ei])),
-                             cmulr(C(CS_C6), nval));
+      const float2 un = radd(A2[ei], cmulr(C(CS_C6), nval));
       U[ei] = un;
       return un;
     }
@@ -257,8 +256,10 @@ __host__ __device__ inline int stage_coef_half(int stage) {
   return (stage == ST_R3_E2 || stage == ST_R4_E3) ? 1 : 0;
 }
 __host__ __device__ inline bool stage_reads_a1(int stage) {
-  return stage == ST_R2_E1 || stage == ST_R3_E1 || stage == ST_R3_E2 || stage == ST_R4_E2 || stage == ST_R4_E3;
+  return stage == ST_R2_E1 || stage == ST_R3_E1 || stage == ST_R3_E2 || stage == ST_R4_E2;
 }
+// the state U: every stage but the last of SETDRK4 (whose partial sum is in A2) and ST_EVAL
+__host__ __device__ inline bool stage_reads_u(int stage) { return stage != ST_R4_E3 && stage != ST_EVAL; }
 __host__ __device__ inline bool stage_reads_a2(int stage) {
   return stage == ST_R3_E2 || stage == ST_R4_E2 || stage == ST_R4_E3;
 }
@@ -274,7 +275,7 @@ __host__ __device__ inline int stage_target(int stage) {
 // Stage combine on values already in registers (same arithmetic and rounding as stage_combine).
 // Returns s; *w is the value the stage stores to its target array (stage_target).
 __device__ __forceinline__ float2 stage_combine_v(int stage, float2 nval, float2 u, float2 a1, float2 a2, float4 c,
-                                                  float2* w) {
+                                                  float c5, float2* w) {
   switch (stage) {
     case ST_R1_E0: {  // c = [E, c1, ...]
       float2 un = radd(rscale(c.x, u), rscale(c.y, nval));
@@ -307,15 +308,14 @@ __device__ __forceinline__ float2 stage_combine_v(int stage, float2 nval, float2
     case ST_R4_E1:
       *w = nval;
       return radd(rscale(c.x, u), rscale(c.y, nval));
-    case ST_R4_E2: {
+    case ST_R4_E2: {  // half 0: [E2, c1, E, c4]; c5 from half 1
       float2 a = radd(rscale(c.x, u), rscale(c.y, a1));
       float2 two_nb = make_float2(2.0f * nval.x, 2.0f * nval.y);
-      *w = radd(a2, nval);
+      *w = radd(radd(rscale(c.z, u), rscale(c.w, a1)), rscale(2.0f * c5, radd(a2, nval)));
       return radd(rscale(c.x, a), rscale(c.y, rsub(two_nb, a1)));
     }
     case ST_R4_E3: {  // half 1: [E, c4, c5, c6]
-      const float c5x2 = 2.0f * c.z;
-      float2 un = radd(radd(radd(rscale(c.x, u), rscale(c.y, a1)), rscale(c5x2, a2)), rscale(c.w, nval));
+      float2 un = radd(a2, rscale(c.w, nval));
       *w = un;
       return un;
     }
@@ -344,7 +344,8 @@ __device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1,
                                                float2 (&s)[RMAX]) {
   static_assert(RMAX % BM == 0, "combine batch must divide the values per thread");
   const float2 z = make_float2(0.f, 0.f);
-  const bool rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
+  const bool rdu = stage_reads_u(stage), rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
+  const bool rd5 = stage == ST_R4_E2;  // c5 of the other coefficient half (16 entries further)
   const int tgt = stage_target(stage);
   float2* dst = tgt == 0 ? U : (tgt == 1 ? A1 : A2);
   const unsigned o0 = (unsigned)j * stride, ostep = (unsigned)NT * stride;
@@ -352,14 +353,24 @@ __device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1,
   for (int m0 = 0; m0 < RMAX; m0 += BM) {
     float2 uu[BM], q1[BM], q2[BM], sv[BM];
     float4 cc[BM];
+    float c5[BM];
 #pragma unroll
     for (int i = 0; i < BM; ++i) {
       const unsigned o = o0 + (unsigned)(m0 + i) * ostep;
-      uu[i] = U[o];
       cc[i] = cpk[2 * o];
+      uu[i] = z;
       q1[i] = z;
       q2[i] = z;
       sv[i] = z;
+      c5[i] = 0.f;
+    }
+    if (rdu) {
+#pragma unroll
+      for (int i = 0; i < BM; ++i) uu[i] = U[o0 + (unsigned)(m0 + i) * ostep];
+    }
+    if (rd5) {
+#pragma unroll
+      for (int i = 0; i < BM; ++i) c5[i] = cpk[2 * (o0 + (unsigned)(m0 + i) * ostep) + 16].z;
     }
     if (rd1) {
 #pragma unroll
@@ -379,7 +390,7 @@ __device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1,
       float2 nv = rscale(coef_t, v[m]);
       if (src) nv = radd(nv, sv[i]);
       float2 w;
-      const float2 r = stage_combine_v(stage, nv, uu[i], q1[i], q2[i], cc[i], &w);
+      const float2 r = stage_combine_v(stage, nv, uu[i], q1[i], q2[i], cc[i], c5[i], &w);
       s[m] = colv ? r : z;
       if (colv) dst[o0 + (unsigned)m * ostep] = w;
     }

@@ -316,8 +316,8 @@ __global__ void k_pack_coefs(CoefArgs a) {
         h0 = make_float4(S(CS_E2), S(CS_C1), S(CS_E), S(CS_C2));
         h1 = make_float4(S(CS_E), S(CS_C3), S(CS_C4), S(CS_C5));
         break;
-      case SETDRK4:
-        h0 = make_float4(S(CS_E2), S(CS_C1), 0.f, 0.f);
+      case SETDRK4:  // the third evaluation forms the partial sum (E U + c4 N0) + 2 c5 (Na + Nb) too
+        h0 = make_float4(S(CS_E2), S(CS_C1), S(CS_E), S(CS_C4));
         h1 = make_float4(S(CS_E), S(CS_C4), S(CS_C5), S(CS_C6));
         break;
       default: break;

@@ -66,6 +66,8 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
   // cpk_index(coff + xo, half) = 2 (coff + xo) + cko: (coff + xo) & 15 == kz & 15
   const int cko = (stage_coef_half(stage) << 4) - (kz & 15);
   constexpr int NC3 = TERM == TERM_KS ? 1 : 3;  // channels
+  // combine batch: the other channels' values stay live, so fewer modes per batch than in 2D
+  constexpr int CBM = RMAX >= 16 ? 1 : (RMAX >= 8 ? 2 : 4);
   float2 s[NC3][RMAX];
   if (stage == ST_PRO) {
 #pragma unroll
@@ -103,7 +105,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
     for (int c = 0; c < 3; ++c) {
       const size_t base = ((size_t)b * 3 + c) * plane + coff;
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride + coff : nullptr;
-      combine_column<RMAX, NT>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko), a.coef_t,
+      combine_column<RMAX, NT, CBM>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko), a.coef_t,
                                (unsigned)inner, j, colv, v3[c], s[c]);
     }
   } else {
@@ -122,7 +124,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
         if (a.dc && ky == 0 && kz == 0 && j == 0) v[0] = rsub(v[0], a.dc[0]);
       }
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride + coff : nullptr;
-      combine_column<RMAX, NT>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko), a.coef_t,
+      combine_column<RMAX, NT, CBM>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko), a.coef_t,
                                (unsigned)inner, j, colv, v, s[c]);
     }
   }
