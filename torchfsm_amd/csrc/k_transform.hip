@@ -258,13 +258,23 @@ __global__ void k_record(const float2* __restrict__ U, float2* __restrict__ traj
 // Convection (C = D = 1): N = coef_t * fft(u * u_x) [+ S], u from the dealiased stage spectrum;
 // ConservativeConvection: N = coef_t * (i a) fft(u * u) [+ S].
 // CPLX: complex-L plan (complex coefficient tables, stage_combine_c).
+// The state U, the stage arrays A1, A2 and (real plans) the per-mode coefficient table stay in LDS for
+// the whole run when they fit (LSTATE, LCOEF): the stage combine then waits on LDS instead of
+// on one memory round trip per mode, which at small N is most of an evaluation.  Each lane only
+// ever touches its own modes k = j + m NT of these copies, so they need no barriers of their own.
 template <int N, int V, bool CPLX>
 __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
   using F = Fft<N>;
   constexpr int RMAX = F::RMAX, NT = F::NT, H = N / 2 + 1;
+  constexpr bool LSTATE = (size_t)V * H * 3 * sizeof(float2) <= 16384;
+  constexpr bool LCOEF = !CPLX && (size_t)V * H * NCOEF * sizeof(float) <= 20480;
   __shared__ float2 s_tw[Fft<N>::tw_size()];
   __shared__ float2 s_buf[LdsSize<N, V>::slots];
   __shared__ float2 s_stage[V * H];
+  __shared__ float2 s_st[LSTATE ? 3 * V * H : 1];
+  __shared__ float s_cf[LCOEF ? V * NCOEF * H : 1];
+  constexpr bool LAX = (size_t)V * H * sizeof(float) <= 8192;  // wavenumbers k = 0..N/2 in LDS too
+  __shared__ float s_ax[LAX ? V * H : 1];
   load_twiddles<N>(s_tw, a.tw);
   const int vi = threadIdx.x % V, j = threadIdx.x / V;
   const int smp = blockIdx.x * V + vi;
@@ -288,6 +298,32 @@ __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
   const float2* src = a.src ? a.src + (size_t)(valid ? smp : 0) * a.src_bstride : nullptr;
   float2* stg = s_stage + vi * H;
   for (int k = j; k < H; k += NT) stg[k] = valid ? U[k] : make_float2(0.f, 0.f);
+  // working copies of U, A1, A2 (written back at the end: U after stepping, A1 after an
+  // evaluation-only call)
+  float2* Uw = U;
+  float2* A1w = A1;
+  float2* A2w = A2;
+  if constexpr (LSTATE) {
+    Uw = s_st + vi * H;
+    A1w = s_st + (V + vi) * H;
+    A2w = s_st + (2 * V + vi) * H;
+    for (int k = j; k < H; k += NT) Uw[k] = valid ? U[k] : make_float2(0.f, 0.f);
+  }
+  const float* axw = ax;
+  if constexpr (LAX) {
+    float* t = s_ax + (size_t)vi * H;
+    for (int k = j; k < H; k += NT) t[k] = ax[k];
+    axw = t;
+  }
+  const float* cfw = cf;
+  size_t cstride = a.P;
+  if constexpr (LCOEF) {
+    float* t = s_cf + (size_t)vi * NCOEF * H;
+    for (int sl = 0; sl < NCOEF; ++sl)
+      for (int k = j; k < H; k += NT) t[sl * H + k] = cf[(size_t)sl * a.P + k];
+    cfw = t;
+    cstride = H;
+  }
   __syncthreads();
   const int nev = a.eval_only ? 1 : scheme_evals(a.scheme);
   const int nloop = a.eval_only ? 1 : nmax;
@@ -296,7 +332,7 @@ __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
     if (a.rec_slot && active) {
       const int sl = a.rec_slot[step];
       if (sl >= 0)
-        for (int k = j; k < H; k += NT) a.traj[((size_t)smp * a.T + sl) * a.P + k] = U[k];
+        for (int k = j; k < H; k += NT) a.traj[((size_t)smp * a.T + sl) * a.P + k] = Uw[k];
     }
     for (int e = 0; e < nev; ++e) {
       const int stage = a.eval_only ? ST_EVAL : scheme_stage(a.scheme, e);
@@ -312,7 +348,7 @@ __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
         if (k <= a.kmax) {
           val = stg[k];
           val = make_float2(val.x * a.scale, val.y * a.scale);
-          axk = a.cons ? 0.f : ax[k];  // u alone: u_x in the imaginary part would add its round-off to u
+          axk = a.cons ? 0.f : axw[k];  // u alone: u_x in the imaginary part would add its round-off to u
         }
         float2 A = val;
         float2 Bv = make_float2(-axk * val.y, axk * val.x);  // i a_k u^
@@ -331,7 +367,7 @@ __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
 #pragma unroll
         for (int m = 0; m < RMAX; ++m) {
           const int pos = j + m * NT;
-          const float ap = pos <= N / 2 ? ax[pos] : 0.f;
+          const float ap = pos <= N / 2 ? axw[pos] : 0.f;
           v[m] = make_float2(-mulr(ap, v[m].y), mulr(ap, v[m].x));  // (i a) fft(u u)
         }
       }
@@ -342,8 +378,8 @@ __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
           float2 sn = make_float2(0.f, 0.f);
           if (active || (a.eval_only && valid)) {
             float2 nv = nl_value(a.coef_t, v[m], src, pos);
-            if constexpr (CPLX) sn = stage_combine_c(stage, nv, U, A1, A2, pos, cfc, a.P, pos);
-            else sn = stage_combine(stage, nv, U, A1, A2, pos, cf, a.P, pos, true);
+            if constexpr (CPLX) sn = stage_combine_c(stage, nv, Uw, A1w, A2w, pos, cfc, a.P, pos);
+            else sn = stage_combine(stage, nv, Uw, A1w, A2w, pos, cfw, cstride, pos, true);
           }
           stg[pos] = sn;
         }
@@ -354,7 +390,13 @@ __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
   if (a.rec_slot && valid && !a.eval_only) {
     const int sl = a.rec_slot[nst];
     if (sl >= 0)
-      for (int k = j; k < H; k += NT) a.traj[((size_t)smp * a.T + sl) * a.P + k] = U[k];
+      for (int k = j; k < H; k += NT) a.traj[((size_t)smp * a.T + sl) * a.P + k] = Uw[k];
+  }
+  if (LSTATE && valid) {
+    if (a.eval_only)
+      for (int k = j; k < H; k += NT) A1[k] = A1w[k];
+    else
+      for (int k = j; k < H; k += NT) U[k] = Uw[k];
   }
 }
 
