@@ -368,10 +368,6 @@ __device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1,
 #pragma unroll
       for (int i = 0; i < BM; ++i) uu[i] = U[o0 + (unsigned)(m0 + i) * ostep];
     }
-    if (rd5) {
-#pragma unroll
-      for (int i = 0; i < BM; ++i) c5[i] = cpk[2 * (o0 + (unsigned)(m0 + i) * ostep) + 16].z;
-    }
     if (rd1) {
 #pragma unroll
       for (int i = 0; i < BM; ++i) q1[i] = A1[o0 + (unsigned)(m0 + i) * ostep];
@@ -383,6 +379,12 @@ __device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1,
     if (src) {
 #pragma unroll
       for (int i = 0; i < BM; ++i) sv[i] = src[o0 + (unsigned)(m0 + i) * ostep];
+    }
+    // last: the compiler forms 2 c5 right after this load, and the wait for it must not hold back
+    // the loads above
+    if (rd5) {
+#pragma unroll
+      for (int i = 0; i < BM; ++i) c5[i] = cpk[2 * (o0 + (unsigned)(m0 + i) * ostep) + 16].z;
     }
 #pragma unroll
     for (int i = 0; i < BM; ++i) {
