@@ -104,6 +104,7 @@ struct tfsm_plan {
   float* L = nullptr;
   float* coef = nullptr;
   float4* cpk = nullptr;  // packed per-mode coefficients of the current scheme
+  float* c5t = nullptr;   // SETDRK4's c5 per mode (read alone by the third evaluation)
   int scheme = -1;
   double dt = 0.0;
   int n_points = 16;
@@ -281,6 +282,7 @@ Col2Args col_args(tfsm_plan* p, float2* U, char* ws) {
   c.coef_bstride = p->coef_bstride();
   c.cpk = p->cpk;
   c.cpk_bstride = p->Bc > 1 ? 2 * p->modes : 0;
+  c.c5t = p->c5t;
   c.src = p->src;
   c.src_bstride = p->src_bstride();
   c.src_cstride = p->src_cstride();
@@ -345,6 +347,7 @@ Col3Args col3_args(tfsm_plan* p, float2* U, char* ws, int b0, int nb) {
   c.A2 = (float2*)(ws + p->off_A2);
   c.cpk = p->cpk;
   c.cpk_bstride = p->Bc > 1 ? 2 * p->modes : 0;
+  c.c5t = p->c5t;
   c.src = p->src;
   c.src_bstride = p->src_bstride();
   c.src_cstride = p->src_cstride();
@@ -783,6 +786,7 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   if (!rc) rc = dev_alloc(p, &p->L, (size_t)Lb * p->modes, "linear symbol");
   if (!rc) rc = dev_alloc(p, &p->coef, (size_t)Lb * NCOEF * p->modes, "coefficient table");
   if (!rc) rc = dev_alloc(p, &p->cpk, (size_t)Lb * 2 * p->modes, "packed coefficient table");
+  if (!rc) rc = dev_alloc(p, &p->c5t, (size_t)Lb * p->modes, "c5 table");
   if (!rc && cplx) rc = dev_alloc(p, &p->Lc, (size_t)Lb * p->modes, "complex linear symbol");
   if (!rc && cplx) rc = dev_alloc(p, &p->coefc, (size_t)Lb * NCOEF * p->modes, "complex coefficient table");
   if (!rc) {
@@ -975,6 +979,7 @@ int tfsm_plan_set_scheme(tfsm_plan* plan, int32_t scheme, double dt, int32_t n_p
   a.L = plan->L;
   a.coef = plan->coef;
   a.cpk = plan->cpk;
+  a.c5t = plan->c5t;
   a.Lc = plan->Lc;
   a.coefc = plan->coefc;
   HIP_TRY(launch_coefs(a, (hipStream_t)stream), "coefficient launch");
@@ -1038,6 +1043,7 @@ int tfsm_plan_set_sample(tfsm_plan* p, int32_t b, const float* linear_coef, doub
       a.L = p->L + (size_t)r * p->modes;
       a.coef = p->coef + (size_t)r * NCOEF * p->modes;
       a.cpk = p->cpk + (size_t)r * 2 * p->modes;
+      a.c5t = p->c5t + (size_t)r * p->modes;
       a.Lc = p->cplx ? p->Lc + (size_t)r * p->modes : nullptr;
       a.coefc = p->cplx ? p->coefc + (size_t)r * NCOEF * p->modes : nullptr;
       HIP_TRY(launch_coefs(a, st), "coefficient launch");

@@ -331,7 +331,7 @@ __device__ __forceinline__ float2 stage_combine_v(int stage, float2 nval, float2
 // Stage combine of one channel's column of RMAX modes per lane into s, storing the stage target
 // (the column kernels, 2D and 3D).  Mode m of lane j is element (j + m NT) * stride of the column
 // whose first element U, A1, A2, src point at; cpk points at the packed coefficient pair of that
-// first element, so mode m's is cpk[2 (j + m NT) stride].  Operands are loaded in batches of BM
+// first element, so mode m's is cpk[2 (j + m NT) stride]; that element's c5 is c5b[c5o].  Operands are loaded in batches of BM
 // modes and every load of a batch is issued before its first use, so a batch costs one memory
 // latency instead of one per operand.  BM defaults to 4 for plans of up to 12 values per thread and
 // to 1 above, where the transforms already hold most of the 128 VGPRs that keep two 512-thread
@@ -339,13 +339,14 @@ __device__ __forceinline__ float2 stage_combine_v(int stage, float2 nval, float2
 // columns of the same rows, which every array and table holds, and its result is discarded.
 // Offsets within one channel are 32-bit (every supported mesh has fewer than 2^28 modes).
 template <int RMAX, int NT, int BM = (RMAX >= 16 ? 1 : 4)>
-__device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1, float2* A2, const float2* src, const float4* cpk, float coef_t,
+__device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1, float2* A2, const float2* src,
+                                               const float4* cpk, const float* c5b, unsigned c5o, float coef_t,
                                                unsigned stride, int j, bool colv, const float2 (&v)[RMAX],
                                                float2 (&s)[RMAX]) {
   static_assert(RMAX % BM == 0, "combine batch must divide the values per thread");
   const float2 z = make_float2(0.f, 0.f);
   const bool rdu = stage_reads_u(stage), rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
-  const bool rd5 = stage == ST_R4_E2;  // c5 of the other coefficient half (16 entries further)
+  const bool rd5 = stage == ST_R4_E2;  // c5, from its own table
   const int tgt = stage_target(stage);
   float2* dst = tgt == 0 ? U : (tgt == 1 ? A1 : A2);
   const unsigned o0 = (unsigned)j * stride, ostep = (unsigned)NT * stride;
@@ -384,7 +385,7 @@ __device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1,
     // the loads above
     if (rd5) {
 #pragma unroll
-      for (int i = 0; i < BM; ++i) c5[i] = cpk[2 * (o0 + (unsigned)(m0 + i) * ostep) + 16].z;
+      for (int i = 0; i < BM; ++i) c5[i] = c5b[c5o + o0 + (unsigned)(m0 + i) * ostep];
     }
 #pragma unroll
     for (int i = 0; i < BM; ++i) {
@@ -441,6 +442,7 @@ struct Col2Args {
   size_t coef_bstride;        // 0 or NCOEF*N0*P1
   const float4* cpk;          // packed coefficients [Bc][N0*P1][2]
   size_t cpk_bstride;         // 0 or 2*N0*P1 (float4 units)
+  const float* c5t;           // SETDRK4's c5 [Bc][N0*P1] (batch stride cpk_bstride / 2)
   const float2* src;          // [Bs][Cs][N0][P1] or null
   size_t src_bstride, src_cstride;
   float2* G;                  // fields [B][NF][N0][PF]
@@ -474,6 +476,7 @@ struct Col3Args {
   float2* A2;
   const float4* cpk;
   size_t cpk_bstride;
+  const float* c5t;           // SETDRK4's c5 [Bc][modes] (batch stride cpk_bstride / 2)
   const float2* src;
   size_t src_bstride, src_cstride;
   float2* G1;                 // [B][2 nc][N0][K1][PF2]
@@ -547,6 +550,7 @@ struct LinArgs {
 
 struct CoefArgs {
   float4* cpk;             // packed coefficients [Bc][modes][2], written after the slot tables
+  float* c5t;              // SETDRK4's c5 alone [Bc][modes] (the third evaluation's fifth coefficient)
   int scheme;
   double dt;
   float dt32, hdt32;       // fl32(dt), fl32(0.5 * dt)

@@ -324,6 +324,7 @@ __global__ void k_pack_coefs(CoefArgs a) {
     }
     a.cpk[cpk_index(e, 0)] = h0;
     a.cpk[cpk_index(e, 1)] = h1;
+    if (a.scheme == SETDRK4) a.c5t[e] = S(CS_C5);
   }
 }
 

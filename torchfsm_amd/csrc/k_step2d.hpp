@@ -58,13 +58,14 @@ struct Term2<TERM_CONS> {  // ConservativeConvection, C = 2: fields u0, u1; prod
 // one mode's operands at a time (BM = 1) to stay within 128 VGPRs.
 template <int N0, int VAR, int BM>
 __device__ __forceinline__ void combine_column2(const Col2Args& a, int stage, size_t base, const float2* src,
-                                                const float4* cpk, int cko, int ky, bool colv, int j,
+                                                const float4* cpk, const float* c5b, int cko, int ky, bool colv, int j,
                                                 const float2 (&v)[Fft<N0, VAR>::RMAX],
                                                 float2 (&s)[Fft<N0, VAR>::RMAX]) {
   using F = Fft<N0, VAR>;
   const size_t cb = base + ky;
   combine_column<F::RMAX, F::NT, BM>(stage, a.U + cb, a.A1 + cb, a.A2 + cb, src ? src + ky : nullptr,
-                                 cpk + (2 * ky + cko), a.coef_t, (unsigned)a.P1, j, colv, v, s);
+                                     cpk + (2 * ky + cko), c5b, (unsigned)ky, a.coef_t, (unsigned)a.P1, j, colv, v,
+                                     s);
 }
 
 // Velocity-form Navier-Stokes (NSPressureConvection): both channels' convection spectra are
@@ -107,12 +108,13 @@ __device__ __forceinline__ void col_nsp_combine(const Col2Args& a, int stage, fl
     v[1][m] = cv[1];
   }
   const float4* cpk = a.cpk + (size_t)b * a.cpk_bstride;
+  const float* c5b = a.c5t + (size_t)b * (a.cpk_bstride / 2);
   // cpk_index(ci, half) = 2 ci + cko: every mode of this lane has ci & 15 == ky & 15
   const int cko = (stage_coef_half(stage) << 4) - (ky & 15);
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride : nullptr;
-    combine_column2<N0, VAR, 1>(a, stage, ((size_t)b * 2 + c) * plane, src, cpk, cko, ky, colv, j, v[c], s[c]);
+    combine_column2<N0, VAR, 1>(a, stage, ((size_t)b * 2 + c) * plane, src, cpk, c5b, cko, ky, colv, j, v[c], s[c]);
   }
 }
 
@@ -147,6 +149,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
   const bool colv = ky < a.H1;
   const size_t plane = (size_t)N0 * a.P1;
   const float4* cpk = a.cpk + (size_t)b * a.cpk_bstride;
+  const float* c5b = a.c5t + (size_t)b * (a.cpk_bstride / 2);
   // cpk_index(ci, half) = 2 ci + cko: every mode of this lane has ci & 15 == ky & 15
   const int cko = (stage_coef_half(stage) << 4) - (ky & 15);
   float2 s[NC][RMAX];
@@ -200,7 +203,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
         if (a.dc && ky == 0 && j == 0) v[0] = rsub(v[0], a.dc[0]);
       }
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride : nullptr;
-      combine_column2<N0, VAR, (NC > 1 || RMAX >= 16) ? 1 : 4>(a, stage, base, src, cpk, cko, ky, colv, j, v, s[c]);
+      combine_column2<N0, VAR, (NC > 1 || RMAX >= 16) ? 1 : 4>(a, stage, base, src, cpk, c5b, cko, ky, colv, j, v, s[c]);
     }
   }
   }

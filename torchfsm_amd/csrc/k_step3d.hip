@@ -105,7 +105,8 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
     for (int c = 0; c < 3; ++c) {
       const size_t base = ((size_t)b * 3 + c) * plane + coff;
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride + coff : nullptr;
-      combine_column<RMAX, NT, CBM>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko), a.coef_t,
+      combine_column<RMAX, NT, CBM>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko),
+                               a.c5t + (size_t)b * (a.cpk_bstride / 2), (unsigned)coff, a.coef_t,
                                (unsigned)inner, j, colv, v3[c], s[c]);
     }
   } else {
@@ -124,7 +125,8 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
         if (a.dc && ky == 0 && kz == 0 && j == 0) v[0] = rsub(v[0], a.dc[0]);
       }
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride + coff : nullptr;
-      combine_column<RMAX, NT, CBM>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko), a.coef_t,
+      combine_column<RMAX, NT, CBM>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko),
+                               a.c5t + (size_t)b * (a.cpk_bstride / 2), (unsigned)coff, a.coef_t,
                                (unsigned)inner, j, colv, v, s[c]);
     }
   }
