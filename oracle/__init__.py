@@ -1,7 +1,7 @@
 """Oracle for the ETD time-stepping path -- TEST INFRASTRUCTURE ONLY.
 
-Only ``tests/``, ``__graft_entry__.smoke()`` and the baseline legs of ``bench.py`` may import this
-package, and only as the checker (or as the timed reference baseline).  The product package
+Only ``tests/``, the smoke check of the entry module and the baseline legs of ``bench.py`` may
+import this package, and only to check results against (or as the timed reference baseline).  The product package
 ``torchfsm_amd`` never imports it: it fails loudly when its HIP library is missing.
 
 Modules:

@@ -2,8 +2,8 @@
 
 This module reproduces what the reference computes, in the reference's order and dtypes, so that
 on the same CPU it matches the reference's float32 output bit for bit (checked against the golden
-fixtures in tests/golden).  It is the checker for the HIP engine, the torch-GPU baseline and the CPU
-baseline; it is never imported by the product package.
+fixtures in tests/golden).  The HIP engine's results are checked against it, and it is the
+torch-GPU and CPU baseline; it is never imported by the product package.
 
 An operator is described by a list of terms in generator order, each ``(coef, kind, payload)``:
   kind "lap"     Laplacian symbol, concatenated over channels   (_laplacian.py:12-15)
