@@ -45,6 +45,7 @@ class NS2D:
     name = "ns2d_kolmogorov_512x512_b64_setdrk4"
     N, B, C, RE, DT, SCHEME = 512, 64, 1, 1000.0, 1e-3, "SETDRK4"
     default_steps = 1000
+    cpu_steps = 15  # about 10 s of the oracle on the host cores
     traffic_file = "k2_col_ns512_b64_traffic.txt"
 
     def __init__(self, dev):
@@ -125,6 +126,7 @@ class Burgers3D:
     name = "burgers3d_128cubed_c3_b8_etdrk2"
     N, B, C, NU, DT, SCHEME = 128, 8, 3, 0.01, 5e-3, "ETDRK2"
     default_steps = 200
+    cpu_steps = 100  # one sample: about 10 s of the oracle on the host cores
     traffic_file = "k3_burgers128_b8_traffic.txt"
     data = "synthetic: per (b, c) 8 seeded cosines, integer |k| in [1, 4], a ~ U(-1,1), phase ~ U(0, 2pi)"
 
@@ -310,7 +312,7 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="ns2d")
     ap.add_argument("--steps", type=int, default=0, help="timed steps (0 = the workload's default)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=0, help="timed oracle steps on the host (0 = the workload's default)")
     ap.add_argument("--torch-steps", type=int, default=20)
     ap.add_argument("--no-baselines", action="store_true", help="engine and kernel legs only")
     ap.add_argument("--tile", type=int, default=0, help="column tile (8/16); 0 = the plan's default launch shape")
@@ -346,6 +348,7 @@ def main():
         log("[bench] copy probe: 3 x 256 MiB device-to-device copies done")
     W = WORKLOADS[args.workload]
     steps = args.steps or W.default_steps
+    args.cpu_steps = args.cpu_steps or W.cpu_steps
     log(f"[bench] {W.name}: building input and plan")
     w = W(dev)
     plan, uh = w.plan, w.uh
