@@ -159,8 +159,10 @@ def test_every_scheme_1d_2d_3d(dev, scheme):
         assert e <= TOL
 
 
-def test_per_sample_viscosity_2d(dev):
-    """Burgers with a [B,1,1,1] viscosity tensor: each sample matches its own scalar-viscosity run."""
+@pytest.mark.parametrize("dims,n", [(2, 64), (3, 32)])
+def test_per_sample_viscosity(dev, dims, n):
+    """Burgers with a [B,1,...] viscosity tensor (per-sample coefficient tables, SETDRK4): each
+    sample matches its own scalar-viscosity run."""
     import torchfsm_amd as tfa
     from oracle import refpath
     from torchfsm_amd import pde
@@ -168,10 +170,10 @@ def test_per_sample_viscosity_2d(dev):
     from torchfsm_amd.workloads import fourier_modes_field
 
     nus = [0.005, 0.01, 0.03]
-    u = 0.5 * fourier_modes_field(64, 2, 2, 3, seed=50)
-    op = pde.Burgers(torch.tensor(nus).reshape(3, 1, 1, 1))
+    u = 0.5 * fourier_modes_field(n, dims, dims, 3, seed=50)
+    op = pde.Burgers(torch.tensor(nus).reshape((3,) + (1,) * (dims + 1)))
     op.set_integrator(SETDRKIntegrator.SETDRK4)
-    mi = [(0, 1, 64)] * 2
+    mi = [(0, 1, n)] * dims
     out = op.integrate(u.to(dev), mesh=tfa.MeshGrid(mi, device=dev), dt=2e-3, step=15)
     for b, nu in enumerate(nus):
         ref = refpath.integrate(refpath.burgers_terms(nu), mi, u[b:b + 1], 2e-3, 15, scheme="SETDRK4")
