@@ -864,10 +864,13 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     p->ws_bytes += 256;
   }
   // default launch shape (the autotuner may refine it): 2D columns of 512 points run the
-  // half-thread radix plan; temporal blocking uses the smallest sub-batch (multiple of 8) that
-  // still gives >= 512 column workgroups (two per CU) while its working set stays under 256 MB
+  // half-thread radix plan for one-channel products (the two-channel ones keep a second column in
+  // registers and run faster with 8 values per thread); temporal blocking uses the smallest
+  // sub-batch (multiple of 8) that still gives >= 512 column workgroups (two per CU) while its
+  // working set stays under 256 MB
   if (!rc && D == 2) {
-    p->col_var = (p->n[0] == 512) ? 1 : 0;
+    const bool one_channel = term == TERM_VC || term == TERM_KS;
+    p->col_var = (p->n[0] == 512 && one_channel) ? 1 : 0;
     const size_t per_sample = 4 * (size_t)p->C * p->modes * sizeof(float2) +
                               (size_t)p->NF * p->n[0] * p->PF * sizeof(float2);
     const int tiles = (p->H + p->V - 1) / p->V;
