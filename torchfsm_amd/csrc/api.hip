@@ -870,7 +870,13 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   // working set stays under 256 MB
   if (!rc && D == 2) {
     const bool one_channel = term == TERM_VC || term == TERM_KS;
-    p->col_var = (p->n[0] == 512 && one_channel) ? 1 : 0;
+    const int n0 = p->n[0], n1 = p->n[1];
+    // measured per axis size (SETDRK4, B = 16..32): 512 keeps 16-column tiles; 256 runs 8-column
+    // tiles with the 8-values-per-thread plan (26 % faster than 16 columns of 16 values); 64 and
+    // 128 run 8-column tiles and one row per 64-thread row group (5-9 %)
+    p->col_var = (n0 == 512 && one_channel) || n0 == 256 ? 1 : 0;
+    if (n0 == 64 || n0 == 128 || n0 == 256) p->V = 8;
+    if (n1 == 64 || n1 == 128) p->row_wg = 64;
     const size_t per_sample = 4 * (size_t)p->C * p->modes * sizeof(float2) +
                               (size_t)p->NF * p->n[0] * p->PF * sizeof(float2);
     const int tiles = (p->H + p->V - 1) / p->V;
