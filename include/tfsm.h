@@ -121,8 +121,10 @@ int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int
                          int32_t sub_batch);
 
 /* Current launch shape: out[6] = column tile, column variant, row variant, row workgroup,
- * sub-batch, streams.  A new plan starts from a default derived from its shape (2D: sub-batch
- * sized to keep >= 512 column workgroups per launch; 3D: radix-4 z rows from 64 points).
+ * sub-batch, streams.  A new plan starts from a default derived from its shape (2D: column tile
+ * and radix variant per axis size and product term, 64-thread row groups for 64- and 128-point
+ * rows, sub-batch sized to keep >= 512 column workgroups per launch; 3D: radix-4 z rows from 64
+ * points).
  * (No reference counterpart: launch configuration.) */
 int tfsm_plan_get_tuning(const tfsm_plan* plan, int32_t* out);
 
