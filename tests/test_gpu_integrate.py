@@ -321,3 +321,39 @@ def test_sub_batch_blocking_is_bit_identical(dev):
         assert torch.equal(ug, ue)
     g.close()
     plan.set_streams(1)
+
+
+def test_ns512_b64_workload_full_size(dev):
+    """The benchmark workload at full size: NavierStokesVorticity(Re=1000) with kolm_force, 512^2,
+    B = 64, SETDRK4, dt 1e-3, default launch shape (temporal blocking over sub-batches).  Samples
+    0, 31 and 63 after 10 steps against the oracle on the same inputs; and for the whole batch the
+    mean vorticity, which only the drag changes (the product and the forcing have zero mean):
+    w0_mean(t) = w0_mean(0) exp(-0.1 t)."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import field, pde
+    from torchfsm_amd.integrator import SETDRKIntegrator
+    from torchfsm_amd.workloads import ns_initial_vorticity
+
+    n, B, dt, steps = 512, 64, 1e-3, 10
+    mi = [(0, 2 * math.pi, n)] * 2
+    mesh = tfa.MeshGrid(mi, device=dev)
+    y = mesh.bc_mesh_grid()[1]
+    u0 = ns_initial_vorticity(n, B, seed=0)
+    u0 = u0 + 0.05 * torch.arange(B, dtype=torch.float32).reshape(B, 1, 1, 1) / B  # nonzero means
+    op = pde.NavierStokesVorticity(1000.0, field.kolm_force(y))
+    op.set_integrator(SETDRKIntegrator.SETDRK4)
+    out = op.integrate(u0.to(dev), mesh=mesh, dt=dt, step=steps)
+    print("launch shape", op._state_dict["integrator"].tuning)
+    terms = refpath.ns_terms(1000.0, y.cpu())
+    for b in (0, 31, 63):
+        ref = refpath.integrate(terms, mi, u0[b:b + 1], dt, steps, scheme="SETDRK4")
+        e = _rel(out[b:b + 1].cpu().numpy(), ref.numpy())
+        print(f"sample {b}: rel-L2 {e:.2e}")
+        assert e <= TOL
+    m0 = u0.double().mean(dim=(1, 2, 3))
+    m = out.double().mean(dim=(1, 2, 3)).cpu()
+    want = m0 * math.exp(-0.1 * dt * steps)
+    err = float((m - want).abs().max() / want.abs().max())
+    print(f"mean vorticity vs drag decay: max rel err {err:.2e}")
+    assert err <= 1e-5
