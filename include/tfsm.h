@@ -182,7 +182,8 @@ int tfsm_full_to_half(tfsm_plan* plan, const void* full, void* uh, int32_t nbatc
 
 /* Advance the state uh (in place) by nsteps steps of the plan's scheme.  If rec_slot (device
  * int32 [nsteps + 1]) is given, the state before step i is copied to traj[b][rec_slot[i]] when
- * rec_slot[i] >= 0 (i = nsteps: the final state); traj is [B][T][C][...][P].
+ * 0 <= rec_slot[i] < T (i = nsteps: the final state; other slot values record nothing); traj is
+ * [B][T][C][...][P].
  * Replaces the hot loop of OperatorLike.integrate (_base.py:729-746) with integrator.step
  * (_setdrk_step.py:5-82, _etdrk.py:23-82), the nonlinear closure (_base.py:375-403) and
  * _TrajRecorder.record (traj_recorder.py:46-55). */

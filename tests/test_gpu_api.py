@@ -388,6 +388,14 @@ def test_plan_rejects_mismatched_tensors(dev):
     with pytest.raises(ValueError, match="record slots"):
         plan.step(uh, 4, torch.zeros(2, dtype=torch.int32, device=dev),
                   torch.zeros((2, 1) + plan.spec_shape[1:], dtype=torch.complex64, device=dev))
+    # slot values outside -1..T-1 would index past the trajectory buffer: rejected on the host,
+    # from the device schedule and from the caller's host copy alike
+    traj2 = torch.zeros((2, 2) + plan.spec_shape[1:], dtype=torch.complex64, device=dev)
+    for bad in ([0, 1, 2], [0, -2, 1]):
+        with pytest.raises(ValueError, match="record slots must lie in -1..1"):
+            plan.step(uh, 2, torch.tensor(bad, dtype=torch.int32, device=dev), traj2)
+        with pytest.raises(ValueError, match="record slots must lie in -1..1"):
+            plan.step(uh, 2, torch.tensor([0, -1, 1], dtype=torch.int32, device=dev), traj2, rec_slot_host=bad)
     with pytest.raises(ValueError, match="channels"):
         plan.spectral_map(0, uh, 3)
     s = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)

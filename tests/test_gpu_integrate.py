@@ -357,3 +357,59 @@ def test_ns512_b64_workload_full_size(dev):
     err = float((m - want).abs().max() / want.abs().max())
     print(f"mean vorticity vs drag decay: max rel err {err:.2e}")
     assert err <= 1e-5
+
+
+def test_ns512_b64_1000_steps_vs_oracle_f32_and_f64(dev):
+    """The benchmark workload over its whole length: NavierStokesVorticity(Re=1000) with kolm_force,
+    512^2, B = 64, SETDRK4, dt 1e-3, 1000 steps, default launch shape.  Samples 0, 31, 32 and 63
+    (both ends of both sub-batches) against the oracle run on the GPU on the same inputs in float32
+    and in float64.  The float32 floor is measured beside it: the oracle's own float32 result
+    against its float64 result, and the oracle in float32 on the CPU against the same on the GPU
+    (two FFT libraries, same op order) for sample 0.  The engine must be within 1e-5 of the oracle's
+    float32 result, or, where the flow's sensitivity lifts float32-vs-float32 differences above
+    that, no farther from the float64 result than the oracle's own float32 run is (within 1.5x),
+    and within 2x the measured floor of it."""
+    import time
+
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import field, pde
+    from torchfsm_amd.integrator import SETDRKIntegrator
+    from torchfsm_amd.workloads import ns_initial_vorticity
+
+    n, B, dt, steps = 512, 64, 1e-3, 1000
+    picks = [0, 31, 32, 63]
+    mi = [(0, 2 * math.pi, n)] * 2
+    mesh = tfa.MeshGrid(mi, device=dev)
+    y = mesh.bc_mesh_grid()[1]
+    u0 = ns_initial_vorticity(n, B, seed=0)
+    op = pde.NavierStokesVorticity(1000.0, field.kolm_force(y))
+    op.set_integrator(SETDRKIntegrator.SETDRK4)
+    t0 = time.perf_counter()
+    out = op.integrate(u0.to(dev), mesh=mesh, dt=dt, step=steps)
+    torch.cuda.synchronize()
+    print(f"engine: {steps} steps of B={B} in {time.perf_counter() - t0:.2f} s, launch shape "
+          f"{op._state_dict['integrator'].tuning}", flush=True)
+    eng = out[picks].cpu().numpy()
+    ref = {}
+    for dtype in (torch.float32, torch.float64):
+        t0 = time.perf_counter()
+        terms = refpath.ns_terms(1000.0, y.to(dtype))
+        ref[dtype] = refpath.integrate(terms, mi, u0[picks].to(dev, dtype), dt, steps, scheme="SETDRK4").cpu().numpy()
+        print(f"oracle {dtype} on the GPU: {time.perf_counter() - t0:.1f} s", flush=True)
+    t0 = time.perf_counter()
+    cpu0 = refpath.integrate(refpath.ns_terms(1000.0, y.cpu()), mi, u0[:1], dt, steps, scheme="SETDRK4").numpy()
+    print(f"oracle float32 on the CPU (sample 0): {time.perf_counter() - t0:.1f} s", flush=True)
+    floor_64 = [_rel(ref[torch.float32][i], ref[torch.float64][i]) for i in range(len(picks))]
+    floor_cg = _rel(cpu0[0], ref[torch.float32][0])
+    e32 = [_rel(eng[i], ref[torch.float32][i]) for i in range(len(picks))]
+    e64 = [_rel(eng[i], ref[torch.float64][i]) for i in range(len(picks))]
+    for i, b in enumerate(picks):
+        print(f"sample {b}: engine vs oracle f32 {e32[i]:.3e}, engine vs oracle f64 {e64[i]:.3e}; "
+              f"oracle f32 vs f64 {floor_64[i]:.3e}", flush=True)
+    print(f"float32 floor: oracle CPU vs GPU (sample 0) {floor_cg:.3e}", flush=True)
+    for i in range(len(picks)):
+        assert np.isfinite(eng[i]).all()
+        if e32[i] > TOL:
+            assert e64[i] <= 1.5 * floor_64[i], (picks[i], e64[i], floor_64[i])
+            assert e32[i] <= 2 * max(floor_64[i], floor_cg), (picks[i], e32[i], floor_64[i], floor_cg)

@@ -78,3 +78,55 @@ def test_hbm_budget_releases_idle_classes(dev):
     assert len(res) == 6
     assert len(srv.classes) == 1
     srv.close()
+
+
+def _oracle(r, dev):
+    """The request solved by the oracle on the GPU in float32 (same inputs, same scheme)."""
+    from oracle import refpath
+    from torchfsm_amd import serving
+
+    dims, ch, dt, scheme, length = serving.SPECS[r.kind]
+    if r.kind == "ks1d":
+        terms, mi = refpath.ks_terms(), [(0, r.param, r.n)]
+    elif r.kind == "ns2d":
+        mi = [(0, length, r.n)] * 2
+        from torchfsm_amd.mesh import MeshGrid
+
+        terms = refpath.ns_terms(r.param, MeshGrid(mi, device=dev).bc_mesh_grid()[1])
+    else:
+        terms, mi = refpath.burgers_terms(r.param), [(0, length, r.n)] * 3
+    return refpath.integrate(terms, mi, r.u0.to(dev), dt, r.steps, scheme=scheme).cpu()
+
+
+def test_serving_at_the_workload_classes_vs_oracle(dev):
+    """The serving workload's classes: KS N in {512..4096}, NS 64^2..512^2 and 3D Burgers 64^3 and
+    128^3, short requests (20-50 steps) arriving mixed.  Every served result is within rel-L2 1e-5
+    of the oracle's float32 solve of the same request; one request per class served alone is
+    bit-identical to its result in the mix."""
+    from torchfsm_amd import serving
+
+    sizes = {"ks1d": (512, 1024, 2048, 4096), "ns2d": (64, 128, 256, 512), "burgers3d": (64, 128)}
+    reqs = serving.synthetic_requests(60, rate=300.0, seed=11, sizes=sizes, steps=(20, 50))
+    classes = {(r.kind, r.n) for r in reqs}
+    want = {(k, n) for k, ns in sizes.items() for n in ns}
+    assert classes == want, want - classes
+    srv = serving.Server(dev)
+    res = srv.run(reqs)
+    assert sorted(r.rid for r in res) == list(range(len(reqs)))
+    print(f"served {len(res)} requests over {len(classes)} classes: {serving.latency_summary(res)}", flush=True)
+    by = {r.rid: r for r in res}
+    worst = {}
+    for q in reqs:
+        e = _rel(by[q.rid].u.numpy(), _oracle(q, dev).numpy())
+        key = (q.kind, q.n)
+        worst[key] = max(worst.get(key, 0.0), e)
+        assert e <= 1e-5, (q.rid, key, e)
+    for key in sorted(worst):
+        print(f"{key}: worst rel-L2 vs oracle f32 {worst[key]:.2e}", flush=True)
+    picked = {}
+    for q in reqs:
+        picked.setdefault((q.kind, q.n), q)
+    for key, q in sorted(picked.items()):
+        alone = srv.run([serving.Request(q.rid, q.kind, q.n, q.steps, q.param, q.u0, 0.0)])[0]
+        assert torch.equal(alone.u, by[q.rid].u), key
+    srv.close()

@@ -249,3 +249,67 @@ def test_rk_tableaux_match_the_oracle():
     assert sorted(m.name for m in RKIntegrator) == sorted(refpath.RK_TABLEAUX)
     for m in RKIntegrator:
         assert rk_tableau(m) == refpath.RK_TABLEAUX[m.name], m
+
+
+def _ctype_key(t):
+    """(kind, element size, length) of a ctypes field type, comparable across definitions."""
+    if hasattr(t, "_length_"):
+        return ("array", ctypes.sizeof(t._type_), t._length_, t._type_._type_)
+    if hasattr(t, "contents") or t.__name__.startswith("LP_"):
+        return ("pointer", ctypes.sizeof(t._type_), 1, t._type_._type_)
+    return ("scalar", ctypes.sizeof(t), 1, t._type_)
+
+
+_C_TYPES = {"int32_t": ctypes.c_int32, "float": ctypes.c_float, "double": ctypes.c_double}
+
+
+def _header_struct(header, name):
+    """Fields of `typedef struct name {...}` in include/tfsm.h as (field, ctypes type)."""
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), header, re.S).group(1)
+    fields = []
+    for line in body.splitlines():
+        line = line.split("/*")[0].strip()
+        if not line:
+            continue
+        m = re.match(r"(?:const\s+)?(\w+)\s*(\*?)\s*(\w+)(?:\[(\d+)\])?;", line)
+        assert m, line
+        base, ptr, fname, n = m.groups()
+        t = _C_TYPES[base]
+        if ptr:
+            t = ctypes.POINTER(t)
+        if n:
+            t = t * int(n)
+        fields.append((fname, t))
+    return fields
+
+
+def _doc_struct(doc, cls):
+    """Fields of the ctypes class `cls` in INTEGRATION.md's binding snippet."""
+    import ast
+
+    code = re.search(r"```python\n(.*?)```", doc, re.S).group(1)
+    tree = ast.parse(code)
+    for node in tree.body:
+        if isinstance(node, ast.ClassDef) and node.name == cls:
+            for st in node.body:
+                if isinstance(st, ast.Assign) and st.targets[0].id == "_fields_":
+                    return [(e.elts[0].value, eval(ast.unparse(e.elts[1]), {"ctypes": ctypes}))
+                            for e in st.value.elts]
+    raise AssertionError(f"{cls} not in INTEGRATION.md")
+
+
+@pytest.mark.parametrize("cname,pyname", [("tfsm_terms", "Terms"), ("tfsm_geometry", "Geometry")])
+def test_boundary_structs_agree(cname, pyname):
+    """The C structs of include/tfsm.h, the package's ctypes binding and the maintainer's binding
+    in INTEGRATION.md have the same fields, in order, with the same types and sizes."""
+    from torchfsm_amd import _lib
+
+    header = open(os.path.join(ROOT, "include", "tfsm.h")).read()
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    want = [(n, _ctype_key(t)) for n, t in _header_struct(header, cname)]
+    got_lib = [(n, _ctype_key(t)) for n, t in getattr(_lib, pyname)._fields_]
+    got_doc = [(n, _ctype_key(t)) for n, t in _doc_struct(doc, pyname)]
+    assert got_lib == want
+    assert got_doc == want
+    assert ctypes.sizeof(getattr(_lib, pyname)) == ctypes.sizeof(
+        type("S", (ctypes.Structure,), {"_fields_": _doc_struct(doc, pyname)}))

@@ -462,3 +462,56 @@ def test_conservative_convection_closure_closed_forms():
     ex = torch.stack([torch.sin(2 * X) + torch.sin(X) * torch.cos(Y), torch.cos(X) * torch.sin(Y) + torch.sin(2 * Y)])[None]
     o2 = R.apply([(1, "cons", None)], [(0, 2 * math.pi, n)] * 2, v)
     assert (o2 - ex).abs().max() < 1e-12
+
+
+# ---------------------------------------------------------------- refpath's SETDRK3 and ETDRK1 branches
+@pytest.mark.parametrize("scheme", ["SETDRK3", "ETDRK1"])
+def test_refpath_setdrk3_etdrk1_match_numref_f64(scheme):
+    """refpath's SETDRK3 (_uncached.py:120-160, _setdrk_step.py:28-52) and ETDRK1 (_etdrk.py:30-51)
+    branches in float64 against numref's independent restatement (phi-function coefficients,
+    half spectrum): 1D Burgers N = 128 for 200 steps and forced Navier-Stokes 64^2 for 20 steps."""
+    g = golden("g1_burgers1d.npz")
+    u0 = torch.from_numpy(g["u0_f64"])
+    o = R.integrate(R.burgers_terms(0.01), [(0, 1, 128)], u0, 0.01, 200, scheme=scheme)
+    t = numref.integrate([(0, 1, 128)], [(0.01, "lap", None), (-1, "conv", None)], g["u0_f64"], 0.01, 200, scheme)
+    assert rel(o.numpy(), t) < 1e-12
+    g = golden("g2_ns64.npz")
+    y = kolm_y(64, torch.float64)
+    o = R.integrate(R.ns_terms(1000, y), [(0, 2 * math.pi, 64)] * 2, torch.from_numpy(g["u0_f64"]), 1e-3, 20,
+                    scheme=scheme)
+    yn = np.broadcast_to(2 * np.pi * np.arange(64) / 64, (1, 1, 64, 64))
+    terms = [(-1, "vc", None), (1 / 1000, "lap", None), (-0.1, "unit", None), (-1, "src", 4 * np.cos(4 * yn))]
+    t = numref.integrate([(0, 2 * np.pi, 64)] * 2, terms, g["u0_f64"], 1e-3, 20, scheme)
+    assert rel(o.numpy(), t) < 1e-12
+
+
+@pytest.mark.parametrize("scheme,order", [("SETDRK3", 3), ("ETDRK1", 1)])
+def test_refpath_observed_order(scheme, order):
+    """Observed order of refpath's own SETDRK3 and ETDRK1 steps on 1D viscous Burgers in float64,
+    against a 400-step SETDRK4 solution: a wrong coefficient or stage formula loses the order."""
+    n = 64
+    x = torch.arange(n, dtype=torch.float64) / n
+    u0 = (torch.sin(2 * math.pi * x) + 0.5).reshape(1, 1, n)
+    terms = R.burgers_terms(0.02)
+    T = 0.2
+    ref = R.integrate(terms, [(0, 1, n)], u0, T / 400, 400, scheme="SETDRK4")
+    errs = [float((R.integrate(terms, [(0, 1, n)], u0, T / m, m, scheme=scheme) - ref).abs().max())
+            for m in (10, 20, 40)]
+    rates = [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    assert abs(rates[-1] - order) < 0.3, rates
+
+
+@pytest.mark.parametrize("scheme", ["ETDRK1", "SETDRK1", "SETDRK3"])
+def test_refpath_constant_forcing_exact(scheme):
+    """u' = L u + s with a constant source: ETDRK1's and SETDRK1's step is exact,
+    u(t) = e^{Lt} u0 + t phi1(Lt) s per mode, and so is SETDRK3's (its c3 + c4 + c5 = dt phi1).
+    refpath in float64 against that closed form."""
+    n, nu, dt, steps = 16, 0.05, 0.1, 5
+    x = torch.arange(n, dtype=torch.float64) / n
+    src = torch.cos(2 * math.pi * 2 * x).reshape(1, 1, n)
+    u0 = torch.sin(2 * math.pi * x).reshape(1, 1, n)
+    o = R.integrate([(nu, "lap", None), (1.0, "src", src)], [(0, 1, n)], u0, dt, steps, scheme=scheme)
+    t = steps * dt
+    lam = -nu * (2 * math.pi * 2) ** 2
+    ex = math.exp(-nu * (2 * math.pi) ** 2 * t) * u0 + t * (math.exp(lam * t) - 1) / (lam * t) * src
+    assert float((o - ex).abs().max()) < 1e-12

@@ -28,6 +28,9 @@ int hip_fail(hipError_t e, const char* what) {
   return fail(TFSM_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// status of a HIP call as a TFSM code (0 on success), for code paths with a single exit
+int launch_status(hipError_t e, const char* what) { return e == hipSuccess ? TFSM_OK : hip_fail(e, what); }
+
 #define HIP_TRY(expr, what)               \
   do {                                    \
     hipError_t _e = (expr);               \
@@ -503,27 +506,27 @@ int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* tra
       HIP_TRY(hipStreamWaitEvent(p->aux, p->ev_fork, 0), "fork wait");
     }
     const hipStream_t caller = st;
-    for (int b0 = 0, k = 0; b0 < p->B; b0 += sb, ++k) {
+    int rc = TFSM_OK;
+    for (int b0 = 0, k = 0; b0 < p->B && !rc; b0 += sb, ++k) {
       st = (two && (k & 1)) ? p->aux : caller;
       const int nb = (p->B - b0) < sb ? (p->B - b0) : sb;
       float2* Ub = U + (size_t)b0 * per;
       float2* Tb = traj ? (float2*)traj + (size_t)b0 * T * per : nullptr;
-      int rc = run_eval(p, U, (char*)ws, ST_PRO, b0, nb, st);
-      if (rc) return rc;
-      for (int i = 0; i < nsteps; ++i) {
-        if (rec_slot) HIP_TRY(launch_record(Ub, Tb, rec_slot, i, nb, per, T, st), "record launch");
-        for (int e = 0; e < nev; ++e) {
-          rc = run_eval(p, U, (char*)ws, scheme_stage(p->scheme, e), b0, nb, st);
-          if (rc) return rc;
-        }
+      rc = run_eval(p, U, (char*)ws, ST_PRO, b0, nb, st);
+      for (int i = 0; i < nsteps && !rc; ++i) {
+        if (rec_slot) rc = launch_status(launch_record(Ub, Tb, rec_slot, i, nb, per, T, st), "record launch");
+        for (int e = 0; e < nev && !rc; ++e) rc = run_eval(p, U, (char*)ws, scheme_stage(p->scheme, e), b0, nb, st);
       }
-      if (rec_slot) HIP_TRY(launch_record(Ub, Tb, rec_slot, nsteps, nb, per, T, st), "record launch");
+      if (rec_slot && !rc) rc = launch_status(launch_record(Ub, Tb, rec_slot, nsteps, nb, per, T, st), "record launch");
     }
+    // the join is recorded on every exit, also after a failed launch: work already queued on the
+    // auxiliary stream is then still ordered before anything the caller queues next
     if (two) {
-      HIP_TRY(hipEventRecord(p->ev_join, p->aux), "join event");
-      HIP_TRY(hipStreamWaitEvent(caller, p->ev_join, 0), "join wait");
+      const int jr = launch_status(hipEventRecord(p->ev_join, p->aux), "join event");
+      const int jw = jr ? jr : launch_status(hipStreamWaitEvent(caller, p->ev_join, 0), "join wait");
+      if (!rc) rc = jw;
     }
-    return TFSM_OK;
+    return rc;
   }
   return fail(TFSM_E_INVALID, "stepping is not available for this dimension");
 }
@@ -737,8 +740,11 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   p->nlin = terms->n_linear;
   for (int t = 0; t < p->nlin; ++t) {
     p->lin_kind[t] = terms->linear_kind[t];
-    p->lin_dim[t] = terms->linear_dim[t];
-    p->lin_order[t] = terms->linear_order[t];
+    // the axis and order fields are read only for the terms that have them
+    if (p->lin_kind[t] == TFSM_LIN_DERIV) {
+      p->lin_dim[t] = terms->linear_dim[t];
+      p->lin_order[t] = terms->linear_order[t];
+    }
   }
   p->cplx = cplx;
   p->per_len = D == 1 && geom->length_per_sample;

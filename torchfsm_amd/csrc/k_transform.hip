@@ -195,7 +195,7 @@ __global__ void k_half_to_full(const float2* __restrict__ half, float2* __restri
 }
 
 // ---------------------------------------------------------------- exact linear stepping (ETDRK0)
-// U <- E U, nsteps times, recording frames before step i where rec_slot[i] >= 0 (and after).
+// U <- E U, nsteps times, recording frames before step i where rec_slot[i] is in [0, T) (and after).
 __global__ void k_linear_steps(float2* __restrict__ U, const float* __restrict__ E, size_t coef_bstride,
                                size_t modes, int C, size_t total, int nsteps, const int* __restrict__ rec_slot,
                                float2* __restrict__ traj, int T) {
@@ -208,7 +208,7 @@ __global__ void k_linear_steps(float2* __restrict__ U, const float* __restrict__
     for (int s = 0; s <= nsteps; ++s) {
       if (rec_slot) {
         const int sl = rec_slot[s];
-        if (sl >= 0) traj[((b * T + sl) * C + c) * modes + mode] = u;
+        if (sl >= 0 && sl < T) traj[((b * T + sl) * C + c) * modes + mode] = u;
       }
       if (s < nsteps) u = rscale(ev, u);
     }
@@ -230,7 +230,7 @@ __global__ void k_linear_steps_c(float2* __restrict__ U, const float2* __restric
     for (int s = 0; s <= nsteps; ++s) {
       if (rec_slot) {
         const int sl = rec_slot[s];
-        if (sl >= 0) traj[((b * T + sl) * C + c) * modes + mode] = u;
+        if (sl >= 0 && sl < T) traj[((b * T + sl) * C + c) * modes + mode] = u;
       }
       if (s < nsteps) u = cmulr(ev, u);
     }
@@ -238,13 +238,13 @@ __global__ void k_linear_steps_c(float2* __restrict__ U, const float2* __restric
   }
 }
 
-// copy the state U [B][per_sample] to traj[b][rec_slot[step]] when that slot is >= 0
+// copy the state U [B][per_sample] to traj[b][rec_slot[step]] when that slot is in [0, T)
 // (_TrajRecorder.record before step i, traj_recorder.py:46-55); the schedule is read on the
 // device so a captured step sequence does not depend on it.
 __global__ void k_record(const float2* __restrict__ U, float2* __restrict__ traj, const int* __restrict__ rec_slot,
                          int step, int B, size_t per_sample, int T) {
   const int sl = rec_slot[step];
-  if (sl < 0) return;
+  if (sl < 0 || sl >= T) return;
   const size_t total = (size_t)B * per_sample;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
     const size_t b = e / per_sample, r = e % per_sample;
@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
     const bool active = valid && step < nst;
     if (a.rec_slot && active) {
       const int sl = a.rec_slot[step];
-      if (sl >= 0)
+      if (sl >= 0 && sl < a.T)
         for (int k = j; k < H; k += NT) a.traj[((size_t)smp * a.T + sl) * a.P + k] = Uw[k];
     }
     for (int e = 0; e < nev; ++e) {
@@ -389,7 +389,7 @@ __global__ void __launch_bounds__(Fft<N>::NT* V) k1_run(Run1Args a) {
   }
   if (a.rec_slot && valid && !a.eval_only) {
     const int sl = a.rec_slot[nst];
-    if (sl >= 0)
+    if (sl >= 0 && sl < a.T)
       for (int k = j; k < H; k += NT) a.traj[((size_t)smp * a.T + sl) * a.P + k] = Uw[k];
   }
   if (LSTATE && valid) {

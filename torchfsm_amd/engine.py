@@ -297,8 +297,12 @@ class SpectralPlan:
 
     # ------------------------------------------------------------------ stepping
     def step(self, uh: torch.Tensor, nsteps: int, rec_slot: Optional[torch.Tensor] = None,
-             traj: Optional[torch.Tensor] = None):
-        """Advance uh in place by nsteps (optionally recording frames into traj [B][T][C][...][P])."""
+             traj: Optional[torch.Tensor] = None, rec_slot_host: Optional[Sequence[int]] = None):
+        """Advance uh in place by nsteps (optionally recording frames into traj [B][T][C][...][P]).
+
+        Every recorded slot must lie in -1..T-1 (-1: no frame at that step).  The check reads
+        rec_slot_host, the caller's host copy of the schedule, when given, and otherwise copies the
+        device schedule to the host (one synchronisation)."""
         if self.scheme is None:
             raise RuntimeError("set_scheme must be called before stepping")
         self._check_state(uh)
@@ -315,6 +319,13 @@ class SpectralPlan:
                 raise ValueError("trajectory buffer must be [B][T][C][...][P]")
             T = int(traj.shape[1])
             self._check(traj, (T,) + tuple(self.spec_shape[1:]), torch.complex64, "trajectory buffer", self.batch)
+            sched = (list(rec_slot_host)[: int(nsteps) + 1] if rec_slot_host is not None
+                     else rec_slot[: int(nsteps) + 1].cpu().tolist())
+            if len(sched) < int(nsteps) + 1:
+                raise ValueError(f"record slots: one per step and one after the last ({int(nsteps) + 1})")
+            bad = [v for v in sched if not -1 <= int(v) < T]
+            if bad:
+                raise ValueError(f"record slots must lie in -1..{T - 1} (trajectory of {T} frames); got {bad[0]}")
         ws = self.workspace()
         with torch.cuda.device(self.device):
             check(lib.tfsm_step(self._plan, _ptr(uh), int(nsteps), _ptr(rec_slot), _ptr(traj), T, _ptr(ws),

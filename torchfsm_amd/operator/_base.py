@@ -376,7 +376,7 @@ class OperatorLike(_MutableMixIn):
                     rec = torch.tensor(slots, dtype=torch.int32, device=plan.device)
                     traj = torch.zeros((plan.batch, t) + tuple(plan.spec_shape[1:]), dtype=torch.complex64,
                                        device=plan.device)
-                    plan.step(uh, step, rec, traj)
+                    plan.step(uh, step, rec, traj, rec_slot_host=slots)
                     trajectory_recorder._take_engine_frames(plan, traj)
             else:
                 # generic recorder protocol: full complex frames handed to record() at recorded steps
@@ -447,7 +447,8 @@ class OperatorLike(_MutableMixIn):
             if busy[h] is not None:
                 compute.wait_event(busy[h][0])
             frames = ring[h][: B * nf * fe].view((B, nf) + half_shape)
-            plan.step(uh, s_end - s0, rec_all[off:off + s_end - s0 + 1], frames)
+            plan.step(uh, s_end - s0, rec_all[off:off + s_end - s0 + 1], frames,
+                      rec_slot_host=slots[off:off + s_end - s0 + 1])
             flat = frames.reshape((B * nf,) + half_shape)
             out = plan.half_to_full(flat) if return_in_fourier else plan.inverse(flat)
             out = out.view((B, nf) + tuple(out.shape[1:]))
