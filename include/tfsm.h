@@ -190,6 +190,24 @@ int tfsm_full_to_half(tfsm_plan* plan, const void* full, void* uh, int32_t nbatc
 int tfsm_step(tfsm_plan* plan, void* uh, int32_t nsteps, const int32_t* rec_slot, void* traj, int32_t T, void* ws,
               void* stream);
 
+/* Bytes of the static device scratch that tfsm_step_noalloc uses as its workspace (64 MiB). */
+#define TFSM_NOALLOC_SCRATCH_BYTES (64ll << 20)
+
+/* tfsm_step without a caller workspace and without any device allocation: the workspace is a
+ * static device array of TFSM_NOALLOC_SCRATCH_BYTES per GPU (loaded with the library's code
+ * object, held until the process ends; its address is resolved once per device on the first
+ * call with that device current).  For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  The plan's device must
+ * be current, and a plan whose tfsm_plan_workspace_bytes exceeds the scratch is rejected
+ * (TFSM_E_INVALID).  Every call on one GPU shares that GPU's scratch: calls that overlap on one GPU
+ * (two streams, two threads) race and must be serialised by the caller.  The scratch keeps the
+ * last call's contents; the step reads no workspace value it has not written in the same call, so
+ * nothing is cleared.  Results are identical to tfsm_step's.
+ * (No reference counterpart: an allocation-free variant of tfsm_step for callers that cannot
+ * allocate.) */
+int tfsm_step_noalloc(tfsm_plan* plan, void* uh, int32_t nsteps, const int32_t* rec_slot, void* traj, int32_t T,
+                      void* stream);
+
 /* Like tfsm_step for 1D plans, with a per-sample step count (device int32 [B]); samples stop
  * independently (continuous batching of 1D requests). */
 int tfsm_step_varying(tfsm_plan* plan, void* uh, const int32_t* nsteps_per_sample, int32_t max_steps, void* ws,

@@ -20,6 +20,7 @@ TERM_NONE, TERM_CONV, TERM_VC, TERM_NSP, TERM_KS, TERM_KS_KEEP_MEAN, TERM_CONS =
 LIN_LAP, LIN_BIHARM, LIN_UNIT, LIN_DERIV = 1, 2, 3, 4
 MAP_DEALIAS, MAP_DIV, MAP_GRAD, MAP_CURL, MAP_V2V, MAP_PRESSURE, MAP_CONSDIV = 0, 1, 2, 3, 4, 5, 6
 ETDRK0, ETDRK1, ETDRK2, SETDRK1, SETDRK2, SETDRK3, SETDRK4 = range(7)
+NOALLOC_SCRATCH_BYTES = 64 << 20  # TFSM_NOALLOC_SCRATCH_BYTES
 
 
 class Geometry(ctypes.Structure):
@@ -74,6 +75,7 @@ SIGNATURES = {
     "tfsm_half_to_full": (ctypes.c_int, [VP, VP, VP, I32, VP]),
     "tfsm_full_to_half": (ctypes.c_int, [VP, VP, VP, I32, VP]),
     "tfsm_step": (ctypes.c_int, [VP, VP, I32, VP, VP, I32, VP, VP]),
+    "tfsm_step_noalloc": (ctypes.c_int, [VP, VP, I32, VP, VP, I32, VP]),
     "tfsm_step_varying": (ctypes.c_int, [VP, VP, VP, I32, VP, VP]),
     "tfsm_apply": (ctypes.c_int, [VP, VP, VP, VP, VP]),
     "tfsm_solve": (ctypes.c_int, [VP, VP, VP, VP]),

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -12,6 +13,12 @@
 #include "tfsm.h"
 
 using namespace tfsm;
+
+// Workspace of tfsm_step_noalloc: a device array of fixed size, one copy per GPU, loaded with the
+// code object and held until the process ends.  Host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (scratch_on_device below); the name itself is
+// a host-side stand-in and never a device address.
+static __device__ __align__(256) unsigned char scratch[TFSM_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -84,6 +91,7 @@ int dealias_kmax(int n, double length, double rate) {
 }  // namespace
 
 struct tfsm_plan {
+  int device = 0;  // HIP device current at tfsm_plan_create (its tables live there)
   int D = 0, n[3] = {1, 1, 1}, B = 0, C = 0;
   double len[3] = {0, 0, 0};
   int H = 0, P = 0;
@@ -645,6 +653,25 @@ AdjStageArgs stage_args(tfsm_plan* p, int op) {
 
 }  // namespace
 
+// Device address of `scratch` on device `dev`, resolved once per device with that device current.
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_on_device(int dev, void** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(TFSM_E_INVALID, "device index out of range for the static scratch");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* q = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "address of the static scratch");
+    if (!q) return fail(TFSM_E_DEVICE, "address of the static scratch is null");
+    g_scratch_ptr[dev] = q;
+  }
+  *out = g_scratch_ptr[dev];
+  return TFSM_OK;
+}
+
 // nbatch blocks of the plan's [C][N...] shape: at least one, and few enough that every launch's row
 // count fits the kernels' 32-bit row index
 static int check_nbatch(const tfsm_plan* p, int32_t nbatch) {
@@ -721,7 +748,10 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   if (geom->length_per_sample && D != 1) return fail(TFSM_E_INVALID, "per-sample lengths are 1D only");
 
   hipStream_t st = (hipStream_t)stream;
+  int device = 0;
+  HIP_TRY(hipGetDevice(&device), "current device");
   tfsm_plan* p = new tfsm_plan();
+  p->device = device;
   p->D = D;
   p->B = geom->batch;
   p->C = geom->channels;
@@ -1229,6 +1259,26 @@ int tfsm_step(tfsm_plan* p, void* uh, int32_t nsteps, const int32_t* rec_slot, v
   if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
   if (nsteps < 0) return fail(TFSM_E_INVALID, "nsteps must be >= 0");
   if (rec_slot && (!traj || T < 1)) return fail(TFSM_E_INVALID, "recording needs a trajectory buffer");
+  return step_impl(p, uh, nsteps, rec_slot, traj, T, ws, (hipStream_t)stream);
+}
+
+int tfsm_step_noalloc(tfsm_plan* p, void* uh, int32_t nsteps, const int32_t* rec_slot, void* traj, int32_t T,
+                      void* stream) {
+  if (!p || !uh) return fail(TFSM_E_INVALID, "null argument");
+  if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
+  if (nsteps < 0) return fail(TFSM_E_INVALID, "nsteps must be >= 0");
+  if (rec_slot && (!traj || T < 1)) return fail(TFSM_E_INVALID, "recording needs a trajectory buffer");
+  if (p->ws_bytes > (size_t)TFSM_NOALLOC_SCRATCH_BYTES)
+    return fail(TFSM_E_INVALID, "the plan's workspace (" + std::to_string(p->ws_bytes) +
+                                    " bytes) exceeds the static scratch of tfsm_step_noalloc (" +
+                                    std::to_string((long long)TFSM_NOALLOC_SCRATCH_BYTES) +
+                                    " bytes): use tfsm_step with a caller workspace or a smaller batch");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev), "current device");
+  if (dev != p->device) return fail(TFSM_E_INVALID, "the plan's device must be the current device");
+  void* ws = nullptr;
+  const int rc = scratch_on_device(dev, &ws);
+  if (rc) return rc;
   return step_impl(p, uh, nsteps, rec_slot, traj, T, ws, (hipStream_t)stream);
 }
 
