@@ -92,6 +92,17 @@ class NS2D:
         """Fill the product/field buffers so single passes see a valid state."""
         self.plan.launch_pass(self.uh.clone(), 1, ST_PRO)
 
+    prime_pass = (1, ST_PRO)
+
+    def step_sequence(self, nb):
+        """The launches of one step, in order: (kernels() name, pass id, stage)."""
+        ks = self.kernels(nb)
+        row, cols = ks[0][0], [k[0] for k in ks[1:]]
+        seq = []
+        for i, st in enumerate((ST_R4_E0, ST_R4_E1, ST_R4_E2, ST_R4_E3)):
+            seq += [(row, 0, st), (cols[i], 1, st)]
+        return seq
+
     def cpu_baseline(self, args):
         import torch
 
@@ -174,6 +185,17 @@ class Burgers3D:
     def prime(self):
         self.plan.launch_pass(self.uh.clone(), 3, ST_PRO)
 
+    prime_pass = (3, ST_PRO)
+
+    def step_sequence(self, nb):
+        ks = {k[0]: k for k in self.kernels(nb)}
+        names = list(ks)
+        row, yfwd, col0, col1, yinv = names
+        seq = []
+        for st, col in ((ST_R2_E0, col0), (ST_R2_E1, col1)):
+            seq += [(row, 4, st), (yfwd, 5, st), (col, 6, st), (yinv, 7, st)]
+        return seq
+
     def cpu_baseline(self, args):
         import torch
 
@@ -206,17 +228,19 @@ class Burgers3D:
 
 WORKLOADS = {"ns2d": NS2D, "burgers3d": Burgers3D}
 
-SERVE_SIZES = {"ks1d": (512, 1024, 2048, 4096), "ns2d": (64, 128, 256, 512), "burgers3d": (64,)}
+SERVE_SIZES = {"ks1d": (512, 1024, 2048, 4096), "ns2d": (64, 128, 256, 512), "burgers3d": (64, 128)}
+SERVE_MIX = ("KS 1D N in {512..4096}, L ~ U[32pi, 128pi], dt 0.1 SETDRK4 | NS 2D N in {64..512}, nu ~ logU(1e-3, "
+             "5e-2), dt 1e-3 SETDRK4 | 3D Burgers 64^3 and 128^3, nu ~ logU(1e-2, 5e-2), dt 2e-3 SETDRK2; "
+             "steps ~ U{100..2000}")
+SERVE_P99_BOUND_S = 2.0  # the p99 latency the rate sweep is held to
 
 
-def serving_leg(dev, n_requests, rate, seed=0, plain_sample=12):
-    """Mixed request stream through the continuous-batching server, then a sample of the same
-    requests solved one by one with plain integrate() calls (the per-request baseline)."""
+def serve_stream(dev, reqs):
+    """Serve `reqs` through the continuous-batching server (classes created before the clock starts)."""
     import torch
 
     from torchfsm_amd import serving
 
-    reqs = serving.synthetic_requests(n_requests, rate, seed=seed, sizes=SERVE_SIZES, steps=(100, 2000))
     srv = serving.Server(dev)
     t0 = time.perf_counter()
     srv.prewarm(sorted({(r.kind, r.n) for r in reqs}))
@@ -224,30 +248,44 @@ def serving_leg(dev, n_requests, rate, seed=0, plain_sample=12):
     t0 = time.perf_counter()
     res = srv.run(reqs)
     t_run = time.perf_counter() - t0
-    work = {r.rid: serving.SPECS[r.kind][1] * r.n ** serving.SPECS[r.kind][0] * r.steps for r in reqs}
+    work = sum(serving.SPECS[r.kind][1] * r.n ** serving.SPECS[r.kind][0] * r.steps for r in reqs)
     lat = serving.latency_summary(res)
     out = {
-        "requests": len(res), "arrival_rate_per_s": rate, "makespan_s": t_run, "setup_s": t_setup,
-        "dof_steps_per_s": sum(work.values()) / t_run, "requests_per_s": len(res) / t_run,
+        "requests": len(res), "makespan_s": t_run, "setup_s": t_setup,
+        "dof_steps_per_s": work / t_run, "requests_per_s": len(res) / t_run,
         "p50_latency_ms": lat["p50_ms"], "p99_latency_ms": lat["p99_ms"], "max_latency_ms": lat["max_ms"],
         "all_finite": all(bool(torch.isfinite(r.u).all()) for r in res),
-        "mix": "KS 1D N in {512..4096}, L ~ U[32pi, 128pi], dt 0.1 SETDRK4 | NS 2D N in {64..512}, nu ~ logU(1e-3, "
-               "5e-2), dt 1e-3 SETDRK4 | 3D Burgers 64^3, nu ~ logU(1e-2, 5e-2), dt 2e-3 SETDRK2; steps ~ U{100..2000}",
+        "by_class": serving.latency_by_class(res),
     }
     srv.close()
-    # plain per-request solves (B = 1 plans through the public API), on a sample of the requests
-    from torchfsm_amd import field, pde
+    return out, work
+
+
+def plain_solves(dev, reqs, warm):
+    """The same requests solved one at a time.  cold: integrate() on a new operator per request (plan
+    and coefficient setup included, as a per-request solve pays it).  warm: one single-slot plan per
+    shape class, kept across requests; each request re-targets it (tfsm_plan_set_sample), transforms
+    its field, steps and transforms back."""
+    import torch
+
+    from torchfsm_amd import field, pde, serving
     from torchfsm_amd.integrator import SETDRKIntegrator
     from torchfsm_amd.mesh import MeshGrid
 
-    sample = reqs[:plain_sample]
-    if not sample:
-        return out
+    classes = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    wsum = 0
-    for r in sample:
+    for r in reqs:
         dims, ch, dt, scheme, length = serving.SPECS[r.kind]
+        if warm:
+            c = classes.get((r.kind, r.n))
+            if c is None:
+                c = classes[(r.kind, r.n)] = serving.ShapeClass(r.kind, r.n, 1, dev)
+            c.plan.set_sample(0, serving.linear_coefs(r.kind, r.param), float(r.param) if r.kind == "ks1d" else 0.0)
+            c.plan.forward(r.u0.to(dev), out=c.uh)
+            c.plan.step(c.uh, r.steps)
+            c.plan.inverse(c.uh).cpu()
+            continue
         if r.kind == "ks1d":
             op, mesh = pde.KuramotoSivashinsky(), MeshGrid([(0, r.param, r.n)], device=dev)
         elif r.kind == "ns2d":
@@ -257,15 +295,98 @@ def serving_leg(dev, n_requests, rate, seed=0, plain_sample=12):
             mesh = MeshGrid([(0, length, r.n)] * 3, device=dev)
             op = pde.Burgers(r.param)
         op.set_integrator(getattr(SETDRKIntegrator, scheme))
-        op.integrate(r.u0.to(dev), mesh=mesh, dt=dt, step=r.steps)
-        wsum += work[r.rid]
+        op.integrate(r.u0.to(dev), mesh=mesh, dt=dt, step=r.steps).cpu()
     torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t0
-    out["plain_per_request"] = {"requests": len(sample), "seconds": t_plain, "dof_steps_per_s": wsum / t_plain,
-                                "note": "first requests of the same stream, solved one at a time with "
-                                        "integrate() (plan setup included, as a per-request solve pays it)"}
-    out["throughput_vs_plain"] = out["dof_steps_per_s"] / (wsum / t_plain)
+    t = time.perf_counter() - t0
+    for c in classes.values():
+        c.close()
+    return t
+
+
+def serving_leg(dev, n_requests, rate, seed=0, rates=(), rate_requests=60):
+    """The mixed request stream through the continuous-batching server; the same requests solved one
+    by one (warm and cold per-request baselines); and a sweep of lighter arrival rates for the
+    highest rate whose p99 latency stays within SERVE_P99_BOUND_S."""
+    from torchfsm_amd import serving
+
+    reqs = serving.synthetic_requests(n_requests, rate, seed=seed, sizes=SERVE_SIZES, steps=(100, 2000))
+    out, work = serve_stream(dev, reqs)
+    out = {"arrival_rate_per_s": rate, **out, "mix": SERVE_MIX}
+    log(f"[bench] serving: {out['requests']} requests in {out['makespan_s']:.2f} s, "
+        f"{out['dof_steps_per_s']:.3e} DOF*steps/s, p50 {out['p50_latency_ms']:.1f} ms, "
+        f"p99 {out['p99_latency_ms']:.1f} ms")
+    for warm in (True, False):
+        t = plain_solves(dev, reqs, warm)
+        name = "plain_per_request_warm" if warm else "plain_per_request_cold"
+        out[name] = {"requests": len(reqs), "seconds": t, "dof_steps_per_s": work / t,
+                     "throughput_vs_plain": out["dof_steps_per_s"] / (work / t)}
+        log(f"[bench] {name}: {t:.2f} s for the same {len(reqs)} requests; served "
+            f"{out[name]['throughput_vs_plain']:.2f}x")
+    sweep = []
+    for i, r in enumerate(rates):
+        lreqs = serving.synthetic_requests(rate_requests, r, seed=seed + 1 + i, sizes=SERVE_SIZES, steps=(100, 2000))
+        lo, _ = serve_stream(dev, lreqs)
+        sweep.append({"arrival_rate_per_s": r, "requests": lo["requests"], "requests_per_s": lo["requests_per_s"],
+                      "p50_latency_ms": lo["p50_latency_ms"], "p99_latency_ms": lo["p99_latency_ms"],
+                      "by_class": lo["by_class"]})
+        log(f"[bench] serving at {r}/s: p50 {lo['p50_latency_ms']:.1f} ms, p99 {lo['p99_latency_ms']:.1f} ms")
+    if sweep:
+        met = [x["arrival_rate_per_s"] for x in sweep if x["p99_latency_ms"] <= SERVE_P99_BOUND_S * 1e3]
+        out["rate_sweep"] = sweep
+        out["p99_bound_ms"] = SERVE_P99_BOUND_S * 1e3
+        out["max_rate_within_p99_bound"] = max(met) if met else None
     return out
+
+
+CPU_SHARE = 16  # host CPUs a one-GPU box allots to the GPU's jobs
+
+
+def host_cpu():
+    """CPU model, os.cpu_count(), the CPUs this process may run on, and the CPU FFT backend."""
+    import torch
+
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"model": model, "os_cpu_count": os.cpu_count(), "usable_cpus": usable,
+            "mkl": bool(torch.backends.mkl.is_available())}
+
+
+def time_in_step(plan, uh, prime, seq, steps=12, skip=2):
+    """Average duration (ms) of each kernel of the step, timed inside the step's own launch
+    sequence: `steps` steps of the first sub-batch launched pass by pass in the step's order (the
+    data flow and cache state of tfsm_step), with HIP events recorded between the launches on the
+    stream the engine launches on (torch's current stream).  The first `skip` steps are not
+    counted."""
+    import torch
+
+    stream = torch.cuda.current_stream()
+    u = uh.clone()
+    torch.cuda.synchronize()
+    plan.launch_pass(u, *prime)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(seq) + 1)] for _ in range(steps)]
+    for k in range(steps):
+        ev[k][0].record(stream)
+        for i, (_, pid, st) in enumerate(seq):
+            plan.launch_pass(u, pid, st)
+            ev[k][i + 1].record(stream)
+    torch.cuda.synchronize()
+    tot, cnt = {}, {}
+    for k in range(skip, steps):
+        for i, (name, _, _) in enumerate(seq):
+            tot[name] = tot.get(name, 0.0) + ev[k][i].elapsed_time(ev[k][i + 1])
+            cnt[name] = cnt.get(name, 0) + 1
+    return {n: tot[n] / cnt[n] for n in tot}
 
 
 def time_kernel(plan, uh, pass_id, stage, reps=20):
@@ -324,8 +445,8 @@ def main():
     ap.add_argument("--streams", type=int, default=1, help="1, or 2 to overlap alternate sub-batches")
     ap.add_argument("--serve-requests", type=int, default=240, help="requests of the serving leg (0 = skip it)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate of the serving leg (1/s)")
-    ap.add_argument("--serve-light-rate", type=float, default=25.0,
-                    help="arrival rate of a second, lightly loaded serving run (1/s; 0 = skip it)")
+    ap.add_argument("--serve-rates", default="5,10,20,40",
+                    help="lighter Poisson rates (1/s, comma-separated) swept for the p99 bound; empty: none")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run 3 plain 256 MiB device copies (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -396,12 +517,8 @@ def main():
     # ---------------------------------------------------------------- kernel roofline
     sb = plan.tuning[4]
     nb = sb if 0 < sb < w.B else w.B
-    w.prime()
-    u = uh.clone()
     ks = w.kernels(nb)
-    kt = {}
-    for name, pid, stage, _, _ in ks:
-        kt[name] = time_kernel(plan, u, pid, stage)
+    kt = time_in_step(plan, uh, w.prime_pass, w.step_sequence(nb))
     # dominant kernel family = the largest share of the step's kernel time
     fam = {}
     for name, pid, stage, nbytes, per_step in ks:
@@ -437,12 +554,17 @@ def main():
         return
 
     # ---------------------------------------------------------------- baselines (oracle restatement)
-    threads = min(16, os.cpu_count() or 1)
+    host = host_cpu()
+    # the host cores this process may use, capped at the 16 CPUs a one-GPU box allots per GPU
+    threads = min(CPU_SHARE, host["usable_cpus"])
     torch.set_num_threads(threads)
-    log(f"[bench] cpu baseline: refpath float32 on {threads} threads, {args.cpu_steps} steps")
+    log(f"[bench] cpu baseline: refpath float32 on {threads} threads ({host['model']}, os.cpu_count() "
+        f"{host['os_cpu_count']}), {args.cpu_steps} steps")
     cpu_val, sample = w.cpu_baseline(args)
     result["cpu_baseline"] = {"value": cpu_val, "unit": "DOF·steps/s", "cores": threads, "kind": "port",
-                              "sample": sample}
+                              "sample": sample, "host": host,
+                              "threads_note": f"threads = min({CPU_SHARE}, usable CPUs): the box allots "
+                                              f"{CPU_SHARE} CPUs per GPU"}
     log(f"[bench] cpu baseline {cpu_val:.3e} DOF*steps/s")
 
     log(f"[bench] torch-GPU baseline: refpath on {dev}, {args.torch_steps} steps")
@@ -470,19 +592,8 @@ def main():
         del sch, ug, w, plan, uh, u
         torch.cuda.empty_cache()
         log(f"[bench] serving: {args.serve_requests} requests at {args.serve_rate}/s")
-        result["serving"] = serving_leg(dev, args.serve_requests, args.serve_rate)
-        sv = result["serving"]
-        log(f"[bench] serving: {sv['requests']} requests in {sv['makespan_s']:.2f} s, {sv['dof_steps_per_s']:.3e} "
-            f"DOF*steps/s, p50 {sv['p50_latency_ms']:.1f} ms, p99 {sv['p99_latency_ms']:.1f} ms; "
-            f"{sv['throughput_vs_plain']:.2f}x plain per-request solves")
-        if args.serve_light_rate > 0:
-            n_light = max(1, args.serve_requests // 3)
-            log(f"[bench] serving (light load): {n_light} requests at {args.serve_light_rate}/s")
-            lt = serving_leg(dev, n_light, args.serve_light_rate, seed=1, plain_sample=0)
-            lt.pop("plain_per_request", None)
-            lt.pop("throughput_vs_plain", None)
-            result["serving_light"] = lt
-            log(f"[bench] serving (light load): p50 {lt['p50_latency_ms']:.1f} ms, p99 {lt['p99_latency_ms']:.1f} ms")
+        rates = [float(x) for x in args.serve_rates.split(",") if x.strip()] if args.serve_rates else []
+        result["serving"] = serving_leg(dev, args.serve_requests, args.serve_rate, rates=rates)
     print(json.dumps(result), flush=True)
 
 

@@ -208,8 +208,14 @@ int tfsm_step(tfsm_plan* plan, void* uh, int32_t nsteps, const int32_t* rec_slot
 int tfsm_step_noalloc(tfsm_plan* plan, void* uh, int32_t nsteps, const int32_t* rec_slot, void* traj, int32_t T,
                       void* stream);
 
-/* Like tfsm_step for 1D plans, with a per-sample step count (device int32 [B]); samples stop
- * independently (continuous batching of 1D requests). */
+/* Like tfsm_step (no recording), with a per-sample step count (device int32 [B]): sample b advances
+ * by min(nsteps_per_sample[b], max_steps) steps and is not touched after that (a count of 0 leaves
+ * it as it is).  Continuous batching of requests: a slot whose request finishes inside a chunk
+ * stops there, an empty slot is not stepped.  1D plans run one persistent launch; 2D / 3D plans
+ * launch their kernels max_steps times and every workgroup of a finished sample returns at once.
+ * Not available for ETDRK0, for 1D KSConvection, or with KSConvection's remove_mean (its mean
+ * couples the samples).  Replaces, per request, the hot loop of OperatorLike.integrate
+ * (_base.py:729-746). */
 int tfsm_step_varying(tfsm_plan* plan, void* uh, const int32_t* nsteps_per_sample, int32_t max_steps, void* ws,
                       void* stream);
 
@@ -285,6 +291,13 @@ int tfsm_adjoint_step(tfsm_plan* plan, void* u, const void* ubar_next, void* uba
 /* Capture nsteps of tfsm_step (no recording) on uh/ws into a HIP graph for repeated launch.
  * (No reference counterpart: the reference launches ~116 kernels per step from Python.) */
 int tfsm_graph_create(tfsm_plan* plan, void* uh, int32_t nsteps, void* ws, void* stream, tfsm_graph** out);
+
+/* Capture max_steps of tfsm_step_varying (2D / 3D plans) into a HIP graph.  The kernels read the
+ * per-sample counts from nsteps_per_sample (device int32 [B]) at every launch of the graph, so the
+ * caller rewrites that array (on the launch stream) before each replay.
+ * (No reference counterpart: launch batching of the serving loop.) */
+int tfsm_graph_create_varying(tfsm_plan* plan, void* uh, const int32_t* nsteps_per_sample, int32_t max_steps,
+                              void* ws, void* stream, tfsm_graph** out);
 int tfsm_graph_launch(tfsm_graph* graph, void* stream);
 int tfsm_graph_destroy(tfsm_graph* graph);
 

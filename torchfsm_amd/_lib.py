@@ -87,6 +87,7 @@ SIGNATURES = {
     "tfsm_adjoint_prepare": (ctypes.c_int, [VP, I32, VP]),
     "tfsm_adjoint_step": (ctypes.c_int, [VP, VP, VP, VP, VP, VP, VP, VP]),
     "tfsm_graph_create": (ctypes.c_int, [VP, VP, I32, VP, VP, ctypes.POINTER(VP)]),
+    "tfsm_graph_create_varying": (ctypes.c_int, [VP, VP, VP, I32, VP, VP, ctypes.POINTER(VP)]),
     "tfsm_graph_launch": (ctypes.c_int, [VP, VP]),
     "tfsm_graph_destroy": (ctypes.c_int, [VP]),
 }

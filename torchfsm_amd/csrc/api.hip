@@ -304,22 +304,25 @@ Col2Args col_args(tfsm_plan* p, float2* U, char* ws) {
   return c;
 }
 
-int launch_row(tfsm_plan* p, char* ws, int b0, int nb, hipStream_t st) {
-  HIP_TRY(launch_row2d(p->term, row_args(p, ws, b0, nb), p->row_var, p->row_wg, st), "row kernel launch");
+int launch_row(tfsm_plan* p, char* ws, int b0, int nb, hipStream_t st, StepMask m = StepMask()) {
+  Row2Args r = row_args(p, ws, b0, nb);
+  r.mask = m;
+  HIP_TRY(launch_row2d(p->term, r, p->row_var, p->row_wg, st), "row kernel launch");
   return TFSM_OK;
 }
 
-int launch_col(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hipStream_t st) {
+int launch_col(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hipStream_t st, StepMask m = StepMask()) {
   Col2Args c = col_args(p, U, ws);
   c.b0 = b0;
+  c.mask = m;
   HIP_TRY(launch_col2d(p->term, c, nb, stage, p->V, p->col_var, st), "column kernel launch");
   return TFSM_OK;
 }
 
 // one nonlinear-evaluation pair of 2D kernels for samples [b0, b0 + nb)
-int run2d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hipStream_t st) {
+int run2d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hipStream_t st, StepMask m) {
   if (stage != ST_PRO) {
-    int rc = launch_row(p, ws, b0, nb, st);
+    int rc = launch_row(p, ws, b0, nb, st, m);
     if (rc) return rc;
     // KSConvection's mean couples the batch (the plan never splits it into sub-batches)
     if (p->ks_mean)
@@ -327,7 +330,7 @@ int run2d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hip
                            (float2*)(ws + p->off_DC), st),
               "batch-mean launch");
   }
-  return launch_col(p, U, ws, stage, b0, nb, st);
+  return launch_col(p, U, ws, stage, b0, nb, st, m);
 }
 
 // ---------------------------------------------------------------- 3D evaluation
@@ -372,7 +375,8 @@ Col3Args col3_args(tfsm_plan* p, float2* U, char* ws, int b0, int nb) {
 // one nonlinear evaluation of the 3D convection term for samples [b0, b0 + nb):
 // z rows (products) -> y columns forward -> x columns (combine, fields) -> y columns inverse
 // `parts` selects kernels (bit 0 z rows, 1 y forward, 2 x columns, 3 y inverse) for profiling.
-int run3d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hipStream_t st, int parts = 15) {
+int run3d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hipStream_t st, int parts = 15,
+               StepMask m = StepMask()) {
   const int N0 = p->n[0], N1 = p->n[1];
   float2* Pp = (float2*)(ws + p->off_P);
   if (stage != ST_PRO && (parts & 1)) {
@@ -390,18 +394,23 @@ int run3d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hip
     r.P = Pp;
     r.b0 = b0;
     r.nb = nb;
+    r.mask = m;
     HIP_TRY(launch_row3d(r, p->row_var, st), "3D row kernel launch");
   }
   if (stage != ST_PRO && (parts & 2)) {
     float2* Pb = Pp + (size_t)b0 * p->C * p->modes;
     HIP_TRY(launch_cols_c2c(N1, -1, Pb, Pb, (size_t)nb * p->C * N0, (size_t)N1 * p->P, (size_t)p->P, p->H, 1.0f,
-                            p->tw[1][0], st),
+                            p->tw[1][0], st, m, b0, (size_t)p->C * N0),
             "3D y-forward launch");
     if (p->ks_mean)
       HIP_TRY(launch_ks_dc(Pp, p->B, p->modes, N0, (size_t)N1 * p->P, (float2*)(ws + p->off_DC), st),
               "batch-mean launch");
   }
-  if (parts & 4) HIP_TRY(launch_col3d(col3_args(p, U, ws, b0, nb), nb, stage, p->col_var, st), "3D column kernel launch");
+  if (parts & 4) {
+    Col3Args c = col3_args(p, U, ws, b0, nb);
+    c.mask = m;
+    HIP_TRY(launch_col3d(c, nb, stage, p->col_var, st), "3D column kernel launch");
+  }
   if (!(parts & 8)) return TFSM_OK;
   Yinv3Args y;
   y.N0 = N0;
@@ -417,12 +426,13 @@ int run3d_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hip
   y.G2 = (float2*)(ws + p->off_G2);
   y.b0 = b0;
   y.nb = nb;
+  y.mask = m;
   HIP_TRY(launch_yinv3d(y, p->col_var, st), "3D y-inverse launch");
   return TFSM_OK;
 }
 
-int run_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hipStream_t st) {
-  return p->D == 3 ? run3d_eval(p, U, ws, stage, b0, nb, st) : run2d_eval(p, U, ws, stage, b0, nb, st);
+int run_eval(tfsm_plan* p, float2* U, char* ws, int stage, int b0, int nb, hipStream_t st, StepMask m = StepMask()) {
+  return p->D == 3 ? run3d_eval(p, U, ws, stage, b0, nb, st, 15, m) : run2d_eval(p, U, ws, stage, b0, nb, st, m);
 }
 
 Run1Args run1_args(tfsm_plan* p, float2* U, char* ws) {
@@ -456,7 +466,10 @@ Run1Args run1_args(tfsm_plan* p, float2* U, char* ws) {
   return a;
 }
 
-int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* traj, int T, void* ws, hipStream_t st) {
+// act: optional per-sample step counts (device int32 [B], 2D / 3D): sample b advances by
+// min(act[b], nsteps) steps and is not touched after that
+int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* traj, int T, void* ws, hipStream_t st,
+              const int* act = nullptr) {
   float2* U = (float2*)uh;
   if (p->scheme == ETDRK0) {
     if (p->cplx)
@@ -520,10 +533,15 @@ int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* tra
       const int nb = (p->B - b0) < sb ? (p->B - b0) : sb;
       float2* Ub = U + (size_t)b0 * per;
       float2* Tb = traj ? (float2*)traj + (size_t)b0 * T * per : nullptr;
-      rc = run_eval(p, U, (char*)ws, ST_PRO, b0, nb, st);
+      StepMask m;
+      m.act = act;
+      m.it = 0;
+      rc = run_eval(p, U, (char*)ws, ST_PRO, b0, nb, st, m);
       for (int i = 0; i < nsteps && !rc; ++i) {
+        m.it = i;
         if (rec_slot) rc = launch_status(launch_record(Ub, Tb, rec_slot, i, nb, per, T, st), "record launch");
-        for (int e = 0; e < nev && !rc; ++e) rc = run_eval(p, U, (char*)ws, scheme_stage(p->scheme, e), b0, nb, st);
+        for (int e = 0; e < nev && !rc; ++e)
+          rc = run_eval(p, U, (char*)ws, scheme_stage(p->scheme, e), b0, nb, st, m);
       }
       if (rec_slot && !rc) rc = launch_status(launch_record(Ub, Tb, rec_slot, nsteps, nb, per, T, st), "record launch");
     }
@@ -1287,13 +1305,18 @@ int tfsm_step_varying(tfsm_plan* p, void* uh, const int32_t* nsteps_per_sample, 
   if (!p || !uh || !nsteps_per_sample || !ws) return fail(TFSM_E_INVALID, "null argument");
   if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
   if (max_steps < 0) return fail(TFSM_E_INVALID, "max_steps must be >= 0");
-  if (p->D != 1 || p->scheme == ETDRK0) return fail(TFSM_E_INVALID, "per-sample step counts need a nonlinear 1D plan");
-  if (p->term == TERM_KS) return fail(TFSM_E_INVALID, "per-sample step counts are not available for KSConvection");
-  Run1Args a = run1_args(p, (float2*)uh, (char*)ws);
-  a.nsteps = max_steps;
-  a.nsteps_per = nsteps_per_sample;
-  HIP_TRY(launch_run1d(a, (hipStream_t)stream), "1D stepper launch");
-  return TFSM_OK;
+  if (p->scheme == ETDRK0) return fail(TFSM_E_INVALID, "per-sample step counts need a nonlinear plan");
+  if (p->term == TERM_KS && (p->D == 1 || p->ks_mean))
+    return fail(TFSM_E_INVALID, "per-sample step counts are not available for KSConvection in 1D or with remove_mean "
+                                "(its mean couples the samples)");
+  if (p->D == 1) {
+    Run1Args a = run1_args(p, (float2*)uh, (char*)ws);
+    a.nsteps = max_steps;
+    a.nsteps_per = nsteps_per_sample;
+    HIP_TRY(launch_run1d(a, (hipStream_t)stream), "1D stepper launch");
+    return TFSM_OK;
+  }
+  return step_impl(p, uh, max_steps, nullptr, nullptr, 0, ws, (hipStream_t)stream, nsteps_per_sample);
 }
 
 int tfsm_apply(tfsm_plan* p, const void* uh, void* out, void* ws, void* stream) {
@@ -1545,10 +1568,11 @@ int tfsm_adjoint_step(tfsm_plan* p, void* u, const void* ubar_next, void* ubar, 
   return TFSM_OK;
 }
 
-int tfsm_graph_create(tfsm_plan* p, void* uh, int32_t nsteps, void* ws, void* stream, tfsm_graph** out) {
-  if (!p || !uh || !out) return fail(TFSM_E_INVALID, "null argument");
-  if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
-  if (nsteps < 1) return fail(TFSM_E_INVALID, "a graph needs nsteps >= 1");
+}  // extern "C"
+
+namespace {
+// nsteps of step_impl (optionally with per-sample counts) captured into an instantiated graph
+int graph_capture(tfsm_plan* p, void* uh, int nsteps, const int* act, void* ws, void* stream, tfsm_graph** out) {
   *out = nullptr;
   hipStream_t cap = nullptr;
   HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "capture stream");
@@ -1566,7 +1590,7 @@ int tfsm_graph_create(tfsm_plan* p, void* uh, int32_t nsteps, void* ws, void* st
     (void)hipStreamDestroy(cap);
     return hip_fail(e, "begin capture");
   }
-  int rc = step_impl(p, uh, nsteps, nullptr, nullptr, 0, ws, cap);
+  int rc = step_impl(p, uh, nsteps, nullptr, nullptr, 0, ws, cap, act);
   e = hipStreamEndCapture(cap, &g->graph);
   if (rc == TFSM_OK && e != hipSuccess) rc = hip_fail(e, "end capture");
   if (rc == TFSM_OK) {
@@ -1580,6 +1604,28 @@ int tfsm_graph_create(tfsm_plan* p, void* uh, int32_t nsteps, void* ws, void* st
   }
   *out = g;
   return TFSM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int tfsm_graph_create(tfsm_plan* p, void* uh, int32_t nsteps, void* ws, void* stream, tfsm_graph** out) {
+  if (!p || !uh || !out) return fail(TFSM_E_INVALID, "null argument");
+  if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
+  if (nsteps < 1) return fail(TFSM_E_INVALID, "a graph needs nsteps >= 1");
+  return graph_capture(p, uh, nsteps, nullptr, ws, stream, out);
+}
+
+int tfsm_graph_create_varying(tfsm_plan* p, void* uh, const int32_t* nsteps_per_sample, int32_t max_steps, void* ws,
+                              void* stream, tfsm_graph** out) {
+  if (!p || !uh || !nsteps_per_sample || !ws || !out) return fail(TFSM_E_INVALID, "null argument");
+  if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
+  if (max_steps < 1) return fail(TFSM_E_INVALID, "a graph needs max_steps >= 1");
+  if (p->D < 2 || p->scheme == ETDRK0) return fail(TFSM_E_INVALID, "per-sample step graphs need a nonlinear 2D or 3D plan");
+  if (p->term == TERM_KS && p->ks_mean)
+    return fail(TFSM_E_INVALID, "per-sample step counts are not available for KSConvection with remove_mean "
+                                "(its mean couples the samples)");
+  return graph_capture(p, uh, max_steps, nsteps_per_sample, ws, stream, out);
 }
 
 int tfsm_graph_launch(tfsm_graph* g, void* stream) {

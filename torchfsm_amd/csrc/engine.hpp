@@ -425,6 +425,15 @@ __device__ __forceinline__ float2 nl_value(float coef_t, float2 f, const float2*
 }
 
 // ---------------------------------------------------------------- kernel argument blocks
+// Per-sample step counts (continuous batching of 2D / 3D requests): a launch belongs to step `it`
+// of its tfsm_step_varying call, and sample b takes part only while it < act[b]; its workgroups
+// return before touching memory otherwise.  act = null: every sample steps.
+struct StepMask {
+  const int* act = nullptr;
+  int it = 0;
+  __device__ __forceinline__ bool skip(int b) const { return act && it >= act[b]; }
+};
+
 struct Col2Args {
   int B, C, N0, N1, H1, P1, PF;
   int kmax0, kmax1;
@@ -449,6 +458,7 @@ struct Col2Args {
   const float2* dc = nullptr;  // KSConvection: batch mean of the product DC (subtracted at k = 0), or null
   int b0;                     // first sample handled by this launch
   int nb;                     // samples handled by this launch (set by the launcher)
+  StepMask mask;
 };
 
 struct Row2Args {
@@ -458,6 +468,7 @@ struct Row2Args {
   const float2* G;
   float2* P;
   int b0;
+  StepMask mask;
 };
 
 struct Col3Args {
@@ -482,6 +493,7 @@ struct Col3Args {
   float2* G1;                 // [B][2 nc][N0][K1][PF2]
   const float2* dc = nullptr;  // KSConvection: batch mean of the product DC, or null
   int b0, nb;
+  StepMask mask;
 };
 
 struct Yinv3Args {
@@ -492,6 +504,7 @@ struct Yinv3Args {
   const float2* G1;
   float2* G2;                 // [B][3 nc][N0][N1][PF2]
   int b0, nb;
+  StepMask mask;
 };
 
 struct Row3Args {
@@ -502,6 +515,7 @@ struct Row3Args {
   const float2* G2;
   float2* P;                  // products [B][nc][N0][N1][P2]
   int b0, nb;
+  StepMask mask;
 };
 
 struct Run1Args {
@@ -582,8 +596,10 @@ hipError_t launch_rows_c2r(int n, const float2* in, size_t in_ld, float* out, si
                            const float2* tw, hipStream_t st);
 hipError_t launch_rows_c2c(int n, int dir, const float2* in, size_t in_ld, float2* out, size_t out_ld, int nrows,
                            float scale, const float2* tw, hipStream_t st);
+// with a mask, outer block o of the launch belongs to sample b0 + o / outer_per_sample
 hipError_t launch_cols_c2c(int n, int dir, const float2* in, float2* out, size_t nouter, size_t outer_stride,
-                           size_t col_stride, int ninner, float scale, const float2* tw, hipStream_t st);
+                           size_t col_stride, int ninner, float scale, const float2* tw, hipStream_t st,
+                           StepMask mask = StepMask(), int b0 = 0, size_t outer_per_sample = 1);
 hipError_t launch_full_to_half(const float2* full, float2* half, const ConvArgs& a, hipStream_t st);
 hipError_t launch_half_to_full(const float2* half, float2* full, const ConvArgs& a, hipStream_t st);
 hipError_t launch_linear_steps(float2* U, const float* E, size_t coef_bstride, size_t modes, int C, size_t total,

@@ -145,6 +145,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
     bl = blockIdx.x / ntiles;
   }
   const int b = a.b0 + bl;
+  if (a.mask.skip(b)) return;  // uniform per workgroup: this sample's steps are done
   const int ky = tile * V + vi;
   const bool colv = ky < a.H1;
   const size_t plane = (size_t)N0 * a.P1;
@@ -266,8 +267,9 @@ __global__ void __launch_bounds__((Fft<N1, VAR>::NT * ROWS)) k2_row(Row2Args a) 
   // with NT a multiple of 64 a wavefront holds one row: the row (and every row pointer) is uniform,
   // so the loads and stores take a scalar base and a 32-bit lane offset
   if constexpr (NT % 64 == 0) row = __builtin_amdgcn_readfirstlane(row);
-  const bool rv = row < a.B * a.N0;
-  const int b = a.b0 + (rv ? row / a.N0 : 0);
+  const bool in_range = row < a.B * a.N0;
+  const int b = a.b0 + (in_range ? row / a.N0 : 0);
+  const bool rv = in_range && !a.mask.skip(b);  // rows of samples whose steps are done stay idle
   const int x = rv ? row % a.N0 : 0;
   float2* lds = s_buf + g * SL;
   const size_t fs = (size_t)a.N0 * a.PF;

@@ -56,6 +56,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
   const int bl = slot % a.nb;
   if (tile >= ntiles) return;
   const int b = a.b0 + bl;
+  if (a.mask.skip(b)) return;
   const int ky = tile / nzt;
   const int kz = (tile % nzt) * V + vi;
   const bool colv = kz < a.H2;
@@ -176,6 +177,7 @@ __global__ void __launch_bounds__((Fft<N1, VAR>::NT * V)) k3_yinv(Yinv3Args a) {
   const int kz = (blockIdx.x % nzt) * V + vi;
   const int x = (blockIdx.x / nzt) % a.N0;
   const int b = a.b0 + blockIdx.x / (nzt * a.N0);
+  if (a.mask.skip(b)) return;
   const bool kv = kz <= a.kmax2;
 #pragma unroll 1
   for (int c = 0; c < NCH; ++c) {
@@ -234,6 +236,7 @@ __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) 
   const int bl = (int)(row0 / nxy);
   const int xy0 = (int)(row0 % nxy);
   const int b = a.b0 + bl;
+  if (a.mask.skip(b)) return;  // whole workgroup (its rows belong to one sample)
   const int PF2 = a.PF2;
   const size_t fs = (size_t)nxy * PF2;  // one G2 field of one sample
   {

@@ -95,11 +95,12 @@ __global__ void __launch_bounds__(Fft<N>::NT* ROWS)
 template <int N, int V, int DIR>
 __global__ void __launch_bounds__(Fft<N>::NT* V)
     k_cols_c2c(const float2* in, float2* out, size_t outer_stride, size_t col_stride, int ninner, float scale,
-               const float2* tw) {
+               const float2* tw, StepMask mask, int b0, unsigned o0, unsigned outer_per_sample) {
   using F = Fft<N>;
   constexpr int RMAX = F::RMAX, NT = F::NT;
   __shared__ float2 s_tw[Fft<N>::tw_size()];
   __shared__ float2 s_buf[LdsSize<N, V>::slots];
+  if (mask.act && mask.skip(b0 + (int)((o0 + blockIdx.y) / outer_per_sample))) return;
   load_twiddles<N>(s_tw, tw);
   __syncthreads();
   const int vi = threadIdx.x % V, j = threadIdx.x / V;
@@ -536,9 +537,11 @@ hipError_t launch_rows_c2c(int n, int dir, const float2* in, size_t in_ld, float
 }
 
 hipError_t launch_cols_c2c(int n, int dir, const float2* in, float2* out, size_t nouter, size_t outer_stride,
-                           size_t col_stride, int ninner, float scale, const float2* tw, hipStream_t st) {
+                           size_t col_stride, int ninner, float scale, const float2* tw, hipStream_t st,
+                           StepMask mask, int b0, size_t outer_per_sample) {
   if (nouter == 0 || ninner <= 0) return hipSuccess;
   if (n > 512) return hipErrorInvalidValue;
+  if (mask.act && (outer_per_sample == 0 || nouter > 0xffffffffu)) return hipErrorInvalidValue;
   TFSM_SWITCH_N(n, {
     constexpr int V = (NN > 512) ? 1 : ((Fft<NN>::NT * 16 <= 1024) ? 16 : 8);
     // grid.y is limited to 65535: launch the outer blocks in chunks
@@ -547,12 +550,13 @@ hipError_t launch_cols_c2c(int n, int dir, const float2* in, float2* out, size_t
       dim3 grid((ninner + V - 1) / V, (unsigned)no);
       const float2* ib = in + o0 * outer_stride;
       float2* ob = out + o0 * outer_stride;
+      const unsigned ops = (unsigned)(outer_per_sample ? outer_per_sample : 1);
       if (dir < 0)
         hipLaunchKernelGGL((k_cols_c2c<NN, V, -1>), grid, dim3(Fft<NN>::NT * V), 0, st, ib, ob, outer_stride,
-                           col_stride, ninner, scale, tw);
+                           col_stride, ninner, scale, tw, mask, b0, (unsigned)o0, ops);
       else
         hipLaunchKernelGGL((k_cols_c2c<NN, V, 1>), grid, dim3(Fft<NN>::NT * V), 0, st, ib, ob, outer_stride,
-                           col_stride, ninner, scale, tw);
+                           col_stride, ninner, scale, tw, mask, b0, (unsigned)o0, ops);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
     }

@@ -333,8 +333,11 @@ class SpectralPlan:
         return uh
 
     def step_varying(self, uh: torch.Tensor, nsteps_per_sample: torch.Tensor, max_steps: int):
+        """Advance sample b of uh by min(nsteps_per_sample[b], max_steps) steps (device int32 [B])."""
         self._check_state(uh)
         self._check(nsteps_per_sample, (), torch.int32, "steps per sample", self.batch)
+        if int(max_steps) < 0:
+            raise ValueError("max_steps must be >= 0")
         ws = self.workspace()
         with torch.cuda.device(self.device):
             check(lib.tfsm_step_varying(self._plan, _ptr(uh), _ptr(nsteps_per_sample), int(max_steps), _ptr(ws),
@@ -407,23 +410,31 @@ class SpectralPlan:
             check(lib.tfsm_launch_pass(self._plan, _ptr(uh), _ptr(ws), int(pass_id), int(stage),
                                        _stream(self.device)), "launch_pass")
 
-    def graph(self, uh: torch.Tensor, nsteps: int) -> "StepGraph":
-        return StepGraph(self, uh, nsteps)
+    def graph(self, uh: torch.Tensor, nsteps: int, counts: Optional[torch.Tensor] = None) -> "StepGraph":
+        return StepGraph(self, uh, nsteps, counts)
 
 
 class StepGraph:
-    """nsteps of tfsm_step captured into a HIP graph on fixed state/workspace buffers."""
+    """nsteps of tfsm_step captured into a HIP graph on fixed state/workspace buffers.  With
+    `counts` (device int32 [B]), nsteps of tfsm_step_varying: every replay reads the per-sample
+    step counts from that tensor, which the caller updates before launching."""
 
-    def __init__(self, plan: SpectralPlan, uh: torch.Tensor, nsteps: int):
+    def __init__(self, plan: SpectralPlan, uh: torch.Tensor, nsteps: int, counts: Optional[torch.Tensor] = None):
         self.plan = plan
         self.uh = uh
         self.nsteps = nsteps
+        self.counts = counts
         self._g = ctypes.c_void_p()
         plan._check_state(uh)
         ws = plan.workspace()
         with torch.cuda.device(plan.device):
-            check(lib.tfsm_graph_create(plan.handle, _ptr(uh), int(nsteps), _ptr(ws), _stream(plan.device),
-                                        ctypes.byref(self._g)), "graph")
+            if counts is None:
+                check(lib.tfsm_graph_create(plan.handle, _ptr(uh), int(nsteps), _ptr(ws), _stream(plan.device),
+                                            ctypes.byref(self._g)), "graph")
+            else:
+                plan._check(counts, (), torch.int32, "steps per sample", plan.batch)
+                check(lib.tfsm_graph_create_varying(plan.handle, _ptr(uh), _ptr(counts), int(nsteps), _ptr(ws),
+                                                    _stream(plan.device), ctypes.byref(self._g)), "graph")
 
     def launch(self):
         with torch.cuda.device(self.plan.device):

@@ -10,10 +10,15 @@ Requests of one (kind, N) form a shape class.  A class owns one engine plan with
 of sample slots and per-sample coefficients: admitting a request re-targets one slot
 (tfsm_plan_set_sample: its L and ETD coefficients, and its axis table for 1D lengths) and writes
 the forward transform of its field into that slot.  Each class steps on its own HIP stream in
-chunks: 1D classes advance every slot by its own count in one persistent launch
-(tfsm_step_varying); 2D and 3D classes advance all slots by the smallest remaining count, replaying
-captured HIP graphs of 2^k steps.  Between chunks, finished slots are harvested (inverse
-transform, copy to the host) and queued requests are admitted.  One host thread drives all classes.
+chunks, and every slot advances by its own count within a chunk (tfsm_step_varying): a slot whose
+request finishes inside the chunk stops there, and an empty slot is not stepped at all.  1D classes
+run one persistent launch per chunk; 2D and 3D classes replay a captured HIP graph of 2^k steps
+whose kernels read the per-slot counts from device memory.  A chunk is at most MAX_CHUNK steps
+(and at most the longest remaining request), so queued requests wait at most one chunk for a free
+slot.  Between chunks, finished slots are harvested asynchronously: the inverse transform and the
+copy into a pinned host buffer are queued on the class stream behind an event, and the result is
+handed out when the host sees that event complete, so the scheduler thread never blocks on a
+copy.  One host thread drives all classes.
 
 Results do not depend on the mix: samples never interact, launch shapes are fixed per shape class
 (never per occupancy), and every slot's coefficients come from the same kernels.  A request
@@ -47,7 +52,8 @@ DEFAULT_SLOTS = {
     ("burgers3d", 32): 16, ("burgers3d", 64): 8, ("burgers3d", 128): 4,
 }
 
-MAX_GRAPH_LOG2 = 8  # graphs of 1, 2, 4, ..., 256 steps
+MAX_GRAPH_LOG2 = 6  # graphs of 1, 2, 4, ..., 64 steps
+MAX_CHUNK = 1 << MAX_GRAPH_LOG2  # steps per chunk at most: bounds the wait of a queued request
 
 
 @dataclass
@@ -68,10 +74,21 @@ class Result:
     arrival: float
     admitted: float
     completed: float
+    key: Tuple[str, int] = ("", 0)  # shape class (kind, N)
 
     @property
     def latency(self) -> float:
         return self.completed - self.arrival
+
+    @property
+    def queue(self) -> float:
+        """Time from arrival to admission into a slot."""
+        return self.admitted - self.arrival
+
+    @property
+    def service(self) -> float:
+        """Time from admission until the result is on the host."""
+        return self.completed - self.admitted
 
 
 def linear_coefs(kind: str, param: float) -> List[float]:
@@ -130,6 +147,12 @@ class ShapeClass:
         self.event = None
         self.graphs: Dict[int, StepGraph] = {}
         self.counts = torch.zeros(self.slots, dtype=torch.int32, device=self.device)
+        # per-slot step counts of the next chunk: staged in pinned host memory, copied on the stream
+        self.counts_host = torch.zeros(self.slots, dtype=torch.int32, pin_memory=True)
+        # harvest buffers: inverse transforms on the device, their copies in pinned host memory
+        self.out_dev = torch.empty((self.slots, ch) + (self.n,) * dims, dtype=torch.float32, device=self.device)
+        self.out_host = torch.empty((self.slots, ch) + (self.n,) * dims, dtype=torch.float32, pin_memory=True)
+        self.pending: List[Tuple[torch.cuda.Event, List[Tuple[int, Request, float]]]] = []
         self.last_used = time.perf_counter()
         self.reserved_bytes = self.bytes_estimate(kind, n, slots)
         # per-class counters: chunks, steps, slot-steps of active requests, device time (ms)
@@ -142,7 +165,8 @@ class ShapeClass:
         dims, ch = SPECS[kind][0], SPECS[kind][1]
         half = (n // 2 + 1 + 15) // 16 * 16 * n ** (dims - 1) * 8
         factor = {1: 4, 2: 7, 3: 12}[dims]  # state + stage arrays + products + pruned fields
-        return slots * ch * half * factor + slots * 8 * 4 * half // 8 + (64 << 20)
+        outputs = 2 * slots * ch * n ** dims * 4  # harvest buffer on the device (and pinned on the host)
+        return slots * ch * half * factor + slots * 8 * 4 * half // 8 + outputs + (64 << 20)
 
     @property
     def nbytes(self) -> int:
@@ -180,60 +204,65 @@ class ShapeClass:
         g = self.graphs.get(nsteps)
         if g is None:
             with torch.cuda.stream(self.stream):
-                g = StepGraph(self.plan, self.uh, nsteps)
+                g = StepGraph(self.plan, self.uh, nsteps, counts=self.counts)
             self.graphs[nsteps] = g
         return g
 
     def launch(self) -> int:
-        """Start the next chunk on the class stream; returns its step count."""
+        """Start the next chunk on the class stream; returns its step count.  The chunk is the
+        largest power of two not above min(MAX_CHUNK, longest remaining request); each slot
+        advances by min(its remaining steps, chunk), empty slots by 0."""
         act = self.active()
         if not act:
             return 0
+        longest = min(max(self.left[b] for b in act), MAX_CHUNK)
+        k = 1 << (longest.bit_length() - 1)
+        counts = [min(self.left[b], k) if self.req[b] is not None else 0 for b in range(self.slots)]
         with torch.cuda.stream(self.stream):
             self.ev0 = torch.cuda.Event(enable_timing=True)
             self.ev0.record(self.stream)
+            # the previous chunk's copy of the counts has completed (the host saw its event)
+            self.counts_host.copy_(torch.tensor(counts, dtype=torch.int32))
+            self.counts.copy_(self.counts_host, non_blocking=True)
             if self.dims == 1:
-                k = min(max(self.left[b] for b in act), 1 << MAX_GRAPH_LOG2)
-                counts = [min(self.left[b], k) if b in act else 0 for b in range(self.slots)]
-                self.counts.copy_(torch.tensor(counts, dtype=torch.int32))
                 self.plan.step_varying(self.uh, self.counts, k)
-                for b in act:
-                    self.left[b] -= counts[b]
-                self.stats["slot_steps"] += sum(counts)
             else:
-                k = min(min(self.left[b] for b in act), (1 << (MAX_GRAPH_LOG2 + 1)) - 1)
-                rest = k
-                for e in range(MAX_GRAPH_LOG2, -1, -1):
-                    if rest >= (1 << e):
-                        self._graph(1 << e).launch()
-                        rest -= 1 << e
-                for b in act:
-                    self.left[b] -= k
-                self.stats["slot_steps"] += k * len(act)
+                self._graph(k).launch()
             self.event = torch.cuda.Event(enable_timing=True)
             self.event.record(self.stream)
+        for b in act:
+            self.left[b] -= counts[b]
+        self.stats["slot_steps"] += sum(counts)
         self.stats["chunks"] += 1
         self.stats["steps"] += k
         self.busy = True
         self.last_used = time.perf_counter()
         return k
 
-    def harvest(self, t0: float) -> List[Result]:
-        """Inverse-transform and return the finished slots (called when the chunk is done)."""
-        out = []
+    def harvest(self, now: float):
+        """Queue the inverse transform and host copy of every finished slot behind an event, and
+        free those slots (later work on the class stream is ordered after the copies)."""
         done = [b for b in range(self.slots) if self.req[b] is not None and self.left[b] == 0]
         if not done:
-            return out
+            return
         with torch.cuda.stream(self.stream):
-            fields = [self.plan.inverse(self.uh[b:b + 1]) for b in done]
             for b in done:
-                self.uh[b].zero_()
-            host = [f.cpu() for f in fields]
-        now = time.perf_counter() - t0
-        for b, u in zip(done, host):
-            r = self.req[b]
-            out.append(Result(r.rid, u, r.arrival, self.admitted[b], now))
+                self.out_dev[b:b + 1].copy_(self.plan.inverse(self.uh[b:b + 1]))
+                self.out_host[b].copy_(self.out_dev[b], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(self.stream)
+        self.pending.append((ev, [(b, self.req[b], self.admitted[b]) for b in done]))
+        for b in done:
             self.req[b] = None
+
+    def collect(self, t0: float) -> List[Result]:
+        """Results whose host copies have completed (never waits)."""
+        out = []
+        while self.pending and self.pending[0][0].query():
+            _, items = self.pending.pop(0)
+            now = time.perf_counter() - t0
+            for b, r, adm in items:
+                out.append(Result(r.rid, self.out_host[b:b + 1].clone(), r.arrival, adm, now, (self.kind, self.n)))
         return out
 
     def close(self):
@@ -269,7 +298,8 @@ class Server:
             raise RuntimeError(f"shape class {key} needs {need / 2**30:.2f} GiB, above the HBM budget "
                                f"({self.budget / 2**30:.2f} GiB)")
         while self.reserved() + need > self.budget:
-            idle = [k for k, v in self.classes.items() if not v.busy and all(r is None for r in v.req)]
+            idle = [k for k, v in self.classes.items()
+                    if not v.busy and not v.pending and all(r is None for r in v.req)]
             if not idle:
                 return None
             k = min(idle, key=lambda x: self.classes[x].last_used)
@@ -303,12 +333,14 @@ class Server:
             launched = False
             for key in list(set(waiting) | set(self.classes)):
                 c = self.classes.get(key)
+                if c is not None:
+                    results.extend(c.collect(t0))
                 if c is not None and c.busy:
                     if not c.event.query():
                         continue
                     c.busy = False
                     c.stats["gpu_ms"] += c.ev0.elapsed_time(c.event)
-                    results.extend(c.harvest(t0))
+                    c.harvest(time.perf_counter() - t0)
                 q = waiting.get(key)
                 if q:
                     if c is None:
@@ -333,16 +365,33 @@ class Server:
         self.classes.clear()
 
 
+def _pct(vals: Sequence[float], p: float) -> float:
+    v = sorted(vals)
+    i = min(len(v) - 1, max(0, int(math.ceil(p / 100.0 * len(v))) - 1))
+    return v[i]
+
+
 def latency_summary(results: Sequence[Result]) -> dict:
-    lat = sorted(r.latency for r in results)
+    lat = [r.latency for r in results]
     if not lat:
         return {}
+    return {"p50_ms": _pct(lat, 50) * 1e3, "p99_ms": _pct(lat, 99) * 1e3, "max_ms": max(lat) * 1e3, "n": len(lat)}
 
-    def pct(p):
-        i = min(len(lat) - 1, max(0, int(math.ceil(p / 100.0 * len(lat))) - 1))
-        return lat[i]
 
-    return {"p50_ms": pct(50) * 1e3, "p99_ms": pct(99) * 1e3, "max_ms": lat[-1] * 1e3, "n": len(lat)}
+def latency_by_class(results: Sequence[Result]) -> dict:
+    """Per shape class: request count, p50/p99 of latency, and its split into queueing (arrival to
+    admission) and service (admission to the result on the host), in milliseconds."""
+    out = {}
+    for key in sorted({r.key for r in results}):
+        rs = [r for r in results if r.key == key]
+        row = {"n": len(rs)}
+        for name, get in (("latency", lambda r: r.latency), ("queue", lambda r: r.queue),
+                          ("service", lambda r: r.service)):
+            vals = [get(r) for r in rs]
+            row[f"{name}_p50_ms"] = round(_pct(vals, 50) * 1e3, 2)
+            row[f"{name}_p99_ms"] = round(_pct(vals, 99) * 1e3, 2)
+        out[f"{key[0]}_{key[1]}"] = row
+    return out
 
 
 def synthetic_requests(n_requests: int, rate: float, seed: int = 0, kinds=("ks1d", "ns2d", "burgers3d"),
