@@ -1,6 +1,7 @@
 """Hand-written adjoint of the ETD step (torchfsm_amd/adjoint.py, tfsm_adjoint_step) against the
-reference's autograd gradients (golden fixture) and float64 central finite differences of the
-oracle.  Tolerances: relative error <= 2e-4 for every gradient (float32 adjoint vs float64 truth)."""
+reference's autograd gradients (golden fixture, SETDRK2), the oracle's float64 autograd and float64
+central finite differences of the oracle (SETDRK4 with Convection in 2D/3D and with
+VorticityConvection, ETDRK1/2, SETDRK1/2).  Tolerances: relative error <= 2e-4 for every gradient (float32 adjoint vs float64 truth)."""
 
 import math
 
@@ -165,4 +166,127 @@ def test_adjoint_1d_and_errors(dev):
         assert e <= TOL and eu <= TOL
     with pytest.raises(ValueError, match="gradients need one of"):
         op = pde.Burgers(torch.tensor(0.01, requires_grad=True))
+        op.set_integrator(SETDRKIntegrator.SETDRK3)
         op.integrate(u0.float().to(dev), mesh=tfa.MeshGrid([(0, 1, n)], device=dev), dt=dt, step=2)
+
+
+def _check_grads(tag, gu, gth, u0, th0, loss64, n_dirs=2, eps=1e-6, seed=41):
+    """Adjoint gradients (gu w.r.t. u0, gth w.r.t. the scalar theta) against float64 central finite
+    differences of loss64(u0, theta) along n_dirs seeded unit directions and along theta."""
+    fd_th = (loss64(u0, th0 + eps) - loss64(u0, th0 - eps)) / (2 * eps)
+    e_th = abs(gth - fd_th) / abs(fd_th)
+    print(f"{tag}: d/dtheta adjoint {gth:.6e} FD {fd_th:.6e} rel {e_th:.2e}", flush=True)
+    assert e_th <= TOL
+    gen = torch.Generator().manual_seed(seed)
+    for i in range(n_dirs):
+        d = torch.randn(u0.shape, generator=gen, dtype=torch.float64).to(u0.device)
+        d = d / d.norm()
+        fd = (loss64(u0 + eps * d, th0) - loss64(u0 - eps * d, th0)) / (2 * eps)
+        ad = float((gu.double() * d).sum())
+        e = abs(ad - fd) / max(abs(fd), 1e-30)
+        print(f"  direction {i}: adjoint {ad:.6e} FD {fd:.6e} rel {e:.2e}", flush=True)
+        assert e <= TOL
+
+
+def test_adjoint_setdrk4_burgers_2d(dev):
+    """The default ('auto' = SETDRK4) path: 2D Burgers(nu) 64^2, C=2, B=2, dt 0.01, 20 steps,
+    checkpoints every 10.  Against the oracle's float64 autograd (u0 and nu gradients) and float64
+    central finite differences."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import pde
+    from torchfsm_amd.workloads import fourier_modes_field
+
+    n, B, steps, dt = 64, 2, 20, 0.01
+    u0 = (0.1 * fourier_modes_field(n, 2, 2, B, seed=51)).to(dev)
+    target = (0.1 * fourier_modes_field(n, 2, 2, B, seed=52)).to(dev)
+    nu = torch.tensor(0.01, device=dev, requires_grad=True)
+    u = u0.clone().requires_grad_(True)
+    res = pde.Burgers(nu).integrate(u, mesh=tfa.MeshGrid([(0, 1, n)] * 2, device=dev), dt=dt, step=steps)
+    ((res - target) ** 2).mean().backward()
+    # the oracle's autograd in float64 through the same SETDRK4 schedule
+    nu64 = torch.tensor(0.01, dtype=torch.float64, device=dev, requires_grad=True)
+    u64 = u0.double().requires_grad_(True)
+    out = refpath.integrate(refpath.burgers_terms(nu64), [(0, 1, n)] * 2, u64, dt, steps, scheme="SETDRK4")
+    ((out - target.double()) ** 2).mean().backward()
+    e_u = _rel(u.grad.cpu().numpy(), u64.grad.cpu().numpy())
+    e_nu = abs(float(nu.grad) - float(nu64.grad)) / abs(float(nu64.grad))
+    print(f"SETDRK4 Burgers 2D vs oracle autograd f64: grad u0 rel {e_u:.2e}, grad nu rel {e_nu:.2e}", flush=True)
+    assert e_u <= TOL and e_nu <= TOL
+
+    def loss64(uu, th):
+        o = refpath.integrate(refpath.burgers_terms(th), [(0, 1, n)] * 2, uu, dt, steps, scheme="SETDRK4")
+        return float(((o - target.double()) ** 2).mean())
+
+    _check_grads("SETDRK4 Burgers 2D", u.grad, float(nu.grad), u0.double(), 0.01, loss64)
+
+
+def test_adjoint_setdrk4_navier_stokes_vorticity(dev):
+    """NavierStokesVorticity(Re, kolm_force(y)) (VorticityConvection, drag, explicit forcing), 64^2,
+    B=2, SETDRK4 ('auto'), dt 1e-3, 20 steps: gradients w.r.t. the initial vorticity and Re against
+    the oracle's float64 autograd and float64 central finite differences."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import field, pde
+    from torchfsm_amd.workloads import ns_initial_vorticity
+
+    n, B, steps, dt = 64, 2, 20, 1e-3
+    mi = [(0, 2 * math.pi, n)] * 2
+    mesh = tfa.MeshGrid(mi, device=dev)
+    y = mesh.bc_mesh_grid()[1]
+    w0 = ns_initial_vorticity(n, B, seed=61).to(dev)
+    target = ns_initial_vorticity(n, B, seed=62).to(dev)
+    Re = torch.tensor(100.0, device=dev, requires_grad=True)
+    w = w0.clone().requires_grad_(True)
+    res = pde.NavierStokesVorticity(Re, field.kolm_force(y)).integrate(w, mesh=mesh, dt=dt, step=steps)
+    ((res - target) ** 2).mean().backward()
+    Re64 = torch.tensor(100.0, dtype=torch.float64, device=dev, requires_grad=True)
+    w64 = w0.double().requires_grad_(True)
+    out = refpath.integrate(refpath.ns_terms(Re64, y.double()), mi, w64, dt, steps, scheme="SETDRK4")
+    ((out - target.double()) ** 2).mean().backward()
+    e_u = _rel(w.grad.cpu().numpy(), w64.grad.cpu().numpy())
+    e_re = abs(float(Re.grad) - float(Re64.grad)) / abs(float(Re64.grad))
+    print(f"SETDRK4 NS vorticity vs oracle autograd f64: grad w0 rel {e_u:.2e}, grad Re rel {e_re:.2e}", flush=True)
+    assert e_u <= TOL and e_re <= TOL
+
+    def loss64(ww, th):
+        o = refpath.integrate(refpath.ns_terms(th, y.double()), mi, ww, dt, steps, scheme="SETDRK4")
+        return float(((o - target.double()) ** 2).mean())
+
+    _check_grads("SETDRK4 NS vorticity", w.grad, float(Re.grad), w0.double(), 100.0, loss64, eps=1e-5)
+
+
+@pytest.mark.parametrize("scheme", ["ETDRK2", "SETDRK4"])
+def test_adjoint_3d_convection(dev, scheme):
+    """3D vector Burgers(nu) 16^3, C=3, B=1, dt 5e-3, 10 steps, checkpoints every 4: gradients
+    w.r.t. u0 and nu against float64 central finite differences of the oracle (and its float64
+    autograd for u0)."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import pde
+    from torchfsm_amd.integrator import ETDRKIntegrator, SETDRKIntegrator
+    from torchfsm_amd.workloads import fourier_modes_field
+
+    n, steps, dt = 16, 10, 5e-3
+    mi = [(0, 1, n)] * 3
+    u0 = (0.2 * fourier_modes_field(n, 3, 3, 1, seed=71)).to(dev)
+    target = (0.2 * fourier_modes_field(n, 3, 3, 1, seed=72)).to(dev)
+    nu = torch.tensor(0.02, device=dev, requires_grad=True)
+    u = u0.clone().requires_grad_(True)
+    op = pde.Burgers(nu)
+    op.set_integrator(ETDRKIntegrator.ETDRK2 if scheme == "ETDRK2" else SETDRKIntegrator.SETDRK4)
+    op.adjoint_checkpoint_every = 4
+    res = op.integrate(u, mesh=tfa.MeshGrid(mi, device=dev), dt=dt, step=steps)
+    ((res - target) ** 2).mean().backward()
+    u64 = u0.double().requires_grad_(True)
+    out = refpath.integrate(refpath.burgers_terms(0.02), mi, u64, dt, steps, scheme=scheme)
+    ((out - target.double()) ** 2).mean().backward()
+    e_u = _rel(u.grad.cpu().numpy(), u64.grad.cpu().numpy())
+    print(f"{scheme} 3D grad u0 vs oracle autograd f64: {e_u:.2e}", flush=True)
+    assert e_u <= TOL
+
+    def loss64(uu, th):
+        o = refpath.integrate(refpath.burgers_terms(th), mi, uu, dt, steps, scheme=scheme)
+        return float(((o - target.double()) ** 2).mean())
+
+    _check_grads(f"{scheme} 3D Burgers", u.grad, float(nu.grad), u0.double(), 0.02, loss64, n_dirs=1)

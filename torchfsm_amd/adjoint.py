@@ -7,9 +7,10 @@ Here the forward pass runs the fused engine and keeps the spectral state every `
 segment from its checkpoint and applies tfsm_adjoint_step to each, from the last step back.
 Memory: ceil(steps / every) + every states, instead of the reference's whole graph.
 
-Supported: Convection-based operators in 1D/2D (e.g. Burgers) with ETDRK1/2 or SETDRK1/2;
-gradients with respect to the initial field and to the coefficient of one linear term (a 0-d
-tensor, or one value per sample).
+Supported: Convection in 1D, 2D and 3D (e.g. Burgers) and VorticityConvection in 2D (e.g.
+NavierStokesVorticity, with explicit sources such as kolm_force), with ETDRK1/2, SETDRK1/2 and
+SETDRK4 (the 'auto' scheme of nonlinear operators, _base.py:451-455); gradients with respect to the
+initial field and to the coefficient of one linear term (a 0-d tensor, or one value per sample).
 """
 
 import math
@@ -21,14 +22,15 @@ from . import _lib
 from ._lib import check, lib
 from .engine import SpectralPlan, _ptr, _stream
 
-ADJOINT_SCHEMES = ("ETDRK1", "ETDRK2", "SETDRK1", "SETDRK2")
+ADJOINT_SCHEMES = ("ETDRK1", "ETDRK2", "SETDRK1", "SETDRK2", "SETDRK4")
 
 
 def _adjoint_workspace(plan: SpectralPlan) -> torch.Tensor:
+    # the size depends on the plan's current scheme (SETDRK4 keeps more stages)
+    nbytes = max(int(lib.tfsm_plan_adjoint_workspace_bytes(plan.handle)), 256)
     ws = getattr(plan, "_adj_ws", None)
-    if ws is None:
-        nbytes = int(lib.tfsm_plan_adjoint_workspace_bytes(plan.handle))
-        ws = torch.zeros(max(nbytes, 256), dtype=torch.uint8, device=plan.device)
+    if ws is None or ws.numel() < nbytes:
+        ws = torch.zeros(nbytes, dtype=torch.uint8, device=plan.device)
         plan._adj_ws = ws
     return ws
 

@@ -574,15 +574,20 @@ AdjArgs adj_args(tfsm_plan* p) {
   a.P = p->P;
   a.modes = p->modes;
   a.coef_t = p->coef_t;
+  a.term = p->term;
   return a;
 }
 
 struct AdjLayout {
-  size_t Aa, N0s, T1, Jv, Ab, Nb, gpart, F, Fs, X, Q, total;
+  size_t Aa, N0s, T1, Jv, Ab, Nb, Na, Ac, Nc, Cb, Bb, Abar, gpart, F, Fs, X, Q, total;
 };
 
+// fields and products of the transposed Jacobian (k_adj_fields / k_adj_products)
+int adj_nfa(const tfsm_plan* p) { return p->term == TERM_VC ? 5 : 2 * p->C + p->C * p->D; }
+int adj_nq(const tfsm_plan* p) { return p->term == TERM_VC ? 4 : p->C + p->C * p->D; }
+
 AdjLayout adj_layout(const tfsm_plan* p) {
-  const int nfa = 2 * p->C + p->C * p->D, nq = p->C + p->C * p->D;
+  const int nfa = adj_nfa(p), nq = adj_nq(p);
   size_t npts = 1;
   for (int d = 0; d < p->D; ++d) npts *= p->n[d];
   const size_t S = roundup(p->state_elems() * sizeof(float2), 256);
@@ -594,6 +599,14 @@ AdjLayout adj_layout(const tfsm_plan* p) {
   l.Jv = o; o += S;
   l.Ab = o; o += S;
   l.Nb = o; o += S;
+  // SETDRK4 only: Na, c, Nc and the stage adjoints
+  const bool four = p->scheme == SETDRK4;
+  l.Na = o; o += four ? S : 0;
+  l.Ac = o; o += four ? S : 0;
+  l.Nc = o; o += four ? S : 0;
+  l.Cb = o; o += four ? S : 0;
+  l.Bb = o; o += four ? S : 0;
+  l.Abar = o; o += four ? S : 0;
   l.gpart = o; o += roundup(p->state_elems() * sizeof(float), 256);
   l.F = o; o += roundup((size_t)p->B * nfa * p->modes * sizeof(float2), 256);
   l.Fs = o; o += roundup((size_t)p->B * (nfa > nq ? nfa : nq) * p->modes * sizeof(float2), 256);
@@ -607,7 +620,7 @@ AdjLayout adj_layout(const tfsm_plan* p) {
 int adj_jt(tfsm_plan* p, const float2* U, const float2* V, float2* W, char* wsa, hipStream_t st) {
   const AdjLayout l = adj_layout(p);
   const AdjArgs a = adj_args(p);
-  const int nfa = 2 * p->C + p->C * p->D, nq = p->C + p->C * p->D;
+  const int nfa = adj_nfa(p), nq = adj_nq(p);
   float2* F = (float2*)(wsa + l.F);
   float2* Fs = (float2*)(wsa + l.Fs);
   float* X = (float*)(wsa + l.X);
@@ -619,6 +632,24 @@ int adj_jt(tfsm_plan* p, const float2* U, const float2* V, float2* W, char* wsa,
     HIP_TRY(launch_rows_c2r(nl, F, (size_t)p->P, X, (size_t)nl, p->B * nfa, scale, p->tw[0][0], st), "adjoint c2r");
     HIP_TRY(launch_adj_products(a, X, Q, st), "adjoint products launch");
     HIP_TRY(launch_rows_r2c(nl, Q, (size_t)nl, Fs, (size_t)p->P, p->B * nq, 1.0f, p->tw[0][0], st), "adjoint r2c");
+  } else if (p->D == 3) {
+    // x columns, y columns, z rows (as tfsm_inverse / tfsm_forward do)
+    const size_t plane = (size_t)p->n[1] * p->P;
+    HIP_TRY(launch_cols_c2c(p->n[0], +1, F, Fs, (size_t)p->B * nfa, p->modes, plane, (int)plane, 1.0f, p->tw[0][0], st),
+            "adjoint x ifft");
+    HIP_TRY(launch_cols_c2c(p->n[1], +1, Fs, Fs, (size_t)p->B * nfa * p->n[0], plane, (size_t)p->P, p->H, 1.0f,
+                            p->tw[1][0], st),
+            "adjoint y ifft");
+    HIP_TRY(launch_rows_c2r(nl, Fs, (size_t)p->P, X, (size_t)nl, p->B * nfa * p->n[0] * p->n[1], scale, p->tw[2][0], st),
+            "adjoint c2r");
+    HIP_TRY(launch_adj_products(a, X, Q, st), "adjoint products launch");
+    HIP_TRY(launch_rows_r2c(nl, Q, (size_t)nl, Fs, (size_t)p->P, p->B * nq * p->n[0] * p->n[1], 1.0f, p->tw[2][0], st),
+            "adjoint r2c");
+    HIP_TRY(launch_cols_c2c(p->n[1], -1, Fs, Fs, (size_t)p->B * nq * p->n[0], plane, (size_t)p->P, p->H, 1.0f,
+                            p->tw[1][0], st),
+            "adjoint y fft");
+    HIP_TRY(launch_cols_c2c(p->n[0], -1, Fs, Fs, (size_t)p->B * nq, p->modes, plane, (int)plane, 1.0f, p->tw[0][0], st),
+            "adjoint x fft");
   } else {
     HIP_TRY(launch_cols_c2c(p->n[0], +1, F, Fs, (size_t)p->B * nfa, p->modes, (size_t)p->P, p->H, 1.0f, p->tw[0][0], st),
             "adjoint column ifft");
@@ -648,6 +679,151 @@ int eval_nonlinear(tfsm_plan* p, float2* U, char* ws, hipStream_t st) {
   return rc;
 }
 
+AdjLinArgs lin_args(tfsm_plan* p, float2* Y) {
+  AdjLinArgs a;
+  a.B = p->B;
+  a.C = p->C;
+  a.P = p->P;
+  a.H = p->H;
+  a.modes = p->modes;
+  a.coef = p->coef;
+  a.coef_bstride = p->coef_bstride();
+  a.nt = 0;
+  a.Y = Y;
+  return a;
+}
+
+void lin_term(AdjLinArgs& a, const float2* X, int slot, float s) {
+  a.X[a.nt] = X;
+  a.slot[a.nt] = slot;
+  a.s[a.nt] = s;
+  ++a.nt;
+}
+
+AdjGradArgs grad_args(tfsm_plan* p, const float2* Xbar, float* gpart, int accumulate) {
+  AdjGradArgs g;
+  g.B = p->B;
+  g.C = p->C;
+  g.P = p->P;
+  g.H = p->H;
+  g.nlast = p->n[p->D - 1];
+  g.modes = p->modes;
+  g.dcoef = p->dcoef;
+  g.dcoef_bstride = p->Bc > 1 ? (size_t)ADJ_NDCOEF * p->modes : 0;
+  g.sym = p->sym;
+  g.Xbar = Xbar;
+  g.nt = 0;
+  g.gpart = gpart;
+  g.accumulate = accumulate;
+  return g;
+}
+
+void grad_term(AdjGradArgs& g, const float2* Y, int dslot, float s) {
+  g.Y[g.nt] = Y;
+  g.dslot[g.nt] = dslot;
+  g.s[g.nt] = s;
+  ++g.nt;
+}
+
+// One adjoint SETDRK4 step (see k_adjoint.hip): recompute N0, a, Na, b, Nb, c (and Nc for the
+// coefficient gradient) from u, then the transposed stages from c back to u.
+int adjoint_step4(tfsm_plan* p, float2* U, const float2* Ubn, float2* Ubo, double* grad, char* W, char* A,
+                  hipStream_t st) {
+  enum { D_E = 0, D_E2 = 1, D_C1 = 2, D_C4 = 3, D_C5 = 4, D_C6 = 5 };
+  const AdjLayout l = adj_layout(p);
+  float2* A1 = (float2*)(W + p->off_A1);
+  float2* N0 = (float2*)(A + l.N0s);
+  float2* Sa = (float2*)(A + l.Aa);
+  float2* Na = (float2*)(A + l.Na);
+  float2* Sb = (float2*)(A + l.Ab);
+  float2* Nb = (float2*)(A + l.Nb);
+  float2* Sc = (float2*)(A + l.Ac);
+  float2* Nc = (float2*)(A + l.Nc);
+  float2* T = (float2*)(A + l.T1);
+  float2* Jv = (float2*)(A + l.Jv);
+  float2* Cb = (float2*)(A + l.Cb);
+  float2* Bb = (float2*)(A + l.Bb);
+  float2* Ab = (float2*)(A + l.Abar);
+  float* gp = (grad && p->adj_term >= 0) ? (float*)(A + l.gpart) : nullptr;
+  const size_t sbytes = p->state_elems() * sizeof(float2);
+  auto nonlin = [&](float2* X, float2* out) -> int {
+    int rc = eval_nonlinear(p, X, W, st);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(out, A1, sbytes, hipMemcpyDeviceToDevice, st), "adjoint copy");
+    return TFSM_OK;
+  };
+  auto comb = [&](AdjLinArgs& a) -> int {
+    HIP_TRY(launch_adj_lincomb(a, st), "adjoint stage launch");
+    return TFSM_OK;
+  };
+  int rc = nonlin(U, N0);
+  if (rc) return rc;
+  AdjLinArgs a = lin_args(p, Sa);  // a = E2 u + c1 N0
+  lin_term(a, U, CS_E2, 1.f);
+  lin_term(a, N0, CS_C1, 1.f);
+  if ((rc = comb(a)) || (rc = nonlin(Sa, Na))) return rc;
+  a = lin_args(p, Sb);  // b = E2 u + c1 Na
+  lin_term(a, U, CS_E2, 1.f);
+  lin_term(a, Na, CS_C1, 1.f);
+  if ((rc = comb(a)) || (rc = nonlin(Sb, Nb))) return rc;
+  a = lin_args(p, Sc);  // c = E2 a + c1 (2 Nb - N0)
+  lin_term(a, Sa, CS_E2, 1.f);
+  lin_term(a, Nb, CS_C1, 2.f);
+  lin_term(a, N0, CS_C1, -1.f);
+  if ((rc = comb(a))) return rc;
+  if (gp && (rc = nonlin(Sc, Nc))) return rc;
+  // backward
+  a = lin_args(p, T);  // Nc bar = c6 u'bar
+  lin_term(a, Ubn, CS_C6, 1.f);
+  if ((rc = comb(a)) || (rc = adj_jt(p, Sc, T, Cb, A, st))) return rc;
+  a = lin_args(p, T);  // Nb bar = 2 c5 u'bar + 2 c1 cbar
+  lin_term(a, Ubn, CS_C5, 2.f);
+  lin_term(a, Cb, CS_C1, 2.f);
+  if ((rc = comb(a)) || (rc = adj_jt(p, Sb, T, Bb, A, st))) return rc;
+  a = lin_args(p, T);  // Na bar = 2 c5 u'bar + c1 bbar
+  lin_term(a, Ubn, CS_C5, 2.f);
+  lin_term(a, Bb, CS_C1, 1.f);
+  if ((rc = comb(a)) || (rc = adj_jt(p, Sa, T, Jv, A, st))) return rc;
+  a = lin_args(p, Ab);  // abar = E2 cbar + J(a)^T Na bar
+  lin_term(a, Cb, CS_E2, 1.f);
+  lin_term(a, Jv, -1, 1.f);
+  if ((rc = comb(a))) return rc;
+  a = lin_args(p, T);  // N0 bar = c4 u'bar - c1 cbar + c1 abar
+  lin_term(a, Ubn, CS_C4, 1.f);
+  lin_term(a, Cb, CS_C1, -1.f);
+  lin_term(a, Ab, CS_C1, 1.f);
+  if ((rc = comb(a)) || (rc = adj_jt(p, U, T, Jv, A, st))) return rc;
+  a = lin_args(p, Ubo);  // ubar = E u'bar + E2 bbar + E2 abar + J(u)^T N0 bar
+  lin_term(a, Ubn, CS_E, 1.f);
+  lin_term(a, Bb, CS_E2, 1.f);
+  lin_term(a, Ab, CS_E2, 1.f);
+  lin_term(a, Jv, -1, 1.f);
+  if ((rc = comb(a))) return rc;
+  if (!gp) return TFSM_OK;
+  AdjGradArgs g = grad_args(p, Ubn, gp, 0);
+  grad_term(g, U, D_E, 1.f);
+  grad_term(g, N0, D_C4, 1.f);
+  grad_term(g, Na, D_C5, 2.f);
+  grad_term(g, Nb, D_C5, 2.f);
+  grad_term(g, Nc, D_C6, 1.f);
+  HIP_TRY(launch_adj_gacc(g, st), "adjoint gradient launch");
+  g = grad_args(p, Cb, gp, 1);
+  grad_term(g, Sa, D_E2, 1.f);
+  grad_term(g, Nb, D_C1, 2.f);
+  grad_term(g, N0, D_C1, -1.f);
+  HIP_TRY(launch_adj_gacc(g, st), "adjoint gradient launch");
+  g = grad_args(p, Bb, gp, 1);
+  grad_term(g, U, D_E2, 1.f);
+  grad_term(g, Na, D_C1, 1.f);
+  HIP_TRY(launch_adj_gacc(g, st), "adjoint gradient launch");
+  g = grad_args(p, Ab, gp, 1);
+  grad_term(g, U, D_E2, 1.f);
+  grad_term(g, N0, D_C1, 1.f);
+  HIP_TRY(launch_adj_gacc(g, st), "adjoint gradient launch");
+  HIP_TRY(launch_adj_reduce(gp, (size_t)p->C * p->modes, p->B, grad, st), "adjoint reduce launch");
+  return TFSM_OK;
+}
+
 AdjStageArgs stage_args(tfsm_plan* p, int op) {
   AdjStageArgs s;
   s.op = op;
@@ -661,7 +837,7 @@ AdjStageArgs stage_args(tfsm_plan* p, int op) {
   s.coef = p->coef;
   s.coef_bstride = p->coef_bstride();
   s.dcoef = p->dcoef;
-  s.dcoef_bstride = p->Bc > 1 ? 3 * p->modes : 0;
+  s.dcoef_bstride = p->Bc > 1 ? (size_t)ADJ_NDCOEF * p->modes : 0;
   s.sym = p->sym;
   s.X0 = s.X1 = s.X2 = s.X3 = nullptr;
   s.Y0 = s.Y1 = nullptr;
@@ -1477,20 +1653,26 @@ int64_t tfsm_plan_adjoint_workspace_bytes(const tfsm_plan* plan) {
 int tfsm_adjoint_prepare(tfsm_plan* p, int32_t grad_term, void* stream) {
   if (!p) return fail(TFSM_E_INVALID, "null plan");
   if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
-  if (p->scheme != ETDRK1 && p->scheme != ETDRK2 && p->scheme != SETDRK1 && p->scheme != SETDRK2)
-    return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint covers ETDRK1/2 and SETDRK1/2");
-  if (p->term != TERM_CONV || p->D > 2)
-    return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint covers Convection in 1D and 2D");
-  if (p->src) return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint does not cover explicit sources");
+  if (p->scheme != ETDRK1 && p->scheme != ETDRK2 && p->scheme != SETDRK1 && p->scheme != SETDRK2 &&
+      p->scheme != SETDRK4)
+    return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint covers ETDRK1/2 and SETDRK1/2/4");
+  if (!(p->term == TERM_CONV || (p->term == TERM_VC && p->D == 2)))
+    return fail(TFSM_E_INVALID,
+                "term not supported by engine: the adjoint covers Convection (1D, 2D, 3D) and VorticityConvection");
+  // explicit sources are constant: they enter the recomputed states but no Jacobian
   if (p->cplx) return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint needs a real linear symbol");
   if (p->per_len) return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint needs shared lengths");
   if (grad_term < -1 || grad_term >= p->nlin) return fail(TFSM_E_INVALID, "grad_term out of range");
   hipStream_t st = (hipStream_t)stream;
   int rc = TFSM_OK;
-  if (!p->dcoef) rc = dev_alloc(p, &p->dcoef, (size_t)p->Bc * 3 * p->modes, "adjoint coefficient table");
+  if (!p->dcoef) rc = dev_alloc(p, &p->dcoef, (size_t)p->Bc * ADJ_NDCOEF * p->modes, "adjoint coefficient table");
   if (!rc && !p->sym) rc = dev_alloc(p, &p->sym, p->modes, "adjoint symbol");
   if (rc) return rc;
-  HIP_TRY(launch_adj_dcoefs(p->L, p->modes, p->Bc, p->dt, p->dcoef, st), "adjoint coefficient launch");
+  if (p->scheme == SETDRK4)
+    HIP_TRY(launch_adj_dcoefs4(p->L, p->modes, p->Bc, p->dt, p->n_points, p->radius, p->dcoef, st),
+            "adjoint coefficient launch");
+  else
+    HIP_TRY(launch_adj_dcoefs(p->L, p->modes, p->Bc, p->dt, p->dcoef, st), "adjoint coefficient launch");
   if (grad_term >= 0) HIP_TRY(launch_adj_sym(adj_args(p), p->lin_kind[grad_term], p->sym, st), "adjoint symbol launch");
   p->adj_term = grad_term;
   p->adj_scheme = p->scheme;
@@ -1518,6 +1700,7 @@ int tfsm_adjoint_step(tfsm_plan* p, void* u, const void* ubar_next, void* ubar, 
   float2* Nb = (float2*)(A + l.Nb);
   float* gp = (grad && p->adj_term >= 0) ? (float*)(A + l.gpart) : nullptr;
   const size_t sbytes = p->state_elems() * sizeof(float2);
+  if (p->scheme == SETDRK4) return adjoint_step4(p, U, Ubn, Ubo, grad, W, A, st);
   int rc = eval_nonlinear(p, U, W, st);
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(N0s, A1, sbytes, hipMemcpyDeviceToDevice, st), "adjoint copy");

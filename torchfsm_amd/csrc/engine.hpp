@@ -630,6 +630,7 @@ hipError_t launch_pack_coefs(const CoefArgs& a, hipStream_t st);
 hipError_t launch_accum_source(float2* S, const float2* X, float coef, size_t total, int first, hipStream_t st);
 // ---------------------------------------------------------------- adjoint (k_adjoint.hip)
 struct AdjArgs {
+  int term = TERM_CONV;     // TERM_CONV or TERM_VC
   int D, C, B;
   int n[3];
   int H, P;
@@ -657,7 +658,38 @@ struct AdjStageArgs {
   float2* Y1;
   float* gpart;             // [B][C][modes] per-mode gradient terms, or null
 };
+constexpr int ADJ_NDCOEF = 6;  // coefficient-derivative slots: 2-stage dE, dc1, dc2; SETDRK4 dE, dE2, dc1, dc4, dc5, dc6
+constexpr int ADJ_MAXT = 5;
+struct AdjLinArgs {
+  int B, C, P, H;
+  size_t modes;
+  const float* coef;        // [Bc][NCOEF][modes]
+  size_t coef_bstride;
+  int nt;
+  const float2* X[ADJ_MAXT];
+  int slot[ADJ_MAXT];       // coefficient slot (CoefSlot) or -1 for 1
+  float s[ADJ_MAXT];
+  float2* Y;
+};
+struct AdjGradArgs {
+  int B, C, P, H, nlast;
+  size_t modes;
+  const float* dcoef;       // [Bc][ADJ_NDCOEF][modes]
+  size_t dcoef_bstride;
+  const float* sym;         // dL/dtheta [modes]
+  const float2* Xbar;
+  int nt;
+  const float2* Y[ADJ_MAXT];
+  int dslot[ADJ_MAXT];
+  float s[ADJ_MAXT];
+  float* gpart;
+  int accumulate;
+};
 hipError_t launch_adj_fields(const AdjArgs& a, const float2* U, const float2* V, float2* F, hipStream_t st);
+hipError_t launch_adj_dcoefs4(const float* L, size_t modes, int Bc, double dt, int M, double r, float* dcoef,
+                              hipStream_t st);
+hipError_t launch_adj_lincomb(const AdjLinArgs& a, hipStream_t st);
+hipError_t launch_adj_gacc(const AdjGradArgs& a, hipStream_t st);
 hipError_t launch_adj_products(const AdjArgs& a, const float* X, float* Q, hipStream_t st);
 hipError_t launch_adj_combine(const AdjArgs& a, const float2* R, float2* W, hipStream_t st);
 hipError_t launch_adj_stage(const AdjStageArgs& a, hipStream_t st);

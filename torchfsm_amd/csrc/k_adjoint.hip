@@ -15,6 +15,21 @@
 // collects  <abar, dE u + dc1 N0> + <u'bar, dc2 (Na - N0)>,  d. = (d./dL) S.
 // 1-stage schemes (ETDRK1 / SETDRK1): u' = E u + c1 N0, ubar = E u'bar + J_N(u)^T (c1 u'bar),
 // gradient <u'bar, dE u + dc1 N0>.
+//
+// VorticityConvection (2D, C = 1; _navier_stokes.py:27-46): N = coef F[psi_y w_x - psi_x w_y] with
+// psi^ = -invlap M w^ (invlap(0) = 1), w_x = Re IFFT(i a_x M w^), ...  Its transposed Jacobian is
+//   J_N(w)^T v^ = coef M [ invlap (i a_y Q1 - i a_x Q2) - i a_x Q3 + i a_y Q4 ]
+// with Q1..Q4 the spectra of p w_x, p w_y, p psi_y, p psi_x.  Explicit sources are constant and
+// drop out of every Jacobian.
+//
+// SETDRK4 (_setdrk_step.py:55-82): N0 = N(u), a = E2 u + c1 N0, Na = N(a), b = E2 u + c1 Na,
+// Nb = N(b), c = E2 a + c1 (2 Nb - N0), Nc = N(c), u' = E u + c4 N0 + 2 c5 (Na + Nb) + c6 Nc.
+// Backward, in this order: cbar = J(c)^T (c6 u'bar); bbar = J(b)^T (2 c5 u'bar + 2 c1 cbar);
+// abar = E2 cbar + J(a)^T (2 c5 u'bar + c1 bbar); ubar = E u'bar + E2 (bbar + abar)
+// + J(u)^T (c4 u'bar - c1 cbar + c1 abar).  Coefficient gradient:
+//   <u'bar, dE u + dc4 N0 + 2 dc5 (Na + Nb) + dc6 Nc> + <cbar, dE2 a + dc1 (2 Nb - N0)>
+//   + <bbar, dE2 u + dc1 Na> + <abar, dE2 u + dc1 N0>,
+// with the derivatives of the contour means taken as contour means of the derivatives (float64).
 #include "engine.hpp"
 
 namespace tfsm {
@@ -38,9 +53,34 @@ __device__ __forceinline__ void adj_index(const AdjArgs& a, size_t mode, int* id
   if (a.D >= 2) idx[0] = (int)r;
 }
 
-// F [B][NFA][modes]: v_c, M u_c, i a_i M u_c  (NFA = 2C + C D)
+// F [B][NFA][modes].  Convection: v_c, M u_c, i a_i M u_c  (NFA = 2C + C D).  VorticityConvection:
+// v, i a_x psi^, i a_y psi^, i a_x M w^, i a_y M w^ (NFA = 5), psi^ = -invlap M w^.
 __global__ void k_adj_fields(AdjArgs a, const float2* __restrict__ U, const float2* __restrict__ V,
                              float2* __restrict__ F) {
+  if (a.term == TERM_VC) {
+    const size_t total = (size_t)a.B * a.modes;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+      const size_t mode = e % a.modes;
+      const size_t b = e / a.modes;
+      int idx[3] = {0, 0, 0};
+      adj_index(a, mode, idx);
+      const bool valid = idx[1] < a.H;
+      const bool keep = valid && adj_keep(2, idx, a.n, a.kmax);
+      const float ax = a.kv[0][idx[0]], ay = a.kv[1][idx[1]];
+      const float2 v = valid ? V[e] : make_float2(0.f, 0.f);
+      const float2 w = keep ? U[e] : make_float2(0.f, 0.f);
+      const float lap = -ax * ax - ay * ay;
+      const float il = lap == 0.f ? 1.f : 1.f / lap;
+      const float2 psi = make_float2(-il * w.x, -il * w.y);
+      float2* fo = F + b * 5 * a.modes + mode;
+      fo[0] = v;
+      fo[a.modes] = make_float2(-ax * psi.y, ax * psi.x);
+      fo[2 * a.modes] = make_float2(-ay * psi.y, ay * psi.x);
+      fo[3 * a.modes] = make_float2(-ax * w.y, ax * w.x);
+      fo[4 * a.modes] = make_float2(-ay * w.y, ay * w.x);
+    }
+    return;
+  }
   const int C = a.C, D = a.D, nfa = 2 * C + C * D;
   const size_t total = (size_t)a.B * a.modes;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
@@ -66,7 +106,23 @@ __global__ void k_adj_fields(AdjArgs a, const float2* __restrict__ U, const floa
 
 // X [B][NFA][npts] physical (p_c, u_j, d_i u_c) -> Q [B][C + C D][npts]:
 //   Q_i = sum_c p_c d_i u_c,  Q_{C + c D + j} = p_c u_j
+// VorticityConvection: X = (p, psi_x, psi_y, w_x, w_y) -> Q = (p w_x, p w_y, p psi_y, p psi_x).
 __global__ void k_adj_products(AdjArgs a, const float* __restrict__ X, float* __restrict__ Q) {
+  if (a.term == TERM_VC) {
+    const size_t total = (size_t)a.B * a.npts;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+      const size_t pt = e % a.npts;
+      const size_t b = e / a.npts;
+      const float* x = X + b * 5 * a.npts + pt;
+      float* q = Q + b * 4 * a.npts + pt;
+      const float pv = x[0];
+      q[0] = pv * x[3 * a.npts];
+      q[a.npts] = pv * x[4 * a.npts];
+      q[2 * a.npts] = pv * x[2 * a.npts];
+      q[3 * a.npts] = pv * x[a.npts];
+    }
+    return;
+  }
   const int C = a.C, D = a.D, nfa = 2 * C + C * D, nq = C + C * D;
   const size_t total = (size_t)a.B * a.npts;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
@@ -90,7 +146,32 @@ __global__ void k_adj_products(AdjArgs a, const float* __restrict__ X, float* __
 }
 
 // W_c = coef M (R_c - i sum_j a_j S_{c,j}),  R [B][C + C D][modes] spectra of Q
+// VorticityConvection: W = coef M [invlap (i a_y R1 - i a_x R2) - i a_x R3 + i a_y R4].
 __global__ void k_adj_combine(AdjArgs a, const float2* __restrict__ R, float2* __restrict__ W) {
+  if (a.term == TERM_VC) {
+    const size_t total = (size_t)a.B * a.modes;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+      const size_t mode = e % a.modes;
+      const size_t b = e / a.modes;
+      int idx[3] = {0, 0, 0};
+      adj_index(a, mode, idx);
+      const bool keep = idx[1] < a.H && adj_keep(2, idx, a.n, a.kmax);
+      float2 w = make_float2(0.f, 0.f);
+      if (keep) {
+        const float ax = a.kv[0][idx[0]], ay = a.kv[1][idx[1]];
+        const float lap = -ax * ax - ay * ay;
+        const float il = lap == 0.f ? 1.f : 1.f / lap;
+        const float2* r = R + b * 4 * a.modes + mode;
+        const float2 r1 = r[0], r2 = r[a.modes], r3 = r[2 * a.modes], r4 = r[3 * a.modes];
+        // i a z = (-a z.y, a z.x)
+        const float2 t = make_float2(-ay * r1.y + ax * r2.y, ay * r1.x - ax * r2.x);
+        w = make_float2(il * t.x + ax * r3.y - ay * r4.y, il * t.y - ax * r3.x + ay * r4.x);
+        w = make_float2(a.coef_t * w.x, a.coef_t * w.y);
+      }
+      W[e] = w;
+    }
+    return;
+  }
   const int C = a.C, D = a.D, nq = C + C * D;
   const size_t total = (size_t)a.B * a.modes;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
@@ -222,7 +303,7 @@ __global__ void k_adj_dcoefs(const float* __restrict__ L, size_t modes, int Bc, 
     const double z = (double)L[e] * dt;
     double d1, d2;
     phi_derivs(z, &d1, &d2);
-    float* o = dcoef + b * 3 * modes + mode;
+    float* o = dcoef + b * ADJ_NDCOEF * modes + mode;
     o[0] = (float)(dt * exp(z));
     o[modes] = (float)(dt * dt * d1);
     o[2 * modes] = (float)(dt * dt * d2);
@@ -244,6 +325,106 @@ __global__ void k_adj_sym(AdjArgs a, int kind, float* __restrict__ S) {
     if (kind == LIN_LAP) s = lap;
     else if (kind == LIN_BIHARM) s = lap * lap;
     S[mode] = idx[a.D - 1] < a.H ? s : 0.f;
+  }
+}
+
+// SETDRK4 coefficient derivatives d(E, E2, c1, c4, c5, c6)/dL into dcoef slots 0..5, float64:
+// E and E2 in closed form; c1, c4, c5, c6 as the contour means (M points of radius r around
+// z = L dt, theta_j = 2 pi (j - 1/2) / M) of dt^2 f'(lr), the derivative of the integrands of
+// _uncached.py:180-211 (f1 = (e^{w/2} - 1)/w, f4 = (-4 - w + e^w (4 - 3w + w^2))/w^3,
+// f5 = (2 + w + e^w (w - 2))/w^3, f6 = (-4 - 3w - w^2 + e^w (4 - w))/w^3).
+struct Cd {
+  double re, im;
+};
+__device__ __forceinline__ Cd cdm(Cd a, Cd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ Cd cda(Cd a, Cd b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ Cd cds(double s, Cd a) { return {s * a.re, s * a.im}; }
+__device__ __forceinline__ Cd cdd(Cd a, Cd b) {
+  const double d = b.re * b.re + b.im * b.im;
+  return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+__device__ __forceinline__ Cd cdexp(Cd a) {
+  const double m = exp(a.re);
+  return {m * cos(a.im), m * sin(a.im)};
+}
+
+__global__ void k_adj_dcoefs4(const float* __restrict__ L, size_t modes, int Bc, double dt, int M, double r,
+                              float* __restrict__ dcoef) {
+  const size_t total = modes * Bc;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t mode = e % modes;
+    const size_t b = e / modes;
+    const double z = (double)L[e] * dt;
+    Cd s1 = {0, 0}, s4 = {0, 0}, s5 = {0, 0}, s6 = {0, 0};
+    for (int j = 1; j <= M; ++j) {
+      const double th = 2.0 * 3.14159265358979323846 * (j - 0.5) / M;
+      const Cd w = {z + r * cos(th), r * sin(th)};
+      const Cd ew = cdexp(w), eh = cdexp(cds(0.5, w));
+      const Cd w2 = cdm(w, w), w4 = cdm(w2, w2);
+      // f1' = ((1/2) e^{w/2} w - (e^{w/2} - 1)) / w^2
+      s1 = cda(s1, cdd(cda(cds(0.5, cdm(eh, w)), cds(-1.0, cda(eh, {-1.0, 0.0}))), w2));
+      // f' = (N' w - 3 N) / w^4 for f = N / w^3
+      const Cd one = {1.0, 0.0};
+      const Cd n4 = cda(cda({-4.0, 0.0}, cds(-1.0, w)), cdm(ew, cda(cda({4.0, 0.0}, cds(-3.0, w)), w2)));
+      const Cd d4 = cda({-1.0, 0.0}, cdm(ew, cda(cda(one, cds(-1.0, w)), w2)));
+      s4 = cda(s4, cdd(cda(cdm(d4, w), cds(-3.0, n4)), w4));
+      const Cd n5 = cda(cda({2.0, 0.0}, w), cdm(ew, cda(w, {-2.0, 0.0})));
+      const Cd d5 = cda(one, cdm(ew, cda(w, {-1.0, 0.0})));
+      s5 = cda(s5, cdd(cda(cdm(d5, w), cds(-3.0, n5)), w4));
+      const Cd n6 = cda(cda(cda({-4.0, 0.0}, cds(-3.0, w)), cds(-1.0, w2)), cdm(ew, cda({4.0, 0.0}, cds(-1.0, w))));
+      const Cd d6 = cda(cda({-3.0, 0.0}, cds(-2.0, w)), cdm(ew, cda({3.0, 0.0}, cds(-1.0, w))));
+      s6 = cda(s6, cdd(cda(cdm(d6, w), cds(-3.0, n6)), w4));
+    }
+    const double k = dt * dt / M;
+    float* o = dcoef + b * ADJ_NDCOEF * modes + mode;
+    o[0] = (float)(dt * exp(z));
+    o[modes] = (float)(0.5 * dt * exp(0.5 * z));
+    o[2 * modes] = (float)(k * s1.re);
+    o[3 * modes] = (float)(k * s4.re);
+    o[4 * modes] = (float)(k * s5.re);
+    o[5 * modes] = (float)(k * s6.re);
+  }
+}
+
+// Y = sum_k s_k c_k X_k per mode (c_k a coefficient table or 1); zero on the padded columns
+__global__ void k_adj_lincomb(AdjLinArgs a) {
+  const size_t total = (size_t)a.B * a.C * a.modes;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t mode = e % a.modes;
+    const size_t b = e / (a.C * a.modes);
+    float2 y = make_float2(0.f, 0.f);
+    if ((int)(mode % a.P) < a.H) {
+      const float* cf = a.coef + b * a.coef_bstride + mode;
+      for (int k = 0; k < a.nt; ++k) {
+        const float c = a.s[k] * (a.slot[k] >= 0 ? cf[(size_t)a.slot[k] * a.modes] : 1.f);
+        const float2 x = a.X[k][e];
+        y = make_float2(y.x + c * x.x, y.y + c * x.y);
+      }
+    }
+    a.Y[e] = y;
+  }
+}
+
+// gpart (+)= w S <Xbar, sum_k s_k dc_k Y_k> per mode (w: half-spectrum weight, S = dL/dtheta)
+__global__ void k_adj_gacc(AdjGradArgs a) {
+  const size_t total = (size_t)a.B * a.C * a.modes;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t mode = e % a.modes;
+    const size_t b = e / (a.C * a.modes);
+    const int k = (int)(mode % a.P);
+    float g = 0.f;
+    if (k < a.H) {
+      const float w = (k == 0 || 2 * k == a.nlast) ? 1.f : 2.f;
+      const float* dc = a.dcoef + b * a.dcoef_bstride + mode;
+      float2 d = make_float2(0.f, 0.f);
+      for (int t = 0; t < a.nt; ++t) {
+        const float c = a.s[t] * dc[(size_t)a.dslot[t] * a.modes];
+        const float2 y = a.Y[t][e];
+        d = make_float2(d.x + c * y.x, d.y + c * y.y);
+      }
+      g = w * a.sym[mode] * rdot(a.Xbar[e], d);
+    }
+    a.gpart[e] = a.accumulate ? a.gpart[e] + g : g;
   }
 }
 
@@ -274,6 +455,19 @@ hipError_t launch_adj_reduce(const float* gpart, size_t per, int B, double* out,
 }
 hipError_t launch_adj_dcoefs(const float* L, size_t modes, int Bc, double dt, float* dcoef, hipStream_t st) {
   hipLaunchKernelGGL(k_adj_dcoefs, dim3(grid_of(modes * Bc)), dim3(256), 0, st, L, modes, Bc, dt, dcoef);
+  return hipGetLastError();
+}
+hipError_t launch_adj_dcoefs4(const float* L, size_t modes, int Bc, double dt, int M, double r, float* dcoef,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_dcoefs4, dim3(grid_of(modes * Bc)), dim3(256), 0, st, L, modes, Bc, dt, M, r, dcoef);
+  return hipGetLastError();
+}
+hipError_t launch_adj_lincomb(const AdjLinArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_lincomb, dim3(grid_of((size_t)a.B * a.C * a.modes)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+hipError_t launch_adj_gacc(const AdjGradArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_gacc, dim3(grid_of((size_t)a.B * a.C * a.modes)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 hipError_t launch_adj_sym(const AdjArgs& a, int kind, float* S, hipStream_t st) {
