@@ -363,29 +363,43 @@ def host_cpu():
 
 
 def time_in_step(plan, uh, prime, seq, steps=12, skip=2):
-    """Average duration (ms) of each kernel of the step, timed inside the step's own launch
-    sequence: `steps` steps of the first sub-batch launched pass by pass in the step's order (the
-    data flow and cache state of tfsm_step), with HIP events recorded between the launches on the
-    stream the engine launches on (torch's current stream).  The first `skip` steps are not
-    counted."""
+    """Average duration (ms) of each kernel of the step, timed inside the step's own schedule:
+    every sub-batch runs `steps` steps launched pass by pass in the step's order, sub-batch k on
+    the stream tfsm_step uses for it (with two streams, alternate sub-batches on torch's current
+    stream and on a second stream, so their kernels overlap as in tfsm_step), and HIP events are
+    recorded between the launches on each kernel's own stream.  The first `skip` steps of every
+    sub-batch are not counted.  With two streams a kernel's duration includes the time it shares
+    the GPU with the other stream's kernel."""
     import torch
 
-    stream = torch.cuda.current_stream()
+    B, sb = plan.batch, plan.tuning[4]
+    nsub = 1 if not 0 < sb < B else (B + sb - 1) // sb
+    main = torch.cuda.current_stream()
+    streams = [main] if plan.streams == 1 or nsub == 1 else [main, torch.cuda.Stream()]
     u = uh.clone()
     torch.cuda.synchronize()
-    plan.launch_pass(u, *prime)
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(seq) + 1)] for _ in range(steps)]
-    for k in range(steps):
-        ev[k][0].record(stream)
-        for i, (_, pid, st) in enumerate(seq):
-            plan.launch_pass(u, pid, st)
-            ev[k][i + 1].record(stream)
+    fork = torch.cuda.Event()
+    fork.record(main)
+    evs = []  # per sub-batch: [steps][len(seq) + 1] events
+    for k in range(nsub):
+        s = streams[k % len(streams)]
+        s.wait_event(fork)
+        with torch.cuda.stream(s):
+            plan.launch_pass(u, *prime, sub=k)
+            ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(seq) + 1)] for _ in range(steps)]
+            for i in range(steps):
+                ev[i][0].record(s)
+                for j, (_, pid, st) in enumerate(seq):
+                    plan.launch_pass(u, pid, st, sub=k)
+                    ev[i][j + 1].record(s)
+        evs.append(ev)
     torch.cuda.synchronize()
     tot, cnt = {}, {}
-    for k in range(skip, steps):
-        for i, (name, _, _) in enumerate(seq):
-            tot[name] = tot.get(name, 0.0) + ev[k][i].elapsed_time(ev[k][i + 1])
-            cnt[name] = cnt.get(name, 0) + 1
+    for ev in evs:
+        for i in range(skip, steps):
+            for j, (name, _, _) in enumerate(seq):
+                tot[name] = tot.get(name, 0.0) + ev[i][j].elapsed_time(ev[i][j + 1])
+                cnt[name] = cnt.get(name, 0) + 1
     return {n: tot[n] / cnt[n] for n in tot}
 
 
@@ -546,6 +560,11 @@ def main():
         "kernel_ms": {k: round(v, 5) for k, v in kt.items()},
         "step_bytes_model": step_bytes,
         "step_frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "streams": plan.streams,
+        "timing": ("in-step: HIP events between the launches of every sub-batch's steps, each on the stream "
+                   "tfsm_step uses for it" + ("; two sub-batches run concurrently, so a kernel's duration "
+                                              "includes the time it shares the GPU (step_frac is the "
+                                              "step-level fraction)" if plan.streams == 2 else "")),
     }
     log(f"[bench] kernels (ms): {kt}; dominant {dom}: {avg_bytes / 1e6:.1f} MB per launch in {avg_ms:.4f} ms "
         f"= {achieved:.0f} GB/s; traffic {traffic}")
@@ -589,7 +608,7 @@ def main():
     }
     log(f"[bench] torch-GPU baseline {tg_val:.3e} DOF*steps/s; engine speedup {value / tg_val:.2f}x")
     if args.serve_requests > 0 and args.workload == "ns2d":
-        del sch, ug, w, plan, uh, u
+        del sch, ug, w, plan, uh
         torch.cuda.empty_cache()
         log(f"[bench] serving: {args.serve_requests} requests at {args.serve_rate}/s")
         rates = [float(x) for x in args.serve_rates.split(",") if x.strip()] if args.serve_rates else []

@@ -134,6 +134,16 @@ int tfsm_plan_get_tuning(const tfsm_plan* plan, int32_t* out);
  * Results are identical.  (No reference counterpart: launch configuration.) */
 int tfsm_plan_set_streams(tfsm_plan* plan, int32_t nstreams);
 
+/* Step 2D plans of square 32^2, 64^2 or 128^2 meshes with Convection or VorticityConvection with
+ * the persistent per-sample kernel (on = 1, the default for those plans: one workgroup per sample
+ * runs every step of a tfsm_step call, one launch per call) or with the tiled row and column
+ * kernels (on = 0).  Results are identical.  TFSM_E_INVALID for other plans.
+ * (No reference counterpart: launch configuration.) */
+int tfsm_plan_set_persistent(tfsm_plan* plan, int32_t on);
+
+/* 1 if tfsm_step runs the persistent per-sample 2D kernel for this plan, else 0. */
+int32_t tfsm_plan_persistent(const tfsm_plan* plan);
+
 /* ETD coefficients on the device: SETDRK1-4 as float64 contour means of n_points points on a
  * circle of radius `radius`; ETDRK0-2 with the reference's complex64 closed forms.
  * Replaces OperatorLike._build_integrator (_base.py:441-526) with ETDRK0/1/2.__init__
@@ -258,15 +268,16 @@ int tfsm_rk_combine(tfsm_plan* plan, void* out, const void* x, const void* const
  * (_base.py:217-262). */
 int tfsm_solve(tfsm_plan* plan, const void* uh, void* out, void* stream);
 
-/* Launch one pass of the fused step on the current state/workspace, for profiling, over the first
- * sub-batch (tfsm_plan_set_tuning; the whole batch when none is set), exactly as tfsm_step
- * launches it: pass 0 = the
+/* Launch one pass of the fused step on the current state/workspace, for profiling, over sub-batch
+ * `sub` (0, 1, ...: samples [sub * sub_batch, ...) of tfsm_plan_set_tuning's sub-batch; the whole
+ * batch when none is set), exactly as tfsm_step launches it: pass 0 = the
  * row kernel (2D), pass 1 = the column kernel with stage code `stage` (0 prologue, 7..10 the four
  * SETDRK4 evaluations), pass 2 = the 1D persistent stepper for one step, pass 3 = one whole 3D
  * evaluation (z-row, y-forward, x-column and y-inverse kernels) with stage code `stage`, passes
  * 4..7 = one of those four kernels alone.  (No reference
  * counterpart: instrumentation of the engine's own kernels.) */
-int tfsm_launch_pass(tfsm_plan* plan, void* uh, void* ws, int32_t pass_id, int32_t stage, void* stream);
+int tfsm_launch_pass(tfsm_plan* plan, void* uh, void* ws, int32_t pass_id, int32_t stage, int32_t sub,
+                     void* stream);
 
 /* Bytes of the adjoint workspace tfsm_adjoint_step needs (stage arrays, transposed-Jacobian
  * fields and products). */

@@ -256,6 +256,7 @@ def test_every_launch_shape_is_exact(dev):
     op.register_mesh(tfa.FourierMesh(mesh), 1)
     plan = op._prepare(1, dev, 1e-3)
     u0 = torch.from_numpy(g["u0_f32"]).to(dev)
+    plan.set_persistent(False)  # the candidates below are the tiled kernels' (128^2 uses them by default)
     cands = autotune.candidates(plan)
     assert len(cands) == 16
     cands = cands + [(16, 0, 0, 256, 8)]
@@ -273,6 +274,7 @@ def test_every_launch_shape_is_exact(dev):
     op = pde.Burgers(0.01)
     op.register_mesh(tfa.FourierMesh([(0, 1, 64)] * 2, device=dev), 2)
     plan = op._prepare(2, dev, 0.005)
+    plan.set_persistent(False)
     for cand in autotune.candidates(plan):
         plan.set_tuning(*cand)
         uh = plan.forward(u.to(dev))
@@ -413,3 +415,99 @@ def test_ns512_b64_1000_steps_vs_oracle_f32_and_f64(dev):
         if e32[i] > TOL:
             assert e64[i] <= 1.5 * floor_64[i], (picks[i], e64[i], floor_64[i])
             assert e32[i] <= 2 * max(floor_64[i], floor_cg), (picks[i], e32[i], floor_64[i], floor_cg)
+
+
+@pytest.mark.parametrize("n,term", [(32, "conv"), (32, "vc"), (64, "vc"), (64, "conv")])
+def test_persistent_2d_stepper_matches_tiled_kernels(dev, n, term):
+    """The persistent per-sample 2D stepper (default for square 32^2..128^2 meshes) against the tiled
+    row and column kernels: plain steps with a trajectory recorded every other step and per-sample
+    step counts agree to rel-L2 1e-6 (the same device code, compiled into different kernels, may
+    fuse the transforms' multiply-adds differently); within the persistent stepper a sample with
+    count 0 is untouched and a full count equals plain stepping bit for bit; sample 0 against the
+    oracle within the float32 bar."""
+    from oracle import refpath
+    from torchfsm_amd.engine import SpectralPlan
+
+    B = 3
+    g = torch.Generator().manual_seed(n + (1 if term == "vc" else 0))
+    if term == "vc":
+        mi = [(0, 2 * math.pi, n)] * 2
+        lin = [("lap", [1e-3, 2e-3, 4e-3]), ("unit", -0.1)]
+        plan = SpectralPlan(mi, B, 1, nonlinear="vc", nonlinear_coef=-1.0, linear=lin, device=dev)
+        plan.set_scheme("SETDRK4", 1e-3)
+        u0 = torch.randn(B, 1, n, n, generator=g)
+        steps = 12
+    else:
+        mi = [(0, 1, n)] * 2
+        plan = SpectralPlan(mi, B, 2, nonlinear="conv", nonlinear_coef=-1.0, linear=[("lap", 0.01)], device=dev)
+        plan.set_scheme("SETDRK2", 2e-3)
+        u0 = 0.3 * torch.randn(B, 2, n, n, generator=g)
+        steps = 10
+    assert plan.persistent
+    if n == 64:  # the persistent kernel ignores the tiled kernels' launch shape
+        plan.set_tuning(16, 1, 1, 64, 0)
+    uh0 = plan.forward(u0.to(dev))
+    T = steps // 2 + 1
+    sched = [i // 2 if i % 2 == 0 else -1 for i in range(steps + 1)]
+    slots = torch.tensor(sched, dtype=torch.int32, device=dev)
+
+    def run(persistent):
+        plan.set_persistent(persistent)
+        u = uh0.clone()
+        traj = torch.zeros((B, T) + tuple(plan.spec_shape[1:]), dtype=torch.complex64, device=dev)
+        plan.step(u, steps, rec_slot=slots, traj=traj, rec_slot_host=sched)
+        v = uh0.clone()
+        cnt = torch.tensor([steps, 0, steps // 2], dtype=torch.int32, device=dev)
+        plan.step_varying(v, cnt, steps)
+        torch.cuda.synchronize(dev)
+        return u, traj, v
+
+    u1, t1, v1 = run(True)
+    u2, t2, v2 = run(False)
+    for name, a_, b_ in (("state", u1, u2), ("trajectory", t1, t2), ("per-sample counts", v1, v2)):
+        d = _rel(torch.view_as_real(a_).cpu().numpy(), torch.view_as_real(b_).cpu().numpy())
+        print(f"persistent vs tiled {n}^2 {term} {name}: rel-L2 {d:.2e}", flush=True)
+        assert d <= 1e-6, (name, d)
+    assert torch.equal(v1[1], uh0[1])
+    assert torch.equal(v1[0], u1[0])
+    out = plan.inverse(u1)[:1].cpu()
+    if term == "vc":
+        terms = [(-1, "vc", None), (1e-3, "lap", None), (-0.1, "unit", None)]
+        ref = refpath.integrate(terms, mi, u0[:1], 1e-3, steps, scheme="SETDRK4")
+    else:
+        ref = refpath.integrate(refpath.burgers_terms(0.01), mi, u0[:1], 2e-3, steps, scheme="SETDRK2")
+    e = _rel(out.numpy(), ref.numpy())
+    print(f"persistent {n}^2 {term}: sample 0 vs oracle {e:.2e}")
+    assert e <= TOL
+    plan.close()
+    # 128^2 keeps the tiled kernels (the sample's fields do not fit LDS)
+    big = SpectralPlan([(0, 1, 128)] * 2, 1, 2, nonlinear="conv", nonlinear_coef=-1.0, linear=[("lap", 0.01)],
+                       device=dev)
+    assert not big.persistent
+    with pytest.raises(ValueError, match="persistent 2D stepper"):
+        big.set_persistent(True)
+    big.close()
+
+
+def test_integrate_replays_graphs_on_small_multi_launch_meshes(dev):
+    """integrate() on a small 3D mesh (several launches per step) replays a captured graph of 16
+    steps: 3D Burgers 32^3, C=3, B=2, SETDRK2, 50 steps (3 replays + 2 eager steps) equals eager
+    stepping bit for bit, also on a second call with a new initial field (the graph is reused) and
+    after a change of dt (the graph is recaptured)."""
+    import torchfsm_amd as tfa
+    from torchfsm_amd import pde
+    from torchfsm_amd.integrator import SETDRKIntegrator
+    from torchfsm_amd.workloads import fourier_modes_field
+
+    n = 32
+    mesh = tfa.MeshGrid([(0, 1, n)] * 3, device=dev)
+    op = pde.Burgers(0.02)
+    op.set_integrator(SETDRKIntegrator.SETDRK2)
+    for seed, dt, steps in ((3, 2e-3, 50), (4, 2e-3, 37), (5, 1e-3, 40)):
+        u0 = (0.3 * fourier_modes_field(n, 3, 3, 2, seed=seed)).to(dev)
+        out = op.integrate(u0, mesh=mesh, dt=dt, step=steps)
+        plan = op._state_dict["integrator"]
+        assert getattr(plan, "_step_graph", None) is not None
+        uh = plan.forward(u0)
+        plan.step(uh, steps)
+        assert torch.equal(out, plan.inverse(uh)), (seed, dt, steps)

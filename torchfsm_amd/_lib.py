@@ -64,6 +64,8 @@ SIGNATURES = {
     "tfsm_plan_inverse_scratch_bytes": (I64, [VP, I32]),
     "tfsm_plan_set_tuning": (ctypes.c_int, [VP, I32, I32, I32, I32, I32]),
     "tfsm_plan_set_streams": (ctypes.c_int, [VP, I32]),
+    "tfsm_plan_set_persistent": (ctypes.c_int, [VP, I32]),
+    "tfsm_plan_persistent": (I32, [VP]),
     "tfsm_plan_get_tuning": (ctypes.c_int, [VP, VP]),
     "tfsm_plan_set_scheme": (ctypes.c_int, [VP, I32, ctypes.c_double, I32, ctypes.c_double, VP]),
     "tfsm_plan_coefficients": (ctypes.c_int, [VP, I32, VP, VP]),
@@ -82,7 +84,7 @@ SIGNATURES = {
     "tfsm_rk_combine": (ctypes.c_int, [VP, VP, VP, VP, VP, I32, ctypes.c_float, VP]),
     "tfsm_spectral_map": (ctypes.c_int, [VP, I32, ctypes.c_float, VP, VP, VP]),
     "tfsm_pointwise_products": (ctypes.c_int, [VP, VP, VP, VP]),
-    "tfsm_launch_pass": (ctypes.c_int, [VP, VP, VP, I32, I32, VP]),
+    "tfsm_launch_pass": (ctypes.c_int, [VP, VP, VP, I32, I32, I32, VP]),
     "tfsm_plan_adjoint_workspace_bytes": (I64, [VP]),
     "tfsm_adjoint_prepare": (ctypes.c_int, [VP, I32, VP]),
     "tfsm_adjoint_step": (ctypes.c_int, [VP, VP, VP, VP, VP, VP, VP, VP]),

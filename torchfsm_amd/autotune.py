@@ -1,9 +1,10 @@
 """Autotuner for the launch shape of the fused 2D step kernels.
 
-The search space per shape class (grid, batch, channels, term): column tile width {8, 16} x
-column radix-plan variant {0, 1} x row radix-plan variant {0, 1} x row workgroup size {64, 256},
-then, for the best launch shape, the temporal-blocking sub-batch (samples stepped together), and
-then whether alternate sub-batches run concurrently on a second stream.
+The search space per shape class (grid, batch, channels, term): the persistent per-sample 2D
+stepper where the plan supports it, and the tiled kernels' column tile width {8, 16} x column
+radix-plan variant {0, 1} x row radix-plan variant {0, 1} x row workgroup size {64, 256}, then, for
+the best tiled launch shape, the temporal-blocking sub-batch (samples stepped together), and then
+whether alternate sub-batches run concurrently on a second stream.
 Candidates are timed on the plan's own buffers (HIP events around a few full steps on the
 current stream) and the fastest is kept for the process; results can be persisted as JSON.
 Choices are fixed per shape class, never per batch occupancy, so results do not depend on the mix.
@@ -24,9 +25,11 @@ def shape_key(plan) -> str:
 
 
 def _apply(plan, cand):
-    """cand = (col tile, col var, row var, row wg, sub-batch[, streams])."""
+    """cand = (col tile, col var, row var, row wg, sub-batch[, streams[, persistent]])."""
     plan.set_tuning(*cand[:5])
     plan.set_streams(cand[5] if len(cand) > 5 else 1)
+    if plan.persistent_supported:
+        plan.set_persistent(bool(cand[6]) if len(cand) > 6 else False)
 
 
 def candidates(plan):
@@ -98,9 +101,11 @@ def autotune(plan, uh, steps=6, log=None, cache_file: Optional[str] = None, fina
         if log:
             log(f"[autotune] {key} {cand}: {results[cand]:.4f} ms/step")
 
+    if plan.persistent_supported:
+        trial((16, 0, 0, 256, 0, 1, 1))  # the persistent per-sample stepper (launch shape unused)
     for cand in candidates(plan):
         trial(cand + (1,))
-    shape = min(results, key=results.get)
+    shape = min((c for c in results if len(c) < 7 or not c[6]), key=results.get)
     for sb in sub_batches(plan):
         trial(shape[:4] + (sb, 1))
     for sb in sub_batches(plan):

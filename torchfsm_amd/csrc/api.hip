@@ -128,6 +128,7 @@ struct tfsm_plan {
   int row_wg = 256;  // row-kernel workgroup size
   int sub_batch = 0; // samples stepped together through all steps (0 = whole batch)
   int nstreams = 1;  // 2: alternate sub-batches run on the caller's stream and an auxiliary one
+  bool persistent = false;  // 2D: step with the persistent per-sample kernel k2_run
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int PF = 0, NF = 0;
@@ -509,6 +510,26 @@ int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* tra
     a.traj = (float2*)traj;
     a.T = T;
     HIP_TRY(launch_run1d(a, st), "1D stepper launch");
+    return TFSM_OK;
+  }
+  if (p->D == 2 && p->persistent && run2d_supported(p->term, p->n[0], p->n[1], p->PF) && !p->cplx) {
+    // one workgroup per sample runs every step (small meshes: launch-bound otherwise)
+    Col2Args ca = col_args(p, U, (char*)ws);
+    Row2Args ra = row_args(p, (char*)ws, 0, p->B);
+    // k2_run runs the default radix plan (variant 0) on both axes, whatever the tiled kernels' tuning
+    ca.tw0 = p->tw[0][0];
+    ra.tw1 = p->tw[1][0];
+    Run2Args r;
+    r.b0 = 0;
+    r.nb = p->B;
+    r.scheme = p->scheme;
+    r.nsteps = nsteps;
+    r.act = act;
+    r.rec_slot = rec_slot;
+    r.traj = (float2*)traj;
+    r.T = T;
+    r.per_sample = (size_t)p->C * p->modes;
+    HIP_TRY(launch_run2d(p->term, ca, ra, r, st), "persistent 2D stepper launch");
     return TFSM_OK;
   }
   if (p->D >= 2) {
@@ -1104,7 +1125,7 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     // measured per axis size (SETDRK4, B = 16..32): 512 keeps 16-column tiles; 256 runs 8-column
     // tiles with the 8-values-per-thread plan (26 % faster than 16 columns of 16 values); 64 and
     // 128 run 8-column tiles and one row per 64-thread row group (5-9 %)
-    p->col_var = (n0 == 512 && one_channel) || n0 == 256 ? 1 : 0;
+    p->col_var = n0 == 256 ? 1 : 0;
     if (n0 == 64 || n0 == 128 || n0 == 256) p->V = 8;
     if (n1 == 64 || n1 == 128) p->row_wg = 64;
     const size_t per_sample = 4 * (size_t)p->C * p->modes * sizeof(float2) +
@@ -1114,7 +1135,22 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     while (sb < p->B && (size_t)tiles * sb < 512) sb += 8;
     if (sb >= p->B || (size_t)sb * per_sample > ((size_t)256 << 20) || p->ks_mean) sb = 0;
     p->sub_batch = sb;
+    // 512-point one-channel columns (VorticityConvection) with 32 or more samples: sub-batches of 16
+    // on two streams, so one sub-batch's VALU-bound row pass overlaps the other's memory-bound
+    // column pass (NS 512^2, B = 64: 0.630 ms/step against 0.662 for one stream and 32-sample
+    // sub-batches, and 0.676 for the 16-values-per-thread columns it replaces)
+    if (n0 == 512 && term == TERM_VC && p->B >= 32) {
+      p->sub_batch = 16;
+      if (hipStreamCreateWithFlags(&p->aux, hipStreamNonBlocking) == hipSuccess &&
+          hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) == hipSuccess &&
+          hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) == hipSuccess)
+        p->nstreams = 2;
+      else
+        rc = hip_fail(hipGetLastError(), "auxiliary stream");
+    }
   }
+  // small square meshes step with the persistent per-sample kernel (one launch per call)
+  if (!rc && D == 2 && run2d_supported(term, p->n[0], p->n[1], p->PF) && !p->cplx) p->persistent = true;
   if (!rc && D == 3) p->row_var = (p->n[2] >= 64 && is_pow2(p->n[2])) ? 1 : 0;  // radix-4 z rows (fewer registers)
   if (rc) {
     tfsm_plan_destroy(p);
@@ -1182,6 +1218,17 @@ int tfsm_plan_get_tuning(const tfsm_plan* plan, int32_t* out) {
   out[5] = plan->nstreams;
   return TFSM_OK;
 }
+
+int tfsm_plan_set_persistent(tfsm_plan* plan, int32_t on) {
+  if (!plan) return fail(TFSM_E_INVALID, "null plan");
+  if (on && !(plan->D == 2 && run2d_supported(plan->term, plan->n[0], plan->n[1], plan->PF) && !plan->cplx))
+    return fail(TFSM_E_INVALID, "the persistent 2D stepper covers square 32^2 and 64^2 meshes with Convection or "
+                                "VorticityConvection (2/3 dealiasing or coarser)");
+  plan->persistent = on != 0;
+  return TFSM_OK;
+}
+
+int32_t tfsm_plan_persistent(const tfsm_plan* plan) { return plan && plan->persistent ? 1 : 0; }
 
 int tfsm_plan_set_streams(tfsm_plan* plan, int32_t nstreams) {
   if (!plan) return fail(TFSM_E_INVALID, "null plan");
@@ -1612,18 +1659,21 @@ int tfsm_solve(tfsm_plan* p, const void* uh, void* out, void* stream) {
   return TFSM_OK;
 }
 
-int tfsm_launch_pass(tfsm_plan* p, void* uh, void* ws, int32_t pass_id, int32_t stage, void* stream) {
+int tfsm_launch_pass(tfsm_plan* p, void* uh, void* ws, int32_t pass_id, int32_t stage, int32_t sub, void* stream) {
   if (!p || !uh || !ws) return fail(TFSM_E_INVALID, "null argument");
   hipStream_t st = (hipStream_t)stream;
-  // the launches cover the first sub-batch, exactly as tfsm_step launches them
-  const int nb = (p->sub_batch > 0 && p->sub_batch < p->B) ? p->sub_batch : p->B;
+  // the launches cover sub-batch `sub`, exactly as tfsm_step launches them
+  const int sb = (p->sub_batch > 0 && p->sub_batch < p->B) ? p->sub_batch : p->B;
+  if (sub < 0 || (int64_t)sub * sb >= p->B) return fail(TFSM_E_INVALID, "sub-batch index out of range");
+  const int b0 = sub * sb;
+  const int nb = (p->B - b0) < sb ? (p->B - b0) : sb;
   if (p->D == 2) {
     if (p->term == TERM_NONE) return fail(TFSM_E_INVALID, "no product term");
-    if (pass_id == 0) return launch_row(p, (char*)ws, 0, nb, st);
+    if (pass_id == 0) return launch_row(p, (char*)ws, b0, nb, st);
     if (pass_id == 1) {
       if (stage < ST_PRO || stage > ST_EVAL) return fail(TFSM_E_INVALID, "bad stage");
       if (stage != ST_PRO && p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
-      return launch_col(p, (float2*)uh, (char*)ws, stage, 0, nb, st);
+      return launch_col(p, (float2*)uh, (char*)ws, stage, b0, nb, st);
     }
   }
   if (p->D == 3) {
@@ -1631,9 +1681,9 @@ int tfsm_launch_pass(tfsm_plan* p, void* uh, void* ws, int32_t pass_id, int32_t 
     if (stage < ST_PRO || stage > ST_EVAL) return fail(TFSM_E_INVALID, "bad stage");
     if (stage != ST_PRO && p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");
     // one whole evaluation (4 kernels) per call; pass ids 0/1 are not split for 3D
-    if (pass_id == 3) return run3d_eval(p, (float2*)uh, (char*)ws, stage, 0, nb, st);
+    if (pass_id == 3) return run3d_eval(p, (float2*)uh, (char*)ws, stage, b0, nb, st);
     if (pass_id >= 4 && pass_id <= 7)
-      return run3d_eval(p, (float2*)uh, (char*)ws, stage, 0, nb, st, 1 << (pass_id - 4));
+      return run3d_eval(p, (float2*)uh, (char*)ws, stage, b0, nb, st, 1 << (pass_id - 4));
   }
   if (p->D == 1 && pass_id == 2) {
     if (p->scheme < 0) return fail(TFSM_E_STATE, "no scheme set on the plan");

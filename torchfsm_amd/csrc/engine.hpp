@@ -338,7 +338,7 @@ __device__ __forceinline__ float2 stage_combine_v(int stage, float2 nval, float2
 // workgroups per CU.  The loads are unconditional: a lane past the last column (colv false) reads the padded
 // columns of the same rows, which every array and table holds, and its result is discarded.
 // Offsets within one channel are 32-bit (every supported mesh has fewer than 2^28 modes).
-template <int RMAX, int NT, int BM = (RMAX >= 16 ? 1 : 4)>
+template <int RMAX, int NT, int BM = (RMAX >= 16 ? 1 : 4), bool FENCE = false>
 __device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1, float2* A2, const float2* src,
                                                const float4* cpk, const float* c5b, unsigned c5o, float coef_t,
                                                unsigned stride, int j, bool colv, const float2 (&v)[RMAX],
@@ -352,6 +352,10 @@ __device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1,
   const unsigned o0 = (unsigned)j * stride, ostep = (unsigned)NT * stride;
 #pragma unroll
   for (int m0 = 0; m0 < RMAX; m0 += BM) {
+    // FENCE (kernels compiled for one stage): keep each batch's loads in their batch; with the stage
+    // known at compile time the scheduler would otherwise issue every mode's operand loads at once,
+    // far beyond the register budget
+    if constexpr (FENCE) __builtin_amdgcn_sched_barrier(0);
     float2 uu[BM], q1[BM], q2[BM], sv[BM];
     float4 cc[BM];
     float c5[BM];
@@ -469,6 +473,17 @@ struct Row2Args {
   float2* P;
   int b0;
   StepMask mask;
+};
+
+// persistent per-sample 2D stepper (k2_run): samples [b0, b0 + nb), one workgroup each
+struct Run2Args {
+  int b0, nb;
+  int scheme, nsteps;
+  const int* act;           // per-sample step counts (device) or null
+  const int* rec_slot;      // device [nsteps + 1]: frame recorded before step i, or -1; or null
+  float2* traj;             // [B][T][per_sample]
+  int T;
+  size_t per_sample;        // C * modes
 };
 
 struct Col3Args {
@@ -589,6 +604,10 @@ struct ConvArgs {
 hipError_t launch_col2d(int term, const Col2Args& a, int nb, int stage, int v, int var, hipStream_t st);
 hipError_t launch_row2d(int term, const Row2Args& a, int var, int wgt, hipStream_t st);
 bool col2d_supported(int n0, int v, int var);
+// persistent per-sample stepper (k2_run): square 32^2 and 64^2 meshes, Convection or
+// VorticityConvection, field pitch pf at most that of the 2/3 rule
+hipError_t launch_run2d(int term, const Col2Args& ca, const Row2Args& ra, const Run2Args& r, hipStream_t st);
+bool run2d_supported(int term, int n0, int n1, int pf);
 bool row2d_supported(int n1, int var, int wgt);
 hipError_t launch_rows_r2c(int n, const float* in, size_t in_ld, float2* out, size_t out_ld, int nrows, float scale,
                            const float2* tw, hipStream_t st);

@@ -24,6 +24,18 @@ hipError_t launch_row2d(int term, const Row2Args& a, int var, int wgt, hipStream
   return hipErrorInvalidValue;
 }
 
+hipError_t launch_run2d(int term, const Col2Args& ca, const Row2Args& ra, const Run2Args& r, hipStream_t st) {
+  if (term == TERM_VC) return launch_run2d_vc(ca, ra, r, st);
+  if (term == TERM_CONV) return launch_run2d_conv(ca, ra, r, st);
+  return hipErrorInvalidValue;
+}
+
+bool run2d_supported(int term, int n0, int n1, int pf) {
+  // fields at the pitch of the 2/3 rule or narrower (k2_run holds them in LDS)
+  return (term == TERM_VC || term == TERM_CONV) && n0 == n1 && (n0 == 32 || n0 == 64) &&
+         pf <= ((n0 / 3 + 1 + 15) / 16) * 16;
+}
+
 bool col2d_supported(int n0, int v, int var) {
   if (!(v == 8 || v == 16)) return false;
   if (n0 == 16 || n0 == 32 || is_mixed_size_2d(n0)) return var == 0;

@@ -56,14 +56,14 @@ struct Term2<TERM_CONS> {  // ConservativeConvection, C = 2: fields u0, u1; prod
 // Stage combine of one channel's column (see combine_column in engine.hpp): column base ky,
 // element stride P1.  With two channels the other channel's values stay live, so those kernels load
 // one mode's operands at a time (BM = 1) to stay within 128 VGPRs.
-template <int N0, int VAR, int BM>
+template <int N0, int VAR, int BM, bool FENCE = false>
 __device__ __forceinline__ void combine_column2(const Col2Args& a, int stage, size_t base, const float2* src,
                                                 const float4* cpk, const float* c5b, int cko, int ky, bool colv, int j,
                                                 const float2 (&v)[Fft<N0, VAR>::RMAX],
                                                 float2 (&s)[Fft<N0, VAR>::RMAX]) {
   using F = Fft<N0, VAR>;
   const size_t cb = base + ky;
-  combine_column<F::RMAX, F::NT, BM>(stage, a.U + cb, a.A1 + cb, a.A2 + cb, src ? src + ky : nullptr,
+  combine_column<F::RMAX, F::NT, BM, FENCE>(stage, a.U + cb, a.A1 + cb, a.A2 + cb, src ? src + ky : nullptr,
                                      cpk + (2 * ky + cko), c5b, (unsigned)ky, a.coef_t, (unsigned)a.P1, j, colv, v,
                                      s);
 }
@@ -118,41 +118,25 @@ __device__ __forceinline__ void col_nsp_combine(const Col2Args& a, int stage, fl
   }
 }
 
-template <int N0, int V, int TERM, int VAR>
-__global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int stage) {
+// Stage combine, fields and inverse x-transforms of column tile `tile` of sample b: the body of
+// k2_col (and of the column pass of k2_run).  ST >= 0: the stage is a compile-time constant.
+template <int N0, int V, int TERM, int VAR, int ST, bool CLAMP = false>
+__device__ __forceinline__ void col2_tile(const Col2Args& a, int stage, int tile, int b, float2* s_buf,
+                                          const float2* s_tw, const float* s_ax, int j, int vi) {
   using F = Fft<N0, VAR>;
   constexpr int RMAX = F::RMAX, NT = F::NT;
   constexpr int NC = Term2<TERM>::NC, NF = Term2<TERM>::NF;
-  __shared__ float2 s_tw[Fft<N0, VAR>::tw_size()];
-  __shared__ float2 s_buf[LdsSize<N0, V>::slots];
-  __shared__ float s_ax[N0];  // x wavenumbers: read per mode by the field pass without a memory wait
-  load_twiddles<N0, VAR>(s_tw, a.tw0);
-  for (int i = threadIdx.x; i < N0; i += NT * V) s_ax[i] = a.ax[i];
-  __syncthreads();
-  const int vi = threadIdx.x % V, j = threadIdx.x / V;
-  // Block -> (tile, sample).  Workgroups are dispatched round-robin over the 8 XCDs (block v on
-  // XCD v % 8); with nb % 8 == 0 each XCD owns a group of nb/8 samples and walks the tiles in
-  // order, so a coefficient tile is fetched once into that XCD's L2 and reused by its samples, and
-  // the heavy tiles (those with kept columns, ky <= kmax1) are dispatched before the light ones.
-  const int ntiles = (a.H1 + V - 1) / V;
-  int tile, bl;
-  if ((a.nb & 7) == 0) {
-    const int x = blockIdx.x & 7, sidx = blockIdx.x >> 3, g = a.nb >> 3;
-    tile = sidx / g;
-    bl = x * g + sidx % g;
-  } else {
-    tile = blockIdx.x % ntiles;
-    bl = blockIdx.x / ntiles;
-  }
-  const int b = a.b0 + bl;
-  if (a.mask.skip(b)) return;  // uniform per workgroup: this sample's steps are done
   const int ky = tile * V + vi;
   const bool colv = ky < a.H1;
+  // the column the lane's loads address: its own, or (CLAMP, tiles wider than the pitch P1) the last
+  // valid column for the lanes past it, whose results are discarded.  Without CLAMP the tiles stay
+  // within the pitch (V <= 16 and P1 a multiple of 16), whose padded columns every array holds.
+  const int kl = (CLAMP && !colv) ? a.H1 - 1 : ky;
   const size_t plane = (size_t)N0 * a.P1;
   const float4* cpk = a.cpk + (size_t)b * a.cpk_bstride;
   const float* c5b = a.c5t + (size_t)b * (a.cpk_bstride / 2);
-  // cpk_index(ci, half) = 2 ci + cko: every mode of this lane has ci & 15 == ky & 15
-  const int cko = (stage_coef_half(stage) << 4) - (ky & 15);
+  // cpk_index(ci, half) = 2 ci + cko: every mode of this lane has ci & 15 == kl & 15
+  const int cko = (stage_coef_half(stage) << 4) - (kl & 15);
   float2 s[NC][RMAX];
   if constexpr (TERM == TERM_NSP) {
     col_nsp_combine<N0, V, VAR>(a, stage, s, s_buf, s_tw, b, ky, colv, j, vi);
@@ -163,7 +147,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
     if (stage == ST_PRO) {
 #pragma unroll
       for (int m = 0; m < RMAX; ++m) {
-        const float2 t = a.U[base + (size_t)(j + m * NT) * a.P1 + ky];
+        const float2 t = a.U[base + (size_t)(j + m * NT) * a.P1 + kl];
         s[c][m] = colv ? t : make_float2(0.f, 0.f);
       }
     } else {
@@ -173,7 +157,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
         const size_t pb = ((size_t)b * 4 + 2 * c) * plane;
 #pragma unroll
         for (int m = 0; m < RMAX; ++m) {
-          const float2 t = a.P[pb + (size_t)(j + m * NT) * a.P1 + ky];
+          const float2 t = a.P[pb + (size_t)(j + m * NT) * a.P1 + kl];
           v[m] = colv ? t : make_float2(0.f, 0.f);
         }
         fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
@@ -185,7 +169,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
         float2 w[RMAX];
 #pragma unroll
         for (int m = 0; m < RMAX; ++m) {
-          const float2 t = a.P[pb + plane + (size_t)(j + m * NT) * a.P1 + ky];
+          const float2 t = a.P[pb + plane + (size_t)(j + m * NT) * a.P1 + kl];
           w[m] = colv ? t : make_float2(0.f, 0.f);
         }
         fft_wg<N0, -1, V, VAR>(w, s_buf, s_tw, j, vi);
@@ -194,7 +178,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
       } else {
 #pragma unroll
         for (int m = 0; m < RMAX; ++m) {
-          const float2 t = a.P[base + (size_t)(j + m * NT) * a.P1 + ky];
+          const float2 t = a.P[base + (size_t)(j + m * NT) * a.P1 + kl];
           v[m] = colv ? t : make_float2(0.f, 0.f);
         }
         fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
@@ -204,7 +188,8 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
         if (a.dc && ky == 0 && j == 0) v[0] = rsub(v[0], a.dc[0]);
       }
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride : nullptr;
-      combine_column2<N0, VAR, (NC > 1 || RMAX >= 16) ? 1 : 4>(a, stage, base, src, cpk, c5b, cko, ky, colv, j, v, s[c]);
+      combine_column2<N0, VAR, (NC > 1 || RMAX >= 16) ? 1 : 4, (ST >= 0)>(a, stage, base, src, cpk, c5b, cko, kl, colv,
+                                                                             j, v, s[c]);
     }
   }
   }
@@ -244,7 +229,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
       v[m] = val;
     }
     fft_wg<N0, +1, V, VAR>(v, s_buf, s_tw, j, vi);
-    if (ky < a.PF) {
+    if (ky <= a.kmax1) {  // kept columns only: the row kernel reads no other
       float2* g = a.G + (((size_t)b * NF + f) * N0) * a.PF + ky;
 #pragma unroll
       for (int m = 0; m < RMAX; ++m) g[(size_t)(j + m * NT) * a.PF] = v[m];
@@ -252,26 +237,53 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
   }
 }
 
-template <int N1, int ROWS, int TERM, int VAR>
-__global__ void __launch_bounds__((Fft<N1, VAR>::NT * ROWS)) k2_row(Row2Args a) {
+
+
+// ST >= 0: the stage is a compile-time constant (only that stage's combine is compiled, which keeps
+// the code a launch executes small); ST = -1: the stage comes at run time.
+template <int N0, int V, int TERM, int VAR, int ST = -1>
+__global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int stage_rt) {
+  const int stage = ST >= 0 ? ST : stage_rt;
+  using F = Fft<N0, VAR>;
+  constexpr int NT = F::NT;
+  __shared__ float2 s_tw[Fft<N0, VAR>::tw_size()];
+  __shared__ float2 s_buf[LdsSize<N0, V>::slots];
+  __shared__ float s_ax[N0];  // x wavenumbers: read per mode by the field pass without a memory wait
+  load_twiddles<N0, VAR>(s_tw, a.tw0);
+  for (int i = threadIdx.x; i < N0; i += NT * V) s_ax[i] = a.ax[i];
+  __syncthreads();
+  const int vi = threadIdx.x % V, j = threadIdx.x / V;
+  // Block -> (tile, sample).  Workgroups are dispatched round-robin over the 8 XCDs (block v on
+  // XCD v % 8); with nb % 8 == 0 each XCD owns a group of nb/8 samples and walks the tiles in
+  // order, so a coefficient tile is fetched once into that XCD's L2 and reused by its samples, and
+  // the heavy tiles (those with kept columns, ky <= kmax1) are dispatched before the light ones.
+  // (A tile-major order, each XCD taking an eighth of the tiles for all samples so that every
+  // coefficient tile is fetched by one or two XCDs only, made the 512-point columns 35 % slower: at
+  // any moment every CU then reads the same band of columns, at one stride across the rows.)
+  const int ntiles = (a.H1 + V - 1) / V;
+  int tile, bl;
+  if ((a.nb & 7) == 0) {
+    const int x = blockIdx.x & 7, sidx = blockIdx.x >> 3, g = a.nb >> 3;
+    tile = sidx / g;
+    bl = x * g + sidx % g;
+  } else {
+    tile = blockIdx.x % ntiles;
+    bl = blockIdx.x / ntiles;
+  }
+  const int b = a.b0 + bl;
+  if (a.mask.skip(b)) return;  // uniform per workgroup: this sample's steps are done
+  col2_tile<N0, V, TERM, VAR, ST>(a, stage, tile, b, s_buf, s_tw, s_ax, j, vi);
+}
+
+// Row x of sample b (rv false: idle row that only takes part in the transforms' synchronisation):
+// packed C2R of the fields, physical product, R2C into the products P.  The body of k2_row (and of
+// the row pass of k2_run); `lds` is the row's own LDS region.
+template <int N1, int TERM, int VAR>
+__device__ __forceinline__ void row2_body(const Row2Args& a, int b, int x, bool rv, float2* lds, const float2* s_tw,
+                                          int j) {
   using F = Fft<N1, VAR>;
   constexpr int RMAX = F::RMAX, NT = F::NT;
-  constexpr int SL = LdsSize<N1, 1>::slots;
   constexpr int NF = Term2<TERM>::NF;
-  __shared__ float2 s_tw[Fft<N1, VAR>::tw_size()];
-  __shared__ float2 s_buf[ROWS * SL];
-  load_twiddles<N1, VAR>(s_tw, a.tw1);
-  __syncthreads();
-  const int g = threadIdx.x / NT, j = threadIdx.x % NT;
-  int row = blockIdx.x * ROWS + g;
-  // with NT a multiple of 64 a wavefront holds one row: the row (and every row pointer) is uniform,
-  // so the loads and stores take a scalar base and a 32-bit lane offset
-  if constexpr (NT % 64 == 0) row = __builtin_amdgcn_readfirstlane(row);
-  const bool in_range = row < a.B * a.N0;
-  const int b = a.b0 + (in_range ? row / a.N0 : 0);
-  const bool rv = in_range && !a.mask.skip(b);  // rows of samples whose steps are done stay idle
-  const int x = rv ? row % a.N0 : 0;
-  float2* lds = s_buf + g * SL;
   const size_t fs = (size_t)a.N0 * a.PF;
   const float2* gb = a.G + ((size_t)b * NF) * fs + (size_t)x * a.PF;
 
@@ -458,16 +470,162 @@ __global__ void __launch_bounds__((Fft<N1, VAR>::NT * ROWS)) k2_row(Row2Args a) 
   }
 }
 
+
+template <int N1, int ROWS, int TERM, int VAR>
+__global__ void __launch_bounds__((Fft<N1, VAR>::NT * ROWS)) k2_row(Row2Args a) {
+  using F = Fft<N1, VAR>;
+  constexpr int NT = F::NT;
+  constexpr int SL = LdsSize<N1, 1>::slots;
+  __shared__ float2 s_tw[Fft<N1, VAR>::tw_size()];
+  __shared__ float2 s_buf[ROWS * SL];
+  load_twiddles<N1, VAR>(s_tw, a.tw1);
+  __syncthreads();
+  const int g = threadIdx.x / NT, j = threadIdx.x % NT;
+  int row = blockIdx.x * ROWS + g;
+  // with NT a multiple of 64 a wavefront holds one row: the row (and every row pointer) is uniform,
+  // so the loads and stores take a scalar base and a 32-bit lane offset
+  if constexpr (NT % 64 == 0) row = __builtin_amdgcn_readfirstlane(row);
+  const bool in_range = row < a.B * a.N0;
+  const int b = a.b0 + (in_range ? row / a.N0 : 0);
+  const bool rv = in_range && !a.mask.skip(b);  // rows of samples whose steps are done stay idle
+  const int x = rv ? row % a.N0 : 0;
+  row2_body<N1, TERM, VAR>(a, b, x, rv, s_buf + g * SL, s_tw, j);
+}
+
+// ---------------------------------------------------------------- persistent per-sample stepper
+// k2_run: for small square meshes (N = 32, 64) one workgroup per sample runs every step of a
+// tfsm_step call, with the sample's fields G and products P held in LDS for the whole run: each
+// evaluation's row pass (all N rows at once, one row per NT threads) reads G from LDS and writes P
+// to LDS, and its column pass (one tile of N columns) reads P and writes G, with a barrier between
+// them; only the state, the stage arrays and the coefficient tables are read from global memory.
+// One launch covers all steps instead of two launches per evaluation.  Same device code as the
+// tiled kernels (col2_tile, row2_body, with G and P addressed through flat pointers into LDS);
+// compiled into another kernel, the transforms' multiply-adds may fuse differently, so results
+// agree with the tiled kernels to float32 rounding rather than bit for bit.  The column tile is
+// wider than the pitch, so the column pass clamps its load column (col2_tile<..., CLAMP>).
+// (At 128^2 G and P do not fit LDS; through L2 the stepper took 160 us per SETDRK4 step against
+// 90-130 us for the tiled kernels, and 128^2 keeps them.)
+template <int N, int TERM>
+struct Run2Shape {
+  using F = Fft<N, 0>;
+  static constexpr int NT = F::NT;               // threads per transform (rows and columns)
+  static constexpr int V = N;                    // one column tile holds all N/2 + 1 columns
+  static constexpr int THREADS = NT * V;         // 128 (N = 32), 512 (N = 64)
+  static constexpr int ROWS = THREADS / NT;      // = N: every row in one pass
+  static constexpr int PDR = F::PD < 16 ? F::PD : 16;
+  static constexpr int SLR = N + N / PDR + 1;    // LDS slots of one row (padding of lds_slot<1, PD>)
+  static constexpr int COL_SLOTS = LdsSize<N, V>::slots;
+  static constexpr int SLOTS = COL_SLOTS > ROWS * SLR ? COL_SLOTS : ROWS * SLR;
+  static constexpr int P1 = (N / 2 + 1 + 15) / 16 * 16;  // pitch of the half spectrum
+  static constexpr int PF = (N / 3 + 1 + 15) / 16 * 16;  // field pitch at the 2/3 rule (kmax = N/3)
+  static constexpr int NF = Term2<TERM>::NF;
+  static constexpr int NP = Term2<TERM>::NC;     // product planes: VorticityConvection 1, Convection 2
+  static constexpr int G_SLOTS = NF * N * PF;
+  static constexpr int P_SLOTS = NP * N * P1;
+  static_assert(ROWS == N, "every row in one pass");
+};
+
+template <int N, int TERM>
+__global__ void __launch_bounds__((Run2Shape<N, TERM>::THREADS)) k2_run(Col2Args ca, Row2Args ra, Run2Args r) {
+  static_assert(TERM == TERM_VC || TERM == TERM_CONV, "the persistent stepper's column loads are clamped for "
+                                                      "Convection and VorticityConvection only");
+  using S = Run2Shape<N, TERM>;
+  constexpr int NT = S::NT, V = S::V;
+  __shared__ float2 s_tw[Fft<N, 0>::tw_size()];
+  __shared__ float2 s_buf[S::SLOTS];
+  __shared__ float2 s_G[S::G_SLOTS];
+  __shared__ float2 s_P[S::P_SLOTS];
+  __shared__ float s_ax[N];
+  const int b = r.b0 + blockIdx.x;
+  int ns = r.nsteps;
+  if (r.act) ns = r.act[b] < ns ? r.act[b] : ns;  // per-sample step count (uniform per workgroup)
+  if (ns <= 0 && !r.rec_slot) return;
+  // per-sample views with sample index 0: global arrays offset to sample b, G and P in LDS
+  const size_t per = r.per_sample;
+  Col2Args c = ca;
+  c.U += (size_t)b * per;
+  c.A1 += (size_t)b * per;
+  c.A2 += (size_t)b * per;
+  c.cpk += (size_t)b * ca.cpk_bstride;
+  c.c5t += (size_t)b * (ca.cpk_bstride / 2);
+  c.coef += (size_t)b * ca.coef_bstride;
+  if (c.src) c.src += (size_t)b * ca.src_bstride;
+  c.G = s_G;
+  c.P = s_P;
+  c.b0 = 0;
+  Row2Args w = ra;
+  w.G = s_G;
+  w.P = s_P;
+  w.b0 = 0;
+  // rows and columns share the plan: a square mesh with the same radix variant on both axes
+  load_twiddles<N, 0>(s_tw, ca.tw0);
+  for (int i = threadIdx.x; i < N; i += S::THREADS) s_ax[i] = ca.ax[i];
+  __syncthreads();
+  const int nev = scheme_evals(r.scheme);
+  auto record = [&](int i) {  // the state before step i (i = ns: the final state), as k_record
+    const int sl = r.rec_slot[i];
+    if (sl < 0 || sl >= r.T) return;
+    float2* dst = r.traj + ((size_t)b * r.T + sl) * per;
+    for (size_t q = threadIdx.x; q < per; q += S::THREADS) dst[q] = c.U[q];
+  };
+  // pass k = 0: fields of the state (ST_PRO); k = 1 + i nev + e: evaluation e of step i
+  const int npass = ns > 0 ? 1 + ns * nev : 0;
+#pragma unroll 1
+  for (int k = 0; k < npass; ++k) {
+    // the lane indices are re-derived in every pass behind an opaque copy: otherwise the compiler
+    // hoists every mode's addresses out of the step loop and keeps them live across both passes
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int vi = tid % V, jc = tid / V;  // column pass: column vi
+    const int g = tid / NT, jr = tid % NT;  // row pass: row g
+    const int e = k > 0 ? (k - 1) % nev : 0;
+    if (k > 0) {
+      __syncthreads();  // fields G of the previous column pass (and the state) complete
+      if (e == 0 && r.rec_slot) record((k - 1) / nev);
+      row2_body<N, TERM, 0>(w, 0, g, true, s_buf + g * S::SLR, s_tw, jr);
+    }
+    __syncthreads();  // products P complete; the LDS exchange buffer is reused
+    col2_tile<N, V, TERM, 0, -1, true>(c, k > 0 ? scheme_stage(r.scheme, e) : ST_PRO, 0, 0, s_buf, s_tw, s_ax, jc,
+                                       vi);
+  }
+  if (r.rec_slot) {
+    __syncthreads();
+    record(ns > 0 ? ns : 0);
+  }
+}
+
 // ---------------------------------------------------------------- host launchers
 // Tuning knobs (chosen per shape class by the autotuner): column tile V in {8, 16}, radix-plan
 // variant of the column and row transforms (0: N/RMAX threads per transform; 1: half as many),
 // and the row-kernel workgroup size (64 or 256 threads).
+// Shapes whose column kernels are also compiled per SETDRK4 stage (and for the field pass of the
+// initial state): the 512-point VorticityConvection columns of the Navier-Stokes workloads with 8
+// values per thread.  One stage's kernel holds a quarter of the combine code of the run-time-stage
+// kernel (13-16 KB of code instead of 21 KB) and 87-128 VGPRs; with 16 values per thread the
+// stage kernels need 133-164 VGPRs (occupancy 3) and the run-time-stage kernel is kept.
+template <int N0, int V, int TERM, int VAR>
+struct ColStageKernels {
+  static constexpr bool value = TERM == TERM_VC && N0 == 512 && VAR == 0;
+};
+
 template <int N0, int V, int TERM, int VAR>
 inline hipError_t launch_col_t(const Col2Args& a, int nb, int stage, hipStream_t st) {
   const int tiles = (a.H1 + V - 1) / V;
   Col2Args b = a;
   b.nb = nb;
-  hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR>), dim3(tiles * nb), dim3(Fft<N0, VAR>::NT * V), 0, st, b, stage);
+  const dim3 grid(tiles * nb), block(Fft<N0, VAR>::NT * V);
+  if constexpr (ColStageKernels<N0, V, TERM, VAR>::value) {
+    switch (stage) {
+      case ST_PRO: hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR, ST_PRO>), grid, block, 0, st, b, stage); break;
+      case ST_R4_E0: hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR, ST_R4_E0>), grid, block, 0, st, b, stage); break;
+      case ST_R4_E1: hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR, ST_R4_E1>), grid, block, 0, st, b, stage); break;
+      case ST_R4_E2: hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR, ST_R4_E2>), grid, block, 0, st, b, stage); break;
+      case ST_R4_E3: hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR, ST_R4_E3>), grid, block, 0, st, b, stage); break;
+      default: hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR>), grid, block, 0, st, b, stage);
+    }
+  } else {
+    hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR>), grid, block, 0, st, b, stage);
+  }
   return hipGetLastError();
 }
 
@@ -501,7 +659,20 @@ inline hipError_t launch_row_t(const Row2Args& a, hipStream_t st) {
 #define TFSM_ROW_CASE(N, VR, W) \
   if (a.N1 == N && var == VR && wgt == W) return launch_row_t<N, TERM, VR, W>(a, st);
 
+template <int N, int TERM>
+inline hipError_t launch_run_t(const Col2Args& ca, const Row2Args& ra, const Run2Args& r, hipStream_t st) {
+  // the kernel's LDS holds G at the field pitch of the 2/3 rule and P at the mesh's pitch
+  if (ca.PF > Run2Shape<N, TERM>::PF || ca.P1 != Run2Shape<N, TERM>::P1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k2_run<N, TERM>), dim3(r.nb), dim3(Run2Shape<N, TERM>::THREADS), 0, st, ca, ra, r);
+  return hipGetLastError();
+}
+#define TFSM_RUN_SIZES(X) X(32) X(64)
+#define TFSM_RUN_CASE(NN) \
+  if (ca.N0 == NN && ca.N1 == NN) return launch_run_t<NN, TERM>(ca, ra, r, st);
+
 // per-unit entry points (k_step2d_*.hip)
+hipError_t launch_run2d_vc(const Col2Args& ca, const Row2Args& ra, const Run2Args& r, hipStream_t st);
+hipError_t launch_run2d_conv(const Col2Args& ca, const Row2Args& ra, const Run2Args& r, hipStream_t st);
 hipError_t launch_col2d_vc(const Col2Args& a, int nb, int stage, int v, int var, hipStream_t st);
 hipError_t launch_col2d_conv(const Col2Args& a, int nb, int stage, int v, int var, hipStream_t st);
 hipError_t launch_col2d_nsp(const Col2Args& a, int nb, int stage, int v, int var, hipStream_t st);

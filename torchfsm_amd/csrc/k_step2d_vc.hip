@@ -15,4 +15,10 @@ hipError_t launch_row2d_vc(const Row2Args& a, int var, int wgt, hipStream_t st) 
   return hipErrorInvalidValue;
 }
 
+hipError_t launch_run2d_vc(const Col2Args& ca, const Row2Args& ra, const Run2Args& r, hipStream_t st) {
+  constexpr int TERM = TERM_VC;
+  TFSM_RUN_SIZES(TFSM_RUN_CASE)
+  return hipErrorInvalidValue;
+}
+
 }  // namespace tfsm

@@ -133,9 +133,12 @@ class SpectralPlan:
         check(lib.tfsm_plan_get_tuning(self._plan, ctypes.cast(cur, ctypes.c_void_p)), "tuning")
         self.tuning = tuple(int(x) for x in cur[:5])
         self.streams = int(cur[5])
+        self.persistent_supported = self.persistent  # on by default wherever the plan supports it
 
     # ------------------------------------------------------------------ lifecycle
     def close(self):
+        if "_step_graph" in self.__dict__:
+            self._drop_step_graph()
         if getattr(self, "_plan", None) is not None and self._plan.value:
             lib.tfsm_plan_destroy(self._plan)
             self._plan = ctypes.c_void_p()
@@ -158,16 +161,35 @@ class SpectralPlan:
     def set_tuning(self, col_tile=16, col_var=0, row_var=0, row_wg=256, sub_batch=0):
         """Launch shape of the fused 2D kernels and the temporal-blocking sub-batch (see
         tfsm_plan_set_tuning)."""
+        self._drop_step_graph()
         check(lib.tfsm_plan_set_tuning(self._plan, int(col_tile), int(col_var), int(row_var), int(row_wg),
                                        int(sub_batch)), "tuning")
         self.tuning = (int(col_tile), int(col_var), int(row_var), int(row_wg), int(sub_batch))
 
+    @property
+    def persistent(self) -> bool:
+        """True when 2D stepping runs the persistent per-sample kernel (small square meshes)."""
+        return bool(lib.tfsm_plan_persistent(self._plan))
+
+    def set_persistent(self, on: bool):
+        """Persistent per-sample 2D stepper (True) or the tiled row and column kernels (False)."""
+        self._drop_step_graph()
+        check(lib.tfsm_plan_set_persistent(self._plan, 1 if on else 0), "persistent")
+
     def set_streams(self, nstreams: int):
         """1: one stream; 2: alternate sub-batches overlap on an auxiliary stream (tfsm_plan_set_streams)."""
+        self._drop_step_graph()
         check(lib.tfsm_plan_set_streams(self._plan, int(nstreams)), "streams")
         self.streams = int(nstreams)
 
+    def _drop_step_graph(self):
+        """A graph captured by integrate() bakes the scheme's stages and the launch shape in."""
+        cache = self.__dict__.pop("_step_graph", None)
+        if cache is not None:
+            cache[1].close()
+
     def set_scheme(self, scheme: str, dt: float, n_points: int = 16, radius: float = 1.0):
+        self._drop_step_graph()
         with torch.cuda.device(self.device):
             check(lib.tfsm_plan_set_scheme(self._plan, SCHEME_IDS[scheme], float(dt), int(n_points), float(radius),
                                            _stream(self.device)), "scheme")
@@ -195,6 +217,7 @@ class SpectralPlan:
 
     def add_source(self, src: torch.Tensor, coef: float):
         """Explicit source coef * fftn(src); src is (Bs, Cs, N...) with Bs in {1, B}, Cs in {1, C}."""
+        self._drop_step_graph()
         if src.dim() != self.ndim + 2:
             raise ValueError(f"explicit source must have shape (B, C, N...), got {tuple(src.shape)}")
         s = src.detach().to(device=self.device, dtype=torch.float32).contiguous()
@@ -402,12 +425,13 @@ class SpectralPlan:
             check(lib.tfsm_solve(self._plan, _ptr(uh), _ptr(out), _stream(self.device)), "solve")
         return out
 
-    def launch_pass(self, uh: torch.Tensor, pass_id: int, stage: int = 0):
-        """One row (0) / column (1, stage code) / 1D step (2) pass, for kernel timing."""
+    def launch_pass(self, uh: torch.Tensor, pass_id: int, stage: int = 0, sub: int = 0):
+        """One row (0) / column (1, stage code) / 1D step (2) / 3D (3..7) pass over sub-batch `sub`,
+        for kernel timing."""
         self._check_state(uh)
         ws = self.workspace()
         with torch.cuda.device(self.device):
-            check(lib.tfsm_launch_pass(self._plan, _ptr(uh), _ptr(ws), int(pass_id), int(stage),
+            check(lib.tfsm_launch_pass(self._plan, _ptr(uh), _ptr(ws), int(pass_id), int(stage), int(sub),
                                        _stream(self.device)), "launch_pass")
 
     def graph(self, uh: torch.Tensor, nsteps: int, counts: Optional[torch.Tensor] = None) -> "StepGraph":

@@ -131,6 +131,21 @@ def _coef_value(coef, batch):
     return float(coef)
 
 
+GRAPH_STEPS = 16  # steps per captured graph replayed by integrate() on small multi-launch meshes
+
+
+def _graph_worthwhile(plan) -> bool:
+    """Plans whose steps are several short launches (2D without the persistent stepper up to 128^2,
+    3D up to 64^3): launch overhead is a large part of a step there, and a replayed HIP graph
+    removes most of it."""
+    if plan.ndim == 1 or plan.persistent:
+        return False
+    pts = 1
+    for n in plan.n:
+        pts *= int(n)
+    return pts <= (64 ** 3 if plan.ndim == 3 else 128 * 128)
+
+
 class OperatorLike(_MutableMixIn):
     """Base of all operators (_base.py:284-805)."""
 
@@ -547,6 +562,20 @@ class OperatorLike(_MutableMixIn):
         if step <= 0:
             return
         if not progressive:
+            if step >= 2 * GRAPH_STEPS and _graph_worthwhile(plan):
+                # several launches per step on a small mesh: replay a captured graph of GRAPH_STEPS
+                # steps (kept on the plan, over its own state buffer) instead of launching each kernel
+                cache = getattr(plan, "_step_graph", None)
+                if cache is None:
+                    buf = plan.empty_spectrum()
+                    cache = plan._step_graph = (buf, plan.graph(buf, GRAPH_STEPS))
+                buf, graph = cache
+                buf.copy_(uh)
+                for _ in range(step // GRAPH_STEPS):
+                    graph.launch()
+                uh.copy_(buf)
+                plan.step(uh, step % GRAPH_STEPS)
+                return
             plan.step(uh, step)
             return
         chunk = max(1, step // 100)
