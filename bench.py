@@ -338,6 +338,71 @@ def serving_leg(dev, n_requests, rate, seed=0, rates=(), rate_requests=60):
     return out
 
 
+def differentiation_leg(dev, reps=5):
+    """The differentiation workload: 2D viscous Burgers 128^2, C=2, B=16, nu=0.01, 100 ETDRK2 steps
+    of dt 0.01, loss = MSE against a seeded target, gradients w.r.t. u0 and nu through the
+    hand-written adjoint with a checkpoint every 10 steps.  Timed: integrate() alone, and
+    integrate() + loss.backward(); beside it the reference's method (torch autograd through the
+    oracle restatement's full-complex torch.fft step, float32 on the same GPU, u0 gradient)."""
+    import torchfsm_amd as tfa
+    from torchfsm_amd import pde
+    from torchfsm_amd.integrator import ETDRKIntegrator
+    from torchfsm_amd.workloads import fourier_modes_field
+
+    n, c, b, steps, dt = 128, 2, 16, 100, 0.01
+    u0 = (0.1 * fourier_modes_field(n, c, 2, b, seed=21)).to(dev)
+    target = (0.1 * fourier_modes_field(n, c, 2, b, seed=22)).to(dev)
+    mesh = tfa.MeshGrid([(0, 1, n)] * 2, device=dev)
+    nu = torch.tensor(0.01, device=dev, requires_grad=True)
+    op = pde.Burgers(nu)
+    op.set_integrator(ETDRKIntegrator.ETDRK2)
+    op.adjoint_checkpoint_every = 10
+
+    def run(grad):
+        u = u0.clone().requires_grad_(grad)
+        if grad:
+            nu.grad = None
+        res = op.integrate(u, mesh=mesh, dt=dt, step=steps)
+        if grad:
+            ((res - target) ** 2).mean().backward()
+        return res
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[len(ts) // 2] * 1e3
+
+    with torch.no_grad():
+        fwd_ms = timed(lambda: run(False))
+    fb_ms = timed(lambda: run(True))
+
+    from oracle import refpath
+
+    def ref():
+        u = u0.clone().requires_grad_(True)
+        out = refpath.integrate(refpath.burgers_terms(0.01), [(0, 1, n)] * 2, u, dt, steps, scheme="ETDRK2")
+        ((out - target) ** 2).mean().backward()
+
+    ref_ms = timed(ref)
+    out = {"workload": f"burgers2d_{n}x{n}_c{c}_b{b}_etdrk2_{steps}steps_ckpt10",
+           "forward_ms": fwd_ms, "forward_backward_ms": fb_ms, "backward_ms": fb_ms - fwd_ms,
+           "backward_ms_per_step": (fb_ms - fwd_ms) / steps,
+           "backward_over_forward": (fb_ms - fwd_ms) / fwd_ms,
+           "reference_autograd_ms": ref_ms, "speedup_vs_reference_autograd": ref_ms / fb_ms,
+           "timing": f"median of {reps} wall-clock runs (device synchronised) after one warm-up",
+           "reference_method": "torch autograd through the oracle restatement of the reference step "
+                               "(full complex torch.fft, float32, same GPU), u0 gradient"}
+    log(f"[bench] differentiation: forward {fwd_ms:.2f} ms, forward+backward {fb_ms:.2f} ms "
+        f"(backward {fb_ms - fwd_ms:.2f} ms); reference autograd {ref_ms:.2f} ms")
+    return out
+
+
 CPU_SHARE = 16  # host CPUs a one-GPU box allots to the GPU's jobs
 
 
@@ -463,6 +528,8 @@ def main():
                     help="lighter Poisson rates (1/s, comma-separated) swept for the p99 bound; empty: none")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run 3 plain 256 MiB device copies (calibrates FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--no-differentiation", action="store_true",
+                    help="skip the differentiation leg (Burgers 128^2 adjoint vs reference autograd)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the engine has no multi-GPU path (--gpus {args.gpus} requested). "
@@ -607,8 +674,12 @@ def main():
         "engine_speedup": value / tg_val,
     }
     log(f"[bench] torch-GPU baseline {tg_val:.3e} DOF*steps/s; engine speedup {value / tg_val:.2f}x")
-    if args.serve_requests > 0 and args.workload == "ns2d":
+    if args.workload == "ns2d":
         del sch, ug, w, plan, uh
+        torch.cuda.empty_cache()
+        if not args.no_differentiation:
+            result["differentiation"] = differentiation_leg(dev)
+    if args.serve_requests > 0 and args.workload == "ns2d":
         torch.cuda.empty_cache()
         log(f"[bench] serving: {args.serve_requests} requests at {args.serve_rate}/s")
         rates = [float(x) for x in args.serve_rates.split(",") if x.strip()] if args.serve_rates else []

@@ -63,7 +63,7 @@ typedef struct tfsm_graph tfsm_graph;
  * state registered by OperatorLike.register_mesh (_base.py:581-624). */
 typedef struct tfsm_geometry {
   int32_t ndim;                      /* 1, 2 or 3 */
-  int32_t n[3];                      /* points per axis: powers of two or a listed mixed-radix size */
+  int32_t n[3];                      /* points per axis, >= 2 (any length; see tfsm_plan_create) */
   double length[3];                  /* end - start per axis */
   int32_t batch;                     /* B */
   int32_t channels;                  /* C */
@@ -92,6 +92,13 @@ const char* tfsm_last_error(void);
 int tfsm_version(void);
 
 /* Compile an operator on a mesh into a plan: symbols, dealias limits, twiddle tables, L.
+ * Any mesh of >= 2 points per axis is accepted (the reference transforms any length,
+ * mesh.py:481-491).  The fused step kernels cover 1D powers of two 32..4096 and 48, 96, 100, 112,
+ * 192, 200, 384, 400, 768, 800 points; 2D axes of powers of two 16..512 and 48, 96, 100, 112, 192,
+ * 200; 3D axes of powers of two 16..128 and cubic 48, 96, 100 meshes.  Other plans (other sizes,
+ * ConservativeConvection in 3D, a complex L with a product term in 2D / 3D) step through the
+ * unfused path: separate field, product and stage launches with general-length FFTs (Stockham with
+ * radices 2..7 in LDS, Bluestein for other lengths, four-step beyond 4096 points).
  * Replaces OperatorLike.register_mesh + _build_linear_coefs + _build_nonlinear_funcs
  * (_base.py:339-406, 581-624) and the FourierMesh symbol caches (mesh.py:399-461). */
 int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* stream, tfsm_plan** out);
@@ -134,15 +141,25 @@ int tfsm_plan_get_tuning(const tfsm_plan* plan, int32_t* out);
  * Results are identical.  (No reference counterpart: launch configuration.) */
 int tfsm_plan_set_streams(tfsm_plan* plan, int32_t nstreams);
 
-/* Step 2D plans of square 32^2, 64^2 or 128^2 meshes with Convection or VorticityConvection with
- * the persistent per-sample kernel (on = 1, the default for those plans: one workgroup per sample
+/* Step 2D plans of square 32^2 or 64^2 meshes with Convection or VorticityConvection with the
+ * persistent per-sample kernel (on = 1, the default for those plans: one workgroup per sample
  * runs every step of a tfsm_step call, one launch per call) or with the tiled row and column
- * kernels (on = 0).  Results are identical.  TFSM_E_INVALID for other plans.
- * (No reference counterpart: launch configuration.) */
+ * kernels (on = 0).  Same arithmetic; the two kernels contract multiply-adds differently, so
+ * results agree to float32 rounding (relative 1e-6), not bit for bit.  TFSM_E_INVALID for other
+ * plans.  (No reference counterpart: launch configuration.) */
 int tfsm_plan_set_persistent(tfsm_plan* plan, int32_t on);
 
 /* 1 if tfsm_step runs the persistent per-sample 2D kernel for this plan, else 0. */
 int32_t tfsm_plan_persistent(const tfsm_plan* plan);
+
+/* Step through the unfused path (on = 1: separate field, product, transform and stage launches)
+ * or the fused kernels (on = 0, TFSM_E_INVALID where none cover the plan).  Plans the fused
+ * kernels do not cover start on the unfused path.  Switching grows or shrinks
+ * tfsm_plan_workspace_bytes.  (No reference counterpart: choice of kernels.) */
+int tfsm_plan_set_general(tfsm_plan* plan, int32_t on);
+
+/* 1 if tfsm_step runs the unfused path for this plan, else 0. */
+int32_t tfsm_plan_general(const tfsm_plan* plan);
 
 /* ETD coefficients on the device: SETDRK1-4 as float64 contour means of n_points points on a
  * circle of radius `radius`; ETDRK0-2 with the reference's complex64 closed forms.
@@ -169,6 +186,17 @@ int tfsm_plan_coefficients(tfsm_plan* plan, int32_t slot, float* out, void* stre
  * spectrum is computed here on the device.  Replaces _ExplicitSourceCore (_base.py:994-1015). */
 int tfsm_plan_add_source(tfsm_plan* plan, const float* src, int32_t src_batch, int32_t src_channels, float coef,
                          void* stream);
+
+/* External force f of NSPressureConvection (NavierStokes(Re, force), pde.py:85-100): adds
+ * coef * fftn(force) to the plan's force spectrum; force is [force_batch][force_channels][N...]
+ * float32 on the device, force_batch in {1, batch}, force_channels in {1, channels}.  The forced
+ * term is N = grad p - (conv - f) + f with p = invlap div (conv - f), and, as in the reference, it
+ * zeroes the stage spectra it evaluates outside the dealiasing box in place; such plans step on the
+ * unfused path.  TFSM_E_INVALID for plans without NSPressureConvection.
+ * Replaces _NSPressureConvectionCore with external_force (_navier_stokes.py:221-262) for forces
+ * made of explicit sources (_base.py:994-1028). */
+int tfsm_plan_add_force(tfsm_plan* p, const float* force, int32_t force_batch, int32_t force_channels, float coef,
+                        void* stream);
 
 /* Real fields -> half spectra, unnormalised: nbatch blocks of [C][N...].
  * Replaces FourierMesh.fft (mesh.py:481-485) for real input. */

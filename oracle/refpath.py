@@ -25,7 +25,7 @@ import torch
 
 Term = Tuple[object, str, Optional[torch.Tensor]]
 LINEAR_KINDS = ("lap", "biharm", "unit", "deriv")
-NONLINEAR_KINDS = ("conv", "vc", "nsp", "ks", "ks_keep_mean", "cons", "src")
+NONLINEAR_KINDS = ("conv", "vc", "nsp", "nsp_f", "ks", "ks_keep_mean", "cons", "src")
 
 
 class Mesh:
@@ -150,6 +150,21 @@ def ns_pressure_convection(u_fft, mesh: Mesh, u=None):
     return nab * p - conv
 
 
+def ns_pressure_convection_forced(u_fft, mesh: Mesh, force, dealias_rate=2 / 3):
+    """_NSPressureConvectionCore with an external force (_navier_stokes.py:238-262): the core runs
+    without the closure's dealiasing; it takes the force's spectrum from the undealiased input,
+    masks the caller's u_fft IN PLACE (the integrator's stage tensor), subtracts the force from
+    the convection before the projection and adds it again after:
+    N = grad p - (conv - f) + f,  p = invlap div (conv - f)."""
+    u_fft *= mesh.mask(dealias_rate)
+    u = mesh.ifft(u_fft).real
+    conv = convection(u_fft, mesh, u)
+    conv -= force
+    nab = mesh.nabla_vector(1)
+    p = mesh.inv_lap() * torch.sum(nab * conv, dim=1, keepdim=True)
+    return nab * p - conv + force
+
+
 def ks_convection(u_fft, mesh: Mesh, u=None, remove_mean=True):
     """_KSConvectionCore (_ks_convection.py:18-38): 1/2 sum_i (d_i phi)^2, minus its mean over the
     whole tensor (batch included) when remove_mean."""
@@ -167,7 +182,7 @@ def source_spectrum(src: torch.Tensor):
 
 def check_terms(mesh: Mesh, terms: List[Term], n_channel: int):
     for _, kind, _ in terms:
-        if kind in ("conv", "nsp") and mesh.n_dim != n_channel:  # _convection.py:57-64
+        if kind in ("conv", "nsp", "nsp_f") and mesh.n_dim != n_channel:  # _convection.py:57-64
             raise ValueError("convection operator only works for vector field with the same dimension as mesh")
         if kind == "vc" and (mesh.n_dim != 2 or n_channel != 1):  # _navier_stokes.py:56-59
             raise ValueError("Only vorticity in 2Dmesh is supported")
@@ -185,7 +200,7 @@ def build_closure(mesh: Mesh, terms: List[Term], dealias_rate=2 / 3) -> Optional
     """nonlinear_funcs_all (_base.py:359-406): dealias only the inputs of dealiased terms."""
     nl = []
     for c, k, p in terms:
-        if k == "src":
+        if k in ("src", "nsp_f"):  # nsp_f payload: the force's spatial field (its explicit sources)
             nl.append((c, k, source_spectrum(p).to(mesh.device)))
         elif k in ("conv", "vc", "nsp", "ks", "ks_keep_mean", "cons"):
             nl.append((c, k, None))
@@ -210,6 +225,10 @@ def build_closure(mesh: Mesh, terms: List[Term], dealias_rate=2 / 3) -> Optional
                 else:
                     f = {"conv": convection, "vc": vorticity_convection, "nsp": ns_pressure_convection}[k]
                     result += c * f(d_fft, mesh, d_u)
+            elif k == "nsp_f":  # dealiasing switched off: the raw stage tensor, masked in place
+                if u is None:
+                    u = mesh.ifft(u_fft).real
+                result += c * ns_pressure_convection_forced(u_fft, mesh, p, dealias_rate)
             else:  # explicit source: no dealiasing, u computed but unused (_base.py:394-401)
                 if u is None:
                     u = mesh.ifft(u_fft).real

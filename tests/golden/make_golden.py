@@ -26,6 +26,13 @@ Fixtures (reference call sites in brackets):
   g8_linear         Laplacian().integrate (ETDRK0 exact linear path), 2D 32^2, 5 steps
   g9_ns128          NavierStokesVorticity(1000, kolm_force(y)), 128^2, B=1, SETDRK4, dt 1e-3,
                     100 steps (float32 and float64) -- a longer run to pin accumulated rounding
+  g10_nsp_force     NavierStokes(100, force=ExplicitSource(f)), velocity form, 2D 32^2 (B=2) and
+                    3D 16^3 (B=1), every ETD scheme (float64: SETDRK4), 6 steps, u0 with content at
+                    every frequency;
+                    and one __call__ -- pins the forced core's in-place masking of the integrator's
+                    stage tensors and its double force   [_navier_stokes.py:221-280, pde.py:85-100]
+
+Usage for one fixture:  python tests/golden/make_golden.py g10
 """
 
 import math
@@ -44,7 +51,7 @@ def _import_reference():
     import torchfsm  # noqa: F401
     from torchfsm.mesh import MeshGrid, FourierMesh
     from torchfsm import pde, field
-    from torchfsm.operator import VorticityConvection, Convection, Laplacian
+    from torchfsm.operator import VorticityConvection, Convection, Laplacian, ExplicitSource
     from torchfsm.integrator import ETDRKIntegrator, SETDRKIntegrator
     from torchfsm.traj_recorder import AutoRecorder
 
@@ -59,6 +66,7 @@ def _import_reference():
         ETDRKIntegrator=ETDRKIntegrator,
         SETDRKIntegrator=SETDRKIntegrator,
         AutoRecorder=AutoRecorder,
+        ExplicitSource=ExplicitSource,
     )
 
 
@@ -252,10 +260,46 @@ def g9(R):
     np.savez_compressed(os.path.join(HERE, "g9_ns128.npz"), **out)
 
 
+def g10(R):
+    out = {}
+    gen = torch.Generator().manual_seed(10)
+    for dims, n, batch in ((2, 32, 2), (3, 16, 1)):
+        u0 = 0.5 * fourier_modes_field(n, dims, dims, batch, seed=100 + dims)
+        u0 = u0 + 0.05 * torch.randn(u0.shape, generator=gen, dtype=torch.float64)
+        xs = torch.meshgrid(*[torch.arange(n, dtype=torch.float64) / n] * dims, indexing="ij")
+        f = torch.zeros(1, dims, *([n] * dims), dtype=torch.float64)
+        f[0, 0] = 0.5 * torch.sin(2 * math.pi * (xs[0] + 2 * xs[1]))
+        f[0, 1] = 0.3 * torch.cos(2 * math.pi * 3 * xs[0])
+        if dims == 3:
+            f[0, 2] = 0.2 * torch.sin(2 * math.pi * (xs[2] - xs[1]))
+        tag = f"{dims}d"
+        out[f"u0_{tag}"] = u0.numpy()
+        out[f"force_{tag}"] = f.numpy()
+        for name, dt_ in (("f32", torch.float32), ("f64", torch.float64)):
+            mesh = R["MeshGrid"]([(0, 1, n)] * dims, dtype=dt_)
+            schemes = [("etdrk1", R["ETDRKIntegrator"].ETDRK1), ("etdrk2", R["ETDRKIntegrator"].ETDRK2),
+                       ("setdrk1", R["SETDRKIntegrator"].SETDRK1), ("setdrk2", R["SETDRKIntegrator"].SETDRK2),
+                       ("setdrk3", R["SETDRKIntegrator"].SETDRK3), ("setdrk4", R["SETDRKIntegrator"].SETDRK4)]
+            for sname, scheme in schemes:
+                if name == "f64" and sname != "setdrk4":  # float64 only for the default scheme (size)
+                    continue
+                op = R["pde"].NavierStokes(100, force=R["ExplicitSource"](f.to(dt_)))
+                op.set_integrator(scheme)
+                res = op.integrate(u0.to(dt_), mesh=mesh, dt=2e-3, step=6)
+                out[f"out_{tag}_{sname}_{name}"] = res.numpy()
+            op = R["pde"].NavierStokes(100, force=R["ExplicitSource"](f.to(dt_)))
+            out[f"call_{tag}_{name}"] = op(u0.to(dt_), mesh=mesh, return_in_fourier=True).numpy()
+    np.savez_compressed(os.path.join(HERE, "g10_nsp_force.npz"), **out)
+
+
 def main():
     torch.set_num_threads(min(8, os.cpu_count() or 1))
     R = _import_reference()
-    for fn in (g1, g2, g3, g4, g5, g6, g7, g8, g9):
+    fns = (g1, g2, g3, g4, g5, g6, g7, g8, g9, g10)
+    only = set(sys.argv[1:])
+    for fn in fns:
+        if only and fn.__name__ not in only:
+            continue
         fn(R)
         print("wrote", fn.__name__, flush=True)
 

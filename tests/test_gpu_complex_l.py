@@ -94,11 +94,13 @@ def test_linear_complex_l_etdrk0_call_and_solve(dev):
         assert e1 <= TOL and e2 <= 2.0 and e3 <= TOL
 
 
-def test_complex_l_with_product_term_in_2d_is_rejected(dev):
+def test_spatial_derivative_on_a_vector_field_is_rejected(dev):
+    """SpatialDerivative only takes a scalar field (_spatial_derivative.py:18-24), also beside a
+    product term; the scalar case with a product term is covered in test_gpu_general.py."""
     import torchfsm_amd as tfa
     from torchfsm_amd.operator import Convection, SpatialDerivative
 
     mesh = tfa.MeshGrid([(0, 1, 32)] * 2, device=dev)
     op = SpatialDerivative(0, 1) + Convection()
-    with pytest.raises(ValueError):
+    with pytest.raises(ValueError, match="scalar field"):
         op.integrate(torch.zeros(1, 2, 32, 32, device=dev), mesh=mesh, dt=0.01, step=1)

@@ -27,13 +27,10 @@ def test_library_exports_every_declared_symbol():
 
 @pytest.mark.parametrize("ndim,n,batch,ch,term,err", [
     (4, (16, 16, 16), 1, 1, 0, "1D, 2D and 3D"),
-    (3, (256, 16, 16), 1, 3, 1, "powers of two 16..128"),
     (3, (16, 16, 16), 1, 2, 1, "convection operator only works"),
-    (2, (90, 90, 1), 1, 1, 0, "2D mesh sizes per axis"),
-    (1, (102, 1, 1), 1, 1, 0, "1D mesh sizes"),
-    (3, (96, 96, 48), 1, 3, 1, "cubic 48, 96 or 100"),
-    (3, (100, 100, 101), 1, 1, 0, "3D mesh sizes per axis"),
-    (1, (16, 1, 1), 1, 1, 0, "powers of two 32..4096"),
+    (1, (1, 1, 1), 1, 1, 0, "at least 2 points"),
+    (2, (64, 1, 1), 1, 1, 0, "at least 2 points"),
+    (3, (96, 96, 0), 1, 3, 1, "at least 2 points"),
     (2, (64, 64, 1), 1, 1, 1, "convection operator only works"),
     (1, (64, 1, 1), 1, 1, 2, "Only vorticity in 2Dmesh"),
     (2, (64, 64, 1), 0, 1, 0, "batch and channels"),
@@ -313,3 +310,20 @@ def test_boundary_structs_agree(cname, pyname):
     assert got_doc == want
     assert ctypes.sizeof(getattr(_lib, pyname)) == ctypes.sizeof(
         type("S", (ctypes.Structure,), {"_fields_": _doc_struct(doc, pyname)}))
+
+
+def test_external_force_must_be_explicit_sources():
+    """NSPressureConvection(external_force): the engine takes forces made of explicit sources
+    (the reference evaluates force(u_fft=...) on the raw stage spectrum, _navier_stokes.py:244);
+    other forces raise ValueError at register_mesh, before any device work."""
+    import torchfsm_amd as tfa
+    from torchfsm_amd.operator import ExplicitSource, Laplacian, NSPressureConvection
+    from torchfsm_amd.operator._cores import _force_sources
+
+    mesh = tfa.FourierMesh([(0, 1, 8)] * 2)
+    f = torch.zeros(1, 2, 8, 8)
+    got = _force_sources(0.5 * ExplicitSource(f) + ExplicitSource(2 * f), mesh, 2)
+    assert [c for c, _ in got] == [0.5, 1] and all(t.shape == f.shape for _, t in got)
+    op = NSPressureConvection(Laplacian())
+    with pytest.raises(ValueError, match="explicit sources"):
+        op.register_mesh(mesh, 2)

@@ -515,3 +515,101 @@ def test_refpath_constant_forcing_exact(scheme):
     lam = -nu * (2 * math.pi * 2) ** 2
     ex = math.exp(-nu * (2 * math.pi) ** 2 * t) * u0 + t * (math.exp(lam * t) - 1) / (lam * t) * src
     assert float((o - ex).abs().max()) < 1e-12
+
+
+def _hermitian_part(x, dims):
+    """(X(k) + conj(X(-k))) / 2 over the given FFT dims."""
+    xm = torch.roll(torch.flip(x, dims=dims), shifts=[1] * len(dims), dims=dims)
+    return 0.5 * (x + xm.conj())
+
+
+@pytest.mark.parametrize("dims,n", [(2, 16), (3, 8)])
+def test_nyquist_plane_parity_rule_is_the_hermitian_part(dims, n):
+    """Terms that differentiate after a product's FFT (ConservativeConvection, the projection of
+    NSPressureConvection) are not Hermitian on Nyquist planes in the reference's full spectrum.
+    Their Hermitian part, which a real output keeps, equals the engine's rule: on axis d's Nyquist
+    plane a product of symbols survives only with an even number of Nyquist-axis factors a_d, except
+    at modes that are their own conjugate partner, which keep the full value.  Checked in float64
+    on fields with content at every frequency (no dealiasing); and the reference's real output after
+    10 steps of each scheme equals stepping with the Hermitian part of N."""
+    torch.manual_seed(3)
+    mi = [(0, 1, n)] * dims
+    ax = tuple(range(-dims, 0))
+    mesh = R.Mesh(mi, dtype=torch.float64)
+    u = torch.randn(2, dims, *([n] * dims), dtype=torch.float64)
+    uf = torch.fft.fftn(u, dim=ax)
+    a = torch.meshgrid(*[2 * math.pi * torch.fft.fftfreq(n, 1 / n, dtype=torch.float64)] * dims, indexing="ij")
+    idx = torch.meshgrid(*[torch.arange(n)] * dims, indexing="ij")
+    ny = [i == n // 2 for i in idx]
+    own = torch.ones_like(ny[0])
+    for d in range(dims):
+        own = own & ((idx[d] == 0) | ny[d])
+    ny = [f & ~own for f in ny]
+    for kind in ("cons", "nsp"):
+        raw = R.build_closure(mesh, [(1, kind, None)], dealias_rate=10.0)(uf)
+        # own-partner modes keep the full value (the real output takes their real part either way)
+        ref = torch.where(own, raw, _hermitian_part(raw, ax))
+        got = torch.zeros_like(uf)
+        if kind == "cons":
+            for c in range(dims):
+                for i in range(dims):
+                    ai = torch.where(ny[i], 0.0, a[i])
+                    got[:, c] += 1j * ai * torch.fft.fftn(u[:, i] * u[:, c], dim=ax)
+        else:
+            conv = R.convection(uf, mesh, u)
+            lap = sum(-(a[i] ** 2) for i in range(dims))
+            il = torch.where(lap == 0, 1.0, 1 / lap)
+            for c in range(dims):
+                acc = sum(torch.where(ny[c] == ny[i], -a[c] * a[i], 0.0) * conv[:, i] for i in range(dims))
+                got[:, c] = acc * il - conv[:, c]
+        e = float((got - ref).abs().max() / ref.abs().max())
+        assert e < 1e-13, (kind, e)
+    # the reference's real output is the dynamics of the Hermitian part
+    for kind in ("cons", "nsp"):
+        terms = [(0.02, "lap", None), (-1 if kind == "cons" else 1, kind, None)]
+        u0 = 0.3 * torch.randn(1, dims, *([n] * dims), dtype=torch.float64)
+        for scheme in ("SETDRK4", "ETDRK2"):
+            ref = R.integrate(terms, mi, u0, 1e-3, 10, scheme=scheme)
+            orig = R.build_closure
+            try:
+                R.build_closure = lambda m, t, r=2 / 3: (lambda f, cl=orig(m, t, r): _hermitian_part(cl(f), ax))
+                emu = R.integrate(terms, mi, u0, 1e-3, 10, scheme=scheme)
+            finally:
+                R.build_closure = orig
+            assert R.rel_l2(emu, ref) < 1e-13, (kind, scheme)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_refpath_forced_ns_velocity_vs_reference_golden(dims):
+    """NavierStokes(100, force=ExplicitSource(f)) (pde.py:85-100): the forced core masks the
+    integrator's stage tensors in place and adds the force twice (_navier_stokes.py:238-262).  The
+    oracle's restatement matches the reference's float64 run (SETDRK4) to 1e-12, every scheme's
+    float32 run to 1e-5, and one __call__ (golden g10)."""
+    g = golden("g10_nsp_force.npz")
+    tag = f"{dims}d"
+    n = g[f"u0_{tag}"].shape[-1]
+    mi = [(0, 1, n)] * dims
+    terms = [(1, "nsp_f", torch.from_numpy(g[f"force_{tag}"])), (1 / 100, "lap", None)]
+    u64 = torch.from_numpy(g[f"u0_{tag}"])
+    out = R.integrate(terms, mi, u64, 2e-3, 6, scheme="SETDRK4")
+    assert R.rel_l2(out, g[f"out_{tag}_setdrk4_f64"]) < 1e-12
+    t32 = [(1, "nsp_f", torch.from_numpy(g[f"force_{tag}"]).float()), (1 / 100, "lap", None)]
+    for scheme in ("ETDRK1", "ETDRK2", "SETDRK1", "SETDRK2", "SETDRK3", "SETDRK4"):
+        out = R.integrate(t32, mi, u64.float(), 2e-3, 6, scheme=scheme)
+        assert R.rel_l2(out, g[f"out_{tag}_{scheme.lower()}_f32"]) < 1e-5, scheme
+    call = R.apply(terms, mi, u64, return_in_fourier=True)
+    assert R.rel_l2(call, g[f"call_{tag}_f64"]) < 1e-12
+
+
+def test_forced_ns_core_doubles_a_divergence_free_force():
+    """At u = 0 the forced core returns grad(invlap div(-f)) + 2 f, which is 2 f for a
+    divergence-free f: f = (sin 2 pi y, sin 2 pi x) on [0,1)^2."""
+    n = 16
+    mesh = R.Mesh([(0, 1, n)] * 2, dtype=torch.float64)
+    x = torch.arange(n, dtype=torch.float64) / n
+    X, Y = torch.meshgrid(x, x, indexing="ij")
+    f = torch.stack([torch.sin(2 * math.pi * Y), torch.sin(2 * math.pi * X)])[None]
+    ff = torch.fft.fftn(f, dim=(-2, -1))
+    u = torch.zeros(1, 2, n, n, dtype=torch.complex128)
+    got = R.ns_pressure_convection_forced(u, mesh, ff)
+    assert float((got - 2 * ff).abs().max()) < 1e-12

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libtfsm.so")
 SOURCES = ["api.hip", "k_coefs.hip", "k_transform.hip", "k_step2d.hip", "k_step2d_vc.hip", "k_step2d_conv.hip",
            "k_step2d_nsp.hip", "k_step2d_ks.hip", "k_step2d_cons.hip",
            "k_step2d_mixed.hip", "k_step3d.hip", "k_adjoint.hip",
-           "k_maps.hip"]
+           "k_maps.hip", "k_gfft.hip", "k_general.hip"]
 ARCH = "gfx950"
 # packed FP32 instructions are disabled for all device code
 CFLAGS = [

@@ -66,11 +66,14 @@ SIGNATURES = {
     "tfsm_plan_set_streams": (ctypes.c_int, [VP, I32]),
     "tfsm_plan_set_persistent": (ctypes.c_int, [VP, I32]),
     "tfsm_plan_persistent": (I32, [VP]),
+    "tfsm_plan_set_general": (ctypes.c_int, [VP, I32]),
+    "tfsm_plan_general": (I32, [VP]),
     "tfsm_plan_get_tuning": (ctypes.c_int, [VP, VP]),
     "tfsm_plan_set_scheme": (ctypes.c_int, [VP, I32, ctypes.c_double, I32, ctypes.c_double, VP]),
     "tfsm_plan_coefficients": (ctypes.c_int, [VP, I32, VP, VP]),
     "tfsm_plan_set_sample": (ctypes.c_int, [VP, I32, VP, ctypes.c_double, VP]),
     "tfsm_plan_add_source": (ctypes.c_int, [VP, VP, I32, I32, ctypes.c_float, VP]),
+    "tfsm_plan_add_force": (ctypes.c_int, [VP, VP, I32, I32, ctypes.c_float, VP]),
     "tfsm_forward": (ctypes.c_int, [VP, VP, VP, I32, VP]),
     "tfsm_inverse": (ctypes.c_int, [VP, VP, VP, I32, VP, VP]),
     "tfsm_c2c": (ctypes.c_int, [VP, VP, VP, I32, I32, VP]),

@@ -81,6 +81,8 @@ def time_steps_sustained(plan, uh, steps=200) -> float:
 
 def autotune(plan, uh, steps=6, log=None, cache_file: Optional[str] = None, final_steps: int = 200):
     """Pick and apply the fastest launch shape for this plan; returns (tuning, {tuning: ms})."""
+    if plan.general:  # the unfused path has no launch shape to choose
+        return plan.tuning, {}
     key = shape_key(plan)
     if cache_file and os.path.exists(cache_file) and key not in _CACHE:
         try:

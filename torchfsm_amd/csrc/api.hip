@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "gfft.hpp"
 #include "tfsm.h"
 
 using namespace tfsm;
@@ -142,6 +143,15 @@ struct tfsm_plan {
   float* sym = nullptr;    // [modes]
   int adj_term = -2;       // linear term whose coefficient gets a gradient (-1 none; -2 not prepared)
   int adj_scheme = -1;
+  // unfused path (k_general.hip): stepping without the fused kernels
+  bool general = false;    // the plan steps through the unfused path
+  bool fused_ok = false;   // the fused kernels cover this plan (the unfused path is then optional)
+  GAxis gax[3];            // general-length FFT tables of the axes without fixed-size kernels
+  float2* gtmp = nullptr;  // four-step scratch of those axes
+  float2* force = nullptr;  // NSPressureConvection's external force spectrum [forceB][forceC][modes]
+  int forceB = 0, forceC = 0;
+  size_t ws_fused = 0;     // workspace of the fused path (the unfused buffers follow it)
+  size_t off_gS = 0, off_gF = 0, off_gR = 0, off_gQ = 0;
 
   size_t state_elems() const { return (size_t)B * C * modes; }
   size_t src_bstride() const { return srcB > 1 ? (size_t)srcC * modes : 0; }
@@ -243,6 +253,224 @@ int build_twiddles(tfsm_plan* p, int d) {
     HIP_TRY(hipMemcpy(p->tw[d][var], h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice), "twiddle upload");
   }
   return TFSM_OK;
+}
+
+// ---------------------------------------------------------------- transforms per axis
+// Fixed-size kernels where the axis has them (rows: every length of the twiddle list; columns: up
+// to 512 points), the general-length FFTs (gfft_run) elsewhere.
+bool fast_rows(const tfsm_plan* p, int d) { return p->tw[d][0] != nullptr; }
+bool fast_cols(const tfsm_plan* p, int d) { return p->tw[d][0] != nullptr && p->n[d] <= 512; }
+bool axis_needs_gfft(const tfsm_plan* p, int d) { return d == p->D - 1 ? !fast_rows(p, d) : !fast_cols(p, d); }
+
+// nrows rows of the last axis: GM_R2C (real in, ld in floats), GM_C2R, GM_C2C
+hipError_t xf_rows(tfsm_plan* p, int mode, int dir, const void* in, size_t in_ld, void* out, size_t out_ld,
+                   size_t nrows, float scale, hipStream_t st) {
+  const int d = p->D - 1, n = p->n[d];
+  if (nrows == 0) return hipSuccess;
+  if (fast_rows(p, d) && nrows <= 0x7fffffffu) {
+    if (mode == GM_R2C)
+      return launch_rows_r2c(n, (const float*)in, in_ld, (float2*)out, out_ld, (int)nrows, scale, p->tw[d][0], st);
+    if (mode == GM_C2R)
+      return launch_rows_c2r(n, (const float2*)in, in_ld, (float*)out, out_ld, (int)nrows, scale, p->tw[d][0], st);
+    return launch_rows_c2c(n, dir, (const float2*)in, in_ld, (float2*)out, out_ld, (int)nrows, scale, p->tw[d][0], st);
+  }
+  if (p->gax[d].n != n) return hipErrorInvalidValue;
+  GLines li, lo;
+  li.n_o = lo.n_o = nrows;
+  li.os = in_ld;
+  lo.os = out_ld;
+  return gfft_run(p->gax[d], mode, dir, in, li, out, lo, scale, p->gtmp, st);
+}
+
+// complex transform along axis d < D - 1: nouter blocks of [n_d][...] with elements k * col_stride
+// + i, i < ninner, of block o at o * outer_stride
+hipError_t xf_cols(tfsm_plan* p, int d, int dir, const float2* in, float2* out, size_t nouter, size_t outer_stride,
+                   size_t col_stride, size_t ninner, hipStream_t st) {
+  if (nouter == 0 || ninner == 0) return hipSuccess;
+  if (fast_cols(p, d) && ninner <= 0x7fffffffu)
+    return launch_cols_c2c(p->n[d], dir, in, out, nouter, outer_stride, col_stride, (int)ninner, 1.0f, p->tw[d][0], st);
+  if (p->gax[d].n != p->n[d]) return hipErrorInvalidValue;
+  GLines l;
+  l.n_o = nouter;
+  l.n_i = ninner;
+  l.os = outer_stride;
+  l.is = 1;
+  l.es = col_stride;
+  return gfft_run(p->gax[d], GM_C2C, dir, in, l, out, l, 1.0f, p->gtmp, st);
+}
+
+// nblk real blocks [N0..N_{D-1}] -> half spectra [N0..][P]
+int xf_forward_blocks(tfsm_plan* p, const float* u, float2* out, size_t nblk, hipStream_t st) {
+  const int D = p->D, nl = p->n[D - 1];
+  size_t rows = nblk;
+  for (int d = 0; d < D - 1; ++d) rows *= p->n[d];
+  HIP_TRY(xf_rows(p, GM_R2C, -1, u, (size_t)nl, out, (size_t)p->P, rows, 1.0f, st), "r2c launch");
+  if (D == 2)
+    HIP_TRY(xf_cols(p, 0, -1, out, out, nblk, p->modes, (size_t)p->P, (size_t)p->H, st), "column fft launch");
+  if (D == 3) {
+    const size_t plane = (size_t)p->n[1] * p->P;
+    HIP_TRY(xf_cols(p, 1, -1, out, out, nblk * p->n[0], plane, (size_t)p->P, (size_t)p->H, st), "y fft launch");
+    HIP_TRY(xf_cols(p, 0, -1, out, out, nblk, p->modes, plane, plane, st), "x fft launch");
+  }
+  return TFSM_OK;
+}
+
+// nblk half spectra -> real blocks, scaled by `scale`; `work` receives the partial transforms
+// (it may be `in`, which is then overwritten; unused in 1D)
+int xf_inverse_blocks(tfsm_plan* p, const float2* in, float2* work, float* u, size_t nblk, float scale,
+                      hipStream_t st) {
+  const int D = p->D, nl = p->n[D - 1];
+  size_t rows = nblk;
+  for (int d = 0; d < D - 1; ++d) rows *= p->n[d];
+  if (D == 2) {
+    HIP_TRY(xf_cols(p, 0, +1, in, work, nblk, p->modes, (size_t)p->P, (size_t)p->H, st), "column ifft launch");
+    in = work;
+  }
+  if (D == 3) {
+    const size_t plane = (size_t)p->n[1] * p->P;
+    HIP_TRY(xf_cols(p, 0, +1, in, work, nblk, p->modes, plane, plane, st), "x ifft launch");
+    HIP_TRY(xf_cols(p, 1, +1, work, work, nblk * p->n[0], plane, (size_t)p->P, (size_t)p->H, st), "y ifft launch");
+    in = work;
+  }
+  HIP_TRY(xf_rows(p, GM_C2R, +1, in, (size_t)p->P, u, (size_t)nl, rows, scale, st), "c2r launch");
+  return TFSM_OK;
+}
+
+// ---------------------------------------------------------------- unfused stepping
+size_t plan_npts(const tfsm_plan* p) {
+  size_t n = 1;
+  for (int d = 0; d < p->D; ++d) n *= p->n[d];
+  return n;
+}
+
+GenArgs gen_args(const tfsm_plan* p) {
+  GenArgs g;
+  g.term = p->term;
+  g.D = p->D;
+  g.B = p->B;
+  g.C = p->C;
+  for (int d = 0; d < 3; ++d) {
+    g.n[d] = p->n[d];
+    g.kmax[d] = p->kmax[d];
+    g.kv[d] = p->kv[d];
+  }
+  g.H = p->H;
+  g.P = p->P;
+  g.modes = p->modes;
+  g.npts = plan_npts(p);
+  g.kv_bstride = p->kv_bstride;
+  g.NF = gen_nf(p->term, p->C, p->D);
+  g.NP = gen_np(p->term, p->C);
+  g.coef_t = p->coef_t;
+  return g;
+}
+
+// unfused buffers after the fused workspace: S (stage spectrum), F (fields, later product spectra),
+// R (real fields), Q (real products)
+void gen_layout(tfsm_plan* p) {
+  const GenArgs g = gen_args(p);
+  const size_t sb = p->state_elems() * sizeof(float2);
+  const size_t nfa = (size_t)(g.NF > g.NP ? g.NF : g.NP);
+  p->off_gS = roundup(p->ws_fused, 256);
+  p->off_gF = p->off_gS + roundup(sb, 256);
+  p->off_gR = p->off_gF + roundup((size_t)p->B * nfa * p->modes * sizeof(float2), 256);
+  p->off_gQ = p->off_gR + roundup((size_t)p->B * g.NF * g.npts * sizeof(float), 256);
+  p->ws_bytes = p->off_gQ + roundup((size_t)p->B * g.NP * g.npts * sizeof(float), 256);
+}
+
+// general-length tables of the axes without fixed-size kernels, and their four-step scratch
+int build_gaxes(tfsm_plan* p) {
+  size_t tmp = 0;
+  // blocks per transform: state channels, fields and products of both paths, adjoint fields
+  int nb = p->C;
+  const int counts[4] = {gen_nf(p->term, p->C, p->D), gen_np(p->term, p->C), 2 * p->C + p->C * p->D, 5};
+  for (int c : counts) nb = c > nb ? c : nb;
+  const size_t blk = (size_t)p->B * nb;
+  for (int d = 0; d < p->D; ++d) {
+    if (!axis_needs_gfft(p, d)) continue;
+    hipError_t e = gaxis_build(p->gax[d], p->n[d], p->allocs);
+    if (e == hipErrorInvalidValue) return fail(TFSM_E_INVALID, "mesh axis too long for the general-length FFT");
+    HIP_TRY(e, "general FFT tables");
+    // lines of one transform along axis d (full complex rows of tfsm_c2c included)
+    size_t lines = blk;
+    for (int e2 = 0; e2 < p->D - 1; ++e2)
+      if (e2 != d) lines *= p->n[e2];
+    if (d != p->D - 1) lines *= (size_t)p->n[p->D - 1] > (size_t)p->P ? (size_t)p->n[p->D - 1] : (size_t)p->P;
+    const size_t t = gfft_tmp_elems(p->gax[d], lines);
+    if (t > tmp) tmp = t;
+  }
+  if (tmp) return dev_alloc(p, &p->gtmp, tmp, "general FFT scratch");
+  return TFSM_OK;
+}
+
+// N(S) -> product spectra in F, and the batch mean of their DC for KSConvection; mask_in_place:
+// S is zeroed outside the dealiasing box (the forced NSPressureConvection core does that)
+int gen_eval(tfsm_plan* p, float2* S, char* ws, hipStream_t st, bool mask_in_place = false) {
+  GenArgs g = gen_args(p);
+  g.mask_in_place = mask_in_place ? 1 : 0;
+  float2* F = (float2*)(ws + p->off_gF);
+  float* R = (float*)(ws + p->off_gR);
+  float* Q = (float*)(ws + p->off_gQ);
+  HIP_TRY(launch_gen_fields(g, S, F, st), "field launch");
+  int rc = xf_inverse_blocks(p, F, F, R, (size_t)p->B * g.NF, (float)(1.0 / (double)g.npts), st);
+  if (rc) return rc;
+  HIP_TRY(launch_gen_products(g, R, Q, st), "product launch");
+  rc = xf_forward_blocks(p, Q, F, (size_t)p->B * g.NP, st);
+  if (rc) return rc;
+  if (p->ks_mean)
+    HIP_TRY(launch_ks_dc(F, p->B, (size_t)g.NP * p->modes, 1, 0, (float2*)(ws + p->off_DC), st), "batch-mean launch");
+  return TFSM_OK;
+}
+
+// evaluation from `in` and the stage combine `stage` (S receives the next stage spectrum).
+// stepping: a forced NSPressureConvection masks the stage tensor it evaluates in place, as the
+// reference's core does, except the state of a one-evaluation scheme, whose E u the reference
+// forms before the evaluation (_etdrk.py:47-51, _setdrk_step.py:5-11)
+int gen_stage(tfsm_plan* p, float2* in, float2* U, char* ws, int stage, StepMask m, hipStream_t st,
+              bool stepping = false) {
+  const bool forced = p->force != nullptr;
+  int rc = gen_eval(p, in, ws, st, forced && stepping && stage != ST_R1_E0);
+  if (rc) return rc;
+  GenStageArgs s;
+  s.stage = stage;
+  s.Qs = (const float2*)(ws + p->off_gF);
+  s.dc = p->ks_mean ? (const float2*)(ws + p->off_DC) : nullptr;
+  s.U = U;
+  s.A1 = (float2*)(ws + p->off_A1);
+  s.A2 = (float2*)(ws + p->off_A2);
+  s.S = (float2*)(ws + p->off_gS);
+  s.coef = p->coef;
+  s.coefc = p->cplx ? p->coefc : nullptr;
+  s.coef_bstride = p->coef_bstride();
+  s.coef_per_block = p->per_len ? 1 : 0;
+  s.src = p->src;
+  s.src_bstride = p->src_bstride();
+  s.src_cstride = p->src_cstride();
+  s.force = p->force;
+  s.force_bstride = p->forceB > 1 ? (size_t)p->forceC * p->modes : 0;
+  s.force_cstride = p->forceC > 1 ? p->modes : 0;
+  s.masked_stage = forced && stepping ? 1 : 0;
+  s.mask = m;
+  HIP_TRY(launch_gen_stage(gen_args(p), s, st), "stage launch");
+  return TFSM_OK;
+}
+
+int gen_step(tfsm_plan* p, float2* U, int nsteps, const int* rec_slot, void* traj, int T, char* ws, hipStream_t st,
+             const int* act) {
+  const int nev = scheme_evals(p->scheme);
+  const size_t per = (size_t)p->C * p->modes;
+  float2* S = (float2*)(ws + p->off_gS);
+  StepMask m;
+  m.act = act;
+  int rc = TFSM_OK;
+  for (int i = 0; i < nsteps && !rc; ++i) {
+    m.it = i;
+    if (rec_slot) rc = launch_status(launch_record(U, (float2*)traj, rec_slot, i, p->B, per, T, st), "record launch");
+    for (int e = 0; e < nev && !rc; ++e)
+      rc = gen_stage(p, e == 0 ? U : S, U, ws, scheme_stage(p->scheme, e), m, st, true);
+  }
+  if (rec_slot && !rc) rc = launch_status(launch_record(U, (float2*)traj, rec_slot, nsteps, p->B, per, T, st), "record launch");
+  return rc;
 }
 
 int check_stream_errors(const char* what) {
@@ -484,6 +712,7 @@ int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* tra
     return TFSM_OK;
   }
   if (!ws) return fail(TFSM_E_INVALID, "workspace pointer is null");
+  if (p->general) return gen_step(p, U, nsteps, rec_slot, traj, T, (char*)ws, st, act);
   if (p->D == 1 && p->term == TERM_KS) {
     // one launch per stage (the batch mean couples the samples): raw product -> mean -> combine
     Run1Args a = run1_args(p, U, (char*)ws);
@@ -647,47 +876,19 @@ int adj_jt(tfsm_plan* p, const float2* U, const float2* V, float2* W, char* wsa,
   float* X = (float*)(wsa + l.X);
   float* Q = (float*)(wsa + l.Q);
   HIP_TRY(launch_adj_fields(a, U, V, F, st), "adjoint fields launch");
-  const int nl = p->n[p->D - 1];
   const float scale = (float)(1.0 / (double)a.npts);
-  if (p->D == 1) {
-    HIP_TRY(launch_rows_c2r(nl, F, (size_t)p->P, X, (size_t)nl, p->B * nfa, scale, p->tw[0][0], st), "adjoint c2r");
-    HIP_TRY(launch_adj_products(a, X, Q, st), "adjoint products launch");
-    HIP_TRY(launch_rows_r2c(nl, Q, (size_t)nl, Fs, (size_t)p->P, p->B * nq, 1.0f, p->tw[0][0], st), "adjoint r2c");
-  } else if (p->D == 3) {
-    // x columns, y columns, z rows (as tfsm_inverse / tfsm_forward do)
-    const size_t plane = (size_t)p->n[1] * p->P;
-    HIP_TRY(launch_cols_c2c(p->n[0], +1, F, Fs, (size_t)p->B * nfa, p->modes, plane, (int)plane, 1.0f, p->tw[0][0], st),
-            "adjoint x ifft");
-    HIP_TRY(launch_cols_c2c(p->n[1], +1, Fs, Fs, (size_t)p->B * nfa * p->n[0], plane, (size_t)p->P, p->H, 1.0f,
-                            p->tw[1][0], st),
-            "adjoint y ifft");
-    HIP_TRY(launch_rows_c2r(nl, Fs, (size_t)p->P, X, (size_t)nl, p->B * nfa * p->n[0] * p->n[1], scale, p->tw[2][0], st),
-            "adjoint c2r");
-    HIP_TRY(launch_adj_products(a, X, Q, st), "adjoint products launch");
-    HIP_TRY(launch_rows_r2c(nl, Q, (size_t)nl, Fs, (size_t)p->P, p->B * nq * p->n[0] * p->n[1], 1.0f, p->tw[2][0], st),
-            "adjoint r2c");
-    HIP_TRY(launch_cols_c2c(p->n[1], -1, Fs, Fs, (size_t)p->B * nq * p->n[0], plane, (size_t)p->P, p->H, 1.0f,
-                            p->tw[1][0], st),
-            "adjoint y fft");
-    HIP_TRY(launch_cols_c2c(p->n[0], -1, Fs, Fs, (size_t)p->B * nq, p->modes, plane, (int)plane, 1.0f, p->tw[0][0], st),
-            "adjoint x fft");
-  } else {
-    HIP_TRY(launch_cols_c2c(p->n[0], +1, F, Fs, (size_t)p->B * nfa, p->modes, (size_t)p->P, p->H, 1.0f, p->tw[0][0], st),
-            "adjoint column ifft");
-    HIP_TRY(launch_rows_c2r(nl, Fs, (size_t)p->P, X, (size_t)nl, p->B * nfa * p->n[0], scale, p->tw[1][0], st),
-            "adjoint c2r");
-    HIP_TRY(launch_adj_products(a, X, Q, st), "adjoint products launch");
-    HIP_TRY(launch_rows_r2c(nl, Q, (size_t)nl, Fs, (size_t)p->P, p->B * nq * p->n[0], 1.0f, p->tw[1][0], st),
-            "adjoint r2c");
-    HIP_TRY(launch_cols_c2c(p->n[0], -1, Fs, Fs, (size_t)p->B * nq, p->modes, (size_t)p->P, p->H, 1.0f, p->tw[0][0], st),
-            "adjoint column fft");
-  }
+  int rc = xf_inverse_blocks(p, F, Fs, X, (size_t)p->B * nfa, scale, st);
+  if (rc) return rc;
+  HIP_TRY(launch_adj_products(a, X, Q, st), "adjoint products launch");
+  rc = xf_forward_blocks(p, Q, Fs, (size_t)p->B * nq, st);
+  if (rc) return rc;
   HIP_TRY(launch_adj_combine(a, Fs, W, st), "adjoint combine launch");
   return TFSM_OK;
 }
 
 // N(U) -> workspace A1
 int eval_nonlinear(tfsm_plan* p, float2* U, char* ws, hipStream_t st) {
+  if (p->general) return gen_stage(p, U, U, ws, ST_EVAL, StepMask(), st);
   if (p->D == 1) {
     Run1Args r = run1_args(p, U, ws);
     r.eval_only = 1;
@@ -909,17 +1110,16 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   const int D = geom->ndim;
   if (D < 1 || D > 3) return fail(TFSM_E_INVALID, "the engine supports 1D, 2D and 3D meshes");
   if (geom->batch < 1 || geom->channels < 1) return fail(TFSM_E_INVALID, "batch and channels must be >= 1");
+  // sizes the fused kernels cover; any other size (>= 2 points per axis) runs the unfused path
+  bool fused_sizes = true;
   for (int d = 0; d < D; ++d) {
     if (!(geom->length[d] > 0)) return fail(TFSM_E_INVALID, "mesh lengths must be positive");
     const int n = geom->n[d];
+    if (n < 2) return fail(TFSM_E_INVALID, "mesh axes need at least 2 points");
     const bool ok = D == 1   ? ((is_pow2(n) && n >= 32 && n <= 4096) || is_mixed_size(n))
                     : D == 2 ? ((is_pow2(n) && n >= 16 && n <= 512) || is_mixed_size_2d(n))
                              : ((is_pow2(n) && n >= 16 && n <= 128) || is_mixed_size_3d(n));
-    if (!ok)
-      return fail(TFSM_E_INVALID, D == 1 ? "1D mesh sizes: powers of two 32..4096, or 48, 96, 100, 112, 192, 200, 384, "
-                                           "400, 768, 800"
-                                  : D == 2 ? "2D mesh sizes per axis: powers of two 16..512, or 48, 96, 100, 112, 192, 200"
-                                           : "3D mesh sizes per axis: powers of two 16..128, or 48, 96, 100");
+    fused_sizes = fused_sizes && ok;
   }
   const bool ks_keep = terms->nonlinear == TFSM_TERM_KS_KEEP_MEAN;
   const int term = ks_keep ? TERM_KS : terms->nonlinear;
@@ -928,19 +1128,13 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     return fail(TFSM_E_INVALID, "unknown nonlinear term");
   if (term == TERM_CONS && geom->channels != D)
     return fail(TFSM_E_INVALID, "convection operator only works for vector field with the same dimension as mesh");
-  if (term == TERM_CONS && D == 3)
-    return fail(TFSM_E_INVALID, "term not supported by engine: stepping ConservativeConvection in 3D");
   if (term == TERM_KS && geom->channels != 1) return fail(TFSM_E_INVALID, "KSConvection only supports scalar field");
-  if (term == TERM_KS && D == 3 && !step3d_supported(geom->n[0], geom->n[1], geom->n[2]))
-    return fail(TFSM_E_INVALID, "3D KSConvection needs 16, 32, 64 or 128 points per axis, or a cubic 48, 96 or 100 mesh");
   if ((term == TERM_CONV || term == TERM_NSP) && geom->channels != D)
     return fail(TFSM_E_INVALID, "convection operator only works for vector field with the same dimension as mesh");
   if (term == TERM_NSP && D == 1)
     return fail(TFSM_E_INVALID, "term not supported by engine: NSPressureConvection on a 1D mesh");
   if (term == TERM_VC && (D != 2 || geom->channels != 1))
     return fail(TFSM_E_INVALID, "Only vorticity in 2Dmesh is supported");
-  if ((term == TERM_CONV || term == TERM_NSP) && D == 3 && !step3d_supported(geom->n[0], geom->n[1], geom->n[2]))
-    return fail(TFSM_E_INVALID, "3D convection needs 16, 32, 64 or 128 points per axis, or a cubic 48, 96 or 100 mesh");
   if (terms->n_linear < 0 || terms->n_linear > 4) return fail(TFSM_E_INVALID, "at most 4 linear terms");
   bool cplx = false;
   for (int t = 0; t < terms->n_linear; ++t) {
@@ -954,9 +1148,11 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     if (geom->channels != 1) return fail(TFSM_E_INVALID, "The SpatialDerivative operator only supports scalar field.");
     if (terms->linear_order[t] & 1) cplx = true;
   }
-  if (cplx && term != TERM_NONE && D > 1)
-    return fail(TFSM_E_INVALID,
-                "term not supported by engine: an odd-order SpatialDerivative (complex L) with a product term in 2D/3D");
+  // the fused kernels cover every product term on their sizes, except ConservativeConvection in 3D
+  // and a complex L with a product term in 2D / 3D; the unfused path covers the rest
+  bool fused = fused_sizes && !(term == TERM_CONS && D == 3) && !(cplx && term != TERM_NONE && D > 1);
+  if (fused && D == 3 && term != TERM_NONE && term != TERM_CONS)
+    fused = step3d_supported(geom->n[0], geom->n[1], geom->n[2]);
   if (terms->n_linear > 0 && !terms->linear_coef) return fail(TFSM_E_INVALID, "linear coefficients missing");
   const int Bc = terms->coef_batch < 1 ? 1 : terms->coef_batch;
   if (Bc != 1 && Bc != geom->batch) return fail(TFSM_E_INVALID, "coef_batch must be 1 or the batch size");
@@ -992,6 +1188,8 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     }
   }
   p->cplx = cplx;
+  p->fused_ok = fused;
+  p->general = !fused && term != TERM_NONE;
   p->per_len = D == 1 && geom->length_per_sample;
   p->Bc = Bc;
   int rc = TFSM_OK;
@@ -1004,6 +1202,7 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     rc = build_twiddles(p, d);
     if (rc) break;
   }
+  if (!rc) rc = build_gaxes(p);
   // axis wavenumber tables
   if (!rc) {
     for (int d = 0; d < D && !rc; ++d) {
@@ -1088,7 +1287,9 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     const size_t sb = p->state_elems() * sizeof(float2);
     p->off_A1 = 0;
     p->off_A2 = roundup(sb, 256);
-    if (D == 3) {
+    if (!p->fused_ok) {
+      p->ws_bytes = p->off_A2 + roundup(sb, 256);
+    } else if (D == 3) {
       // products P [B][3][N0][N1][P2]; fields G1 [B][6][N0][K1][PF2]; G2 [B][9][N0][N1][PF2]
       p->PF = (int)roundup((size_t)p->kmax[2] + 1, 16);
       p->K1 = 2 * p->kmax[1] + 1;
@@ -1113,13 +1314,15 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     }
     p->off_DC = p->ws_bytes;
     p->ws_bytes += 256;
+    p->ws_fused = p->ws_bytes;
+    if (p->general) gen_layout(p);
   }
   // default launch shape (the autotuner may refine it): 2D columns of 512 points run the
   // half-thread radix plan for one-channel products (the two-channel ones keep a second column in
   // registers and run faster with 8 values per thread); temporal blocking uses the smallest
   // sub-batch (multiple of 8) that still gives >= 512 column workgroups (two per CU) while its
   // working set stays under 256 MB
-  if (!rc && D == 2) {
+  if (!rc && D == 2 && p->fused_ok) {
     const bool one_channel = term == TERM_VC || term == TERM_KS;
     const int n0 = p->n[0], n1 = p->n[1];
     // measured per axis size (SETDRK4, B = 16..32): 512 keeps 16-column tiles; 256 runs 8-column
@@ -1150,7 +1353,8 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     }
   }
   // small square meshes step with the persistent per-sample kernel (one launch per call)
-  if (!rc && D == 2 && run2d_supported(term, p->n[0], p->n[1], p->PF) && !p->cplx) p->persistent = true;
+  if (!rc && D == 2 && p->fused_ok && run2d_supported(term, p->n[0], p->n[1], p->PF) && !p->cplx)
+    p->persistent = true;
   if (!rc && D == 3) p->row_var = (p->n[2] >= 64 && is_pow2(p->n[2])) ? 1 : 0;  // radix-4 z rows (fewer registers)
   if (rc) {
     tfsm_plan_destroy(p);
@@ -1188,6 +1392,7 @@ int64_t tfsm_plan_inverse_scratch_bytes(const tfsm_plan* plan, int32_t nbatch) {
 int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int32_t row_var, int32_t row_wg,
                          int32_t sub_batch) {
   if (!plan) return fail(TFSM_E_INVALID, "null plan");
+  if (!plan->fused_ok) return fail(TFSM_E_INVALID, "no fused kernels for this plan: it has no launch shape");
   if (plan->D == 3 && (row_var < 0 || row_var > 1 || col_var < 0 || col_var > 1))
     return fail(TFSM_E_INVALID, "unsupported 3D tuning");
   if (plan->D == 3 && (col_var == 1 || row_var == 1) &&
@@ -1221,6 +1426,7 @@ int tfsm_plan_get_tuning(const tfsm_plan* plan, int32_t* out) {
 
 int tfsm_plan_set_persistent(tfsm_plan* plan, int32_t on) {
   if (!plan) return fail(TFSM_E_INVALID, "null plan");
+  if (on && plan->general) return fail(TFSM_E_INVALID, "the plan steps through the unfused path");
   if (on && !(plan->D == 2 && run2d_supported(plan->term, plan->n[0], plan->n[1], plan->PF) && !plan->cplx))
     return fail(TFSM_E_INVALID, "the persistent 2D stepper covers square 32^2 and 64^2 meshes with Convection or "
                                 "VorticityConvection (2/3 dealiasing or coarser)");
@@ -1229,6 +1435,23 @@ int tfsm_plan_set_persistent(tfsm_plan* plan, int32_t on) {
 }
 
 int32_t tfsm_plan_persistent(const tfsm_plan* plan) { return plan && plan->persistent ? 1 : 0; }
+
+int tfsm_plan_set_general(tfsm_plan* plan, int32_t on) {
+  if (!plan) return fail(TFSM_E_INVALID, "null plan");
+  if (on) {
+    if (plan->term == TERM_NONE) return fail(TFSM_E_INVALID, "a plan without a product term has no nonlinear evaluation");
+    plan->general = true;
+    plan->persistent = false;
+    gen_layout(plan);
+    return TFSM_OK;
+  }
+  if (!plan->fused_ok) return fail(TFSM_E_INVALID, "no fused kernels for this plan: it steps through the unfused path");
+  plan->general = false;
+  plan->ws_bytes = plan->ws_fused;
+  return TFSM_OK;
+}
+
+int32_t tfsm_plan_general(const tfsm_plan* plan) { return plan && plan->general ? 1 : 0; }
 
 int tfsm_plan_set_streams(tfsm_plan* plan, int32_t nstreams) {
   if (!plan) return fail(TFSM_E_INVALID, "null plan");
@@ -1354,57 +1577,16 @@ int tfsm_plan_coefficients(tfsm_plan* p, int32_t slot, float* out, void* stream)
 int tfsm_forward(tfsm_plan* p, const float* u, void* uh, int32_t nbatch, void* stream) {
   if (!p || !u || !uh) return fail(TFSM_E_INVALID, "null argument");
   if (int rc = check_nbatch(p, nbatch)) return rc;
-  hipStream_t st = (hipStream_t)stream;
-  float2* out = (float2*)uh;
-  const size_t blocks = (size_t)nbatch * p->C;
-  const int nl = p->n[p->D - 1];
-  size_t rows = blocks;
-  for (int d = 0; d < p->D - 1; ++d) rows *= p->n[d];
-  HIP_TRY(launch_rows_r2c(nl, u, (size_t)nl, out, (size_t)p->P, (int)rows, 1.0f, p->tw[p->D - 1][0], st), "r2c launch");
-  if (p->D == 2)
-    HIP_TRY(launch_cols_c2c(p->n[0], -1, out, out, blocks, p->modes, (size_t)p->P, p->H, 1.0f, p->tw[0][0], st),
-            "column fft launch");
-  if (p->D == 3) {
-    const size_t plane = (size_t)p->n[1] * p->P;
-    HIP_TRY(launch_cols_c2c(p->n[1], -1, out, out, blocks * p->n[0], plane, (size_t)p->P, p->H, 1.0f, p->tw[1][0], st),
-            "y fft launch");
-    HIP_TRY(launch_cols_c2c(p->n[0], -1, out, out, blocks, p->modes, plane, (int)plane, 1.0f, p->tw[0][0], st),
-            "x fft launch");
-  }
-  return TFSM_OK;
+  return xf_forward_blocks(p, u, (float2*)uh, (size_t)nbatch * p->C, (hipStream_t)stream);
 }
 
 int tfsm_inverse(tfsm_plan* p, const void* uh, float* u, int32_t nbatch, void* scratch, void* stream) {
   if (!p || !u || !uh) return fail(TFSM_E_INVALID, "null argument");
   if (int rc = check_nbatch(p, nbatch)) return rc;
-  hipStream_t st = (hipStream_t)stream;
-  const size_t blocks = (size_t)nbatch * p->C;
-  const int nl = p->n[p->D - 1];
-  double total = 1.0;
-  for (int d = 0; d < p->D; ++d) total *= p->n[d];
-  const float scale = (float)(1.0 / total);
-  const float2* in = (const float2*)uh;
-  size_t rows = blocks;
-  for (int d = 0; d < p->D - 1; ++d) rows *= p->n[d];
-  if (p->D == 2) {
-    if (!scratch) return fail(TFSM_E_INVALID, "inverse transform needs scratch");
-    float2* sc = (float2*)scratch;
-    HIP_TRY(launch_cols_c2c(p->n[0], +1, in, sc, blocks, p->modes, (size_t)p->P, p->H, 1.0f, p->tw[0][0], st),
-            "column ifft launch");
-    in = sc;
-  }
-  if (p->D == 3) {
-    if (!scratch) return fail(TFSM_E_INVALID, "inverse transform needs scratch");
-    float2* sc = (float2*)scratch;
-    const size_t plane = (size_t)p->n[1] * p->P;
-    HIP_TRY(launch_cols_c2c(p->n[0], +1, in, sc, blocks, p->modes, plane, (int)plane, 1.0f, p->tw[0][0], st),
-            "x ifft launch");
-    HIP_TRY(launch_cols_c2c(p->n[1], +1, sc, sc, blocks * p->n[0], plane, (size_t)p->P, p->H, 1.0f, p->tw[1][0], st),
-            "y ifft launch");
-    in = sc;
-  }
-  HIP_TRY(launch_rows_c2r(nl, in, (size_t)p->P, u, (size_t)nl, (int)rows, scale, p->tw[p->D - 1][0], st), "c2r launch");
-  return TFSM_OK;
+  if (p->D > 1 && !scratch) return fail(TFSM_E_INVALID, "inverse transform needs scratch");
+  const float scale = (float)(1.0 / (double)plan_npts(p));
+  return xf_inverse_blocks(p, (const float2*)uh, (float2*)scratch, u, (size_t)nbatch * p->C, scale,
+                           (hipStream_t)stream);
 }
 
 int tfsm_c2c(tfsm_plan* p, const void* in, void* out, int32_t nbatch, int32_t dir, void* stream) {
@@ -1414,26 +1596,17 @@ int tfsm_c2c(tfsm_plan* p, const void* in, void* out, int32_t nbatch, int32_t di
   hipStream_t st = (hipStream_t)stream;
   const size_t blocks = (size_t)nbatch * p->C;
   const int nl = p->n[p->D - 1];
-  double total = 1.0;
-  for (int d = 0; d < p->D; ++d) total *= p->n[d];
-  const float scale = dir > 0 ? (float)(1.0 / total) : 1.0f;
+  const float scale = dir > 0 ? (float)(1.0 / (double)plan_npts(p)) : 1.0f;
   size_t rows = blocks;
   for (int d = 0; d < p->D - 1; ++d) rows *= p->n[d];
-  HIP_TRY(launch_rows_c2c(nl, dir, (const float2*)in, (size_t)nl, (float2*)out, (size_t)nl, (int)rows, scale,
-                          p->tw[p->D - 1][0], st),
-          "row c2c launch");
+  HIP_TRY(xf_rows(p, GM_C2C, dir, in, (size_t)nl, out, (size_t)nl, rows, scale, st), "row c2c launch");
+  float2* o = (float2*)out;
   if (p->D == 2)
-    HIP_TRY(launch_cols_c2c(p->n[0], dir, (const float2*)out, (float2*)out, blocks, (size_t)p->n[0] * nl, (size_t)nl,
-                            nl, 1.0f, p->tw[0][0], st),
-            "column c2c launch");
+    HIP_TRY(xf_cols(p, 0, dir, o, o, blocks, (size_t)p->n[0] * nl, (size_t)nl, (size_t)nl, st), "column c2c launch");
   if (p->D == 3) {
     const size_t plane = (size_t)p->n[1] * nl;
-    HIP_TRY(launch_cols_c2c(p->n[1], dir, (const float2*)out, (float2*)out, blocks * p->n[0], plane, (size_t)nl, nl,
-                            1.0f, p->tw[1][0], st),
-            "y c2c launch");
-    HIP_TRY(launch_cols_c2c(p->n[0], dir, (const float2*)out, (float2*)out, blocks, (size_t)p->n[0] * plane, plane,
-                            (int)plane, 1.0f, p->tw[0][0], st),
-            "x c2c launch");
+    HIP_TRY(xf_cols(p, 1, dir, o, o, blocks * p->n[0], plane, (size_t)nl, (size_t)nl, st), "y c2c launch");
+    HIP_TRY(xf_cols(p, 0, dir, o, o, blocks, (size_t)p->n[0] * plane, plane, plane, st), "x c2c launch");
   }
   return TFSM_OK;
 }
@@ -1494,6 +1667,44 @@ int tfsm_plan_add_source(tfsm_plan* p, const float* src, int32_t src_batch, int3
   return TFSM_OK;
 }
 
+int tfsm_plan_add_force(tfsm_plan* p, const float* force, int32_t force_batch, int32_t force_channels, float coef,
+                        void* stream) {
+  if (!p || !force) return fail(TFSM_E_INVALID, "null argument");
+  if (p->term != TERM_NSP)
+    return fail(TFSM_E_INVALID, "an external force belongs to NSPressureConvection (velocity-form Navier-Stokes)");
+  if (!(force_batch == 1 || force_batch == p->B) || !(force_channels == 1 || force_channels == p->C))
+    return fail(TFSM_E_INVALID, "force batch/channels must be 1 or match the state");
+  if (p->force && (force_batch != p->forceB || force_channels != p->forceC))
+    return fail(TFSM_E_INVALID, "all explicit sources of one force must have the same shape");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t total = (size_t)force_batch * force_channels * p->modes;
+  float2* spec = nullptr;
+  HIP_TRY(hipMalloc(&spec, total * sizeof(float2)), "force spectrum scratch");
+  const int saveC = p->C;
+  p->C = force_channels;
+  int rc = tfsm_forward(p, force, spec, force_batch, stream);
+  p->C = saveC;
+  const bool first = p->force == nullptr;
+  if (!rc && first) rc = dev_alloc(p, &p->force, total, "force spectrum");
+  if (!rc) {
+    hipError_t e = launch_accum_source(p->force, spec, coef, total, first ? 1 : 0, st);
+    if (e != hipSuccess) rc = hip_fail(e, "force accumulate launch");
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(spec);
+  if (rc) return rc;
+  p->forceB = force_batch;
+  p->forceC = force_channels;
+  // the forced core masks the integrator's stage tensors in place: only the unfused path does that
+  p->fused_ok = false;
+  p->persistent = false;
+  if (!p->general) {
+    p->general = true;
+    gen_layout(p);
+  }
+  return TFSM_OK;
+}
+
 int tfsm_step(tfsm_plan* p, void* uh, int32_t nsteps, const int32_t* rec_slot, void* traj, int32_t T, void* ws,
               void* stream) {
   if (!p || !uh) return fail(TFSM_E_INVALID, "null argument");
@@ -1550,7 +1761,9 @@ int tfsm_apply(tfsm_plan* p, const void* uh, void* out, void* ws, void* stream) 
   float2* U = (float2*)uh;
   if (have_nl) {
     if (!ws) return fail(TFSM_E_INVALID, "workspace pointer is null");
-    if (p->D == 1 && p->term == TERM_KS) {
+    if (p->general) {
+      if (int rc = gen_stage(p, U, U, (char*)ws, ST_EVAL, StepMask(), st)) return rc;
+    } else if (p->D == 1 && p->term == TERM_KS) {
       Run1Args a = run1_args(p, U, (char*)ws);
       float2* Nr = (float2*)((char*)ws + p->off_N);
       float2* dc = p->ks_mean ? (float2*)((char*)ws + p->off_DC) : nullptr;
@@ -1661,6 +1874,7 @@ int tfsm_solve(tfsm_plan* p, const void* uh, void* out, void* stream) {
 
 int tfsm_launch_pass(tfsm_plan* p, void* uh, void* ws, int32_t pass_id, int32_t stage, int32_t sub, void* stream) {
   if (!p || !uh || !ws) return fail(TFSM_E_INVALID, "null argument");
+  if (p->general) return fail(TFSM_E_INVALID, "the plan steps through the unfused path: no fused passes");
   hipStream_t st = (hipStream_t)stream;
   // the launches cover sub-batch `sub`, exactly as tfsm_step launches them
   const int sb = (p->sub_batch > 0 && p->sub_batch < p->B) ? p->sub_batch : p->B;
@@ -1711,6 +1925,7 @@ int tfsm_adjoint_prepare(tfsm_plan* p, int32_t grad_term, void* stream) {
                 "term not supported by engine: the adjoint covers Convection (1D, 2D, 3D) and VorticityConvection");
   // explicit sources are constant: they enter the recomputed states but no Jacobian
   if (p->cplx) return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint needs a real linear symbol");
+  if (p->force) return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint of a forced NSPressureConvection");
   if (p->per_len) return fail(TFSM_E_INVALID, "term not supported by engine: the adjoint needs shared lengths");
   if (grad_term < -1 || grad_term >= p->nlin) return fail(TFSM_E_INVALID, "grad_term out of range");
   hipStream_t st = (hipStream_t)stream;

@@ -421,6 +421,37 @@ __device__ __forceinline__ void nsp_project(float2 (&cv)[C], const float (&av)[C
   for (int c = 0; c < C; ++c) cv[c] = rsub(make_float2(-mulr(av[c], p.y), mulr(av[c], p.x)), cv[c]);
 }
 
+// Hermitian part on Nyquist planes.  The reference steps the full complex spectrum.  Where a
+// derivative is applied after a product's FFT (the projection of NSPressureConvection,
+// ConservativeConvection), its value on the Nyquist plane of an even axis d (index n_d / 2, where
+// a_d is even in k instead of odd) is not Hermitian, and its real output keeps only the Hermitian
+// part, which the half spectrum stores: a product of symbols survives there only when it has an
+// even number of factors a_d of Nyquist axes.  ny[d] flags the Nyquist axes of a mode; callers
+// clear every flag at modes that are their own conjugate partner (every index 0 or Nyquist), whose
+// full value the half spectrum holds.
+template <int C>
+__device__ __forceinline__ void nsp_project(float2 (&cv)[C], const float (&av)[C], const bool (&ny)[C]) {
+  float lap = -mulr(av[0], av[0]);
+#pragma unroll
+  for (int i = 1; i < C; ++i) lap = addr(lap, -mulr(av[i], av[i]));
+  const float il = (lap == 0.f) ? 1.0f : __fdiv_rn(1.0f, lap);
+  float2 sum[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+  bool have[2] = {false, false};
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    const float2 t = make_float2(-mulr(av[i], cv[i].y), mulr(av[i], cv[i].x));
+    const int q = ny[i] ? 1 : 0;
+    sum[q] = have[q] ? radd(sum[q], t) : t;
+    have[q] = true;
+  }
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int q = ny[c] ? 1 : 0;
+    const float2 p = make_float2(mulr(il, sum[q].x), mulr(il, sum[q].y));
+    cv[c] = rsub(make_float2(-mulr(av[c], p.y), mulr(av[c], p.x)), cv[c]);
+  }
+}
+
 // Nonlinear value of one mode: N = coef_t * F + S   (S = sum of coef_s * source spectra)
 __device__ __forceinline__ float2 nl_value(float coef_t, float2 f, const float2* __restrict__ src, size_t si) {
   float2 n = rscale(coef_t, f);
@@ -715,6 +746,66 @@ hipError_t launch_adj_stage(const AdjStageArgs& a, hipStream_t st);
 hipError_t launch_adj_reduce(const float* gpart, size_t per, int B, double* out, hipStream_t st);
 hipError_t launch_adj_dcoefs(const float* L, size_t modes, int Bc, double dt, float* dcoef, hipStream_t st);
 hipError_t launch_adj_sym(const AdjArgs& a, int kind, float* S, hipStream_t st);
+
+// ---------------------------------------------------------------- unfused path (k_general.hip)
+// Meshes and term combinations without fused kernels (any transform length, ConservativeConvection
+// in 3D, a complex L with a product term in 2D / 3D) evaluate N(S) in separate launches:
+//   fields   F[b][f]  = mask(S) times the field symbol          (half spectrum, [B][NFA][modes])
+//   inverse  R[b][f]  = real fields                              ([B][NF][npts] float)
+//   products Q[b][p]  = pointwise products of the fields         ([B][NP][npts] float)
+//   forward  Qs[b][p] = spectra of the products                  (in F's memory)
+//   stage    N = the term's combination of Qs, then the stage combine of U / A1 / A2 -> S
+struct GenArgs {
+  int term, D, B, C;
+  int n[3];
+  int H, P;
+  size_t modes, npts;
+  int kmax[3];
+  const float* kv[3];
+  size_t kv_bstride;  // 1D per-sample lengths: axis table stride per (b, c) block, else 0
+  int NF, NP;
+  float coef_t;
+  int mask_in_place = 0;  // the evaluation zeroes its input outside the dealiasing box (forced NSP)
+};
+// fields and products per term (ConservativeConvection: the C (C + 1) / 2 products u_i u_j, i <= j)
+inline int gen_nf(int term, int C, int D) {
+  switch (term) {
+    case TERM_CONV: case TERM_NSP: return C + C * D;
+    case TERM_VC: return 4;
+    case TERM_KS: return D;
+    case TERM_CONS: return C;
+    default: return 0;
+  }
+}
+inline int gen_np(int term, int C) {
+  switch (term) {
+    case TERM_CONV: case TERM_NSP: return C;
+    case TERM_CONS: return C * (C + 1) / 2;
+    default: return 1;
+  }
+}
+struct GenStageArgs {
+  int stage;
+  const float2* Qs;
+  const float2* dc;      // KSConvection with the batch mean removed: (1/B) sum_b Qs_b(k = 0)
+  float2 *U, *A1, *A2, *S;
+  const float* coef;     // [Bc][NCOEF][modes]
+  const float2* coefc;   // complex-L plans
+  size_t coef_bstride;
+  int coef_per_block;    // tables per (b, c) block (1D per-sample lengths) instead of per sample
+  const float2* src;
+  size_t src_bstride, src_cstride;
+  // NSPressureConvection with an external force: force spectrum [Bf][Cf][modes] (null: none); with
+  // masked_stage set, a stage that recomputes a masked stage spectrum from U and A1 (ST_R2_E1,
+  // ST_R4_E2) takes it as zero outside the dealiasing box
+  const float2* force = nullptr;
+  size_t force_bstride = 0, force_cstride = 0;
+  int masked_stage = 0;
+  StepMask mask;
+};
+hipError_t launch_gen_fields(const GenArgs& a, float2* S, float2* F, hipStream_t st);
+hipError_t launch_gen_products(const GenArgs& a, const float* R, float* Q, hipStream_t st);
+hipError_t launch_gen_stage(const GenArgs& a, const GenStageArgs& s, hipStream_t st);
 
 // ---------------------------------------------------------------- diagnostic maps (k_maps.hip)
 enum MapOp : int { MAP_DEALIAS = 0, MAP_DIV = 1, MAP_GRAD = 2, MAP_CURL = 3, MAP_V2V = 4, MAP_PRESSURE = 5,

@@ -99,11 +99,16 @@ __device__ __forceinline__ void col_nsp_combine(const Col2Args& a, int stage, fl
     fft_wg<N0, -1, V, VAR>(v[c], s_buf, s_tw, j, vi);
   }
   const float ayv = colv ? a.ay[ky] : 0.f;
+  // Nyquist-plane parity of nsp_project (N0 is even for every fused size)
+  const bool nyy = 2 * ky == a.N1, zy = ky == 0;
 #pragma unroll
   for (int m = 0; m < RMAX; ++m) {
     float2 cv[2] = {v[0][m], v[1][m]};
-    const float av[2] = {a.ax[j + m * NT], ayv};
-    nsp_project<2>(cv, av);
+    const int x = j + m * NT;
+    const float av[2] = {a.ax[x], ayv};
+    const bool nyx = 2 * x == N0, own = (x == 0 || nyx) && (zy || nyy);
+    const bool ny[2] = {nyx && !own, nyy && !own};
+    nsp_project<2>(cv, av, ny);
     v[0][m] = cv[0];
     v[1][m] = cv[1];
   }
@@ -161,9 +166,15 @@ __device__ __forceinline__ void col2_tile(const Col2Args& a, int stage, int tile
           v[m] = colv ? t : make_float2(0.f, 0.f);
         }
         fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
+        // Nyquist-plane parity (nsp_project, engine.hpp): (i a_x) drops out on the x-Nyquist row,
+        // (i a_y), applied by the row kernel, on the y-Nyquist column, except at modes that are
+        // their own conjugate partner
+        const bool nyy = 2 * ky == a.N1, zy = ky == 0;
 #pragma unroll
         for (int m = 0; m < RMAX; ++m) {
-          const float axm = a.ax[j + m * NT];
+          const int x = j + m * NT;
+          const bool own = (x == 0 || 2 * x == N0) && (zy || nyy);
+          const float axm = (2 * x == N0 && !own) ? 0.f : a.ax[x];
           v[m] = make_float2(-mulr(axm, v[m].y), mulr(axm, v[m].x));
         }
         float2 w[RMAX];
@@ -174,7 +185,11 @@ __device__ __forceinline__ void col2_tile(const Col2Args& a, int stage, int tile
         }
         fft_wg<N0, -1, V, VAR>(w, s_buf, s_tw, j, vi);
 #pragma unroll
-        for (int m = 0; m < RMAX; ++m) v[m] = radd(v[m], w[m]);
+        for (int m = 0; m < RMAX; ++m) {
+          const int x = j + m * NT;
+          if (nyy && !(x == 0 || 2 * x == N0)) w[m] = make_float2(0.f, 0.f);
+          v[m] = radd(v[m], w[m]);
+        }
       } else {
 #pragma unroll
         for (int m = 0; m < RMAX; ++m) {

@@ -94,11 +94,16 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
       fft_wg<N0, -1, V, VAR>(v3[c], s_buf, s_tw, j, vi);
     }
     const float ayv = a.ay[ky], azv = colv ? a.az[kz] : 0.f;
+    // Nyquist-plane parity of nsp_project (every fused 3D size is even)
+    const bool nyy = 2 * ky == a.N1, nyz = 2 * kz == a.N2, oyz = (ky == 0 || nyy) && (kz == 0 || nyz);
 #pragma unroll
     for (int m = 0; m < RMAX; ++m) {
       float2 cv[3] = {v3[0][m], v3[1][m], v3[2][m]};
-      const float av[3] = {a.ax[j + m * NT], ayv, azv};
-      nsp_project<3>(cv, av);
+      const int x = j + m * NT;
+      const float av[3] = {a.ax[x], ayv, azv};
+      const bool nyx = 2 * x == N0, own = (x == 0 || nyx) && oyz;
+      const bool ny[3] = {nyx && !own, nyy && !own, nyz && !own};
+      nsp_project<3>(cv, av, ny);
 #pragma unroll
       for (int c = 0; c < 3; ++c) v3[c][m] = cv[c];
     }

@@ -176,6 +176,21 @@ class SpectralPlan:
         self._drop_step_graph()
         check(lib.tfsm_plan_set_persistent(self._plan, 1 if on else 0), "persistent")
 
+    @property
+    def general(self) -> bool:
+        """True when stepping runs the unfused path (separate field, product, transform and stage
+        launches): meshes and terms the fused kernels do not cover, or set_general(True)."""
+        return bool(lib.tfsm_plan_general(self._plan))
+
+    def set_general(self, on: bool):
+        """Unfused path (True) or the fused kernels (False, where they cover the plan)."""
+        self._drop_step_graph()
+        check(lib.tfsm_plan_set_general(self._plan, 1 if on else 0), "general")
+        ws = int(lib.tfsm_plan_workspace_bytes(self._plan))
+        if ws != self.ws_bytes:
+            self.ws_bytes = ws
+            self._ws = None
+
     def set_streams(self, nstreams: int):
         """1: one stream; 2: alternate sub-batches overlap on an auxiliary stream (tfsm_plan_set_streams)."""
         self._drop_step_graph()
@@ -225,6 +240,22 @@ class SpectralPlan:
             check(lib.tfsm_plan_add_source(self._plan, _ptr(s), int(s.shape[0]), int(s.shape[1]), float(coef),
                                            _stream(self.device)), "source")
             torch.cuda.current_stream(self.device).synchronize()
+
+    def add_force(self, force: torch.Tensor, coef: float):
+        """External force coef * fftn(force) of NSPressureConvection (tfsm_plan_add_force); the plan
+        then steps on the unfused path, whose workspace is larger."""
+        self._drop_step_graph()
+        if force.dim() != self.ndim + 2:
+            raise ValueError(f"external force must have shape (B, C, N...), got {tuple(force.shape)}")
+        f = force.detach().to(device=self.device, dtype=torch.float32).contiguous()
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_plan_add_force(self._plan, _ptr(f), int(f.shape[0]), int(f.shape[1]), float(coef),
+                                          _stream(self.device)), "force")
+            torch.cuda.current_stream(self.device).synchronize()
+        ws = int(lib.tfsm_plan_workspace_bytes(self._plan))
+        if ws != self.ws_bytes:
+            self.ws_bytes = ws
+            self._ws = None
 
     # ------------------------------------------------------------------ argument checks
     def _check(self, t: torch.Tensor, trailing, dtype, what: str, nb: Optional[int] = None) -> int:

@@ -136,13 +136,17 @@ GRAPH_STEPS = 16  # steps per captured graph replayed by integrate() on small mu
 
 def _graph_worthwhile(plan) -> bool:
     """Plans whose steps are several short launches (2D without the persistent stepper up to 128^2,
-    3D up to 64^3): launch overhead is a large part of a step there, and a replayed HIP graph
-    removes most of it."""
-    if plan.ndim == 1 or plan.persistent:
+    3D up to 64^3, the unfused path up to 2^16 points): launch overhead is a large part of a step
+    there, and a replayed HIP graph removes most of it."""
+    if plan.persistent:
         return False
     pts = 1
     for n in plan.n:
         pts *= int(n)
+    if plan.general:
+        return pts <= 1 << 16
+    if plan.ndim == 1:
+        return False
     return pts <= (64 ** 3 if plan.ndim == 3 else 128 * 128)
 
 
@@ -258,6 +262,13 @@ class OperatorLike(_MutableMixIn):
             raise ValueError("term not supported by engine: explicit sources need a product term")
         return lin, prod, prod_coef, sources
 
+    def _engine_forces(self):
+        """Explicit sources of NSPressureConvection's external force, as (coefficient, field)."""
+        out = []
+        for _, core in self._nonlinear_funcs:
+            out += list(getattr(core, "_engine_force", ()))
+        return out
+
     def _engine(self, batch: int, device) -> SpectralPlan:
         cache = self._state_dict.get("engine")
         if cache is None:
@@ -274,6 +285,8 @@ class OperatorLike(_MutableMixIn):
                                     dealias_rate=self._de_aliasing_rate)
                 for c, src in sources:
                     plan.add_source(src, c)
+                for c, f in self._engine_forces():
+                    plan.add_force(f, c)
             except torch.cuda.OutOfMemoryError as e:
                 raise RuntimeError(os.linesep.join([
                     "Cuda out of memory when building the integrator.",

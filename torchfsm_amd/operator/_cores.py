@@ -171,16 +171,44 @@ class VorticityConvection(NonlinearOperator):
 
 
 class _NSPressureConvectionCore(NonlinearFunc):
-    """Velocity-form Navier-Stokes nonlinear term without external force
-    (_navier_stokes.py:221-262): p = lap^-1 (nabla . conv), N = nabla p - conv, conv = (u . grad) u."""
+    """Velocity-form Navier-Stokes nonlinear term (_navier_stokes.py:221-262):
+    p = lap^-1 (nabla . (conv - f)), N = nabla p - (conv - f) + f, conv = (u . grad) u.  With an
+    external force the reference's core masks its input spectrum in place; the engine plan
+    reproduces that (tfsm_plan_add_force).  `_engine_force` lists the force's explicit sources as
+    (coefficient, spatial field)."""
 
     _engine_kind = "nsp"
 
-    def __init__(self) -> None:
+    def __init__(self, force_sources=()) -> None:
         super().__init__()
+        self._engine_force = list(force_sources)
 
     def __call__(self, u_fft, f_mesh: FourierMesh, u=None):
+        if self._engine_force:
+            raise ValueError("term not supported by engine: the forced NSPressureConvection core alone "
+                             "(call or integrate the operator)")
         return _core_direct("nsp", u_fft, f_mesh)
+
+
+def _force_sources(force, f_mesh: FourierMesh, n_channel: int):
+    """The explicit sources an external-force operator is made of, as (coefficient, spatial field):
+    the reference evaluates force(u_fft=..., return_in_fourier=True) (_navier_stokes.py:244-247),
+    which for explicit sources is the sum of coef * fftn(source)."""
+    force.register_mesh(f_mesh, n_channel)
+    if force._linear_cores:
+        raise ValueError("term not supported by engine: an external force with linear terms "
+                         "(the engine takes forces made of explicit sources)")
+    out = []
+    for coef, core in force._nonlinear_funcs:
+        if getattr(core, "_engine_kind", None) != "src":
+            raise ValueError(f"term not supported by engine: an external force with {type(core).__name__} "
+                             "(the engine takes forces made of explicit sources)")
+        if isinstance(coef, torch.Tensor):
+            if coef.numel() != 1:
+                raise ValueError("term not supported by engine: per-sample coefficient on an external force")
+            coef = float(coef)
+        out.append((float(coef), core._source_spatial))
+    return out
 
 
 class _NSPressureConvectionGenerator(CoreGenerator):
@@ -188,17 +216,17 @@ class _NSPressureConvectionGenerator(CoreGenerator):
         self.external_force = external_force
 
     def __call__(self, f_mesh: FourierMesh, n_channel: int) -> NonlinearFunc:
-        if self.external_force is not None:
-            raise ValueError("term not supported by engine: NSPressureConvection with an external force "
-                             "(add the force to the operator instead)")
         if f_mesh.n_dim != n_channel or f_mesh.n_dim == 1:
             raise ValueError("term not supported by engine: NSPressureConvection needs a 2D or 3D velocity field "
                              "with one channel per dimension")
+        if self.external_force is not None:
+            return _NSPressureConvectionCore(_force_sources(self.external_force, f_mesh, n_channel))
         return _NSPressureConvectionCore()
 
 
 class NSPressureConvection(NonlinearOperator):
-    """-grad(lap^-1 div((u . grad) u - f)) - (u . grad) u + f with f = None (_navier_stokes.py:265-280)."""
+    """-grad(lap^-1 div((u . grad) u - f)) - (u . grad) u + f (_navier_stokes.py:265-280); f is an
+    operator made of explicit sources, or None."""
 
     def __init__(self, external_force=None) -> None:
         super().__init__(_NSPressureConvectionGenerator(external_force))
