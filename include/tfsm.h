@@ -94,7 +94,7 @@ int tfsm_version(void);
 /* Compile an operator on a mesh into a plan: symbols, dealias limits, twiddle tables, L.
  * Any mesh of >= 2 points per axis is accepted (the reference transforms any length,
  * mesh.py:481-491).  The fused step kernels cover 1D powers of two 32..4096 and 48, 96, 100, 112,
- * 192, 200, 384, 400, 768, 800 points; 2D axes of powers of two 16..512 and 48, 96, 100, 112, 192,
+ * 192, 200, 384, 400, 768, 800 points; 2D axes of powers of two 16..1024 and 48, 96, 100, 112, 192,
  * 200; 3D axes of powers of two 16..128 and cubic 48, 96, 100 meshes.  Other plans (other sizes,
  * ConservativeConvection in 3D, a complex L with a product term in 2D / 3D) step through the
  * unfused path: separate field, product and stage launches with general-length FFTs (Stockham with

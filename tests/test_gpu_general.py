@@ -310,3 +310,46 @@ def test_forced_ns_velocity_vs_reference_golden(dev, dims):
     e = _rel(call, ref)
     print(f"forced NS {dims}D __call__: rel-L2 vs reference f64 {e:.2e}")
     assert e <= TOL
+
+
+def test_fused_1024_columns_and_rows(dev):
+    """2D meshes with 1024-point axes run the fused kernels (8-column tiles, 8 values per thread):
+    NS vorticity 1024^2 (B=2, SETDRK4) against the unfused path and the oracle, and 2-channel
+    Burgers on 1024 x 256 and 256 x 1024 (ETDRK2) against the oracle."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import field, pde
+    from torchfsm_amd.integrator import ETDRKIntegrator
+    from torchfsm_amd.workloads import fourier_modes_field, ns_initial_vorticity
+
+    n = 1024
+    mi = [(0, 2 * math.pi, n)] * 2
+    mesh = tfa.MeshGrid(mi, device=dev)
+    y = mesh.bc_mesh_grid()[1]
+    op = pde.NavierStokesVorticity(1000, field.kolm_force(y))
+    op.register_mesh(tfa.FourierMesh(mesh), 1)
+    plan = op._prepare(2, dev, 5e-4)
+    assert not plan.general and plan.tuning[:2] == (8, 1)
+    w0 = ns_initial_vorticity(n, 2, seed=17).to(dev)
+    uh = plan.forward(w0)
+    plan.step(uh, 10)
+    fused = plan.inverse(uh)
+    plan.set_general(True)
+    uh = plan.forward(w0)
+    plan.step(uh, 10)
+    unfused = plan.inverse(uh)
+    plan.set_general(False)
+    ref = refpath.integrate(refpath.ns_terms(1000, y), mi, w0, 5e-4, 10, scheme="SETDRK4")
+    e_f, e_u, e_fu = _rel(fused, ref), _rel(unfused, ref), _rel(fused, unfused)
+    print(f"NS 1024^2: fused vs oracle {e_f:.2e}, unfused vs oracle {e_u:.2e}, fused vs unfused {e_fu:.2e}")
+    assert max(e_f, e_u, e_fu) <= TOL
+    for shape in ((1024, 256), (256, 1024)):
+        mi = [(0, 1, shape[0]), (0, 1, shape[1])]
+        u = 0.5 * fourier_modes_field(shape, 2, 2, 1, seed=18)
+        op = pde.Burgers(0.01)
+        op.set_integrator(ETDRKIntegrator.ETDRK2)
+        out = op.integrate(u.to(dev), mesh=tfa.MeshGrid(mi, device=dev), dt=1e-3, step=10)
+        assert not op._prepare(1, dev, 1e-3).general
+        e = _rel(out, refpath.integrate(refpath.burgers_terms(0.01), mi, u, 1e-3, 10, scheme="ETDRK2"))
+        print(f"Burgers {shape}: fused vs oracle {e:.2e}")
+        assert e <= TOL

@@ -237,6 +237,7 @@ bool twiddle_table(int n, int var, std::vector<float2>& h) {
     case 128: twiddle_table_t<128, 1>(h); return true;
     case 256: twiddle_table_t<256, 1>(h); return true;
     case 512: twiddle_table_t<512, 1>(h); return true;
+    case 1024: twiddle_table_t<1024, 1>(h); return true;
     default: return false;
   }
 }
@@ -1117,7 +1118,7 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     const int n = geom->n[d];
     if (n < 2) return fail(TFSM_E_INVALID, "mesh axes need at least 2 points");
     const bool ok = D == 1   ? ((is_pow2(n) && n >= 32 && n <= 4096) || is_mixed_size(n))
-                    : D == 2 ? ((is_pow2(n) && n >= 16 && n <= 512) || is_mixed_size_2d(n))
+                    : D == 2 ? ((is_pow2(n) && n >= 16 && n <= 1024) || is_mixed_size_2d(n))
                              : ((is_pow2(n) && n >= 16 && n <= 128) || is_mixed_size_3d(n));
     fused_sizes = fused_sizes && ok;
   }
@@ -1330,6 +1331,12 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     // 128 run 8-column tiles and one row per 64-thread row group (5-9 %)
     p->col_var = n0 == 256 ? 1 : 0;
     if (n0 == 64 || n0 == 128 || n0 == 256) p->V = 8;
+    // 1024-point columns: 8 columns per tile (LDS), the 8-values-per-thread plan (the 16-value
+    // plan holds 2 waves per SIMD and spills for NSPressureConvection and ConservativeConvection)
+    if (n0 == 1024) {
+      p->V = 8;
+      p->col_var = 1;
+    }
     if (n1 == 64 || n1 == 128) p->row_wg = 64;
     const size_t per_sample = 4 * (size_t)p->C * p->modes * sizeof(float2) +
                               (size_t)p->NF * p->n[0] * p->PF * sizeof(float2);

@@ -214,6 +214,8 @@ template <> struct FftPlan<64, 1> { static constexpr int RMAX = 16; static const
 template <> struct FftPlan<128, 1> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[5] = {8, 4, 4, 0, 0}; };
 template <> struct FftPlan<256, 1> { static constexpr int RMAX = 8; static constexpr int NST = 3; static constexpr int R[5] = {8, 8, 4, 0, 0}; };
 template <> struct FftPlan<512, 1> { static constexpr int RMAX = 16; static constexpr int NST = 3; static constexpr int R[5] = {8, 8, 8, 0, 0}; };
+// (1024: half the elements per thread of VAR 0, for 2D columns of 1024 points)
+template <> struct FftPlan<1024, 1> { static constexpr int RMAX = 8; static constexpr int NST = 4; static constexpr int R[5] = {8, 8, 4, 4, 0}; };
 // VAR 2: 8 elements per thread where VAR 0 holds 16 (lower register use in the 3D kernels)
 template <> struct FftPlan<128, 2> { static constexpr int RMAX = 8; static constexpr int NST = 3; static constexpr int R[5] = {8, 4, 4, 0, 0}; };
 // VAR 3: 4 elements per thread (radix 4, one radix-2 stage where needed): the fewest registers

@@ -39,10 +39,12 @@ bool run2d_supported(int term, int n0, int n1, int pf) {
 bool col2d_supported(int n0, int v, int var) {
   if (!(v == 8 || v == 16)) return false;
   if (n0 == 16 || n0 == 32 || is_mixed_size_2d(n0)) return var == 0;
+  if (n0 == 1024) return v == 8 && (var == 0 || var == 1);  // 16 columns of 1024 points exceed the LDS
   return (n0 == 64 || n0 == 128 || n0 == 256 || n0 == 512) && (var == 0 || var == 1);
 }
 bool row2d_supported(int n1, int var, int wgt) {
   if (n1 == 16 || n1 == 32 || is_mixed_size_2d(n1)) return var == 0 && wgt == 256;
+  if (n1 == 1024) return var == 0 && (wgt == 64 || wgt == 256);
   return (n1 == 64 || n1 == 128 || n1 == 256 || n1 == 512) && (var == 0 || var == 1) && (wgt == 64 || wgt == 256);
 }
 

@@ -659,14 +659,14 @@ inline hipError_t launch_row_t(const Row2Args& a, hipStream_t st) {
 #define TFSM_COL_P2(X)                                                                                        \
   X(16, 8, 0) X(16, 16, 0) X(32, 8, 0) X(32, 16, 0) X(64, 8, 0) X(64, 16, 0) X(64, 8, 1) X(64, 16, 1)          \
   X(128, 8, 0) X(128, 16, 0) X(128, 8, 1) X(128, 16, 1) X(256, 8, 0) X(256, 16, 0) X(256, 8, 1) X(256, 16, 1) \
-  X(512, 8, 0) X(512, 16, 0) X(512, 8, 1) X(512, 16, 1)
+  X(512, 8, 0) X(512, 16, 0) X(512, 8, 1) X(512, 16, 1) X(1024, 8, 0) X(1024, 8, 1)
 #define TFSM_COL_MX(X)                                                                                       \
   X(48, 8, 0) X(48, 16, 0) X(96, 8, 0) X(96, 16, 0) X(100, 8, 0) X(100, 16, 0) X(112, 8, 0) X(112, 16, 0)    \
   X(192, 8, 0) X(192, 16, 0) X(200, 8, 0) X(200, 16, 0)
 #define TFSM_ROW_P2(X)                                                                                        \
   X(16, 0, 256) X(32, 0, 256) X(64, 0, 256) X(64, 1, 256) X(64, 0, 64) X(64, 1, 64) X(128, 0, 256)            \
   X(128, 1, 256) X(128, 0, 64) X(128, 1, 64) X(256, 0, 256) X(256, 1, 256) X(256, 0, 64) X(256, 1, 64)        \
-  X(512, 0, 256) X(512, 1, 256) X(512, 0, 64) X(512, 1, 64)
+  X(512, 0, 256) X(512, 1, 256) X(512, 0, 64) X(512, 1, 64) X(1024, 0, 256) X(1024, 0, 64)
 #define TFSM_ROW_MX(X) X(48, 0, 256) X(96, 0, 256) X(100, 0, 256) X(112, 0, 256) X(192, 0, 256) X(200, 0, 256)
 
 #define TFSM_COL_CASE(N, VV, VR) \
