@@ -362,14 +362,18 @@ def differentiation_leg(dev, reps=5):
     op = pde.Burgers(nu)
     op.set_integrator(ETDRKIntegrator.ETDRK2)
     op.adjoint_checkpoint_every = 10
+    op_fwd = pde.Burgers(0.01)  # the same operator with a plain coefficient: integrate() alone
+    op_fwd.set_integrator(ETDRKIntegrator.ETDRK2)
 
     def run(grad):
         u = u0.clone().requires_grad_(grad)
         if grad:
             nu.grad = None
-        res = op.integrate(u, mesh=mesh, dt=dt, step=steps)
+        res = (op if grad else op_fwd).integrate(u, mesh=mesh, dt=dt, step=steps)
         if grad:
-            ((res - target) ** 2).mean().backward()
+            # the operator's coefficient (nu times the Laplacian's unit coefficient) is one graph
+            # node shared by every run: keep it for the next backward
+            ((res - target) ** 2).mean().backward(retain_graph=True)
         return res
 
     def timed(fn):
@@ -434,18 +438,18 @@ def host_cpu():
 
 def time_in_step(plan, uh, prime, seq, steps=12, skip=2):
     """Average duration (ms) of each kernel of the step, timed inside the step's own schedule:
-    every sub-batch runs `steps` steps launched pass by pass in the step's order, sub-batch k on
-    the stream tfsm_step uses for it (with two streams, alternate sub-batches on torch's current
-    stream and on a second stream, so their kernels overlap as in tfsm_step), and HIP events are
-    recorded between the launches on each kernel's own stream.  The first `skip` steps of every
-    sub-batch are not counted.  With two streams a kernel's duration includes the time it shares
-    the GPU with the other stream's kernel."""
+    every sub-batch runs `steps` steps launched pass by pass in the step's order, with HIP events
+    recorded between the launches on the stream they run on.  The sub-batches run one after the
+    other on torch's current stream, so each duration is the kernel's own (as a profiler that
+    serialises kernels sees it); the engine leg overlaps two sub-batches where the plan uses two
+    streams, which the step-level fraction reflects.  The first `skip` steps of every sub-batch are
+    not counted."""
     import torch
 
     B, sb = plan.batch, plan.tuning[4]
     nsub = 1 if not 0 < sb < B else (B + sb - 1) // sb
     main = torch.cuda.current_stream()
-    streams = [main] if plan.streams == 1 or nsub == 1 else [main, torch.cuda.Stream()]
+    streams = [main]
     u = uh.clone()
     torch.cuda.synchronize()
     fork = torch.cuda.Event()
@@ -635,10 +639,10 @@ def main():
         "step_bytes_model": step_bytes,
         "step_frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "streams": plan.streams,
-        "timing": ("in-step: HIP events between the launches of every sub-batch's steps, each on the stream "
-                   "tfsm_step uses for it" + ("; two sub-batches run concurrently, so a kernel's duration "
-                                              "includes the time it shares the GPU (step_frac is the "
-                                              "step-level fraction)" if plan.streams == 2 else "")),
+        "timing": ("in-step: HIP events between the launches of every sub-batch's steps, the sub-batches "
+                   "one after the other on one stream (each kernel's own duration)" +
+                   ("; the engine leg overlaps two sub-batches on two streams (step_frac is the step-level "
+                    "fraction)" if plan.streams == 2 else "")),
     }
     log(f"[bench] kernels (ms): {kt}; dominant {dom}: {avg_bytes / 1e6:.1f} MB per launch in {avg_ms:.4f} ms "
         f"= {achieved:.0f} GB/s; traffic {traffic}")

@@ -96,7 +96,8 @@ int tfsm_version(void);
  * mesh.py:481-491).  The fused step kernels cover 1D powers of two 32..4096 and 48, 96, 100, 112,
  * 192, 200, 384, 400, 768, 800 points; 2D axes of powers of two 16..1024 and 48, 96, 100, 112, 192,
  * 200; 3D axes of powers of two 16..128 and cubic 48, 96, 100 meshes.  Other plans (other sizes,
- * ConservativeConvection in 3D, a complex L with a product term in 2D / 3D) step through the
+ * ConservativeConvection in 3D, a complex L with a product term in 2D / 3D, NSPressureConvection
+ * with an external force, tfsm_plan_add_force) step through the
  * unfused path: separate field, product and stage launches with general-length FFTs (Stockham with
  * radices 2..7 in LDS, Bluestein for other lengths, four-step beyond 4096 points).
  * Replaces OperatorLike.register_mesh + _build_linear_coefs + _build_nonlinear_funcs

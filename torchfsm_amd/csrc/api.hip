@@ -117,12 +117,15 @@ struct tfsm_plan {
   float* coef = nullptr;
   float4* cpk = nullptr;  // packed per-mode coefficients of the current scheme
   float* c5t = nullptr;   // SETDRK4's c5 per mode (read alone by the third evaluation)
+  float2* ck01 = nullptr; // SETDRK4's [E2, c1] per mode (first and second evaluations)
+  float* c6t = nullptr;   // SETDRK4's c6 per mode (last evaluation)
   int scheme = -1;
   double dt = 0.0;
   int n_points = 16;
   double radius = 1.0;
   float2* src = nullptr;
   int srcB = 0, srcC = 0;
+  unsigned char* src_cols = nullptr;  // 2D: [P] 1 where a source mode of last-axis column k is nonzero
   int V = 16;        // column tile of the fused 2D column kernel
   int col_var = 0;   // radix-plan variant of the column transforms
   int row_var = 0;   // radix-plan variant of the row transforms
@@ -524,9 +527,12 @@ Col2Args col_args(tfsm_plan* p, float2* U, char* ws) {
   c.cpk = p->cpk;
   c.cpk_bstride = p->Bc > 1 ? 2 * p->modes : 0;
   c.c5t = p->c5t;
+  c.ck01 = p->scheme == SETDRK4 ? p->ck01 : nullptr;
+  c.c6t = p->scheme == SETDRK4 ? p->c6t : nullptr;
   c.src = p->src;
   c.src_bstride = p->src_bstride();
   c.src_cstride = p->src_cstride();
+  c.src_cols = p->src_cols;
   c.G = (float2*)(ws + p->off_G);
   c.dc = p->ks_mean ? (const float2*)(ws + p->off_DC) : nullptr;
   c.b0 = 0;
@@ -592,6 +598,8 @@ Col3Args col3_args(tfsm_plan* p, float2* U, char* ws, int b0, int nb) {
   c.cpk = p->cpk;
   c.cpk_bstride = p->Bc > 1 ? 2 * p->modes : 0;
   c.c5t = p->c5t;
+  c.ck01 = p->scheme == SETDRK4 ? p->ck01 : nullptr;
+  c.c6t = p->scheme == SETDRK4 ? p->c6t : nullptr;
   c.src = p->src;
   c.src_bstride = p->src_bstride();
   c.src_cstride = p->src_cstride();
@@ -1238,6 +1246,8 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   if (!rc) rc = dev_alloc(p, &p->coef, (size_t)Lb * NCOEF * p->modes, "coefficient table");
   if (!rc) rc = dev_alloc(p, &p->cpk, (size_t)Lb * 2 * p->modes, "packed coefficient table");
   if (!rc) rc = dev_alloc(p, &p->c5t, (size_t)Lb * p->modes, "c5 table");
+  if (!rc) rc = dev_alloc(p, &p->ck01, (size_t)Lb * p->modes, "SETDRK4 [E2, c1] table");
+  if (!rc) rc = dev_alloc(p, &p->c6t, (size_t)Lb * p->modes, "SETDRK4 c6 table");
   if (!rc && cplx) rc = dev_alloc(p, &p->Lc, (size_t)Lb * p->modes, "complex linear symbol");
   if (!rc && cplx) rc = dev_alloc(p, &p->coefc, (size_t)Lb * NCOEF * p->modes, "complex coefficient table");
   if (!rc) {
@@ -1496,6 +1506,8 @@ int tfsm_plan_set_scheme(tfsm_plan* plan, int32_t scheme, double dt, int32_t n_p
   a.coef = plan->coef;
   a.cpk = plan->cpk;
   a.c5t = plan->c5t;
+  a.ck01 = plan->ck01;
+  a.c6t = plan->c6t;
   a.Lc = plan->Lc;
   a.coefc = plan->coefc;
   HIP_TRY(launch_coefs(a, (hipStream_t)stream), "coefficient launch");
@@ -1560,6 +1572,8 @@ int tfsm_plan_set_sample(tfsm_plan* p, int32_t b, const float* linear_coef, doub
       a.coef = p->coef + (size_t)r * NCOEF * p->modes;
       a.cpk = p->cpk + (size_t)r * 2 * p->modes;
       a.c5t = p->c5t + (size_t)r * p->modes;
+      a.ck01 = p->ck01 + (size_t)r * p->modes;
+      a.c6t = p->c6t + (size_t)r * p->modes;
       a.Lc = p->cplx ? p->Lc + (size_t)r * p->modes : nullptr;
       a.coefc = p->cplx ? p->coefc + (size_t)r * NCOEF * p->modes : nullptr;
       HIP_TRY(launch_coefs(a, st), "coefficient launch");
@@ -1671,6 +1685,18 @@ int tfsm_plan_add_source(tfsm_plan* p, const float* src, int32_t src_batch, int3
   if (rc) return rc;
   p->srcB = src_batch;
   p->srcC = src_channels;
+  if (p->D == 2) {
+    // columns (last-axis frequencies) where the accumulated source has a nonzero mode: the fused
+    // column kernels skip the source loads elsewhere (adding an exact zero changes nothing)
+    std::vector<float2> h(total);
+    HIP_TRY(hipMemcpy(h.data(), p->src, total * sizeof(float2), hipMemcpyDeviceToHost), "source spectrum download");
+    std::vector<unsigned char> cols(p->P, 0);
+    for (size_t e = 0; e < total; ++e)
+      if (h[e].x != 0.f || h[e].y != 0.f) cols[e % p->P] = 1;
+    if (!p->src_cols) rc = dev_alloc(p, &p->src_cols, (size_t)p->P, "source column flags");
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(p->src_cols, cols.data(), cols.size(), hipMemcpyHostToDevice), "source column flags upload");
+  }
   return TFSM_OK;
 }
 

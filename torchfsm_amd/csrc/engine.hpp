@@ -338,15 +338,21 @@ __device__ __forceinline__ float2 stage_combine_v(int stage, float2 nval, float2
 // workgroups per CU.  The loads are unconditional: a lane past the last column (colv false) reads the padded
 // columns of the same rows, which every array and table holds, and its result is discarded.
 // Offsets within one channel are 32-bit (every supported mesh has fewer than 2^28 modes).
+// SETDRK4's first, second and last evaluations need 2, 2 and 1 of the packed entry's floats: with
+// k01b ([E2, c1] per mode) and c6b (c6 per mode), tables laid out like c5b, those stages read 8 and
+// 4 bytes per mode instead of the 16-byte entry.
 template <int RMAX, int NT, int BM = (RMAX >= 16 ? 1 : 4), bool FENCE = false>
 __device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1, float2* A2, const float2* src,
                                                const float4* cpk, const float* c5b, unsigned c5o, float coef_t,
                                                unsigned stride, int j, bool colv, const float2 (&v)[RMAX],
-                                               float2 (&s)[RMAX]) {
+                                               float2 (&s)[RMAX], const float2* k01b = nullptr,
+                                               const float* c6b = nullptr) {
   static_assert(RMAX % BM == 0, "combine batch must divide the values per thread");
   const float2 z = make_float2(0.f, 0.f);
   const bool rdu = stage_reads_u(stage), rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
   const bool rd5 = stage == ST_R4_E2;  // c5, from its own table
+  const bool rk01 = k01b && (stage == ST_R4_E0 || stage == ST_R4_E1);
+  const bool rk6 = c6b && stage == ST_R4_E3;
   const int tgt = stage_target(stage);
   float2* dst = tgt == 0 ? U : (tgt == 1 ? A1 : A2);
   const unsigned o0 = (unsigned)j * stride, ostep = (unsigned)NT * stride;
@@ -362,7 +368,14 @@ __device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1,
 #pragma unroll
     for (int i = 0; i < BM; ++i) {
       const unsigned o = o0 + (unsigned)(m0 + i) * ostep;
-      cc[i] = cpk[2 * o];
+      if (rk01) {
+        const float2 t = k01b[c5o + o];
+        cc[i] = make_float4(t.x, t.y, 0.f, 0.f);
+      } else if (rk6) {
+        cc[i] = make_float4(0.f, 0.f, 0.f, c6b[c5o + o]);
+      } else {
+        cc[i] = cpk[2 * o];
+      }
       uu[i] = z;
       q1[i] = z;
       q2[i] = z;
@@ -487,8 +500,11 @@ struct Col2Args {
   const float4* cpk;          // packed coefficients [Bc][N0*P1][2]
   size_t cpk_bstride;         // 0 or 2*N0*P1 (float4 units)
   const float* c5t;           // SETDRK4's c5 [Bc][N0*P1] (batch stride cpk_bstride / 2)
+  const float2* ck01 = nullptr;  // SETDRK4's [E2, c1] per mode (same layout as c5t), or null
+  const float* c6t = nullptr;    // SETDRK4's c6 per mode, or null
   const float2* src;          // [Bs][Cs][N0][P1] or null
   size_t src_bstride, src_cstride;
+  const unsigned char* src_cols = nullptr;  // [P1]: 1 where some source mode of column ky is nonzero
   float2* G;                  // fields [B][NF][N0][PF]
   const float2* dc = nullptr;  // KSConvection: batch mean of the product DC (subtracted at k = 0), or null
   int b0;                     // first sample handled by this launch
@@ -534,6 +550,8 @@ struct Col3Args {
   const float4* cpk;
   size_t cpk_bstride;
   const float* c5t;           // SETDRK4's c5 [Bc][modes] (batch stride cpk_bstride / 2)
+  const float2* ck01 = nullptr;  // SETDRK4's [E2, c1] per mode (same layout as c5t), or null
+  const float* c6t = nullptr;    // SETDRK4's c6 per mode, or null
   const float2* src;
   size_t src_bstride, src_cstride;
   float2* G1;                 // [B][2 nc][N0][K1][PF2]
@@ -611,6 +629,8 @@ struct LinArgs {
 struct CoefArgs {
   float4* cpk;             // packed coefficients [Bc][modes][2], written after the slot tables
   float* c5t;              // SETDRK4's c5 alone [Bc][modes] (the third evaluation's fifth coefficient)
+  float2* ck01 = nullptr;  // SETDRK4's [E2, c1] [Bc][modes] (first and second evaluations)
+  float* c6t = nullptr;    // SETDRK4's c6 [Bc][modes] (last evaluation)
   int scheme;
   double dt;
   float dt32, hdt32;       // fl32(dt), fl32(0.5 * dt)

@@ -39,6 +39,7 @@ struct GAxis {
   int L = 0;          // n, or M for Bluestein
   bool four = false;  // L > GFFT_LDS_MAX: L = g1.L * g2.L
   GRadix g, g1, g2;
+  const float2* tw4 = nullptr;  // four-step: exp(-2 pi i t / L), t < L (twiddles between the passes)
   const float2* chirp[2] = {nullptr, nullptr};  // [n] exp(-/+ i pi (t^2 mod 2n) / n): forward, inverse
   const float2* bhat[2] = {nullptr, nullptr};   // [M] FFT_M of the conjugate chirp kernel, times 1/M
 };

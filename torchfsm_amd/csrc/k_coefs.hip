@@ -324,7 +324,11 @@ __global__ void k_pack_coefs(CoefArgs a) {
     }
     a.cpk[cpk_index(e, 0)] = h0;
     a.cpk[cpk_index(e, 1)] = h1;
-    if (a.scheme == SETDRK4) a.c5t[e] = S(CS_C5);
+    if (a.scheme == SETDRK4) {
+      a.c5t[e] = S(CS_C5);
+      if (a.ck01) a.ck01[e] = make_float2(S(CS_E2), S(CS_C1));
+      if (a.c6t) a.c6t[e] = S(CS_C6);
+    }
   }
 }
 

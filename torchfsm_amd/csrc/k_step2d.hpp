@@ -58,14 +58,17 @@ struct Term2<TERM_CONS> {  // ConservativeConvection, C = 2: fields u0, u1; prod
 // one mode's operands at a time (BM = 1) to stay within 128 VGPRs.
 template <int N0, int VAR, int BM, bool FENCE = false>
 __device__ __forceinline__ void combine_column2(const Col2Args& a, int stage, size_t base, const float2* src,
-                                                const float4* cpk, const float* c5b, int cko, int ky, bool colv, int j,
+                                                const float4* cpk, const float* c5b, size_t tb, int cko, int ky,
+                                                bool colv, int j,
                                                 const float2 (&v)[Fft<N0, VAR>::RMAX],
                                                 float2 (&s)[Fft<N0, VAR>::RMAX]) {
   using F = Fft<N0, VAR>;
   const size_t cb = base + ky;
-  combine_column<F::RMAX, F::NT, BM, FENCE>(stage, a.U + cb, a.A1 + cb, a.A2 + cb, src ? src + ky : nullptr,
+  // a source spectrum is read only in the columns where it has a nonzero mode (src_cols)
+  const bool rs = src && (!a.src_cols || a.src_cols[ky]);
+  combine_column<F::RMAX, F::NT, BM, FENCE>(stage, a.U + cb, a.A1 + cb, a.A2 + cb, rs ? src + ky : nullptr,
                                      cpk + (2 * ky + cko), c5b, (unsigned)ky, a.coef_t, (unsigned)a.P1, j, colv, v,
-                                     s);
+                                     s, a.ck01 ? a.ck01 + tb : nullptr, a.c6t ? a.c6t + tb : nullptr);
 }
 
 // Velocity-form Navier-Stokes (NSPressureConvection): both channels' convection spectra are
@@ -113,13 +116,15 @@ __device__ __forceinline__ void col_nsp_combine(const Col2Args& a, int stage, fl
     v[1][m] = cv[1];
   }
   const float4* cpk = a.cpk + (size_t)b * a.cpk_bstride;
-  const float* c5b = a.c5t + (size_t)b * (a.cpk_bstride / 2);
+  const size_t tb = (size_t)b * (a.cpk_bstride / 2);
+  const float* c5b = a.c5t + tb;
   // cpk_index(ci, half) = 2 ci + cko: every mode of this lane has ci & 15 == ky & 15
   const int cko = (stage_coef_half(stage) << 4) - (ky & 15);
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride : nullptr;
-    combine_column2<N0, VAR, 1>(a, stage, ((size_t)b * 2 + c) * plane, src, cpk, c5b, cko, ky, colv, j, v[c], s[c]);
+    combine_column2<N0, VAR, 1>(a, stage, ((size_t)b * 2 + c) * plane, src, cpk, c5b, tb, cko, ky, colv, j, v[c],
+                                s[c]);
   }
 }
 
@@ -139,7 +144,8 @@ __device__ __forceinline__ void col2_tile(const Col2Args& a, int stage, int tile
   const int kl = (CLAMP && !colv) ? a.H1 - 1 : ky;
   const size_t plane = (size_t)N0 * a.P1;
   const float4* cpk = a.cpk + (size_t)b * a.cpk_bstride;
-  const float* c5b = a.c5t + (size_t)b * (a.cpk_bstride / 2);
+  const size_t tb = (size_t)b * (a.cpk_bstride / 2);
+  const float* c5b = a.c5t + tb;
   // cpk_index(ci, half) = 2 ci + cko: every mode of this lane has ci & 15 == kl & 15
   const int cko = (stage_coef_half(stage) << 4) - (kl & 15);
   float2 s[NC][RMAX];
@@ -203,7 +209,8 @@ __device__ __forceinline__ void col2_tile(const Col2Args& a, int stage, int tile
         if (a.dc && ky == 0 && j == 0) v[0] = rsub(v[0], a.dc[0]);
       }
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride : nullptr;
-      combine_column2<N0, VAR, (NC > 1 || RMAX >= 16) ? 1 : 4, (ST >= 0)>(a, stage, base, src, cpk, c5b, cko, kl, colv,
+      combine_column2<N0, VAR, (NC > 1 || RMAX >= 16) ? 1 : 4, (ST >= 0)>(a, stage, base, src, cpk, c5b, tb, cko, kl,
+                                                                             colv,
                                                                              j, v, s[c]);
     }
   }
@@ -563,6 +570,8 @@ __global__ void __launch_bounds__((Run2Shape<N, TERM>::THREADS)) k2_run(Col2Args
   c.A2 += (size_t)b * per;
   c.cpk += (size_t)b * ca.cpk_bstride;
   c.c5t += (size_t)b * (ca.cpk_bstride / 2);
+  if (c.ck01) c.ck01 += (size_t)b * (ca.cpk_bstride / 2);
+  if (c.c6t) c.c6t += (size_t)b * (ca.cpk_bstride / 2);
   c.coef += (size_t)b * ca.coef_bstride;
   if (c.src) c.src += (size_t)b * ca.src_bstride;
   c.G = s_G;

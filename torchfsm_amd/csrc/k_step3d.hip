@@ -64,6 +64,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
   const size_t plane = (size_t)N0 * inner;   // modes
   const size_t coff = (size_t)ky * a.P2 + kz;
   const float4* cpk = a.cpk + (size_t)b * a.cpk_bstride;
+  const size_t tb = (size_t)b * (a.cpk_bstride / 2);  // per-mode tables (c5t, ck01, c6t) of this sample
   // cpk_index(coff + xo, half) = 2 (coff + xo) + cko: (coff + xo) & 15 == kz & 15
   const int cko = (stage_coef_half(stage) << 4) - (kz & 15);
   constexpr int NC3 = TERM == TERM_KS ? 1 : 3;  // channels
@@ -112,8 +113,8 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
       const size_t base = ((size_t)b * 3 + c) * plane + coff;
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride + coff : nullptr;
       combine_column<RMAX, NT, CBM>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko),
-                               a.c5t + (size_t)b * (a.cpk_bstride / 2), (unsigned)coff, a.coef_t,
-                               (unsigned)inner, j, colv, v3[c], s[c]);
+                               a.c5t + tb, (unsigned)coff, a.coef_t, (unsigned)inner, j, colv, v3[c], s[c],
+                               a.ck01 ? a.ck01 + tb : nullptr, a.c6t ? a.c6t + tb : nullptr);
     }
   } else {
 #pragma unroll
@@ -132,8 +133,8 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
       }
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride + coff : nullptr;
       combine_column<RMAX, NT, CBM>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko),
-                               a.c5t + (size_t)b * (a.cpk_bstride / 2), (unsigned)coff, a.coef_t,
-                               (unsigned)inner, j, colv, v, s[c]);
+                               a.c5t + tb, (unsigned)coff, a.coef_t, (unsigned)inner, j, colv, v, s[c],
+                               a.ck01 ? a.ck01 + tb : nullptr, a.c6t ? a.c6t + tb : nullptr);
     }
   }
   const int kyc = ky_compress(ky, a.N1, a.kmax1);
