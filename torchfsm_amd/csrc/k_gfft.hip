@@ -33,8 +33,13 @@ struct GLaunch {
   const float2* post;    // output multiplier per logical r (Bluestein chirp or kernel spectrum)
   long long twn;         // four-step twiddle exp(twdir 2 pi i k s / twn) (0: none)
   int twdir;
+  const float2* twt;     // its table exp(-2 pi i t / twn), t < twn (null: computed in float64)
   float scale;
   int T;
+  // the workgroup's lines are consecutive s (n_s > 1) or i (n_s == 1) of one (o, i) / o, and
+  // adjacent lines are adjacent in memory on input (ilv_in) / output (ilv_out): the threads then
+  // walk those lines interleaved, so neighbouring threads touch neighbouring addresses
+  int ilv_in, ilv_out;
 };
 
 __device__ __forceinline__ float2 gmul(float2 a, float2 b) {
@@ -142,8 +147,43 @@ __global__ void __launch_bounds__(GFFT_THREADS) k_gfft(GLaunch a) {
     i = r % a.n_i;
     o = r / a.n_i;
   }
+  const long long L0 = (long long)blockIdx.x * lpw;  // first line of the workgroup
+  long long s0 = 0, i0 = 0, o0 = 0;
+  if (a.ilv_in || a.ilv_out) {
+    s0 = L0 % a.n_s;
+    const long long r0 = L0 / a.n_s;
+    i0 = r0 % a.n_i;
+    o0 = r0 / a.n_i;
+  }
+  if (a.ilv_in) {
+    for (int e = threadIdx.x; e < lpw * L; e += GFFT_THREADS) {
+      const int ls = e % lpw, j = e / lpw;
+      float2 v = make_float2(0.f, 0.f);
+      const long long sl = a.n_s > 1 ? s0 + ls : 0, il = a.n_s > 1 ? i0 : i0 + ls;
+      const long long q = (long long)j * a.qj + sl * a.qs;
+      if (L0 + ls < nlines && q < a.nload) {
+        const long long ibl = o0 * a.ios + il * a.iis;
+        if (a.in_mode == 1) {
+          v.x = ((const float*)a.in)[ibl + q * a.ies];
+        } else if (a.in_mode == 2) {
+          const float2* x = (const float2*)a.in;
+          if (q < a.hbins) {
+            v = x[ibl + q * a.ies];
+          } else {
+            const float2 m = x[ibl + (a.nfull - q) * a.ies];
+            v = make_float2(m.x, -m.y);
+          }
+          if (q == 0 || 2 * q == a.nfull) v.y = 0.f;
+        } else {
+          v = ((const float2*)a.in)[ibl + q * a.ies];
+        }
+        if (a.pre) v = gmul(v, a.pre[q]);
+      }
+      g_lds[(size_t)ls * 2 * L + j] = v;
+    }
+  }
   const long long ib = o * a.ios + i * a.iis;
-  for (int j = t; j < L; j += T) {
+  for (int j = t; j < L && !a.ilv_in; j += T) {
     float2 v = make_float2(0.f, 0.f);
     const long long q = (long long)j * a.qj + s * a.qs;
     if (valid && q < a.nload) {
@@ -172,6 +212,37 @@ __global__ void __launch_bounds__(GFFT_THREADS) k_gfft(GLaunch a) {
     __syncthreads();
     gfft_lds(a.g, +1, src, dst, t, T);
   }
+  if (a.ilv_out) {
+    // every line ends in the same buffer of its pair: this thread's offset holds for all of them
+    const int fo = (int)(src - (g_lds + (size_t)sub * 2 * L));
+    for (int e = threadIdx.x; e < lpw * L; e += GFFT_THREADS) {
+      const int ls = e % lpw, k = e / lpw;
+      if (L0 + ls >= nlines) continue;
+      const long long sl = a.n_s > 1 ? s0 + ls : 0, il = a.n_s > 1 ? i0 : i0 + ls;
+      const long long r = (long long)k * a.rk + sl * a.rs;
+      if (r >= a.nstore) continue;
+      float2 v = g_lds[(size_t)ls * 2 * L + fo + k];
+      if (a.twn) {
+        const long long e2 = ((long long)k * sl) % a.twn;
+        if (a.twt) {
+          v = gmul(v, gtw(a.twt, (int)e2, a.twdir));
+        } else {
+          double sn, cs;
+          sincos((double)a.twdir * 2.0 * M_PI * (double)e2 / (double)a.twn, &sn, &cs);
+          v = gmul(v, make_float2((float)cs, (float)sn));
+        }
+      }
+      if (a.post) v = gmul(v, a.post[r]);
+      v.x *= a.scale;
+      v.y *= a.scale;
+      const long long obl = o0 * a.oos + il * a.ois;
+      if (a.out_mode == 1)
+        ((float*)a.out)[obl + r * a.oes] = v.x;
+      else
+        ((float2*)a.out)[obl + r * a.oes] = v;
+    }
+    return;
+  }
   if (!valid) return;
   const long long ob = o * a.oos + i * a.ois;
   for (int k = t; k < L; k += T) {
@@ -180,9 +251,13 @@ __global__ void __launch_bounds__(GFFT_THREADS) k_gfft(GLaunch a) {
     float2 v = src[k];
     if (a.twn) {
       const long long e = ((long long)k * s) % a.twn;
-      double sn, cs;
-      sincos((double)a.twdir * 2.0 * M_PI * (double)e / (double)a.twn, &sn, &cs);
-      v = gmul(v, make_float2((float)cs, (float)sn));
+      if (a.twt) {
+        v = gmul(v, gtw(a.twt, (int)e, a.twdir));
+      } else {
+        double sn, cs;
+        sincos((double)a.twdir * 2.0 * M_PI * (double)e / (double)a.twn, &sn, &cs);
+        v = gmul(v, make_float2((float)cs, (float)sn));
+      }
     }
     if (a.post) v = gmul(v, a.post[r]);
     v.x *= a.scale;
@@ -202,6 +277,20 @@ int threads_per_line(int L) {
 
 hipError_t launch_g(GLaunch a, hipStream_t st) {
   a.T = threads_per_line(a.g.L);
+  // lines adjacent in memory (strided transforms: columns, four-step passes): up to 16 lines per
+  // workgroup, walked interleaved, so that their loads and stores coalesce
+  const long long run = a.n_s > 1 ? a.n_s : a.n_i;
+  const long long din = a.n_s > 1 ? a.qs * a.ies : a.iis, dout = a.n_s > 1 ? a.rs * a.oes : a.ois;
+  a.ilv_in = a.ilv_out = 0;
+  if (din == 1 || dout == 1) {
+    int l = 1;
+    while (l < 16 && 2 * l * a.g.L <= GFFT_LDS_MAX && run % (2 * l) == 0) l *= 2;
+    if (l > 1) {
+      a.T = GFFT_THREADS / l;
+      a.ilv_in = din == 1;
+      a.ilv_out = dout == 1;
+    }
+  }
   const long long lpw = GFFT_THREADS / a.T;
   const long long nlines = a.n_o * a.n_i * a.n_s;
   if (nlines <= 0) return hipSuccess;
@@ -314,6 +403,15 @@ hipError_t gaxis_build(GAxis& a, int n, std::vector<void*>& allocs) {
     if (!l1) return hipErrorInvalidValue;
     e = build_radix(a.g1, l1, allocs);
     if (e == hipSuccess) e = build_radix(a.g2, a.L / l1, allocs);
+    if (e == hipSuccess) {
+      // the L-point twiddle table of the four-step (float32 of float64 values, 8 L bytes)
+      std::vector<float2> h(a.L);
+      for (int t = 0; t < a.L; ++t) {
+        const double ang = -2.0 * M_PI * (double)t / (double)a.L;
+        h[t] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+      }
+      e = upload(h, &a.tw4, allocs);
+    }
   }
   if (e != hipSuccess || !a.blu) return e;
   // chirp c_d[t] = exp(-/+ i pi (t^2 mod 2n) / n); kernel b[m] = conj(c_d[|m|]) for |m| < n on the
@@ -408,6 +506,7 @@ hipError_t gfft_run(const GAxis& a, int mode, int dir, const void* in, const GLi
   A.post = nullptr;
   A.twn = L;
   A.twdir = A.dir;
+  A.twt = a.tw4;
   A.scale = 1.0f;
   // pass B: L2-point FFTs over j2 of t1[k1 L2 + j2]; result X[k1 + L1 k2]
   GLaunch B = base;
