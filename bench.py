@@ -438,8 +438,9 @@ def host_cpu():
 
 def time_in_step(plan, uh, prime, seq, steps=12, skip=2):
     """Average duration (ms) of each kernel of the step, timed inside the step's own schedule:
-    every sub-batch runs `steps` steps launched pass by pass in the step's order, with HIP events
-    recorded between the launches on the stream they run on.  The sub-batches run one after the
+    every sub-batch runs `steps` steps launched pass by pass in the step's order, each kernel
+    recording a pair of HIP events at its own start and end (hipExtLaunchKernelGGL through
+    tfsm_launch_pass_timed, so launch gaps are not counted).  The sub-batches run one after the
     other on torch's current stream, so each duration is the kernel's own (as a profiler that
     serialises kernels sees it); the engine leg overlaps two sub-batches where the plan uses two
     streams, which the step-level fraction reflects.  The first `skip` steps of every sub-batch are
@@ -460,19 +461,22 @@ def time_in_step(plan, uh, prime, seq, steps=12, skip=2):
         s.wait_event(fork)
         with torch.cuda.stream(s):
             plan.launch_pass(u, *prime, sub=k)
-            ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(seq) + 1)] for _ in range(steps)]
+            ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in seq]
+                  for _ in range(steps)]
+            for pair in (p for row in ev for p in row):  # torch creates an event at its first record
+                for e in pair:
+                    e.record(s)
             for i in range(steps):
-                ev[i][0].record(s)
                 for j, (_, pid, st) in enumerate(seq):
-                    plan.launch_pass(u, pid, st, sub=k)
-                    ev[i][j + 1].record(s)
+                    # the kernel itself records the pair at its start and end (tfsm_launch_pass_timed)
+                    plan.launch_pass(u, pid, st, sub=k, events=ev[i][j])
         evs.append(ev)
     torch.cuda.synchronize()
     tot, cnt = {}, {}
     for ev in evs:
         for i in range(skip, steps):
             for j, (name, _, _) in enumerate(seq):
-                tot[name] = tot.get(name, 0.0) + ev[i][j].elapsed_time(ev[i][j + 1])
+                tot[name] = tot.get(name, 0.0) + ev[i][j][0].elapsed_time(ev[i][j][1])
                 cnt[name] = cnt.get(name, 0) + 1
     return {n: tot[n] / cnt[n] for n in tot}
 
@@ -639,8 +643,8 @@ def main():
         "step_bytes_model": step_bytes,
         "step_frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "streams": plan.streams,
-        "timing": ("in-step: HIP events between the launches of every sub-batch's steps, the sub-batches "
-                   "one after the other on one stream (each kernel's own duration)" +
+        "timing": ("in-step: HIP events recorded by each kernel at its start and end, every sub-batch's steps in "
+                   "order, the sub-batches one after the other on one stream (each kernel's own duration)" +
                    ("; the engine leg overlaps two sub-batches on two streams (step_frac is the step-level "
                     "fraction)" if plan.streams == 2 else "")),
     }

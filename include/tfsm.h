@@ -308,6 +308,13 @@ int tfsm_solve(tfsm_plan* plan, const void* uh, void* out, void* stream);
 int tfsm_launch_pass(tfsm_plan* plan, void* uh, void* ws, int32_t pass_id, int32_t stage, int32_t sub,
                      void* stream);
 
+/* tfsm_launch_pass with its (first) kernel timed: ev_start and ev_stop (hipEvent_t, created by the
+ * caller with timing enabled) are recorded at that kernel's start and end by the dispatch itself
+ * (hipExtLaunchKernelGGL), so hipEventElapsedTime gives the kernel's own duration, without the
+ * launch gap an event recorded between launches includes.  (No reference counterpart: profiling.) */
+int tfsm_launch_pass_timed(tfsm_plan* p, void* uh, void* ws, int32_t pass_id, int32_t stage, int32_t sub,
+                           void* ev_start, void* ev_stop, void* stream);
+
 /* Bytes of the adjoint workspace tfsm_adjoint_step needs (stage arrays, transposed-Jacobian
  * fields and products). */
 int64_t tfsm_plan_adjoint_workspace_bytes(const tfsm_plan* plan);

@@ -88,6 +88,7 @@ SIGNATURES = {
     "tfsm_spectral_map": (ctypes.c_int, [VP, I32, ctypes.c_float, VP, VP, VP]),
     "tfsm_pointwise_products": (ctypes.c_int, [VP, VP, VP, VP]),
     "tfsm_launch_pass": (ctypes.c_int, [VP, VP, VP, I32, I32, I32, VP]),
+    "tfsm_launch_pass_timed": (ctypes.c_int, [VP, VP, VP, I32, I32, I32, VP, VP, VP]),
     "tfsm_plan_adjoint_workspace_bytes": (I64, [VP]),
     "tfsm_adjoint_prepare": (ctypes.c_int, [VP, I32, VP]),
     "tfsm_adjoint_step": (ctypes.c_int, [VP, VP, VP, VP, VP, VP, VP, VP]),

@@ -15,6 +15,10 @@
 
 using namespace tfsm;
 
+namespace tfsm {
+thread_local hipEvent_t g_launch_ev[2] = {nullptr, nullptr};
+}
+
 // Workspace of tfsm_step_noalloc: a device array of fixed size, one copy per GPU, loaded with the
 // code object and held until the process ends.  Host code reaches it only through the address
 // hipGetSymbolAddress returns for the current device (scratch_on_device below); the name itself is
@@ -1941,6 +1945,16 @@ int tfsm_launch_pass(tfsm_plan* p, void* uh, void* ws, int32_t pass_id, int32_t 
     return TFSM_OK;
   }
   return fail(TFSM_E_INVALID, "unknown pass for this plan");
+}
+
+int tfsm_launch_pass_timed(tfsm_plan* p, void* uh, void* ws, int32_t pass_id, int32_t stage, int32_t sub,
+                           void* ev_start, void* ev_stop, void* stream) {
+  if (!ev_start || !ev_stop) return fail(TFSM_E_INVALID, "null event");
+  g_launch_ev[0] = (hipEvent_t)ev_start;
+  g_launch_ev[1] = (hipEvent_t)ev_stop;
+  const int rc = tfsm_launch_pass(p, uh, ws, pass_id, stage, sub, stream);
+  g_launch_ev[0] = g_launch_ev[1] = nullptr;
+  return rc;
 }
 
 int64_t tfsm_plan_adjoint_workspace_bytes(const tfsm_plan* plan) {

@@ -8,12 +8,28 @@
 //   2D fields      [B][NF][N0][PF] float2: inverse-x-transformed fields, ky <= kmax1 only
 //   2D products    [B][NP][N0][P]  float2: forward-y-transformed physical products
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "fft_core.hpp"
 
 namespace tfsm {
+
+// Kernel timing (tfsm_launch_pass_timed): when set, the next pass launch records these events at
+// its kernel's start and end (hipExtLaunchKernelGGL, no launch gap inside the interval) and clears
+// them; every other launch is a plain one.
+extern thread_local hipEvent_t g_launch_ev[2];
+#define TFSM_LAUNCH(K, G, B, SH, ST, ...)                                                  \
+  do {                                                                                     \
+    if (::tfsm::g_launch_ev[0]) {                                                           \
+      hipEvent_t e0_ = ::tfsm::g_launch_ev[0], e1_ = ::tfsm::g_launch_ev[1];                \
+      ::tfsm::g_launch_ev[0] = ::tfsm::g_launch_ev[1] = nullptr;                            \
+      hipExtLaunchKernelGGL(K, G, B, SH, ST, e0_, e1_, 0, __VA_ARGS__);                     \
+    } else {                                                                               \
+      hipLaunchKernelGGL(K, G, B, SH, ST, __VA_ARGS__);                                     \
+    }                                                                                      \
+  } while (0)
 
 enum TermKind : int { TERM_NONE = 0, TERM_CONV = 1, TERM_VC = 2, TERM_NSP = 3, TERM_KS = 4, TERM_CONS = 6 };
 

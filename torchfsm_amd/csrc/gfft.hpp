@@ -29,6 +29,7 @@ struct GRadix {
   int L = 0;
   int nr = 0;
   int r[GFFT_MAX_RADICES] = {};
+  int pow2 = 0;  // every radix is 2 or 4: stage indices by shifts and masks
   const float2* tw = nullptr;
 };
 

@@ -92,10 +92,13 @@ __device__ __forceinline__ void gbfly(float2 (&v)[7], const float2* tw, int L, i
 // one Stockham stage: Ns points already combined; butterfly b reads b + q L/R, twiddles by
 // w^(q (b mod Ns)) with w = exp(dir 2 pi i / (Ns R)), writes (b / Ns) Ns R + (b mod Ns) + q Ns
 template <int R>
-__device__ void gstage(const float2* src, float2* dst, const float2* tw, int L, int Ns, int dir, int t, int T) {
+__device__ void gstage(const float2* src, float2* dst, const float2* tw, int L, int Ns, int nsl, int dir, int t,
+                       int T) {
   const int nb = L / R, span = L / (Ns * R);
   for (int b = t; b < nb; b += T) {
-    const int k = b % Ns;
+    // nsl = log2(Ns) for power-of-two plans (no integer division), else -1
+    const int k = nsl >= 0 ? (b & (Ns - 1)) : b % Ns;
+    const int grp = nsl >= 0 ? (b >> nsl) : b / Ns;
     float2 v[7];
 #pragma unroll
     for (int q = 0; q < R; ++q) v[q] = src[b + q * nb];
@@ -104,7 +107,7 @@ __device__ void gstage(const float2* src, float2* dst, const float2* tw, int L, 
       for (int q = 1; q < R; ++q) v[q] = gmul(v[q], gtw(tw, q * k * span, dir));
     }
     gbfly<R>(v, tw, L, dir);
-    const int d0 = (b / Ns) * Ns * R + k;
+    const int d0 = grp * Ns * R + k;
 #pragma unroll
     for (int q = 0; q < R; ++q) dst[d0 + q * Ns] = v[q];
   }
@@ -112,21 +115,22 @@ __device__ void gstage(const float2* src, float2* dst, const float2* tw, int L, 
 
 // every stage of g; the result ends in *src (every thread of the workgroup takes part)
 __device__ void gfft_lds(const GRadix& g, int dir, float2*& src, float2*& dst, int t, int T) {
-  int Ns = 1;
+  int Ns = 1, nsl = g.pow2 ? 0 : -1;
   for (int s = 0; s < g.nr; ++s) {
     const int R = g.r[s];
     switch (R) {
-      case 4: gstage<4>(src, dst, g.tw, g.L, Ns, dir, t, T); break;
-      case 2: gstage<2>(src, dst, g.tw, g.L, Ns, dir, t, T); break;
-      case 3: gstage<3>(src, dst, g.tw, g.L, Ns, dir, t, T); break;
-      case 5: gstage<5>(src, dst, g.tw, g.L, Ns, dir, t, T); break;
-      default: gstage<7>(src, dst, g.tw, g.L, Ns, dir, t, T); break;
+      case 4: gstage<4>(src, dst, g.tw, g.L, Ns, nsl, dir, t, T); break;
+      case 2: gstage<2>(src, dst, g.tw, g.L, Ns, nsl, dir, t, T); break;
+      case 3: gstage<3>(src, dst, g.tw, g.L, Ns, nsl, dir, t, T); break;
+      case 5: gstage<5>(src, dst, g.tw, g.L, Ns, nsl, dir, t, T); break;
+      default: gstage<7>(src, dst, g.tw, g.L, Ns, nsl, dir, t, T); break;
     }
     __syncthreads();
     float2* x = src;
     src = dst;
     dst = x;
     Ns *= R;
+    if (nsl >= 0) nsl += R == 4 ? 2 : 1;
   }
 }
 
@@ -134,7 +138,9 @@ __global__ void __launch_bounds__(GFFT_THREADS) k_gfft(GLaunch a) {
   extern __shared__ float2 g_lds[];
   const int T = a.T, L = a.g.L;
   const int lpw = GFFT_THREADS / T;
-  const int sub = threadIdx.x / T, t = threadIdx.x % T;
+  // T and lpw are powers of two
+  const int tsh = __ffs(T) - 1, lsh = __ffs(lpw) - 1;
+  const int sub = threadIdx.x >> tsh, t = threadIdx.x & (T - 1);
   float2* src = g_lds + (size_t)sub * 2 * L;
   float2* dst = src + L;
   const long long nlines = a.n_o * a.n_i * a.n_s;
@@ -157,7 +163,7 @@ __global__ void __launch_bounds__(GFFT_THREADS) k_gfft(GLaunch a) {
   }
   if (a.ilv_in) {
     for (int e = threadIdx.x; e < lpw * L; e += GFFT_THREADS) {
-      const int ls = e % lpw, j = e / lpw;
+      const int ls = e & (lpw - 1), j = e >> lsh;
       float2 v = make_float2(0.f, 0.f);
       const long long sl = a.n_s > 1 ? s0 + ls : 0, il = a.n_s > 1 ? i0 : i0 + ls;
       const long long q = (long long)j * a.qj + sl * a.qs;
@@ -216,14 +222,15 @@ __global__ void __launch_bounds__(GFFT_THREADS) k_gfft(GLaunch a) {
     // every line ends in the same buffer of its pair: this thread's offset holds for all of them
     const int fo = (int)(src - (g_lds + (size_t)sub * 2 * L));
     for (int e = threadIdx.x; e < lpw * L; e += GFFT_THREADS) {
-      const int ls = e % lpw, k = e / lpw;
+      const int ls = e & (lpw - 1), k = e >> lsh;
       if (L0 + ls >= nlines) continue;
       const long long sl = a.n_s > 1 ? s0 + ls : 0, il = a.n_s > 1 ? i0 : i0 + ls;
       const long long r = (long long)k * a.rk + sl * a.rs;
       if (r >= a.nstore) continue;
       float2 v = g_lds[(size_t)ls * 2 * L + fo + k];
       if (a.twn) {
-        const long long e2 = ((long long)k * sl) % a.twn;
+        long long e2 = (long long)k * sl;  // k < L1, s < L2: below twn = L1 L2
+        if (e2 >= a.twn) e2 %= a.twn;
         if (a.twt) {
           v = gmul(v, gtw(a.twt, (int)e2, a.twdir));
         } else {
@@ -250,7 +257,8 @@ __global__ void __launch_bounds__(GFFT_THREADS) k_gfft(GLaunch a) {
     if (r >= a.nstore) continue;
     float2 v = src[k];
     if (a.twn) {
-      const long long e = ((long long)k * s) % a.twn;
+      long long e = (long long)k * s;  // k < L1, s < L2: below twn = L1 L2
+      if (e >= a.twn) e %= a.twn;
       if (a.twt) {
         v = gmul(v, gtw(a.twt, (int)e, a.twdir));
       } else {
@@ -304,6 +312,7 @@ hipError_t launch_g(GLaunch a, hipStream_t st) {
 bool radices(int L, GRadix& g) {
   g.L = L;
   g.nr = 0;
+  g.pow2 = (L & (L - 1)) == 0 ? 1 : 0;
   int n = L;
   while (n % 4 == 0 && g.nr < GFFT_MAX_RADICES) {
     g.r[g.nr++] = 4;

@@ -640,15 +640,15 @@ inline hipError_t launch_col_t(const Col2Args& a, int nb, int stage, hipStream_t
   const dim3 grid(tiles * nb), block(Fft<N0, VAR>::NT * V);
   if constexpr (ColStageKernels<N0, V, TERM, VAR>::value) {
     switch (stage) {
-      case ST_PRO: hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR, ST_PRO>), grid, block, 0, st, b, stage); break;
-      case ST_R4_E0: hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR, ST_R4_E0>), grid, block, 0, st, b, stage); break;
-      case ST_R4_E1: hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR, ST_R4_E1>), grid, block, 0, st, b, stage); break;
-      case ST_R4_E2: hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR, ST_R4_E2>), grid, block, 0, st, b, stage); break;
-      case ST_R4_E3: hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR, ST_R4_E3>), grid, block, 0, st, b, stage); break;
-      default: hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR>), grid, block, 0, st, b, stage);
+      case ST_PRO: TFSM_LAUNCH((k2_col<N0, V, TERM, VAR, ST_PRO>), grid, block, 0, st, b, stage); break;
+      case ST_R4_E0: TFSM_LAUNCH((k2_col<N0, V, TERM, VAR, ST_R4_E0>), grid, block, 0, st, b, stage); break;
+      case ST_R4_E1: TFSM_LAUNCH((k2_col<N0, V, TERM, VAR, ST_R4_E1>), grid, block, 0, st, b, stage); break;
+      case ST_R4_E2: TFSM_LAUNCH((k2_col<N0, V, TERM, VAR, ST_R4_E2>), grid, block, 0, st, b, stage); break;
+      case ST_R4_E3: TFSM_LAUNCH((k2_col<N0, V, TERM, VAR, ST_R4_E3>), grid, block, 0, st, b, stage); break;
+      default: TFSM_LAUNCH((k2_col<N0, V, TERM, VAR>), grid, block, 0, st, b, stage);
     }
   } else {
-    hipLaunchKernelGGL((k2_col<N0, V, TERM, VAR>), grid, block, 0, st, b, stage);
+    TFSM_LAUNCH((k2_col<N0, V, TERM, VAR>), grid, block, 0, st, b, stage);
   }
   return hipGetLastError();
 }
@@ -658,7 +658,7 @@ inline hipError_t launch_row_t(const Row2Args& a, hipStream_t st) {
   constexpr int NT = Fft<N1, VAR>::NT;
   constexpr int ROWS = (NT >= WGT) ? 1 : WGT / NT;
   const int rows = a.B * a.N0;
-  hipLaunchKernelGGL((k2_row<N1, ROWS, TERM, VAR>), dim3((rows + ROWS - 1) / ROWS), dim3(NT * ROWS), 0, st, a);
+  TFSM_LAUNCH((k2_row<N1, ROWS, TERM, VAR>), dim3((rows + ROWS - 1) / ROWS), dim3(NT * ROWS), 0, st, a);
   return hipGetLastError();
 }
 

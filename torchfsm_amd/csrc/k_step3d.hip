@@ -385,11 +385,11 @@ static hipError_t launch_col3d_t(const Col3Args& a, int nb, int stage, hipStream
   Col3Args b = a;
   b.nb = nb;
   if (a.term == TERM_NSP)
-    hipLaunchKernelGGL((k3_col<NN, V, VR, TERM_NSP>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b, stage);
+    TFSM_LAUNCH((k3_col<NN, V, VR, TERM_NSP>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b, stage);
   else if (a.term == TERM_KS)
-    hipLaunchKernelGGL((k3_col<NN, V, VR, TERM_KS>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b, stage);
+    TFSM_LAUNCH((k3_col<NN, V, VR, TERM_KS>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b, stage);
   else
-    hipLaunchKernelGGL((k3_col<NN, V, VR, TERM_CONV>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b,
+    TFSM_LAUNCH((k3_col<NN, V, VR, TERM_CONV>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b,
                        stage);
   return hipGetLastError();
 }
@@ -399,9 +399,9 @@ static hipError_t launch_yinv3d_t(const Yinv3Args& a, hipStream_t st) {
   constexpr int V = 16;
   const int nzt = a.PF2 / V;
   if (a.nc == 1)
-    hipLaunchKernelGGL((k3_yinv<NN, V, VR, 1>), dim3(nzt * a.N0 * a.nb), dim3(Fft<NN, VR>::NT * V), 0, st, a);
+    TFSM_LAUNCH((k3_yinv<NN, V, VR, 1>), dim3(nzt * a.N0 * a.nb), dim3(Fft<NN, VR>::NT * V), 0, st, a);
   else
-    hipLaunchKernelGGL((k3_yinv<NN, V, VR, 3>), dim3(nzt * a.N0 * a.nb), dim3(Fft<NN, VR>::NT * V), 0, st, a);
+    TFSM_LAUNCH((k3_yinv<NN, V, VR, 3>), dim3(nzt * a.N0 * a.nb), dim3(Fft<NN, VR>::NT * V), 0, st, a);
   return hipGetLastError();
 }
 
@@ -446,9 +446,9 @@ static hipError_t launch_row3d_t(const Row3Args& a, hipStream_t st) {
   const size_t lds = (size_t)(ks ? 3 : 9) * R * a.PF2 * sizeof(float2);
   if (lds > 96 * 1024) return hipErrorInvalidValue;
   if (ks)
-    hipLaunchKernelGGL((k3_row<NN, R, VR, TERM_KS>), dim3((unsigned)(rows / R)), dim3(NT * R), lds, st, a);
+    TFSM_LAUNCH((k3_row<NN, R, VR, TERM_KS>), dim3((unsigned)(rows / R)), dim3(NT * R), lds, st, a);
   else
-    hipLaunchKernelGGL((k3_row<NN, R, VR, TERM_CONV>), dim3((unsigned)(rows / R)), dim3(NT * R), lds, st, a);
+    TFSM_LAUNCH((k3_row<NN, R, VR, TERM_CONV>), dim3((unsigned)(rows / R)), dim3(NT * R), lds, st, a);
   return hipGetLastError();
 }
 

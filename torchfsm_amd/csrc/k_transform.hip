@@ -552,10 +552,10 @@ hipError_t launch_cols_c2c(int n, int dir, const float2* in, float2* out, size_t
       float2* ob = out + o0 * outer_stride;
       const unsigned ops = (unsigned)(outer_per_sample ? outer_per_sample : 1);
       if (dir < 0)
-        hipLaunchKernelGGL((k_cols_c2c<NN, V, -1>), grid, dim3(Fft<NN>::NT * V), 0, st, ib, ob, outer_stride,
+        TFSM_LAUNCH((k_cols_c2c<NN, V, -1>), grid, dim3(Fft<NN>::NT * V), 0, st, ib, ob, outer_stride,
                            col_stride, ninner, scale, tw, mask, b0, (unsigned)o0, ops);
       else
-        hipLaunchKernelGGL((k_cols_c2c<NN, V, 1>), grid, dim3(Fft<NN>::NT * V), 0, st, ib, ob, outer_stride,
+        TFSM_LAUNCH((k_cols_c2c<NN, V, 1>), grid, dim3(Fft<NN>::NT * V), 0, st, ib, ob, outer_stride,
                            col_stride, ninner, scale, tw, mask, b0, (unsigned)o0, ops);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;

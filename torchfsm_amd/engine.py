@@ -456,14 +456,21 @@ class SpectralPlan:
             check(lib.tfsm_solve(self._plan, _ptr(uh), _ptr(out), _stream(self.device)), "solve")
         return out
 
-    def launch_pass(self, uh: torch.Tensor, pass_id: int, stage: int = 0, sub: int = 0):
+    def launch_pass(self, uh: torch.Tensor, pass_id: int, stage: int = 0, sub: int = 0, events=None):
         """One row (0) / column (1, stage code) / 1D step (2) / 3D (3..7) pass over sub-batch `sub`,
-        for kernel timing."""
+        for kernel timing.  events: optional (start, stop) torch.cuda.Event pair (enable_timing) that
+        the pass's kernel records at its own start and end (tfsm_launch_pass_timed)."""
         self._check_state(uh)
         ws = self.workspace()
         with torch.cuda.device(self.device):
-            check(lib.tfsm_launch_pass(self._plan, _ptr(uh), _ptr(ws), int(pass_id), int(stage), int(sub),
-                                       _stream(self.device)), "launch_pass")
+            if events is None:
+                check(lib.tfsm_launch_pass(self._plan, _ptr(uh), _ptr(ws), int(pass_id), int(stage), int(sub),
+                                           _stream(self.device)), "launch_pass")
+            else:
+                e0, e1 = events
+                check(lib.tfsm_launch_pass_timed(self._plan, _ptr(uh), _ptr(ws), int(pass_id), int(stage), int(sub),
+                                                 ctypes.c_void_p(e0.cuda_event), ctypes.c_void_p(e1.cuda_event),
+                                                 _stream(self.device)), "launch_pass_timed")
 
     def graph(self, uh: torch.Tensor, nsteps: int, counts: Optional[torch.Tensor] = None) -> "StepGraph":
         return StepGraph(self, uh, nsteps, counts)
