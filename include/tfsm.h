@@ -136,10 +136,13 @@ int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int
  * (No reference counterpart: launch configuration.) */
 int tfsm_plan_get_tuning(const tfsm_plan* plan, int32_t* out);
 
-/* Run alternate sub-batches of tfsm_step (see tfsm_plan_set_tuning) on the caller's stream and
- * on an auxiliary stream of the plan (nstreams = 2), forked from and joined back into the
- * caller's stream with events, so that their kernels overlap; 1 (default) keeps one stream.
- * Results are identical.  (No reference counterpart: launch configuration.) */
+/* Spread the sub-batches of tfsm_step (see tfsm_plan_set_tuning) over nstreams streams
+ * (1 <= nstreams <= TFSM_MAX_STREAMS): sub-batch k runs on the caller's stream when
+ * k % nstreams == 0 and otherwise on one of the plan's auxiliary streams (default priority),
+ * forked from and joined back into the caller's stream with events, so that their kernels
+ * overlap; 1 (the default, except 2 for 512^2 VorticityConvection with 32 or more samples) keeps
+ * one stream.  Results are identical.  (No reference counterpart: launch configuration.) */
+#define TFSM_MAX_STREAMS 4
 int tfsm_plan_set_streams(tfsm_plan* plan, int32_t nstreams);
 
 /* Step 2D plans of square 32^2 or 64^2 meshes with Convection or VorticityConvection with the

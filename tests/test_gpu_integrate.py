@@ -286,8 +286,8 @@ def test_every_launch_shape_is_exact(dev):
 
 def test_sub_batch_blocking_is_bit_identical(dev):
     """Temporal blocking (sub-batches stepped through all steps in turn, on one stream or
-    alternating over two) changes the launch order only: states and recorded frames are
-    bit-identical to whole-batch stepping."""
+    spread over two to four streams) changes the launch order only: states and recorded frames
+    are bit-identical to whole-batch stepping."""
     import torchfsm_amd as tfa
     from torchfsm_amd import field, pde
     from torchfsm_amd.workloads import ns_initial_vorticity
@@ -299,7 +299,7 @@ def test_sub_batch_blocking_is_bit_identical(dev):
     plan = op._prepare(16, dev, 1e-3)
     w0 = ns_initial_vorticity(64, 16, seed=3).to(dev)
     outs = []
-    for sb, ns in ((0, 1), (8, 1), (24, 1), (8, 2), (4, 2)):
+    for sb, ns in ((0, 1), (8, 1), (24, 1), (8, 2), (4, 2), (4, 3), (4, 4), (8, 4)):
         plan.set_tuning(16, 0, 0, 256, sb)
         plan.set_streams(ns)
         uh = plan.forward(w0)
@@ -310,18 +310,34 @@ def test_sub_batch_blocking_is_bit_identical(dev):
     for uh, traj in outs[1:]:
         assert torch.equal(uh, outs[0][0])
         assert torch.equal(traj, outs[0][1])
-    # two streams under HIP-graph capture (fork/join by events): replays equal eager stepping
-    plan.set_tuning(16, 0, 0, 256, 4)
-    plan.set_streams(2)
-    ug = plan.forward(w0)
-    ue = ug.clone()
-    g = plan.graph(ug, 3)
-    for _ in range(3):
-        g.launch()
-        plan.step(ue, 3)
-        torch.cuda.synchronize()
-        assert torch.equal(ug, ue)
-    g.close()
+    # 40 steps on several streams are queued in rounds of 1, 16, 16 and 7 steps per sub-batch:
+    # states and frames recorded across the round boundaries equal one stream's
+    rec = torch.full((41,), -1, dtype=torch.int32, device=dev)
+    rec[0], rec[1], rec[16], rec[17], rec[33], rec[40] = 0, 1, 2, 3, 4, 5
+    long = []
+    for ns in (1, 2, 3):
+        plan.set_tuning(16, 0, 0, 256, 4)
+        plan.set_streams(ns)
+        uh = plan.forward(w0)
+        traj = torch.zeros((16, 6) + tuple(plan.spec_shape[1:]), dtype=torch.complex64, device=dev)
+        plan.step(uh, 40, rec, traj)
+        long.append((uh, traj))
+    for uh, traj in long[1:]:
+        assert torch.equal(uh, long[0][0])
+        assert torch.equal(traj, long[0][1])
+    # several streams under HIP-graph capture (fork/join by events): replays equal eager stepping
+    for ns in (2, 4):
+        plan.set_tuning(16, 0, 0, 256, 4)
+        plan.set_streams(ns)
+        ug = plan.forward(w0)
+        ue = ug.clone()
+        g = plan.graph(ug, 3)
+        for _ in range(3):
+            g.launch()
+            plan.step(ue, 3)
+            torch.cuda.synchronize()
+            assert torch.equal(ug, ue)
+        g.close()
     plan.set_streams(1)
 
 

@@ -135,10 +135,10 @@ struct tfsm_plan {
   int row_var = 0;   // radix-plan variant of the row transforms
   int row_wg = 256;  // row-kernel workgroup size
   int sub_batch = 0; // samples stepped together through all steps (0 = whole batch)
-  int nstreams = 1;  // 2: alternate sub-batches run on the caller's stream and an auxiliary one
+  int nstreams = 1;  // > 1: sub-batch k runs on stream k % nstreams (0: the caller's, i: aux[i - 1])
   bool persistent = false;  // 2D: step with the persistent per-sample kernel k2_run
-  hipStream_t aux = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t aux[TFSM_MAX_STREAMS - 1] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[TFSM_MAX_STREAMS - 1] = {};
   int PF = 0, NF = 0;
   int K1 = 0;  // 3D: kept ky values (2 kmax1 + 1)
   size_t off_A1 = 0, off_A2 = 0, off_P = 0, off_G = 0, off_G2 = 0, ws_bytes = 0;
@@ -775,44 +775,60 @@ int step_impl(tfsm_plan* p, void* uh, int nsteps, const int* rec_slot, void* tra
     return TFSM_OK;
   }
   if (p->D >= 2) {
-    // Temporal blocking: a sub-batch of samples goes through all nsteps before the next one, so
+    // Temporal blocking: a sub-batch of samples goes through many steps before the next one, so
     // its state, auxiliary stages, products and fields stay resident in the last-level cache
     // across the 2 * evals launches of every step.  Samples are independent: same results.
-    // With two streams, alternate sub-batches run concurrently (fork from and join back into the
+    // With several streams, sub-batch k runs on stream k % ns (fork from and join back into the
     // caller's stream with events, which also holds under graph capture): their kernels overlap
     // and fill each other's latency.  Sub-batches use disjoint parts of the workspace.
     const int nev = scheme_evals(p->scheme);
     const int sb = (p->sub_batch > 0 && p->sub_batch < p->B) ? p->sub_batch : p->B;
     const size_t per = (size_t)p->C * p->modes;
-    const bool two = p->nstreams == 2 && sb < p->B && p->aux;
-    if (two) {
+    // streams in use: no more than there are sub-batches
+    const int nsub = (p->B + sb - 1) / sb;
+    const int ns = p->nstreams < nsub ? p->nstreams : nsub;
+    if (ns > 1) {
       HIP_TRY(hipEventRecord(p->ev_fork, st), "fork event");
-      HIP_TRY(hipStreamWaitEvent(p->aux, p->ev_fork, 0), "fork wait");
+      for (int i = 0; i + 1 < ns; ++i) HIP_TRY(hipStreamWaitEvent(p->aux[i], p->ev_fork, 0), "fork wait");
     }
+    // On several streams the steps are queued in rounds (every sub-batch gets its next steps
+    // before the following round starts): a stream's hardware queue holds a bounded number of
+    // packets and the host blocks once one is full, so queuing all steps of one sub-batch first
+    // would leave the other streams idle until that sub-batch is nearly done.  A first round of
+    // one step starts every stream early.  One stream: a single round, every step of a
+    // sub-batch before the next.
+    const int round0 = ns > 1 ? 1 : nsteps, round = ns > 1 ? 16 : nsteps;
     const hipStream_t caller = st;
     int rc = TFSM_OK;
-    for (int b0 = 0, k = 0; b0 < p->B && !rc; b0 += sb, ++k) {
-      st = (two && (k & 1)) ? p->aux : caller;
-      const int nb = (p->B - b0) < sb ? (p->B - b0) : sb;
-      float2* Ub = U + (size_t)b0 * per;
-      float2* Tb = traj ? (float2*)traj + (size_t)b0 * T * per : nullptr;
-      StepMask m;
-      m.act = act;
-      m.it = 0;
-      rc = run_eval(p, U, (char*)ws, ST_PRO, b0, nb, st, m);
-      for (int i = 0; i < nsteps && !rc; ++i) {
-        m.it = i;
-        if (rec_slot) rc = launch_status(launch_record(Ub, Tb, rec_slot, i, nb, per, T, st), "record launch");
-        for (int e = 0; e < nev && !rc; ++e)
-          rc = run_eval(p, U, (char*)ws, scheme_stage(p->scheme, e), b0, nb, st, m);
+    for (int s0 = 0, r = 0; !rc; ++r) {
+      const int len = r == 0 ? round0 : round;
+      const int s1 = nsteps - s0 < len ? nsteps : s0 + len;
+      for (int b0 = 0, k = 0; b0 < p->B && !rc; b0 += sb, ++k) {
+        st = (k % ns) ? p->aux[k % ns - 1] : caller;
+        const int nb = (p->B - b0) < sb ? (p->B - b0) : sb;
+        float2* Ub = U + (size_t)b0 * per;
+        float2* Tb = traj ? (float2*)traj + (size_t)b0 * T * per : nullptr;
+        StepMask m;
+        m.act = act;
+        m.it = s0;
+        if (s0 == 0) rc = run_eval(p, U, (char*)ws, ST_PRO, b0, nb, st, m);
+        for (int i = s0; i < s1 && !rc; ++i) {
+          m.it = i;
+          if (rec_slot) rc = launch_status(launch_record(Ub, Tb, rec_slot, i, nb, per, T, st), "record launch");
+          for (int e = 0; e < nev && !rc; ++e)
+            rc = run_eval(p, U, (char*)ws, scheme_stage(p->scheme, e), b0, nb, st, m);
+        }
+        if (s1 == nsteps && rec_slot && !rc)
+          rc = launch_status(launch_record(Ub, Tb, rec_slot, nsteps, nb, per, T, st), "record launch");
       }
-      if (rec_slot && !rc) rc = launch_status(launch_record(Ub, Tb, rec_slot, nsteps, nb, per, T, st), "record launch");
+      if (s1 == nsteps) break;
+      s0 = s1;
     }
-    // the join is recorded on every exit, also after a failed launch: work already queued on the
-    // auxiliary stream is then still ordered before anything the caller queues next
-    if (two) {
-      const int jr = launch_status(hipEventRecord(p->ev_join, p->aux), "join event");
-      const int jw = jr ? jr : launch_status(hipStreamWaitEvent(caller, p->ev_join, 0), "join wait");
+    // the joins are recorded on every exit, also after a failed launch: work already queued on the
+    // auxiliary streams is then still ordered before anything the caller queues next
+    for (int i = 0; i + 1 < ns; ++i) {
+      const int jr = launch_status(hipEventRecord(p->ev_join[i], p->aux[i]), "join event");
+      const int jw = jr ? jr : launch_status(hipStreamWaitEvent(caller, p->ev_join[i], 0), "join wait");
       if (!rc) rc = jw;
     }
     return rc;
@@ -1365,12 +1381,7 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
     // sub-batches, and 0.676 for the 16-values-per-thread columns it replaces)
     if (n0 == 512 && term == TERM_VC && p->B >= 32) {
       p->sub_batch = 16;
-      if (hipStreamCreateWithFlags(&p->aux, hipStreamNonBlocking) == hipSuccess &&
-          hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) == hipSuccess &&
-          hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) == hipSuccess)
-        p->nstreams = 2;
-      else
-        rc = hip_fail(hipGetLastError(), "auxiliary stream");
+      rc = tfsm_plan_set_streams(p, 2);
     }
   }
   // small square meshes step with the persistent per-sample kernel (one launch per call)
@@ -1387,12 +1398,14 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
 
 int tfsm_plan_destroy(tfsm_plan* plan) {
   if (!plan) return TFSM_OK;
-  if (plan->aux) {
-    (void)hipStreamSynchronize(plan->aux);
-    (void)hipStreamDestroy(plan->aux);
+  for (int i = 0; i + 1 < TFSM_MAX_STREAMS; ++i) {
+    if (plan->aux[i]) {
+      (void)hipStreamSynchronize(plan->aux[i]);
+      (void)hipStreamDestroy(plan->aux[i]);
+    }
+    if (plan->ev_join[i]) (void)hipEventDestroy(plan->ev_join[i]);
   }
   if (plan->ev_fork) (void)hipEventDestroy(plan->ev_fork);
-  if (plan->ev_join) (void)hipEventDestroy(plan->ev_join);
   for (void* q : plan->allocs) (void)hipFree(q);
   delete plan;
   return TFSM_OK;
@@ -1476,14 +1489,16 @@ int32_t tfsm_plan_general(const tfsm_plan* plan) { return plan && plan->general 
 
 int tfsm_plan_set_streams(tfsm_plan* plan, int32_t nstreams) {
   if (!plan) return fail(TFSM_E_INVALID, "null plan");
-  if (nstreams != 1 && nstreams != 2) return fail(TFSM_E_INVALID, "nstreams must be 1 or 2");
-  if (nstreams == 2 && plan->ks_mean)
+  if (nstreams < 1 || nstreams > TFSM_MAX_STREAMS) return fail(TFSM_E_INVALID, "nstreams must be 1 to 4");
+  if (nstreams > 1 && plan->ks_mean)
     return fail(TFSM_E_INVALID, "KSConvection couples the batch through its mean: one stream only");
-  if (nstreams == 2 && !plan->aux) {
-    // default priority, non-blocking with respect to the null stream
-    HIP_TRY(hipStreamCreateWithFlags(&plan->aux, hipStreamNonBlocking), "auxiliary stream");
+  if (nstreams > 1 && !plan->ev_fork)
     HIP_TRY(hipEventCreateWithFlags(&plan->ev_fork, hipEventDisableTiming), "fork event");
-    HIP_TRY(hipEventCreateWithFlags(&plan->ev_join, hipEventDisableTiming), "join event");
+  for (int i = 0; i + 1 < nstreams; ++i) {
+    if (plan->aux[i]) continue;
+    // default priority, non-blocking with respect to the null stream
+    HIP_TRY(hipStreamCreateWithFlags(&plan->aux[i], hipStreamNonBlocking), "auxiliary stream");
+    HIP_TRY(hipEventCreateWithFlags(&plan->ev_join[i], hipEventDisableTiming), "join event");
   }
   plan->nstreams = nstreams;
   return TFSM_OK;

@@ -192,7 +192,7 @@ class SpectralPlan:
             self._ws = None
 
     def set_streams(self, nstreams: int):
-        """1: one stream; 2: alternate sub-batches overlap on an auxiliary stream (tfsm_plan_set_streams)."""
+        """1: one stream; 2-4: sub-batch k runs on stream k % nstreams, overlapping (tfsm_plan_set_streams)."""
         self._drop_step_graph()
         check(lib.tfsm_plan_set_streams(self._plan, int(nstreams)), "streams")
         self.streams = int(nstreams)
