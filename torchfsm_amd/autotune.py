@@ -49,6 +49,8 @@ def candidates(plan):
 
 def sub_batches(plan):
     b = plan.batch
+    if len(plan.spec_shape) == 5:  # 3D: a few samples per sub-batch already fill the GPU
+        return [s for s in (0, 1, 2, 4) if s == 0 or s < b]
     return [s for s in (0, 8, 16, 24, 32) if s == 0 or (s < b and s % 8 == 0)]
 
 

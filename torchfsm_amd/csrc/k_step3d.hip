@@ -36,8 +36,51 @@ __device__ __forceinline__ int ky_compress(int ky, int N1, int kmax1) {
 }
 
 // ---------------------------------------------------------------- x columns (fused combine)
-template <int N0, int V, int VAR, int TERM>
-__global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int stage) {
+// Fields f0 = u^_c and f0 + 1 = a_x u^_c of one channel's combined spectrum s (kept modes, scaled),
+// inverse x FFT, stored to G1 at the compressed ky index kyc.
+template <int N0, int V, int VAR>
+__device__ __forceinline__ void col3_fields(const Col3Args& a, const float2 (&s)[Fft<N0, VAR>::RMAX], int f0, int b,
+                                            int nc, int kyc, int kz, bool colv, float2* s_buf, const float2* s_tw,
+                                            const float* s_ax, int j, int vi) {
+  using F = Fft<N0, VAR>;
+  constexpr int RMAX = F::RMAX, NT = F::NT;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    float2 v[RMAX];
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) {
+      const int kx = j + m * NT;
+      const int ks = kx <= N0 / 2 ? kx : N0 - kx;
+      const bool keep = colv && kz <= a.kmax2 && ks <= a.kmax0;
+      float2 val = make_float2(0.f, 0.f);
+      if (keep) {
+        const float2 u = s[m];
+        if (h) {
+          const float ax = s_ax[kx];
+          val = make_float2(mulr(ax, u.x), mulr(ax, u.y));
+        } else {
+          val = u;
+        }
+        val = make_float2(val.x * a.scale, val.y * a.scale);
+      }
+      v[m] = val;
+    }
+    fft_wg<N0, +1, V, VAR>(v, s_buf, s_tw, j, vi);
+    if (kz < a.PF2) {
+      float2* g = a.G1 + ((((size_t)b * 2 * nc + f0 + h) * N0) * a.K1 + kyc) * a.PF2 + kz;
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) g[(size_t)(j + m * NT) * a.K1 * a.PF2] = v[m];
+    }
+  }
+}
+
+// ST >= 0: the stage is a compile-time constant (only that stage's combine is compiled, which
+// lowers the register count: 128-point Convection columns 193 -> 133 / 148 VGPRs for ETDRK2's
+// evaluations, three waves per SIMD instead of two); ST = -1: the stage comes at run time.
+// (Holding the stage kernels to 128 VGPRs, four waves, spills a few values and measured slower.)
+template <int N0, int V, int VAR, int TERM, int ST = -1>
+__global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int stage_rt) {
+  const int stage = ST >= 0 ? ST : stage_rt;
   using F = Fft<N0, VAR>;
   constexpr int RMAX = F::RMAX, NT = F::NT;
   __shared__ float2 s_tw[Fft<N0, VAR>::tw_size()];
@@ -68,103 +111,94 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k3_col(Col3Args a, int
   // cpk_index(coff + xo, half) = 2 (coff + xo) + cko: (coff + xo) & 15 == kz & 15
   const int cko = (stage_coef_half(stage) << 4) - (kz & 15);
   constexpr int NC3 = TERM == TERM_KS ? 1 : 3;  // channels
-  // combine batch: the other channels' values stay live, so fewer modes per batch than in 2D
+  // combine batch (modes whose operands are loaded together): fewer than in 2D, the NSP kernel keeps
+  // all three channels' values live
   constexpr int CBM = RMAX >= 16 ? 1 : (RMAX >= 8 ? 2 : 4);
-  float2 s[NC3][RMAX];
-  if (stage == ST_PRO) {
-#pragma unroll
-    for (int c = 0; c < NC3; ++c) {
-      const size_t base = ((size_t)b * NC3 + c) * plane + coff;
-#pragma unroll
-      for (int m = 0; m < RMAX; ++m) {
-        const float2 t = a.U[base + (size_t)(j + m * NT) * inner];
-        s[c][m] = colv ? t : make_float2(0.f, 0.f);
-      }
-    }
-  } else if constexpr (TERM == TERM_NSP) {
-    // velocity-form Navier-Stokes: the three convection spectra first, projected mode by mode
-    float2 v3[3][RMAX];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const size_t base = ((size_t)b * 3 + c) * plane + coff;
-#pragma unroll
-      for (int m = 0; m < RMAX; ++m) {
-        const float2 t = a.P[base + (size_t)(j + m * NT) * inner];
-        v3[c][m] = colv ? t : make_float2(0.f, 0.f);
-      }
-      fft_wg<N0, -1, V, VAR>(v3[c], s_buf, s_tw, j, vi);
-    }
-    const float ayv = a.ay[ky], azv = colv ? a.az[kz] : 0.f;
-    // Nyquist-plane parity of nsp_project (every fused 3D size is even)
-    const bool nyy = 2 * ky == a.N1, nyz = 2 * kz == a.N2, oyz = (ky == 0 || nyy) && (kz == 0 || nyz);
-#pragma unroll
-    for (int m = 0; m < RMAX; ++m) {
-      float2 cv[3] = {v3[0][m], v3[1][m], v3[2][m]};
-      const int x = j + m * NT;
-      const float av[3] = {a.ax[x], ayv, azv};
-      const bool nyx = 2 * x == N0, own = (x == 0 || nyx) && oyz;
-      const bool ny[3] = {nyx && !own, nyy && !own, nyz && !own};
-      nsp_project<3>(cv, av, ny);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) v3[c][m] = cv[c];
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const size_t base = ((size_t)b * 3 + c) * plane + coff;
-      const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride + coff : nullptr;
-      combine_column<RMAX, NT, CBM>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko),
-                               a.c5t + tb, (unsigned)coff, a.coef_t, (unsigned)inner, j, colv, v3[c], s[c],
-                               a.ck01 ? a.ck01 + tb : nullptr, a.c6t ? a.c6t + tb : nullptr);
-    }
-  } else {
-#pragma unroll
-    for (int c = 0; c < NC3; ++c) {
-      const size_t base = ((size_t)b * NC3 + c) * plane + coff;
-      float2 v[RMAX];
-#pragma unroll
-      for (int m = 0; m < RMAX; ++m) {
-        const float2 t = a.P[base + (size_t)(j + m * NT) * inner];
-        v[m] = colv ? t : make_float2(0.f, 0.f);
-      }
-      fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
-      if constexpr (TERM == TERM_KS) {
-        // remove_mean: fft(p - mean(p)) differs from fft(p) only at k = 0, by the batch mean of the DC
-        if (a.dc && ky == 0 && kz == 0 && j == 0) v[0] = rsub(v[0], a.dc[0]);
-      }
-      const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride + coff : nullptr;
-      combine_column<RMAX, NT, CBM>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko),
-                               a.c5t + tb, (unsigned)coff, a.coef_t, (unsigned)inner, j, colv, v, s[c],
-                               a.ck01 ? a.ck01 + tb : nullptr, a.c6t ? a.c6t + tb : nullptr);
-    }
-  }
   const int kyc = ky_compress(ky, a.N1, a.kmax1);
-  if (kyc < 0 || (tile % nzt) * V > a.kmax2) return;  // uniform: no kept column in this tile
+  const bool kept = kyc >= 0 && (tile % nzt) * V <= a.kmax2;  // uniform: the tile has kept columns
+  if constexpr (TERM == TERM_NSP) {
+    float2 s[3][RMAX];
+    if (stage == ST_PRO) {
 #pragma unroll
-  for (int f = 0; f < 2 * NC3; ++f) {
-    float2 v[RMAX];
+      for (int c = 0; c < 3; ++c) {
+        const size_t base = ((size_t)b * 3 + c) * plane + coff;
 #pragma unroll
-    for (int m = 0; m < RMAX; ++m) {
-      const int kx = j + m * NT;
-      const int ks = kx <= N0 / 2 ? kx : N0 - kx;
-      const bool keep = colv && kz <= a.kmax2 && ks <= a.kmax0;
-      float2 val = make_float2(0.f, 0.f);
-      if (keep) {
-        const float2 u = s[f >> 1][m];
-        if (f & 1) {
-          const float ax = s_ax[kx];
-          val = make_float2(mulr(ax, u.x), mulr(ax, u.y));
-        } else {
-          val = u;
+        for (int m = 0; m < RMAX; ++m) {
+          const float2 t = a.U[base + (size_t)(j + m * NT) * inner];
+          s[c][m] = colv ? t : make_float2(0.f, 0.f);
         }
-        val = make_float2(val.x * a.scale, val.y * a.scale);
       }
-      v[m] = val;
-    }
-    fft_wg<N0, +1, V, VAR>(v, s_buf, s_tw, j, vi);
-    if (kz < a.PF2) {
-      float2* g = a.G1 + ((((size_t)b * 2 * NC3 + f) * N0) * a.K1 + kyc) * a.PF2 + kz;
+    } else {
+      // velocity-form Navier-Stokes: the three convection spectra first, projected mode by mode
+      float2 v3[3][RMAX];
 #pragma unroll
-      for (int m = 0; m < RMAX; ++m) g[(size_t)(j + m * NT) * a.K1 * a.PF2] = v[m];
+      for (int c = 0; c < 3; ++c) {
+        const size_t base = ((size_t)b * 3 + c) * plane + coff;
+#pragma unroll
+        for (int m = 0; m < RMAX; ++m) {
+          const float2 t = a.P[base + (size_t)(j + m * NT) * inner];
+          v3[c][m] = colv ? t : make_float2(0.f, 0.f);
+        }
+        fft_wg<N0, -1, V, VAR>(v3[c], s_buf, s_tw, j, vi);
+      }
+      const float ayv = a.ay[ky], azv = colv ? a.az[kz] : 0.f;
+      // Nyquist-plane parity of nsp_project (every fused 3D size is even)
+      const bool nyy = 2 * ky == a.N1, nyz = 2 * kz == a.N2, oyz = (ky == 0 || nyy) && (kz == 0 || nyz);
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) {
+        float2 cv[3] = {v3[0][m], v3[1][m], v3[2][m]};
+        const int x = j + m * NT;
+        const float av[3] = {a.ax[x], ayv, azv};
+        const bool nyx = 2 * x == N0, own = (x == 0 || nyx) && oyz;
+        const bool ny[3] = {nyx && !own, nyy && !own, nyz && !own};
+        nsp_project<3>(cv, av, ny);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v3[c][m] = cv[c];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const size_t base = ((size_t)b * 3 + c) * plane + coff;
+        const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride + coff : nullptr;
+        combine_column<RMAX, NT, CBM, (ST >= 0)>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko),
+                                      a.c5t + tb, (unsigned)coff, a.coef_t, (unsigned)inner, j, colv, v3[c], s[c],
+                                      a.ck01 ? a.ck01 + tb : nullptr, a.c6t ? a.c6t + tb : nullptr);
+      }
+    }
+    if (!kept) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      col3_fields<N0, V, VAR>(a, s[c], 2 * c, b, 3, kyc, kz, colv, s_buf, s_tw, s_ax, j, vi);
+  } else {
+    // one channel at a time: its spectrum, combine and two fields (u^_c, a_x u^_c), so that only
+    // that channel's values are live (the fields of channel c need nothing of the others)
+#pragma unroll
+    for (int c = 0; c < NC3; ++c) {
+      const size_t base = ((size_t)b * NC3 + c) * plane + coff;
+      float2 s[RMAX];
+      if (stage == ST_PRO) {
+#pragma unroll
+        for (int m = 0; m < RMAX; ++m) {
+          const float2 t = a.U[base + (size_t)(j + m * NT) * inner];
+          s[m] = colv ? t : make_float2(0.f, 0.f);
+        }
+      } else {
+        float2 v[RMAX];
+#pragma unroll
+        for (int m = 0; m < RMAX; ++m) {
+          const float2 t = a.P[base + (size_t)(j + m * NT) * inner];
+          v[m] = colv ? t : make_float2(0.f, 0.f);
+        }
+        fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
+        if constexpr (TERM == TERM_KS) {
+          // remove_mean: fft(p - mean(p)) differs from fft(p) only at k = 0, by the batch mean of the DC
+          if (a.dc && ky == 0 && kz == 0 && j == 0) v[0] = rsub(v[0], a.dc[0]);
+        }
+        const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride + coff : nullptr;
+        combine_column<RMAX, NT, CBM, (ST >= 0)>(stage, a.U + base, a.A1 + base, a.A2 + base, src, cpk + (2 * coff + cko),
+                                      a.c5t + tb, (unsigned)coff, a.coef_t, (unsigned)inner, j, colv, v, s,
+                                      a.ck01 ? a.ck01 + tb : nullptr, a.c6t ? a.c6t + tb : nullptr);
+      }
+      if (kept) col3_fields<N0, V, VAR>(a, s, 2 * c, b, NC3, kyc, kz, colv, s_buf, s_tw, s_ax, j, vi);
     }
   }
 }
@@ -377,6 +411,13 @@ __global__ void __launch_bounds__((Fft<N2, VAR>::NT * ROWS)) k3_row(Row3Args a) 
     default: return hipErrorInvalidValue;              \
   }
 
+// Convection column kernels also compiled per ETDRK2 / SETDRK2 and SETDRK4 stage (and for the
+// field pass of the initial state): the 64- and 128-point x columns with the default radix plan
+template <int NN, int VR>
+struct Col3StageKernels {
+  static constexpr bool value = (NN == 64 || NN == 128) && VR == (NN == 128 ? 2 : 0);
+};
+
 template <int NN, int VR>
 static hipError_t launch_col3d_t(const Col3Args& a, int nb, int stage, hipStream_t st) {
   constexpr int V = 16;
@@ -388,7 +429,19 @@ static hipError_t launch_col3d_t(const Col3Args& a, int nb, int stage, hipStream
     TFSM_LAUNCH((k3_col<NN, V, VR, TERM_NSP>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b, stage);
   else if (a.term == TERM_KS)
     TFSM_LAUNCH((k3_col<NN, V, VR, TERM_KS>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b, stage);
-  else
+  else if constexpr (Col3StageKernels<NN, VR>::value) {
+    const dim3 grid(slots * 8 * nb), block(Fft<NN, VR>::NT * V);
+    switch (stage) {
+      case ST_PRO: TFSM_LAUNCH((k3_col<NN, V, VR, TERM_CONV, ST_PRO>), grid, block, 0, st, b, stage); break;
+      case ST_R2_E0: TFSM_LAUNCH((k3_col<NN, V, VR, TERM_CONV, ST_R2_E0>), grid, block, 0, st, b, stage); break;
+      case ST_R2_E1: TFSM_LAUNCH((k3_col<NN, V, VR, TERM_CONV, ST_R2_E1>), grid, block, 0, st, b, stage); break;
+      case ST_R4_E0: TFSM_LAUNCH((k3_col<NN, V, VR, TERM_CONV, ST_R4_E0>), grid, block, 0, st, b, stage); break;
+      case ST_R4_E1: TFSM_LAUNCH((k3_col<NN, V, VR, TERM_CONV, ST_R4_E1>), grid, block, 0, st, b, stage); break;
+      case ST_R4_E2: TFSM_LAUNCH((k3_col<NN, V, VR, TERM_CONV, ST_R4_E2>), grid, block, 0, st, b, stage); break;
+      case ST_R4_E3: TFSM_LAUNCH((k3_col<NN, V, VR, TERM_CONV, ST_R4_E3>), grid, block, 0, st, b, stage); break;
+      default: TFSM_LAUNCH((k3_col<NN, V, VR, TERM_CONV>), grid, block, 0, st, b, stage);
+    }
+  } else
     TFSM_LAUNCH((k3_col<NN, V, VR, TERM_CONV>), dim3(slots * 8 * nb), dim3(Fft<NN, VR>::NT * V), 0, st, b,
                        stage);
   return hipGetLastError();
