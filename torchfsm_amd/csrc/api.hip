@@ -130,6 +130,8 @@ struct tfsm_plan {
   float2* src = nullptr;
   int srcB = 0, srcC = 0;
   unsigned char* src_cols = nullptr;  // 2D: [P] 1 where a source mode of last-axis column k is nonzero
+  uint32_t src_bits[Col2Args::SRC_BIT_WORDS] = {};  // the same flags as bits (P <= 544)
+  bool src_bits_ok = false;
   int V = 16;        // column tile of the fused 2D column kernel
   int col_var = 0;   // radix-plan variant of the column transforms
   int row_var = 0;   // radix-plan variant of the row transforms
@@ -537,6 +539,10 @@ Col2Args col_args(tfsm_plan* p, float2* U, char* ws) {
   c.src_bstride = p->src_bstride();
   c.src_cstride = p->src_cstride();
   c.src_cols = p->src_cols;
+  if (p->src_bits_ok) {
+    for (int w = 0; w < Col2Args::SRC_BIT_WORDS; ++w) c.src_bits[w] = p->src_bits[w];
+    c.src_bits_ok = 1;
+  }
   c.G = (float2*)(ws + p->off_G);
   c.dc = p->ks_mean ? (const float2*)(ws + p->off_DC) : nullptr;
   c.b0 = 0;
@@ -1723,6 +1729,11 @@ int tfsm_plan_add_source(tfsm_plan* p, const float* src, int32_t src_batch, int3
     if (!p->src_cols) rc = dev_alloc(p, &p->src_cols, (size_t)p->P, "source column flags");
     if (rc) return rc;
     HIP_TRY(hipMemcpy(p->src_cols, cols.data(), cols.size(), hipMemcpyHostToDevice), "source column flags upload");
+    p->src_bits_ok = p->P <= 32 * Col2Args::SRC_BIT_WORDS;
+    for (int w = 0; w < Col2Args::SRC_BIT_WORDS; ++w) p->src_bits[w] = 0;
+    if (p->src_bits_ok)
+      for (int k = 0; k < p->P; ++k)
+        if (cols[k]) p->src_bits[k >> 5] |= 1u << (k & 31);
   }
   return TFSM_OK;
 }

@@ -357,12 +357,23 @@ __device__ __forceinline__ float2 stage_combine_v(int stage, float2 nval, float2
 // SETDRK4's first, second and last evaluations need 2, 2 and 1 of the packed entry's floats: with
 // k01b ([E2, c1] per mode) and c6b (c6 per mode), tables laid out like c5b, those stages read 8 and
 // 4 bytes per mode instead of the 16-byte entry.
-template <int RMAX, int NT, int BM = (RMAX >= 16 ? 1 : 4), bool FENCE = false>
+// PRE (kernels compiled for one SETDRK4 stage): operands the caller loaded before its forward
+// transform, so that their memory latency overlaps the transform.  PRE = 1 (first and second
+// evaluations): pre.op holds the state U and, when k01b is set, pre.k01 this lane's k01b entries;
+// PRE = 2 (last evaluation): pre.op holds A2 and, when c6b is set, pre.c6 this lane's c6b entries.
+template <int RMAX>
+struct ColPre {
+  float2 op[RMAX];
+  float2 k01[RMAX];
+  float c6[RMAX];
+};
+
+template <int RMAX, int NT, int BM = (RMAX >= 16 ? 1 : 4), bool FENCE = false, int PRE = 0>
 __device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1, float2* A2, const float2* src,
                                                const float4* cpk, const float* c5b, unsigned c5o, float coef_t,
                                                unsigned stride, int j, bool colv, const float2 (&v)[RMAX],
                                                float2 (&s)[RMAX], const float2* k01b = nullptr,
-                                               const float* c6b = nullptr) {
+                                               const float* c6b = nullptr, const ColPre<RMAX>* pre = nullptr) {
   static_assert(RMAX % BM == 0, "combine batch must divide the values per thread");
   const float2 z = make_float2(0.f, 0.f);
   const bool rdu = stage_reads_u(stage), rd1 = stage_reads_a1(stage), rd2 = stage_reads_a2(stage);
@@ -385,10 +396,10 @@ __device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1,
     for (int i = 0; i < BM; ++i) {
       const unsigned o = o0 + (unsigned)(m0 + i) * ostep;
       if (rk01) {
-        const float2 t = k01b[c5o + o];
+        const float2 t = PRE == 1 ? pre->k01[m0 + i] : k01b[c5o + o];
         cc[i] = make_float4(t.x, t.y, 0.f, 0.f);
       } else if (rk6) {
-        cc[i] = make_float4(0.f, 0.f, 0.f, c6b[c5o + o]);
+        cc[i] = make_float4(0.f, 0.f, 0.f, PRE == 2 ? pre->c6[m0 + i] : c6b[c5o + o]);
       } else {
         cc[i] = cpk[2 * o];
       }
@@ -400,7 +411,7 @@ __device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1,
     }
     if (rdu) {
 #pragma unroll
-      for (int i = 0; i < BM; ++i) uu[i] = U[o0 + (unsigned)(m0 + i) * ostep];
+      for (int i = 0; i < BM; ++i) uu[i] = PRE == 1 ? pre->op[m0 + i] : U[o0 + (unsigned)(m0 + i) * ostep];
     }
     if (rd1) {
 #pragma unroll
@@ -408,7 +419,7 @@ __device__ __forceinline__ void combine_column(int stage, float2* U, float2* A1,
     }
     if (rd2) {
 #pragma unroll
-      for (int i = 0; i < BM; ++i) q2[i] = A2[o0 + (unsigned)(m0 + i) * ostep];
+      for (int i = 0; i < BM; ++i) q2[i] = PRE == 2 ? pre->op[m0 + i] : A2[o0 + (unsigned)(m0 + i) * ostep];
     }
     if (src) {
 #pragma unroll
@@ -521,6 +532,11 @@ struct Col2Args {
   const float2* src;          // [Bs][Cs][N0][P1] or null
   size_t src_bstride, src_cstride;
   const unsigned char* src_cols = nullptr;  // [P1]: 1 where some source mode of column ky is nonzero
+  // the same flags as bits (column ky: bit ky & 31 of word ky >> 5), when src_bits_ok: a tile of 8
+  // or 16 columns reads its flags from the kernel arguments instead of memory
+  static constexpr int SRC_BIT_WORDS = 17;  // P1 <= 544
+  uint32_t src_bits[SRC_BIT_WORDS] = {};
+  int src_bits_ok = 0;
   float2* G;                  // fields [B][NF][N0][PF]
   const float2* dc = nullptr;  // KSConvection: batch mean of the product DC (subtracted at k = 0), or null
   int b0;                     // first sample handled by this launch

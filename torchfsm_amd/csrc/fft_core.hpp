@@ -345,6 +345,69 @@ __device__ __forceinline__ void fft_wg(float2 (&v)[(Fft<N, VAR>::RMAX)], float2*
   if constexpr (F::NST > 4) fft_stage<N, 4, DIR, V, VAR, WS>(v, lds, tw, j, vi);
 }
 
+// One Stockham stage reading the previous stage's values from `in` and writing this stage's to
+// `out` (two LDS buffers used alternately): no barrier is needed between the reads and the writes
+// of one stage, only after the writes.
+template <int N, int S, int DIR, int V, int VAR = 0>
+__device__ __forceinline__ void fft_stage_pp(float2 (&v)[(Fft<N, VAR>::RMAX)], const float2* in, float2* out,
+                                             const float2* tw, int j, int vi) {
+  using F = Fft<N, VAR>;
+  constexpr int RMAX = F::RMAX;
+  constexpr int NT = F::NT;
+  constexpr int R = F::radix(S);
+  constexpr int NS = F::ns_before(S);
+  constexpr int Q = RMAX / R;
+  constexpr bool FIRST = (S == 0);
+  constexpr bool LAST = (S == F::NST - 1);
+  constexpr int TWO = F::tw_offset(S);
+  if constexpr (!FIRST) {
+#pragma unroll
+    for (int m = 0; m < RMAX; ++m) v[m] = in[lds_slot<V, F::PD>(j + m * NT, vi)];
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int jb = j + q * NT;
+    float2 x[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = v[q + r * Q];
+    if constexpr (NS > 1) {
+      const int k = jb % NS;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[TWO + r * NS + k];
+        x[r] = (DIR < 0) ? cmul(x[r], w) : cmulc(x[r], w);
+      }
+    }
+    dft<R, DIR>(x);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q + r * Q] = x[r];
+    } else {
+      const int idxd = (jb / NS) * NS * R + (jb % NS);
+#pragma unroll
+      for (int r = 0; r < R; ++r) out[lds_slot<V, F::PD>(idxd + r * NS, vi)] = x[r];
+    }
+  }
+  if constexpr (!LAST) __syncthreads();
+}
+
+// fft_wg over two LDS buffers: stage s writes buffer s % 2 and stage s + 1 reads it, so each
+// exchange costs one workgroup barrier instead of two.  The last stage reads buffer (NST - 2) % 2;
+// with an odd number of stages that is lds1, and the next call's first write (lds0) is already
+// ordered after every read of lds0 by this call's barriers; with an even number a trailing barrier
+// orders it.  Same arithmetic as fft_wg, so the same results.
+template <int N, int DIR, int V, int VAR = 0>
+__device__ __forceinline__ void fft_wg_pp(float2 (&v)[(Fft<N, VAR>::RMAX)], float2* lds0, float2* lds1,
+                                          const float2* tw, int j, int vi) {
+  using F = Fft<N, VAR>;
+  fft_stage_pp<N, 0, DIR, V, VAR>(v, lds1, lds0, tw, j, vi);
+  if constexpr (F::NST > 1) fft_stage_pp<N, 1, DIR, V, VAR>(v, lds0, lds1, tw, j, vi);
+  if constexpr (F::NST > 2) fft_stage_pp<N, 2, DIR, V, VAR>(v, lds1, lds0, tw, j, vi);
+  if constexpr (F::NST > 3) fft_stage_pp<N, 3, DIR, V, VAR>(v, lds0, lds1, tw, j, vi);
+  if constexpr (F::NST > 4) fft_stage_pp<N, 4, DIR, V, VAR>(v, lds1, lds0, tw, j, vi);
+  if constexpr (F::NST % 2 == 0) __syncthreads();
+}
+
 // Supported lengths.  Powers of two, and the mixed-radix lengths above (each also needs its
 // kernels instantiated, which the size lists below drive).
 #define TFSM_MIXED_SIZES_1D(X) X(48) X(96) X(100) X(112) X(192) X(200) X(384) X(400) X(768) X(800)

@@ -56,24 +56,90 @@ struct Term2<TERM_CONS> {  // ConservativeConvection, C = 2: fields u0, u1; prod
 // Stage combine of one channel's column (see combine_column in engine.hpp): column base ky,
 // element stride P1.  With two channels the other channel's values stay live, so those kernels load
 // one mode's operands at a time (BM = 1) to stay within 128 VGPRs.
-template <int N0, int VAR, int BM, bool FENCE = false>
+// rs: read the source spectrum src in this column (src_col); callers look it up before their
+// forward transform, so that its load does not stall the combine.
+template <int N0, int VAR, int BM, bool FENCE = false, int PRE = 0>
 __device__ __forceinline__ void combine_column2(const Col2Args& a, int stage, size_t base, const float2* src,
                                                 const float4* cpk, const float* c5b, size_t tb, int cko, int ky,
                                                 bool colv, int j,
                                                 const float2 (&v)[Fft<N0, VAR>::RMAX],
-                                                float2 (&s)[Fft<N0, VAR>::RMAX]) {
+                                                float2 (&s)[Fft<N0, VAR>::RMAX], bool rs,
+                                                const ColPre<Fft<N0, VAR>::RMAX>* pre = nullptr) {
   using F = Fft<N0, VAR>;
   const size_t cb = base + ky;
-  // a source spectrum is read only in the columns where it has a nonzero mode (src_cols)
-  const bool rs = src && (!a.src_cols || a.src_cols[ky]);
-  combine_column<F::RMAX, F::NT, BM, FENCE>(stage, a.U + cb, a.A1 + cb, a.A2 + cb, rs ? src + ky : nullptr,
+  combine_column<F::RMAX, F::NT, BM, FENCE, PRE>(stage, a.U + cb, a.A1 + cb, a.A2 + cb, rs ? src + ky : nullptr,
                                      cpk + (2 * ky + cko), c5b, (unsigned)ky, a.coef_t, (unsigned)a.P1, j, colv, v,
-                                     s, a.ck01 ? a.ck01 + tb : nullptr, a.c6t ? a.c6t + tb : nullptr);
+                                     s, a.ck01 ? a.ck01 + tb : nullptr, a.c6t ? a.c6t + tb : nullptr, pre);
 }
+
+// Column transforms of the tiled column kernel.  PP: two LDS exchange buffers used alternately
+// (fft_wg_pp, one workgroup barrier per exchange instead of two), for the 1024-thread tiles, which
+// hold one workgroup per CU anyway and have the LDS for a second buffer.
+template <int N0, int V, int VAR>
+struct ColPP {
+  static constexpr bool value = Fft<N0, VAR>::NT * V == 1024 &&
+                                (2 * LdsSize<N0, V>::slots + Fft<N0, VAR>::tw_size()) * 8 + N0 * 4 <= 152 * 1024;
+};
+
+template <int N0, int DIR, int V, int VAR, bool PP>
+__device__ __forceinline__ void col_fft(float2 (&v)[(Fft<N0, VAR>::RMAX)], float2* s_buf, const float2* s_tw, int j,
+                                        int vi) {
+  if constexpr (PP) fft_wg_pp<N0, DIR, V, VAR>(v, s_buf, s_buf + LdsSize<N0, V>::slots, s_tw, j, vi);
+  else fft_wg<N0, DIR, V, VAR>(v, s_buf, s_tw, j, vi);
+}
+
+// A source spectrum is read only in the columns where it has a nonzero mode (src_cols).
+__device__ __forceinline__ bool src_col(const Col2Args& a, int ky) {
+  return a.src && (!a.src_cols || a.src_cols[ky]);
+}
+// The same for lane vi of a tile of V in {8, 16} columns starting at c0 (a multiple of V, so the
+// tile's flags lie in one 32-bit word): from the bits in the kernel arguments when present.
+template <int V>
+__device__ __forceinline__ bool src_col_tile(const Col2Args& a, int c0, int vi) {
+  static_assert(V == 8 || V == 16, "a tile's flags in one word");
+  if (!a.src) return false;
+  if (a.src_bits_ok) return (a.src_bits[c0 >> 5] >> ((c0 & 31) + vi)) & 1u;
+  return !a.src_cols || a.src_cols[c0 + vi];
+}
+
+// Twiddles and x wavenumbers of a column kernel's LDS, staged through registers: load() issues the
+// global loads (before the tile's own loads), store() writes LDS once the tile's loads are in
+// flight, so that one memory latency covers both instead of two in a row.
+template <int N0, int VAR, int NTH>
+struct LdsFill {
+  static constexpr int NTW = Fft<N0, VAR>::tw_size();
+  static constexpr int KT = (NTW + NTH - 1) / NTH, KA = (N0 + NTH - 1) / NTH;
+  float2 t[KT];
+  float x[KA];
+  __device__ __forceinline__ void load(const float2* tw, const float* ax) {
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const int i = (int)threadIdx.x + k * NTH;
+      if (i < NTW) t[k] = tw[i];
+    }
+#pragma unroll
+    for (int k = 0; k < KA; ++k) {
+      const int i = (int)threadIdx.x + k * NTH;
+      if (i < N0) x[k] = ax[i];
+    }
+  }
+  __device__ __forceinline__ void store(float2* s_tw, float* s_ax) const {
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const int i = (int)threadIdx.x + k * NTH;
+      if (i < NTW) s_tw[i] = t[k];
+    }
+#pragma unroll
+    for (int k = 0; k < KA; ++k) {
+      const int i = (int)threadIdx.x + k * NTH;
+      if (i < N0) s_ax[i] = x[k];
+    }
+  }
+};
 
 // Velocity-form Navier-Stokes (NSPressureConvection): both channels' convection spectra are
 // transformed first, projected mode by mode (nsp_project), then combined channel by channel.
-template <int N0, int V, int VAR>
+template <int N0, int V, int VAR, bool PP = false>
 __device__ __forceinline__ void col_nsp_combine(const Col2Args& a, int stage, float2 (&s)[2][Fft<N0, VAR>::RMAX],
                                                 float2* s_buf, const float2* s_tw, int b, int ky, bool colv, int j,
                                                 int vi) {
@@ -91,6 +157,9 @@ __device__ __forceinline__ void col_nsp_combine(const Col2Args& a, int stage, fl
       }
     return;
   }
+  // per-column values the projection and the combine need, loaded before the transforms
+  const float ayv = colv ? a.ay[ky] : 0.f;
+  const bool rs = src_col(a, ky);
   float2 v[2][RMAX];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
@@ -99,9 +168,8 @@ __device__ __forceinline__ void col_nsp_combine(const Col2Args& a, int stage, fl
       const float2 t = a.P[((size_t)b * 2 + c) * plane + (size_t)(j + m * NT) * a.P1 + ky];
       v[c][m] = colv ? t : z;
     }
-    fft_wg<N0, -1, V, VAR>(v[c], s_buf, s_tw, j, vi);
+    col_fft<N0, -1, V, VAR, PP>(v[c], s_buf, s_tw, j, vi);
   }
-  const float ayv = colv ? a.ay[ky] : 0.f;
   // Nyquist-plane parity of nsp_project (N0 is even for every fused size)
   const bool nyy = 2 * ky == a.N1, zy = ky == 0;
 #pragma unroll
@@ -124,15 +192,18 @@ __device__ __forceinline__ void col_nsp_combine(const Col2Args& a, int stage, fl
   for (int c = 0; c < 2; ++c) {
     const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride : nullptr;
     combine_column2<N0, VAR, 1>(a, stage, ((size_t)b * 2 + c) * plane, src, cpk, c5b, tb, cko, ky, colv, j, v[c],
-                                s[c]);
+                                s[c], rs);
   }
 }
 
 // Stage combine, fields and inverse x-transforms of column tile `tile` of sample b: the body of
 // k2_col (and of the column pass of k2_run).  ST >= 0: the stage is a compile-time constant.
-template <int N0, int V, int TERM, int VAR, int ST, bool CLAMP = false>
+// FILL: the caller has issued the LDS twiddle and wavenumber loads (fill) and this function writes
+// them to LDS, with a barrier, right before its first LDS use; otherwise they are in LDS already.
+template <int N0, int V, int TERM, int VAR, int ST, bool CLAMP = false, bool FILL = false, bool PP = false>
 __device__ __forceinline__ void col2_tile(const Col2Args& a, int stage, int tile, int b, float2* s_buf,
-                                          const float2* s_tw, const float* s_ax, int j, int vi) {
+                                          float2* s_tw, float* s_ax, int j, int vi,
+                                          const LdsFill<N0, VAR, Fft<N0, VAR>::NT * V>* fill = nullptr) {
   using F = Fft<N0, VAR>;
   constexpr int RMAX = F::RMAX, NT = F::NT;
   constexpr int NC = Term2<TERM>::NC, NF = Term2<TERM>::NF;
@@ -148,10 +219,24 @@ __device__ __forceinline__ void col2_tile(const Col2Args& a, int stage, int tile
   const float* c5b = a.c5t + tb;
   // cpk_index(ci, half) = 2 ci + cko: every mode of this lane has ci & 15 == kl & 15
   const int cko = (stage_coef_half(stage) << 4) - (kl & 15);
+  // per-column values of the combine and the field pass, loaded before the transforms so that
+  // their latency overlaps the product's loads (kl < P1 <= N1: in the wavenumber table)
+  bool rs;
+  if constexpr (!CLAMP && (V == 8 || V == 16)) rs = src_col_tile<V>(a, tile * V, vi);
+  else rs = src_col(a, kl);
+  const float ay = a.ay[kl];
+  auto fill_lds = [&]() {
+    if constexpr (FILL) {
+      fill->store(s_tw, s_ax);
+      __syncthreads();
+    }
+  };
   float2 s[NC][RMAX];
   if constexpr (TERM == TERM_NSP) {
-    col_nsp_combine<N0, V, VAR>(a, stage, s, s_buf, s_tw, b, ky, colv, j, vi);
+    fill_lds();
+    col_nsp_combine<N0, V, VAR, PP>(a, stage, s, s_buf, s_tw, b, ky, colv, j, vi);
   } else {
+  if constexpr (TERM == TERM_CONS) fill_lds();
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const size_t base = ((size_t)b * NC + c) * plane;
@@ -163,6 +248,13 @@ __device__ __forceinline__ void col2_tile(const Col2Args& a, int stage, int tile
       }
     } else {
       float2 v[RMAX];
+      // stage kernels of one-channel products whose combine reads one state array (U, or A2 in the
+      // last SETDRK4 evaluation): that array and the compact coefficients are loaded before the
+      // forward transform (PRE)
+      constexpr int PRE = (NC == 1 && TERM != TERM_CONS)
+                              ? ((ST == ST_R4_E0 || ST == ST_R4_E1) ? 1 : (ST == ST_R4_E3 ? 2 : 0))
+                              : 0;
+      ColPre<RMAX> pre;
       if constexpr (TERM == TERM_CONS) {
         // N_c = (i a_x) F_x(F_y(u0 u_c)) + F_x(i a_y F_y(u1 u_c))   (_conservative_convection.py:23-27)
         const size_t pb = ((size_t)b * 4 + 2 * c) * plane;
@@ -171,7 +263,7 @@ __device__ __forceinline__ void col2_tile(const Col2Args& a, int stage, int tile
           const float2 t = a.P[pb + (size_t)(j + m * NT) * a.P1 + kl];
           v[m] = colv ? t : make_float2(0.f, 0.f);
         }
-        fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
+        col_fft<N0, -1, V, VAR, PP>(v, s_buf, s_tw, j, vi);
         // Nyquist-plane parity (nsp_project, engine.hpp): (i a_x) drops out on the x-Nyquist row,
         // (i a_y), applied by the row kernel, on the y-Nyquist column, except at modes that are
         // their own conjugate partner
@@ -189,7 +281,7 @@ __device__ __forceinline__ void col2_tile(const Col2Args& a, int stage, int tile
           const float2 t = a.P[pb + plane + (size_t)(j + m * NT) * a.P1 + kl];
           w[m] = colv ? t : make_float2(0.f, 0.f);
         }
-        fft_wg<N0, -1, V, VAR>(w, s_buf, s_tw, j, vi);
+        col_fft<N0, -1, V, VAR, PP>(w, s_buf, s_tw, j, vi);
 #pragma unroll
         for (int m = 0; m < RMAX; ++m) {
           const int x = j + m * NT;
@@ -202,21 +294,38 @@ __device__ __forceinline__ void col2_tile(const Col2Args& a, int stage, int tile
           const float2 t = a.P[base + (size_t)(j + m * NT) * a.P1 + kl];
           v[m] = colv ? t : make_float2(0.f, 0.f);
         }
-        fft_wg<N0, -1, V, VAR>(v, s_buf, s_tw, j, vi);
+        if constexpr (PRE > 0) {
+          // the combine's state operand and compact coefficients, issued behind the product's loads
+          // and before the forward transform, whose first use waits only for the product
+          const unsigned o0 = (unsigned)j * a.P1 + kl, ostep = (unsigned)NT * a.P1;
+          const float2* opb = (PRE == 2 ? a.A2 : a.U) + base;
+#pragma unroll
+          for (int m = 0; m < RMAX; ++m) pre.op[m] = opb[o0 + (unsigned)m * ostep];
+          // (the stage kernels run only with both compact tables present: launch_col_t)
+          if constexpr (PRE == 2) {
+#pragma unroll
+            for (int m = 0; m < RMAX; ++m) pre.c6[m] = a.c6t[tb + o0 + (unsigned)m * ostep];
+          } else {
+#pragma unroll
+            for (int m = 0; m < RMAX; ++m) pre.k01[m] = a.ck01[tb + o0 + (unsigned)m * ostep];
+          }
+        }
+        if (c == 0) fill_lds();
+        col_fft<N0, -1, V, VAR, PP>(v, s_buf, s_tw, j, vi);
       }
       if constexpr (TERM == TERM_KS) {
         // remove_mean: fft(p - mean(p)) differs from fft(p) only at k = 0, by the batch mean of the DC
         if (a.dc && ky == 0 && j == 0) v[0] = rsub(v[0], a.dc[0]);
       }
       const float2* src = a.src ? a.src + (size_t)b * a.src_bstride + (size_t)c * a.src_cstride : nullptr;
-      combine_column2<N0, VAR, (NC > 1 || RMAX >= 16) ? 1 : 4, (ST >= 0)>(a, stage, base, src, cpk, c5b, tb, cko, kl,
-                                                                             colv,
-                                                                             j, v, s[c]);
+      combine_column2<N0, VAR, (NC > 1 || RMAX >= 16) ? 1 : 4, (ST >= 0), PRE>(a, stage, base, src, cpk, c5b, tb,
+                                                                                  cko, kl, colv, j, v, s[c], rs,
+                                                                                  &pre);
     }
   }
   }
   if (tile * V > a.kmax1) return;  // uniform per workgroup: no kept column in this tile
-  const float ay = a.ay[ky];  // ky < P1 <= N1: in the table
+  if (stage == ST_PRO && TERM != TERM_NSP && TERM != TERM_CONS) fill_lds();  // no transform so far
   (void)ay;
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
@@ -250,7 +359,7 @@ __device__ __forceinline__ void col2_tile(const Col2Args& a, int stage, int tile
       }
       v[m] = val;
     }
-    fft_wg<N0, +1, V, VAR>(v, s_buf, s_tw, j, vi);
+    col_fft<N0, +1, V, VAR, PP>(v, s_buf, s_tw, j, vi);
     if (ky <= a.kmax1) {  // kept columns only: the row kernel reads no other
       float2* g = a.G + (((size_t)b * NF + f) * N0) * a.PF + ky;
 #pragma unroll
@@ -268,12 +377,14 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
   const int stage = ST >= 0 ? ST : stage_rt;
   using F = Fft<N0, VAR>;
   constexpr int NT = F::NT;
+  // (the run-time-stage kernel keeps one buffer: the second one's addressing would make it spill)
+  constexpr bool PP = ColPP<N0, V, VAR>::value && ST >= 0;
   __shared__ float2 s_tw[Fft<N0, VAR>::tw_size()];
-  __shared__ float2 s_buf[LdsSize<N0, V>::slots];
+  __shared__ float2 s_buf[(PP ? 2 : 1) * LdsSize<N0, V>::slots];
   __shared__ float s_ax[N0];  // x wavenumbers: read per mode by the field pass without a memory wait
-  load_twiddles<N0, VAR>(s_tw, a.tw0);
-  for (int i = threadIdx.x; i < N0; i += NT * V) s_ax[i] = a.ax[i];
-  __syncthreads();
+  // the LDS tables' loads go out first; col2_tile writes them to LDS once its own loads are issued
+  LdsFill<N0, VAR, NT * V> fill;
+  fill.load(a.tw0, a.ax);
   const int vi = threadIdx.x % V, j = threadIdx.x / V;
   // Block -> (tile, sample).  Workgroups are dispatched round-robin over the 8 XCDs (block v on
   // XCD v % 8); with nb % 8 == 0 each XCD owns a group of nb/8 samples and walks the tiles in
@@ -294,7 +405,7 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
   }
   const int b = a.b0 + bl;
   if (a.mask.skip(b)) return;  // uniform per workgroup: this sample's steps are done
-  col2_tile<N0, V, TERM, VAR, ST>(a, stage, tile, b, s_buf, s_tw, s_ax, j, vi);
+  col2_tile<N0, V, TERM, VAR, ST, false, true, PP>(a, stage, tile, b, s_buf, s_tw, s_ax, j, vi, &fill);
 }
 
 // Row x of sample b (rv false: idle row that only takes part in the transforms' synchronisation):
@@ -638,8 +749,10 @@ inline hipError_t launch_col_t(const Col2Args& a, int nb, int stage, hipStream_t
   Col2Args b = a;
   b.nb = nb;
   const dim3 grid(tiles * nb), block(Fft<N0, VAR>::NT * V);
+  // the stage kernels read SETDRK4's compact coefficient tables unconditionally
   if constexpr (ColStageKernels<N0, V, TERM, VAR>::value) {
-    switch (stage) {
+    const bool tables = a.ck01 && a.c6t;
+    switch (tables ? stage : -1) {
       case ST_PRO: TFSM_LAUNCH((k2_col<N0, V, TERM, VAR, ST_PRO>), grid, block, 0, st, b, stage); break;
       case ST_R4_E0: TFSM_LAUNCH((k2_col<N0, V, TERM, VAR, ST_R4_E0>), grid, block, 0, st, b, stage); break;
       case ST_R4_E1: TFSM_LAUNCH((k2_col<N0, V, TERM, VAR, ST_R4_E1>), grid, block, 0, st, b, stage); break;
