@@ -411,7 +411,11 @@ __global__ void __launch_bounds__((Fft<N0, VAR>::NT * V)) k2_col(Col2Args a, int
 // Row x of sample b (rv false: idle row that only takes part in the transforms' synchronisation):
 // packed C2R of the fields, physical product, R2C into the products P.  The body of k2_row (and of
 // the row pass of k2_run); `lds` is the row's own LDS region.
-template <int N1, int TERM, int VAR>
+// PAIR (VorticityConvection): rows x and x + 1 (x even, same sample) by the same threads, one after
+// the other up to their real products p_x, p_x+1, which then share one forward transform,
+// Z = F(p_x + i p_x+1), unpacked as F(p_x)[k] = (Z[k] + conj Z[N-k]) / 2 and
+// F(p_x+1)[k] = (Z[k] - conj Z[N-k]) / (2i): five transforms per two rows instead of six.
+template <int N1, int TERM, int VAR, bool PAIR = false>
 __device__ __forceinline__ void row2_body(const Row2Args& a, int b, int x, bool rv, float2* lds, const float2* s_tw,
                                           int j) {
   using F = Fft<N1, VAR>;
@@ -421,37 +425,76 @@ __device__ __forceinline__ void row2_body(const Row2Args& a, int b, int x, bool 
   const float2* gb = a.G + ((size_t)b * NF) * fs + (size_t)x * a.PF;
 
   if constexpr (TERM == TERM_VC) {
-    float2 z1[RMAX], z2[RMAX];
-#pragma unroll
-    for (int m = 0; m < RMAX; ++m) {
-      const int pos = j + m * NT;
-      const bool cj = pos > N1 / 2;
-      const int k = cj ? N1 - pos : pos;
-      const bool edge = (k == 0) || (k == N1 / 2);
-      float2 f0 = make_float2(0.f, 0.f), f1 = f0, f2 = f0, f3 = f0;
-      float ay = 0.f;
-      if (rv && k <= a.kmax1) {
-        f0 = gb[k];
-        f1 = gb[fs + k];
-        f2 = gb[2 * fs + k];
-        f3 = gb[3 * fs + k];
-        ay = a.ay[k];
-      }
-      // ux = C2R(i ay psi), uy = C2R(-i kx psi), wx = C2R(i kx w), wy = C2R(i ay w)
-      float2 A = make_float2(-ay * f0.y, ay * f0.x);
-      float2 Bv = make_float2(f1.y, -f1.x);
-      z1[m] = pack_pair(A, Bv, cj, edge);
-      A = make_float2(-f2.y, f2.x);
-      Bv = make_float2(-ay * f3.y, ay * f3.x);
-      z2[m] = pack_pair(A, Bv, cj, edge);
-    }
-    fft_wg<N1, +1, 1, VAR, RowSync<N1, VAR>::WS>(z1, lds, s_tw, j, 0);
-    fft_wg<N1, +1, 1, VAR, RowSync<N1, VAR>::WS>(z2, lds, s_tw, j, 0);
+    constexpr bool WS = RowSync<N1, VAR>::WS;
     float2 v[RMAX];
+    // one loop body for both rows of a pair (unrolled, the second row's loads and addresses are
+    // hoisted above the first row's transforms: both rows' fields live at once, beyond the
+    // register budget)
+#pragma unroll 1
+    for (int h = 0; h < (PAIR ? 2 : 1); ++h) {
+      const float2* gr = gb + (size_t)h * a.PF;  // row x + h
+      float2 z1[RMAX], z2[RMAX];
 #pragma unroll
-    for (int m = 0; m < RMAX; ++m) v[m] = make_float2(z1[m].x * z2[m].x + z1[m].y * z2[m].y, 0.f);
-    fft_wg<N1, -1, 1, VAR, RowSync<N1, VAR>::WS>(v, lds, s_tw, j, 0);
-    if (rv) {
+      for (int m = 0; m < RMAX; ++m) {
+        const int pos = j + m * NT;
+        const bool cj = pos > N1 / 2;
+        const int k = cj ? N1 - pos : pos;
+        const bool edge = (k == 0) || (k == N1 / 2);
+        float2 f0 = make_float2(0.f, 0.f), f1 = f0, f2 = f0, f3 = f0;
+        float ay = 0.f;
+        if (rv && k <= a.kmax1) {
+          f0 = gr[k];
+          f1 = gr[fs + k];
+          f2 = gr[2 * fs + k];
+          f3 = gr[3 * fs + k];
+          ay = a.ay[k];
+        }
+        // ux = C2R(i ay psi), uy = C2R(-i kx psi), wx = C2R(i kx w), wy = C2R(i ay w)
+        float2 A = make_float2(-ay * f0.y, ay * f0.x);
+        float2 Bv = make_float2(f1.y, -f1.x);
+        z1[m] = pack_pair(A, Bv, cj, edge);
+        A = make_float2(-f2.y, f2.x);
+        Bv = make_float2(-ay * f3.y, ay * f3.x);
+        z2[m] = pack_pair(A, Bv, cj, edge);
+      }
+      fft_wg<N1, +1, 1, VAR, WS>(z1, lds, s_tw, j, 0);
+      fft_wg<N1, +1, 1, VAR, WS>(z2, lds, s_tw, j, 0);
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) {
+        const float pr = z1[m].x * z2[m].x + z1[m].y * z2[m].y;
+        if (h == 0) v[m] = make_float2(pr, 0.f);
+        else v[m].y = pr;
+      }
+    }
+    fft_wg<N1, -1, 1, VAR, WS>(v, lds, s_tw, j, 0);
+    if constexpr (PAIR) {
+      // Z[N - k] of every position this lane holds, through the row's LDS region (the transform's
+      // last stage read it before its closing synchronisation)
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) lds[lds_slot<1>(j + m * NT, 0)] = v[m];
+      lds_sync<WS>();
+      float2 pa[RMAX], pb[RMAX];
+#pragma unroll
+      for (int m = 0; m < RMAX; ++m) {
+        const int pos = j + m * NT;
+        const float2 zk = v[m];
+        const float2 zm = lds[lds_slot<1>(pos == 0 ? 0 : N1 - pos, 0)];
+        pa[m] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);
+        pb[m] = make_float2(0.5f * d.y, -0.5f * d.x);
+      }
+      if (rv) {
+        float2* p = a.P + ((size_t)b * a.N0 + x) * a.P1;
+#pragma unroll
+        for (int m = 0; m < RMAX; ++m) {
+          const int pos = j + m * NT;
+          if (pos <= N1 / 2) {
+            p[pos] = pa[m];
+            p[a.P1 + pos] = pb[m];
+          }
+        }
+      }
+    } else if (rv) {
       float2* p = a.P + ((size_t)b * a.N0 + x) * a.P1;
 #pragma unroll
       for (int m = 0; m < RMAX; ++m) {
@@ -604,6 +647,14 @@ __device__ __forceinline__ void row2_body(const Row2Args& a, int b, int x, bool 
 }
 
 
+// row kernels that take rows in pairs sharing one forward transform (row2_body<..., PAIR>):
+// 512-point VorticityConvection rows with the default radix plan (126 VGPRs against 94 unpaired;
+// the other sizes' paired kernels need 120-250)
+template <int TERM, int N1, int VAR>
+struct Row2Pair {
+  static constexpr bool value = TERM == TERM_VC && N1 == 512 && VAR == 0;
+};
+
 template <int N1, int ROWS, int TERM, int VAR>
 __global__ void __launch_bounds__((Fft<N1, VAR>::NT * ROWS)) k2_row(Row2Args a) {
   using F = Fft<N1, VAR>;
@@ -614,7 +665,10 @@ __global__ void __launch_bounds__((Fft<N1, VAR>::NT * ROWS)) k2_row(Row2Args a) 
   load_twiddles<N1, VAR>(s_tw, a.tw1);
   __syncthreads();
   const int g = threadIdx.x / NT, j = threadIdx.x % NT;
-  int row = blockIdx.x * ROWS + g;
+  // VorticityConvection: each thread group takes a pair of rows (row2_body<..., PAIR>); N0 is even,
+  // so the two rows belong to the same sample
+  constexpr int RP = Row2Pair<TERM, N1, VAR>::value ? 2 : 1;
+  int row = (blockIdx.x * ROWS + g) * RP;
   // with NT a multiple of 64 a wavefront holds one row: the row (and every row pointer) is uniform,
   // so the loads and stores take a scalar base and a 32-bit lane offset
   if constexpr (NT % 64 == 0) row = __builtin_amdgcn_readfirstlane(row);
@@ -622,7 +676,7 @@ __global__ void __launch_bounds__((Fft<N1, VAR>::NT * ROWS)) k2_row(Row2Args a) 
   const int b = a.b0 + (in_range ? row / a.N0 : 0);
   const bool rv = in_range && !a.mask.skip(b);  // rows of samples whose steps are done stay idle
   const int x = rv ? row % a.N0 : 0;
-  row2_body<N1, TERM, VAR>(a, b, x, rv, s_buf + g * SL, s_tw, j);
+  row2_body<N1, TERM, VAR, (RP == 2)>(a, b, x, rv, s_buf + g * SL, s_tw, j);
 }
 
 // ---------------------------------------------------------------- persistent per-sample stepper
@@ -770,7 +824,7 @@ template <int N1, int TERM, int VAR, int WGT>
 inline hipError_t launch_row_t(const Row2Args& a, hipStream_t st) {
   constexpr int NT = Fft<N1, VAR>::NT;
   constexpr int ROWS = (NT >= WGT) ? 1 : WGT / NT;
-  const int rows = a.B * a.N0;
+  const int rows = a.B * a.N0 / (Row2Pair<TERM, N1, VAR>::value ? 2 : 1);  // thread groups (row pairs)
   TFSM_LAUNCH((k2_row<N1, ROWS, TERM, VAR>), dim3((rows + ROWS - 1) / ROWS), dim3(NT * ROWS), 0, st, a);
   return hipGetLastError();
 }
