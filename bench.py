@@ -412,7 +412,7 @@ def differentiation_leg(dev, reps=5):
     return out
 
 
-CPU_SHARE = 16  # host CPUs a one-GPU box allots to the GPU's jobs
+CPU_SHARE = 16  # host CPUs allotted to one GPU's jobs on the machines this runs on
 
 
 def host_cpu():
@@ -656,7 +656,7 @@ def main():
 
     # ---------------------------------------------------------------- baselines (oracle restatement)
     host = host_cpu()
-    # the host cores this process may use, capped at the 16 CPUs a one-GPU box allots per GPU
+    # the host cores this process may use, capped at the 16 CPUs allotted per GPU
     threads = min(CPU_SHARE, host["usable_cpus"])
     torch.set_num_threads(threads)
     log(f"[bench] cpu baseline: refpath float32 on {threads} threads ({host['model']}, os.cpu_count() "
@@ -664,8 +664,8 @@ def main():
     cpu_val, sample = w.cpu_baseline(args)
     result["cpu_baseline"] = {"value": cpu_val, "unit": "DOF·steps/s", "cores": threads, "kind": "port",
                               "sample": sample, "host": host,
-                              "threads_note": f"threads = min({CPU_SHARE}, usable CPUs): the box allots "
-                                              f"{CPU_SHARE} CPUs per GPU"}
+                              "threads_note": f"threads = min({CPU_SHARE}, usable CPUs): {CPU_SHARE} host "
+                                              f"CPUs are allotted per GPU"}
     log(f"[bench] cpu baseline {cpu_val:.3e} DOF*steps/s")
 
     log(f"[bench] torch-GPU baseline: refpath on {dev}, {args.torch_steps} steps")

@@ -1,4 +1,4 @@
-"""Operator algebra and the integrate driver with the reference's interface
+"""Operator algebra and the integrate loop with the reference's interface
 (operator/_base.py:16-1028), compiled onto the MI355X engine.
 
 register_mesh turns the generator list into an engine term descriptor: linear symbols
