@@ -327,3 +327,37 @@ def test_external_force_must_be_explicit_sources():
     op = NSPressureConvection(Laplacian())
     with pytest.raises(ValueError, match="explicit sources"):
         op.register_mesh(mesh, 2)
+
+
+def test_native_handles_are_destroyed_once_even_at_shutdown(monkeypatch):
+    """SpectralPlan.close and StepGraph.close destroy their native handle exactly once, also when
+    they run from a finalizer at interpreter shutdown after the module's globals (here ctypes) are
+    gone: the handle is cleared before the destroy call, so a second close (a plan closing the
+    graph it caches, after that graph's own finalizer) does not destroy it again."""
+    import ctypes
+
+    import torchfsm_amd.engine as eng
+
+    calls = []
+
+    class FakeLib:
+        def tfsm_graph_destroy(self, g):
+            calls.append(("graph", g.value))
+
+        def tfsm_plan_destroy(self, p):
+            calls.append(("plan", p.value))
+
+    g = object.__new__(eng.StepGraph)
+    g._g = ctypes.c_void_p(0x1230)
+    plan = object.__new__(eng.SpectralPlan)
+    plan._plan = ctypes.c_void_p(0x4560)
+    plan._step_graph = (None, g)
+    monkeypatch.setattr(eng, "lib", FakeLib())
+    monkeypatch.setattr(eng, "ctypes", None)
+    g.__del__()       # the cached graph's finalizer first
+    plan.__del__()    # then the plan's: closes its cached graph again, then itself
+    plan.close()
+    g.close()
+    assert calls == [("graph", 0x1230), ("plan", 0x4560)]
+    with pytest.raises(RuntimeError, match="closed"):
+        g.launch()

@@ -137,11 +137,15 @@ class SpectralPlan:
 
     # ------------------------------------------------------------------ lifecycle
     def close(self):
+        """Destroy the native plan (idempotent).  The handle is cleared before the destroy call and
+        without touching module globals: at interpreter shutdown this can run from a finalizer after
+        the module's globals are gone, and a handle left set would be destroyed a second time."""
         if "_step_graph" in self.__dict__:
             self._drop_step_graph()
-        if getattr(self, "_plan", None) is not None and self._plan.value:
-            lib.tfsm_plan_destroy(self._plan)
-            self._plan = ctypes.c_void_p()
+        h = self.__dict__.get("_plan")
+        self._plan = None
+        if h is not None and h.value and lib is not None:
+            lib.tfsm_plan_destroy(h)
 
     def __del__(self):
         try:
@@ -472,8 +476,11 @@ class SpectralPlan:
                                                  ctypes.c_void_p(e0.cuda_event), ctypes.c_void_p(e1.cuda_event),
                                                  _stream(self.device)), "launch_pass_timed")
 
-    def graph(self, uh: torch.Tensor, nsteps: int, counts: Optional[torch.Tensor] = None) -> "StepGraph":
-        return StepGraph(self, uh, nsteps, counts)
+    def graph(self, uh: torch.Tensor, nsteps: int, counts: Optional[torch.Tensor] = None,
+              keep_plan: bool = True) -> "StepGraph":
+        """Capture nsteps into a HIP graph (StepGraph).  keep_plan=False for a graph the plan itself
+        holds (and closes with it), which must not keep the plan alive."""
+        return StepGraph(self, uh, nsteps, counts, keep_plan)
 
 
 class StepGraph:
@@ -481,8 +488,14 @@ class StepGraph:
     `counts` (device int32 [B]), nsteps of tfsm_step_varying: every replay reads the per-sample
     step counts from that tensor, which the caller updates before launching."""
 
-    def __init__(self, plan: SpectralPlan, uh: torch.Tensor, nsteps: int, counts: Optional[torch.Tensor] = None):
-        self.plan = plan
+    def __init__(self, plan: SpectralPlan, uh: torch.Tensor, nsteps: int, counts: Optional[torch.Tensor] = None,
+                 keep_plan: bool = True):
+        # the graph's kernels use the plan's tables: a graph the caller holds keeps the plan alive; a
+        # graph the plan caches for itself (keep_plan False) is closed by the plan's close() instead,
+        # and holding the plan there would make a reference cycle that only the cycle collector (or
+        # interpreter shutdown) frees
+        self.plan = plan if keep_plan else None
+        self.device = plan.device
         self.uh = uh
         self.nsteps = nsteps
         self.counts = counts
@@ -499,13 +512,17 @@ class StepGraph:
                                                     _stream(plan.device), ctypes.byref(self._g)), "graph")
 
     def launch(self):
-        with torch.cuda.device(self.plan.device):
-            check(lib.tfsm_graph_launch(self._g, _stream(self.plan.device)), "graph launch")
+        if self._g is None:
+            raise RuntimeError("graph launch: the graph is closed")
+        with torch.cuda.device(self.device):
+            check(lib.tfsm_graph_launch(self._g, _stream(self.device)), "graph launch")
 
     def close(self):
-        if self._g.value:
-            lib.tfsm_graph_destroy(self._g)
-            self._g = ctypes.c_void_p()
+        """Destroy the native graph (idempotent; the handle is cleared first, see SpectralPlan.close)."""
+        g = self.__dict__.get("_g")
+        self._g = None
+        if g is not None and g.value and lib is not None:
+            lib.tfsm_graph_destroy(g)
 
     def __del__(self):
         try:

@@ -581,7 +581,7 @@ class OperatorLike(_MutableMixIn):
                 cache = getattr(plan, "_step_graph", None)
                 if cache is None:
                     buf = plan.empty_spectrum()
-                    cache = plan._step_graph = (buf, plan.graph(buf, GRAPH_STEPS))
+                    cache = plan._step_graph = (buf, plan.graph(buf, GRAPH_STEPS, keep_plan=False))
                 buf, graph = cache
                 buf.copy_(uh)
                 for _ in range(step // GRAPH_STEPS):
