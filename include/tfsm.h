@@ -131,8 +131,10 @@ int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int
 /* Current launch shape: out[6] = column tile, column variant, row variant, row workgroup,
  * sub-batch, streams.  A new plan starts from a default derived from its shape (2D: column tile
  * and radix variant per axis size and product term, 64-thread row groups for 64- and 128-point
- * rows, sub-batch sized to keep >= 512 column workgroups per launch; 3D: radix-4 z rows from 64
- * points).
+ * rows, sub-batch sized to keep >= 512 column workgroups per launch, and for 512-point
+ * VorticityConvection columns with 32 or more samples sub-batches of 16 on two streams; 3D: radix-4
+ * z rows from 64 points, and for 128^3 Convection with 4 or more samples sub-batches of 2 on two
+ * streams).
  * (No reference counterpart: launch configuration.) */
 int tfsm_plan_get_tuning(const tfsm_plan* plan, int32_t* out);
 
@@ -140,8 +142,9 @@ int tfsm_plan_get_tuning(const tfsm_plan* plan, int32_t* out);
  * (1 <= nstreams <= TFSM_MAX_STREAMS): sub-batch k runs on the caller's stream when
  * k % nstreams == 0 and otherwise on one of the plan's auxiliary streams (default priority),
  * forked from and joined back into the caller's stream with events, so that their kernels
- * overlap; 1 (the default, except 2 for 512^2 VorticityConvection with 32 or more samples) keeps
- * one stream.  Results are identical.  (No reference counterpart: launch configuration.) */
+ * overlap; steps are queued in rounds over the sub-batches so that every stream stays fed.  1
+ * (the default, except for the two-stream defaults of tfsm_plan_get_tuning) keeps one stream.
+ * Results are identical.  (No reference counterpart: launch configuration.) */
 #define TFSM_MAX_STREAMS 4
 int tfsm_plan_set_streams(tfsm_plan* plan, int32_t nstreams);
 
