@@ -122,8 +122,9 @@ int64_t tfsm_plan_inverse_scratch_bytes(const tfsm_plan* plan, int32_t nbatch);
 /* Launch shape of the fused 2D kernels (chosen per shape class by the autotuner): column tile
  * width (8 or 16 columns), radix-plan variant of the column and row transforms (0 or 1: variant 1
  * carries a transform with half as many threads), row-kernel workgroup size (64 or 256 threads),
- * and the sub-batch of samples stepped together through all steps of a tfsm_step call (0 = the
- * whole batch; smaller sub-batches keep their working set in the last-level cache).
+ * and the sub-batch of samples stepped together (through all steps of a tfsm_step call on one
+ * stream, in rounds of 16 steps on several; 0 = the whole batch; smaller sub-batches keep their
+ * working set in the last-level cache).
  * (No reference counterpart: launch configuration of the engine's kernels.) */
 int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int32_t row_var, int32_t row_wg,
                          int32_t sub_batch);
