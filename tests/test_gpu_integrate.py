@@ -411,6 +411,44 @@ def test_ns512_b64_workload_full_size(dev):
     assert err <= 1e-5
 
 
+def test_ns512_tail_sub_batch_on_two_streams(dev):
+    """NS 512^2 with B = 36: the default shape steps sub-batches of 16, 16 and 4 samples on two
+    streams (the last one a tail whose column launches do not fill the XCD groups, and whose rows
+    still go in pairs).  Samples 0, 17 and 35 (one per sub-batch) against the oracle after 5 steps,
+    and the whole batch equals one-stream, whole-batch stepping of the same kernels' results to
+    float32 rounding of identical operations (bit for bit)."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import field, pde
+    from torchfsm_amd.integrator import SETDRKIntegrator
+    from torchfsm_amd.workloads import ns_initial_vorticity
+
+    n, B, dt, steps = 512, 36, 1e-3, 5
+    mi = [(0, 2 * math.pi, n)] * 2
+    mesh = tfa.MeshGrid(mi, device=dev)
+    y = mesh.bc_mesh_grid()[1]
+    u0 = ns_initial_vorticity(n, B, seed=4)
+    op = pde.NavierStokesVorticity(1000.0, field.kolm_force(y))
+    op.set_integrator(SETDRKIntegrator.SETDRK4)
+    op.register_mesh(tfa.FourierMesh(mesh), 1)
+    plan = op._prepare(B, dev, dt)
+    assert plan.tuning[4] == 16 and plan.streams == 2
+    uh = plan.forward(u0.to(dev))
+    plan.step(uh, steps)
+    out = plan.inverse(uh)
+    terms = refpath.ns_terms(1000.0, y.cpu())
+    for b in (0, 17, 35):
+        ref = refpath.integrate(terms, mi, u0[b:b + 1], dt, steps, scheme="SETDRK4")
+        e = _rel(out[b:b + 1].cpu().numpy(), ref.numpy())
+        print(f"sample {b}: rel-L2 {e:.2e}")
+        assert e <= TOL
+    plan.set_tuning(*plan.tuning[:4], 0)
+    plan.set_streams(1)
+    uw = plan.forward(u0.to(dev))
+    plan.step(uw, steps)
+    assert torch.equal(uw, uh)
+
+
 def test_ns512_b64_1000_steps_vs_oracle_f32_and_f64(dev):
     """The benchmark workload over its whole length: NavierStokesVorticity(Re=1000) with kolm_force,
     512^2, B = 64, SETDRK4, dt 1e-3, 1000 steps, default launch shape.  Samples 0, 31, 32 and 63
