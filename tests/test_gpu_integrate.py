@@ -214,6 +214,40 @@ def test_burgers3d_128_b8_etdrk2_tracks_oracle(dev):
     assert max(e) <= TOL
 
 
+def test_burgers3d_128_tail_sub_batch_on_two_streams(dev):
+    """3D Burgers 128^3 with B = 5: the default shape steps sub-batches of 2, 2 and 1 samples on two
+    streams (the last a one-sample tail).  Samples 0, 2 and 4 (one per sub-batch) against the oracle
+    after 6 SETDRK4 steps, and the whole batch equals one-stream, whole-batch stepping bit for bit."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import pde
+    from torchfsm_amd.integrator import SETDRKIntegrator
+    from torchfsm_amd.workloads import fourier_modes_field
+
+    mi = [(0, 1, 128)] * 3
+    B, dt, steps = 5, 1e-3, 6
+    u0 = fourier_modes_field(128, 3, 3, B, seed=2)
+    op = pde.Burgers(0.01)
+    op.set_integrator(SETDRKIntegrator.SETDRK4)
+    op.register_mesh(tfa.FourierMesh(mi, device=dev), 3)
+    plan = op._prepare(B, dev, dt)
+    assert plan.tuning[4] == 2 and plan.streams == 2
+    uh = plan.forward(u0.to(dev))
+    plan.step(uh, steps)
+    out = plan.inverse(uh).double()
+    for b in (0, 2, 4):
+        ref = refpath.integrate(refpath.burgers_terms(0.01), mi, u0[b:b + 1].to(dev), dt, steps,
+                                scheme="SETDRK4").double()
+        e = float((out[b:b + 1] - ref).norm() / ref.norm())
+        print(f"sample {b}: rel-L2 {e:.2e}")
+        assert e <= TOL
+    plan.set_tuning(*plan.tuning[:4], 0)
+    plan.set_streams(1)
+    uw = plan.forward(u0.to(dev))
+    plan.step(uw, steps)
+    assert torch.equal(uw, uh)
+
+
 def test_nonlinear_terms_golden(dev):
     """One evaluation through Operator.__call__(return_in_fourier=True) (_base.py:753-790)."""
     import torchfsm_amd as tfa
