@@ -361,3 +361,39 @@ def test_native_handles_are_destroyed_once_even_at_shutdown(monkeypatch):
     assert calls == [("graph", 0x1230), ("plan", 0x4560)]
     with pytest.raises(RuntimeError, match="closed"):
         g.launch()
+
+
+def test_autotune_keeps_the_starting_shape_in_the_final_comparison(monkeypatch):
+    """Short trials that rank the plan's own launch shape outside the top three must not lose it:
+    the sustained comparison always includes the shape the search started from."""
+    from torchfsm_amd import autotune as at
+
+    class FakePlan:
+        general = False
+        persistent_supported = False
+        ndim, n, batch, channels = 2, (512, 512), 64, 1
+        nonlinear, scheme = "vc", "SETDRK4"
+        spec_shape = (64, 1, 512, 272)
+
+        def __init__(self):
+            self.tuning, self.streams = (16, 0, 0, 256, 16), 2
+
+        def set_tuning(self, *t):
+            self.tuning = tuple(t)
+
+        def set_streams(self, n):
+            self.streams = n
+
+    start = (16, 0, 0, 256, 16, 2)
+
+    def shape(p):
+        return tuple(p.tuning) + (p.streams,)
+
+    # short trials rank the starting shape last, sustained runs rank it first
+    monkeypatch.setattr(at, "time_steps", lambda p, uh, steps=6: 2.0 if shape(p) == start else 1.0 + 0.01 * p.tuning[0])
+    monkeypatch.setattr(at, "time_steps_sustained", lambda p, uh, steps=200: 0.5 if shape(p) == start else 0.9)
+    monkeypatch.setattr(at, "_CACHE", {})
+    plan = FakePlan()
+    best, results = at.autotune(plan, None)
+    assert best == start and shape(plan) == start
+    assert start in results and len(results) > 4

@@ -6,7 +6,9 @@ radix-plan variant {0, 1} x row radix-plan variant {0, 1} x row workgroup size {
 the best tiled launch shape, the temporal-blocking sub-batch (samples stepped together), and then
 whether alternate sub-batches run concurrently on a second stream.
 Candidates are timed on the plan's own buffers (HIP events around a few full steps on the
-current stream) and the fastest is kept for the process; results can be persisted as JSON.
+current stream); the three fastest and the shape the plan started from are then re-timed over one
+long step call (wall clock), and the fastest of those is kept for the process; results can be
+persisted as JSON.
 Choices are fixed per shape class, never per batch occupancy, so results do not depend on the mix.
 """
 
@@ -96,6 +98,12 @@ def autotune(plan, uh, steps=6, log=None, cache_file: Optional[str] = None, fina
         _apply(plan, _CACHE[key])
         return _CACHE[key], {}
     results = {}
+    # the plan's own shape (its default for this shape class, or what the caller set) always takes
+    # part in the sustained comparison below, so the search never ends on a slower shape than it
+    # started from only because short trials ranked it outside the top three
+    start = tuple(plan.tuning) + (plan.streams,)
+    if plan.persistent_supported:
+        start += (int(plan.persistent),)
 
     def trial(cand):
         if cand in results:
@@ -117,8 +125,9 @@ def autotune(plan, uh, steps=6, log=None, cache_file: Optional[str] = None, fina
             trial(shape[:4] + (sb, 2))
     # short trials are noisy and favour bursts: re-time the three fastest over sustained runs
     # (wall clock around a long step call, as a production run sees it) before choosing
+    trial(start)
     final = {}
-    for cand in sorted(results, key=results.get)[:3]:
+    for cand in dict.fromkeys(sorted(results, key=results.get)[:3] + [start]):
         _apply(plan, cand)
         final[cand] = time_steps_sustained(plan, uh, final_steps)
         if log:
