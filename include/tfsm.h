@@ -134,8 +134,8 @@ int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int
  * and radix variant per axis size and product term, 64-thread row groups for 64- and 128-point
  * rows, sub-batch sized to keep >= 512 column workgroups per launch, and for 512-point
  * VorticityConvection columns with 32 or more samples sub-batches of 16 on two streams; 3D: radix-4
- * z rows from 64 points, and for 128^3 Convection with 4 or more samples sub-batches of 2 on two
- * streams).
+ * z rows from 64 points, for 128^3 Convection with 4 or more samples sub-batches of 2 on two
+ * streams, and for 64^3 Convection with 8 or more samples sub-batches of 2 on four streams).
  * (No reference counterpart: launch configuration.) */
 int tfsm_plan_get_tuning(const tfsm_plan* plan, int32_t* out);
 
@@ -144,7 +144,7 @@ int tfsm_plan_get_tuning(const tfsm_plan* plan, int32_t* out);
  * k % nstreams == 0 and otherwise on one of the plan's auxiliary streams (default priority),
  * forked from and joined back into the caller's stream with events, so that their kernels
  * overlap; steps are queued in rounds over the sub-batches so that every stream stays fed.  1
- * (the default, except for the two-stream defaults of tfsm_plan_get_tuning) keeps one stream.
+ * (the default, except for the multi-stream defaults of tfsm_plan_get_tuning) keeps one stream.
  * Results are identical.  (No reference counterpart: launch configuration.) */
 #define TFSM_MAX_STREAMS 4
 int tfsm_plan_set_streams(tfsm_plan* plan, int32_t nstreams);
