@@ -293,6 +293,50 @@ def test_burgers3d_128_tail_sub_batch_on_two_streams(dev):
     assert torch.equal(uw, uh)
 
 
+def test_burgers3d_64_default_four_streams(dev):
+    """3D Burgers 64^3 with B = 9: the default shape steps sub-batches of 2 (and a one-sample tail)
+    on four streams, so the fifth sub-batch shares the first one's stream.  Samples 0, 5 and 8
+    against the oracle after 10 ETDRK2 steps, and the whole batch equals one-stream, whole-batch
+    stepping bit for bit.  These fields reach |u| ~ 4, where the oracle's own float32 and float64
+    results differ by ~6e-5: the bar is 1e-5 of the oracle's float32 result, or no farther from
+    its float64 result than its float32 run (1.5x) and within 2x of that floor."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import pde
+    from torchfsm_amd.integrator import ETDRKIntegrator
+    from torchfsm_amd.workloads import fourier_modes_field
+
+    mi = [(0, 1, 64)] * 3
+    B, dt, steps = 9, 2e-3, 10
+    u0 = fourier_modes_field(64, 3, 3, B, seed=3)
+    op = pde.Burgers(0.01)
+    op.set_integrator(ETDRKIntegrator.ETDRK2)
+    op.register_mesh(tfa.FourierMesh(mi, device=dev), 3)
+    plan = op._prepare(B, dev, dt)
+    assert plan.tuning[4] == 2 and plan.streams == 4
+    uh = plan.forward(u0.to(dev))
+    plan.step(uh, steps)
+    out = plan.inverse(uh).double()
+
+    def rel(a, b):
+        return float((a - b).norm() / b.norm())
+
+    for b in (0, 5, 8):
+        ref = {dt_: refpath.integrate(refpath.burgers_terms(0.01), mi, u0[b:b + 1].to(dev, dt_), dt, steps,
+                                      scheme="ETDRK2").double() for dt_ in (torch.float32, torch.float64)}
+        e32, e64 = rel(out[b:b + 1], ref[torch.float32]), rel(out[b:b + 1], ref[torch.float64])
+        floor = rel(ref[torch.float32], ref[torch.float64])
+        print(f"sample {b}: vs oracle f32 {e32:.2e}, vs f64 {e64:.2e}; oracle f32 vs f64 {floor:.2e}")
+        if e32 > TOL:
+            assert e64 <= 1.5 * floor, (b, e64, floor)
+            assert e32 <= 2 * floor, (b, e32, floor)
+    plan.set_tuning(*plan.tuning[:4], 0)
+    plan.set_streams(1)
+    uw = plan.forward(u0.to(dev))
+    plan.step(uw, steps)
+    assert torch.equal(uw, uh)
+
+
 def test_nonlinear_terms_golden(dev):
     """One evaluation through Operator.__call__(return_in_fourier=True) (_base.py:753-790)."""
     import torchfsm_amd as tfa
