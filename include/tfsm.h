@@ -134,8 +134,8 @@ int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int
  * and radix variant per axis size and product term, 64-thread row groups for 64- and 128-point
  * rows, sub-batch sized to keep >= 512 column workgroups per launch, and for 512-point
  * VorticityConvection columns with 32 or more samples sub-batches of 16 on two streams; 3D: radix-4
- * z rows from 64 points, for 128^3 Convection with 4 or more samples sub-batches of 2 on two
- * streams, and for 64^3 Convection with 8 or more samples sub-batches of 2 on four streams).
+ * z rows from 64 points, and for Convection and NSPressureConvection sub-batches of 2 on two
+ * streams at 128^3 with 4 or more samples, on four streams at 64^3 with 8 or more).
  * (No reference counterpart: launch configuration.) */
 int tfsm_plan_get_tuning(const tfsm_plan* plan, int32_t* out);
 

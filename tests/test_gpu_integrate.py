@@ -337,6 +337,40 @@ def test_burgers3d_64_default_four_streams(dev):
     assert torch.equal(uw, uh)
 
 
+@pytest.mark.parametrize("n,B,ns", [(64, 8, 4), (128, 4, 2)])
+def test_navier_stokes3d_default_sub_batches(dev, n, B, ns):
+    """3D NavierStokes (velocity form, NSPressureConvection) steps by default in sub-batches of 2 on
+    four streams at 64^3 (B >= 8) and on two at 128^3 (B >= 4).  The first and last samples against
+    the oracle after 5 SETDRK4 steps, and the whole batch equals one-stream, whole-batch stepping
+    bit for bit."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import pde
+    from torchfsm_amd.workloads import fourier_modes_field
+
+    mi = [(0, 1, n)] * 3
+    dt, steps = 1e-3, 5
+    u0 = fourier_modes_field(n, 3, 3, B, seed=6)
+    op = pde.NavierStokes(100.0)
+    op.register_mesh(tfa.FourierMesh(mi, device=dev), 3)
+    plan = op._prepare(B, dev, dt)
+    assert plan.tuning[4] == 2 and plan.streams == ns and not plan.general
+    uh = plan.forward(u0.to(dev))
+    plan.step(uh, steps)
+    out = plan.inverse(uh).double()
+    for b in (0, B - 1):
+        ref = refpath.integrate(refpath.ns_velocity_terms(100.0), mi, u0[b:b + 1].to(dev), dt, steps,
+                                scheme="SETDRK4").double()
+        e = float((out[b:b + 1] - ref).norm() / ref.norm())
+        print(f"sample {b}: rel-L2 {e:.2e}")
+        assert e <= TOL
+    plan.set_tuning(*plan.tuning[:4], 0)
+    plan.set_streams(1)
+    uw = plan.forward(u0.to(dev))
+    plan.step(uw, steps)
+    assert torch.equal(uw, uh)
+
+
 def test_nonlinear_terms_golden(dev):
     """One evaluation through Operator.__call__(return_in_fourier=True) (_base.py:753-790)."""
     import torchfsm_amd as tfa

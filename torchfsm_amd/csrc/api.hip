@@ -1394,19 +1394,19 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
   if (!rc && D == 2 && p->fused_ok && run2d_supported(term, p->n[0], p->n[1], p->PF) && !p->cplx)
     p->persistent = true;
   if (!rc && D == 3) p->row_var = (p->n[2] >= 64 && is_pow2(p->n[2])) ? 1 : 0;  // radix-4 z rows (fewer registers)
-  // 3D Convection at 128^3 with 4 or more samples: sub-batches of 2 on two streams, so one
-  // sub-batch's VALU-bound z-row pass overlaps the other's memory-bound column passes (Burgers,
-  // B = 8: ETDRK2 1.52 -> 1.34 ms/step, SETDRK4 3.04 -> 2.71)
-  if (!rc && D == 3 && p->fused_ok && term == TERM_CONV && p->n[0] == 128 && p->n[1] == 128 && p->n[2] == 128 &&
-      p->B >= 4) {
+  // 3D Convection and NSPressureConvection at 128^3 with 4 or more samples: sub-batches of 2 on two
+  // streams, so one sub-batch's VALU-bound z-row pass overlaps the other's memory-bound column
+  // passes (Burgers, B = 8: ETDRK2 1.52 -> 1.34 ms/step, SETDRK4 3.04 -> 2.71; NavierStokes SETDRK4,
+  // B = 4: 1.74 -> 1.66)
+  const bool conv3 = D == 3 && p->fused_ok && (term == TERM_CONV || term == TERM_NSP);
+  if (!rc && conv3 && p->n[0] == 128 && p->n[1] == 128 && p->n[2] == 128 && p->B >= 4) {
     p->sub_batch = 2;
     rc = tfsm_plan_set_streams(p, 2);
   }
   // 64^3 with 8 or more samples: sub-batches of 2 on four streams (a 2-sample pass is too short to
   // fill the GPU alone; four in flight do): Burgers SETDRK4, B = 8: 0.41 -> 0.37 ms/step, B = 16:
-  // 0.93 -> 0.78
-  if (!rc && D == 3 && p->fused_ok && term == TERM_CONV && p->n[0] == 64 && p->n[1] == 64 && p->n[2] == 64 &&
-      p->B >= 8) {
+  // 0.93 -> 0.78; NavierStokes SETDRK4, B = 16: 1.09 -> 0.91
+  if (!rc && conv3 && p->n[0] == 64 && p->n[1] == 64 && p->n[2] == 64 && p->B >= 8) {
     p->sub_batch = 2;
     rc = tfsm_plan_set_streams(p, 4);
   }
