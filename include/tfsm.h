@@ -132,8 +132,9 @@ int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int
 /* Current launch shape: out[6] = column tile, column variant, row variant, row workgroup,
  * sub-batch, streams.  A new plan starts from a default derived from its shape (2D: column tile
  * and radix variant per axis size and product term, 64-thread row groups for 64- and 128-point
- * rows, sub-batch sized to keep >= 512 column workgroups per launch, and for 512-point
- * VorticityConvection columns with 32 or more samples sub-batches of 16 on two streams; 3D: radix-4
+ * rows, sub-batch sized to keep >= 512 column workgroups per launch, and sub-batches of 16 on two
+ * streams for 512^2 VorticityConvection, Convection and NSPressureConvection with 32 or more
+ * samples, on four streams for 256^2 VorticityConvection and Convection with 64 or more; 3D: radix-4
  * z rows from 64 points, and for Convection and NSPressureConvection sub-batches of 2 on two
  * streams at 128^3 with 4 or more samples, on four streams at 64^3 with 8 or more).
  * (No reference counterpart: launch configuration.) */

@@ -371,6 +371,45 @@ def test_navier_stokes3d_default_sub_batches(dev, n, B, ns):
     assert torch.equal(uw, uh)
 
 
+@pytest.mark.parametrize("term,n,B,ns", [("conv", 512, 32, 2), ("nsp", 512, 32, 2), ("vc", 256, 64, 4),
+                                         ("conv", 256, 64, 4)])
+def test_2d_default_sub_batches_on_streams(dev, term, n, B, ns):
+    """2D shape classes whose default steps 16-sample sub-batches on several streams: the first and
+    last samples against the oracle after 5 SETDRK4 steps, and the whole batch equals one-stream,
+    whole-batch stepping bit for bit."""
+    import torchfsm_amd as tfa
+    from oracle import refpath
+    from torchfsm_amd import field, pde
+    from torchfsm_amd.workloads import fourier_modes_field
+
+    dt, steps = 1e-3, 5
+    if term == "vc":
+        mi = [(0, 2 * math.pi, n)] * 2
+        y = tfa.MeshGrid(mi, device=dev).bc_mesh_grid()[1]
+        op, C, terms = pde.NavierStokesVorticity(1000.0, field.kolm_force(y)), 1, refpath.ns_terms(1000.0, y)
+    else:
+        mi = [(0, 1, n)] * 2
+        op, C = (pde.Burgers(0.01), 2) if term == "conv" else (pde.NavierStokes(100.0), 2)
+        terms = refpath.burgers_terms(0.01) if term == "conv" else refpath.ns_velocity_terms(100.0)
+    u0 = fourier_modes_field(n, 2, C, B, seed=8, length=mi[0][1])
+    op.register_mesh(tfa.FourierMesh(mi, device=dev), C)
+    plan = op._prepare(B, dev, dt)
+    assert plan.tuning[4] == 16 and plan.streams == ns and not plan.general
+    uh = plan.forward(u0.to(dev))
+    plan.step(uh, steps)
+    out = plan.inverse(uh).double()
+    for b in (0, B - 1):
+        ref = refpath.integrate(terms, mi, u0[b:b + 1].to(dev), dt, steps, scheme="SETDRK4").double()
+        e = float((out[b:b + 1] - ref).norm() / ref.norm())
+        print(f"{term} {n}^2 sample {b}: rel-L2 {e:.2e}")
+        assert e <= TOL
+    plan.set_tuning(*plan.tuning[:4], 0)
+    plan.set_streams(1)
+    uw = plan.forward(u0.to(dev))
+    plan.step(uw, steps)
+    assert torch.equal(uw, uh)
+
+
 def test_nonlinear_terms_golden(dev):
     """One evaluation through Operator.__call__(return_in_fourier=True) (_base.py:753-790)."""
     import torchfsm_amd as tfa
