@@ -132,9 +132,10 @@ int tfsm_plan_set_tuning(tfsm_plan* plan, int32_t col_tile, int32_t col_var, int
 /* Current launch shape: out[6] = column tile, column variant, row variant, row workgroup,
  * sub-batch, streams.  A new plan starts from a default derived from its shape (2D: column tile
  * and radix variant per axis size and product term, 64-thread row groups for 64- and 128-point
- * rows, sub-batch sized to keep >= 512 column workgroups per launch, and sub-batches of 16 on two
- * streams for 512^2 VorticityConvection, Convection and NSPressureConvection with 32 or more
- * samples, on four streams for 256^2 VorticityConvection and Convection with 64 or more; 3D: radix-4
+ * rows, sub-batch sized to keep >= 512 column workgroups per launch, and sub-batches on two
+ * streams for 512^2 VorticityConvection (16 samples, B >= 32), Convection and
+ * NSPressureConvection (16 samples from B = 32, 8 at B = 16..31) and 256^2 VorticityConvection
+ * and Convection (16 samples, B >= 32; four streams from B = 64); 3D: radix-4
  * z rows from 64 points, and for Convection and NSPressureConvection sub-batches of 2 on two
  * streams at 128^3 with 4 or more samples, on four streams at 64^3 with 8 or more).
  * (No reference counterpart: launch configuration.) */

@@ -1390,17 +1390,19 @@ int tfsm_plan_create(const tfsm_geometry* geom, const tfsm_terms* terms, void* s
       rc = tfsm_plan_set_streams(p, 2);
     }
     // the same overlap for the other square shape classes where sweeps showed it (SETDRK4, 200
-    // steps): 512^2 Convection and NSPressureConvection, B = 32, two streams (0.606 -> 0.544 and
-    // 0.734 -> 0.682 ms/step); 256^2 VorticityConvection and Convection, B = 64, four 16-sample
-    // sub-batches on four streams (0.259 -> 0.164, 0.337 -> 0.213)
+    // steps): 512^2 Convection and NSPressureConvection on two streams, sub-batches of 16 from
+    // B = 32 (0.606 -> 0.544 and 0.734 -> 0.682 ms/step), of 8 at B = 16..31 (Convection 0.285 ->
+    // 0.248); 256^2 VorticityConvection and Convection in 16-sample sub-batches, on two streams
+    // from B = 32 (0.130 -> 0.119, 0.169 -> 0.157) and on four from B = 64 (0.259 -> 0.164, 0.337
+    // -> 0.213)
     const bool square = p->n[0] == p->n[1];
-    if (!rc && square && n0 == 512 && (term == TERM_CONV || term == TERM_NSP) && p->B >= 32) {
-      p->sub_batch = 16;
+    if (!rc && square && n0 == 512 && (term == TERM_CONV || term == TERM_NSP) && p->B >= 16) {
+      p->sub_batch = p->B >= 32 ? 16 : 8;
       rc = tfsm_plan_set_streams(p, 2);
     }
-    if (!rc && square && n0 == 256 && (term == TERM_VC || term == TERM_CONV) && p->B >= 64) {
+    if (!rc && square && n0 == 256 && (term == TERM_VC || term == TERM_CONV) && p->B >= 32) {
       p->sub_batch = 16;
-      rc = tfsm_plan_set_streams(p, 4);
+      rc = tfsm_plan_set_streams(p, p->B >= 64 ? 4 : 2);
     }
   }
   // small square meshes step with the persistent per-sample kernel (one launch per call)
